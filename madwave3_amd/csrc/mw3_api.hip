@@ -1,0 +1,563 @@
+// mw3_api.hip -- C ABI of libmw3 (include/mw3.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "mw3_internal.h"
+
+using namespace mw3;
+
+static thread_local std::string g_create_err;
+
+namespace mw3 {
+
+void timer_begin(Ctx* c, int id) {
+  if (!c->timing) return;
+  Timer t;
+  if (!c->pool.empty()) {
+    t = c->pool.back();
+    c->pool.pop_back();
+  } else {
+    (void)hipEventCreate(&t.a);
+    (void)hipEventCreate(&t.b);
+  }
+  (void)hipEventRecord(t.a, c->stream);
+  c->pending.push_back({id, t});
+}
+
+void timer_end(Ctx* c, int id) {
+  if (!c->timing) return;
+  for (auto it = c->pending.rbegin(); it != c->pending.rend(); ++it)
+    if (it->first == id) {
+      (void)hipEventRecord(it->second.b, c->stream);
+      return;
+    }
+}
+
+static void timer_collect(Ctx* c) {
+  if (c->pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& p : c->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.second.a, p.second.b) == hipSuccess) {
+      c->t_ms[p.first] += ms;
+      c->t_n[p.first] += 1;
+    }
+    c->pool.push_back(p.second);
+  }
+  c->pending.clear();
+}
+
+}  // namespace mw3
+
+static int seterr(Ctx* c, int code, const char* what, hipError_t e = hipSuccess) {
+  char buf[512];
+  if (e != hipSuccess)
+    snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  else
+    snprintf(buf, sizeof buf, "%s", what);
+  if (c)
+    c->err = buf;
+  else
+    g_create_err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, call, what)                                      \
+  do {                                                               \
+    hipError_t _e = (call);                                          \
+    if (_e != hipSuccess) return seterr(ctx, MW3_EDEVICE, what, _e); \
+  } while (0)
+
+template <typename T>
+static hipError_t upload(void** dst, const std::vector<double>& v) {
+  std::vector<T> tmp(v.begin(), v.end());
+  size_t bytes = sizeof(T) * std::max<size_t>(tmp.size(), 1);
+  hipError_t e = hipMalloc(dst, bytes);
+  if (e != hipSuccess) return e;
+  if (tmp.empty()) return hipMemset(*dst, 0, bytes);
+  return hipMemcpy(*dst, tmp.data(), sizeof(T) * tmp.size(), hipMemcpyHostToDevice);
+}
+
+template <typename T>
+static hipError_t upload_raw(T** dst, const T* src, size_t n) {
+  size_t bytes = sizeof(T) * std::max<size_t>(n, 1);
+  hipError_t e = hipMalloc((void**)dst, bytes);
+  if (e != hipSuccess) return e;
+  if (n == 0) return hipMemset(*dst, 0, bytes);
+  return hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice);
+}
+
+static hipError_t alloc_zero(void** p, size_t bytes) {
+  bytes = std::max<size_t>(bytes, 8);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return e;
+  return hipMemset(*p, 0, bytes);
+}
+
+static void free_all(Ctx* c) {
+  void* ptrs[] = {c->d_pt_dense, c->d_mask, c->d_v, c->d_absfr1, c->d_absfr2, c->d_psi,
+                  c->d_psim1, c->d_psi0, c->d_a0, c->d_a0b, c->d_t1, c->d_t2, c->d_fbr,
+                  c->d_ang, c->d_hout, c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi,
+                  c->d_dst_tables, c->d_Dmat, c->d_acoef, c->d_bcoef, c->d_nbr,
+                  c->d_flux_dense, c->d_flux_col, c->d_zar1flux, c->d_ring_zr, c->d_ring_zdr,
+                  c->d_ring_cvj, c->d_zcr, c->d_zcdr, c->d_zs2, c->d_etot, c->d_fdv, c->d_dcvj,
+                  c->d_cvj_skip, c->d_norm_part, c->d_ring_norm, c->d_efe, c->d_zfe, c->d_phi,
+                  c->d_wmat, c->d_zero_skip, c->d_ch_elec, c->d_ch_omega, c->d_potiflux,
+                  c->d_flux_elec, c->d_scratch};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  free_plans(c);
+  for (auto& p : c->pending) c->pool.push_back(p.second);
+  c->pending.clear();
+  for (auto& t : c->pool) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  c->pool.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+extern "C" {
+
+int32_t mw3_abi_version(void) { return MW3_ABI_VERSION; }
+
+const char* mw3_last_error(const mw3_ctx* ctx) {
+  if (!ctx) return g_create_err.c_str();
+  return reinterpret_cast<const Ctx*>(ctx)->err.c_str();
+}
+
+const char* mw3_kernel_name(int32_t id) {
+  static const char* names[MW3_NKERNELS] = {"angular_fwd",   "angular_bwd",     "dst_R_fwd",
+                                            "dst_r_scale",   "dst_R_inv",       "update_fused",
+                                            "projections",   "energy_accum"};
+  if (id < 0 || id >= MW3_NKERNELS) return "";
+  return names[id];
+}
+
+int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
+  if (!out) return seterr(nullptr, MW3_EINVAL, "mw3_create: out is NULL");
+  *out = nullptr;
+  if (precision != MW3_FP64 && precision != MW3_MIXED)
+    return seterr(nullptr, MW3_EINVAL, "mw3_create: precision must be MW3_FP64 or MW3_MIXED");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return seterr(nullptr, MW3_EDEVICE, "mw3_create: no HIP device");
+  Ctx* c = new (std::nothrow) Ctx();
+  if (!c) return seterr(nullptr, MW3_ENOMEM, "mw3_create: out of host memory");
+  std::string err;
+  int rc = build_tables(setup, &c->t, &err);
+  if (rc != MW3_OK) {
+    delete c;
+    return seterr(nullptr, rc, err.c_str());
+  }
+  if (c->t.ne > 8) {
+    delete c;
+    return seterr(nullptr, MW3_EUNSUP, "mw3_create: more than 8 electronic states");
+  }
+  c->s = *setup;
+  c->precision = precision;
+  (void)hipGetDevice(&c->device);
+  const HostTables& t = c->t;
+  const size_t nd = (size_t)t.nc * t.nslab;
+  const size_t es = precision == MW3_MIXED ? sizeof(float) : sizeof(double);
+  hipError_t e;
+#define CK(call, what)                                            \
+  do {                                                            \
+    if ((e = (call)) != hipSuccess) {                             \
+      free_all(c);                                                \
+      delete c;                                                   \
+      return seterr(nullptr, e == hipErrorOutOfMemory ? MW3_ENOMEM : MW3_EDEVICE, what, e); \
+    }                                                             \
+  } while (0)
+  CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "mw3_create: stream");
+  CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
+  CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
+  if (precision == MW3_MIXED)
+    CK(upload<float>(&c->d_v, t.vdense), "mw3_create: V");
+  else
+    CK(upload<double>(&c->d_v, t.vdense), "mw3_create: V");
+  CK(upload_raw(&c->d_absfr1, setup->absfr1, (size_t)t.n1), "mw3_create: absfr1");
+  CK(upload_raw(&c->d_absfr2, setup->absfr2, (size_t)t.n2), "mw3_create: absfr2");
+  CK(alloc_zero((void**)&c->d_psi, nd * sizeof(double)), "mw3_create: psi");
+  CK(alloc_zero((void**)&c->d_psim1, nd * sizeof(double)), "mw3_create: psi_m1");
+  CK(alloc_zero((void**)&c->d_psi0, nd * sizeof(double)), "mw3_create: psi0");
+  CK(alloc_zero((void**)&c->d_hout, nd * sizeof(double)), "mw3_create: hout");
+  CK(alloc_zero((void**)&c->d_scratch, nd * sizeof(double)), "mw3_create: scratch");
+  if (setup->netot > 0) {
+    c->reacfct_host.assign(setup->reacfct, setup->reacfct + setup->netot);
+    c->etot_host.assign(setup->etots2, setup->etots2 + setup->netot);
+  }
+  CK(alloc_zero((void**)&c->d_a0, (size_t)t.ntot * sizeof(double)), "mw3_create: a0");
+  CK(alloc_zero((void**)&c->d_a0b, (size_t)t.ntot * sizeof(double)), "mw3_create: a0b");
+  CK(alloc_zero(&c->d_t1, nd * es), "mw3_create: work slab");
+  CK(alloc_zero(&c->d_t2, nd * es), "mw3_create: work slab");
+  CK(alloc_zero(&c->d_ang, nd * es), "mw3_create: angular slab");
+  CK(alloc_zero(&c->d_fbr, (size_t)t.nc * t.nJ * t.nplane * es), "mw3_create: FBR slab");
+  CK(alloc_zero(&c->d_phi, (size_t)t.nc * t.nJ * t.nplane * es), "mw3_create: FBR slab");
+  CK(upload_raw(&c->d_nn1, t.nn1.data(), t.nn1.size()), "mw3_create: tables");
+  CK(upload_raw(&c->d_p2r1, t.p2r1.data(), t.p2r1.size()), "mw3_create: tables");
+  CK(upload_raw(&c->d_p2r2, t.p2r2.data(), t.p2r2.size()), "mw3_create: tables");
+  CK(upload_raw(&c->d_divi, t.divi.data(), t.divi.size()), "mw3_create: tables");
+  if (precision == MW3_MIXED) {
+    CK(upload<float>(&c->d_dst_tables, t.dst_tables), "mw3_create: DST tables");
+    CK(upload<float>(&c->d_Dmat, t.Dmat), "mw3_create: D");
+  } else {
+    CK(upload<double>(&c->d_dst_tables, t.dst_tables), "mw3_create: DST tables");
+    CK(upload<double>(&c->d_Dmat, t.Dmat), "mw3_create: D");
+  }
+  CK(upload<double>(&c->d_acoef, t.acoef), "mw3_create: coefficients");
+  CK(upload<double>(&c->d_bcoef, t.bcoef), "mw3_create: coefficients");
+  CK(upload_raw(&c->d_nbr, t.nbr.data(), t.nbr.size()), "mw3_create: tables");
+  CK(upload_raw(&c->d_ch_elec, t.ch_elec.data(), t.ch_elec.size()), "mw3_create: tables");
+  CK(upload_raw(&c->d_ch_omega, t.ch_omega.data(), t.ch_omega.size()), "mw3_create: tables");
+  // projections
+  int netot = std::max(setup->netot, 0);
+  CK(upload_raw(&c->d_flux_dense, t.flux_dense.data(), t.flux_dense.size()), "mw3_create: flux");
+  CK(upload_raw(&c->d_flux_col, t.flux_col.data(), t.flux_col.size()), "mw3_create: flux");
+  CK(upload_raw(&c->d_zar1flux, t.zar1flux.data(), t.zar1flux.size()), "mw3_create: flux");
+  CK(upload_raw(&c->d_potiflux, t.potiflux.data(), t.potiflux.size()), "mw3_create: flux");
+  CK(upload_raw(&c->d_flux_elec, t.flux_elec.data(), t.flux_elec.size()), "mw3_create: flux");
+  {
+    std::vector<uint8_t> ones((size_t)std::max<int64_t>(t.nflux, 1), 1);
+    CK(upload_raw(&c->d_zero_skip, ones.data(), ones.size()), "mw3_create: flux");
+  }
+  CK(alloc_zero((void**)&c->d_ring_zr, sizeof(double) * kRing * t.nflux), "mw3_create: ring");
+  CK(alloc_zero((void**)&c->d_ring_zdr, sizeof(double) * 2 * kRing * t.nflux), "mw3_create: ring");
+  CK(alloc_zero((void**)&c->d_ring_cvj, sizeof(double) * kRing * t.ncvj), "mw3_create: ring");
+  CK(alloc_zero((void**)&c->d_zcr, sizeof(double) * 2 * t.nflux * netot), "mw3_create: zCR");
+  CK(alloc_zero((void**)&c->d_zcdr, sizeof(double) * 2 * t.nflux * netot), "mw3_create: zCdR");
+  CK(alloc_zero((void**)&c->d_zs2, sizeof(double) * 2 * (size_t)t.ncvj * netot), "mw3_create: zS2");
+  CK(alloc_zero((void**)&c->d_wmat, sizeof(double) * 2 * kRing * netot), "mw3_create: W");
+  if (netot > 0)
+    CK(upload_raw(&c->d_etot, setup->etots2, (size_t)netot), "mw3_create: energies");
+  CK(upload_raw(&c->d_fdv, t.fdv.data(), t.fdv.size()), "mw3_create: fd");
+  CK(upload_raw(&c->d_dcvj, t.dcvj.data(), t.dcvj.size()), "mw3_create: D");
+  CK(upload_raw(&c->d_cvj_skip, t.cvj_isref.data(), t.cvj_isref.size()), "mw3_create: flags");
+  c->nnorm_blocks = 1024;
+  CK(alloc_zero((void**)&c->d_norm_part, sizeof(double) * 2 * c->nnorm_blocks), "mw3_create: norms");
+  CK(alloc_zero((void**)&c->d_ring_norm, sizeof(double) * 2 * kRing), "mw3_create: norms");
+  if (setup->nofe > 0) {
+    if (!setup->efe) {
+      free_all(c);
+      delete c;
+      return seterr(nullptr, MW3_EINVAL, "mw3_create: nofe > 0 needs efe");
+    }
+    CK(upload_raw(&c->d_efe, setup->efe, (size_t)setup->nofe), "mw3_create: efe");
+    CK(alloc_zero((void**)&c->d_zfe, sizeof(double) * 2 * setup->nofe * nd), "mw3_create: zfe");
+  }
+  CK(build_plans(c), "mw3_create: plans");
+  CK(hipStreamSynchronize(c->stream), "mw3_create: sync");
+#undef CK
+  *out = reinterpret_cast<mw3_ctx*>(c);
+  return MW3_OK;
+}
+
+void mw3_destroy(mw3_ctx* ctx) {
+  if (!ctx) return;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+}
+
+int64_t mw3_ntot(const mw3_ctx* ctx) { return ctx ? reinterpret_cast<const Ctx*>(ctx)->t.ntot : -1; }
+int64_t mw3_nflux(const mw3_ctx* ctx) {
+  return ctx ? reinterpret_cast<const Ctx*>(ctx)->t.nflux : -1;
+}
+int32_t mw3_cheby_k(const mw3_ctx* ctx) { return ctx ? reinterpret_cast<const Ctx*>(ctx)->k : -1; }
+
+int32_t mw3_hphi(mw3_ctx* ctx, const double* psi, double* hpsi) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !psi || !hpsi) return seterr(c, MW3_EINVAL, "mw3_hphi: NULL argument");
+  size_t n = (size_t)c->t.ntot;
+  HIPCHK(c, hipMemcpyAsync(c->d_a0, psi, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+         "mw3_hphi: upload");
+  HIPCHK(c, launch_scatter_a0(c, c->d_a0, c->d_scratch), "mw3_hphi: scatter");
+  HIPCHK(c, launch_hpsi(c, c->d_scratch), "mw3_hphi: H psi");
+  HIPCHK(c, launch_update(c, 0, c->d_scratch), "mw3_hphi: H psi");
+  HIPCHK(c, launch_gather_a0(c, c->d_hout, c->d_a0b), "mw3_hphi: gather");
+  HIPCHK(c, hipMemcpyAsync(hpsi, c->d_a0b, n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+         "mw3_hphi: download");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_hphi: sync");
+  return MW3_OK;
+}
+
+static int clear_accumulators(Ctx* c) {
+  const HostTables& t = c->t;
+  int netot = std::max(c->s.netot, 0);
+  HIPCHK(c, launch_zero(c, c->d_zcr, sizeof(double) * 2 * t.nflux * netot), "clear");
+  HIPCHK(c, launch_zero(c, c->d_zcdr, sizeof(double) * 2 * t.nflux * netot), "clear");
+  HIPCHK(c, launch_zero(c, c->d_zs2, sizeof(double) * 2 * (size_t)t.ncvj * netot), "clear");
+  if (c->s.nofe > 0)
+    HIPCHK(c, launch_zero(c, c->d_zfe, sizeof(double) * 2 * c->s.nofe * (size_t)t.nc * t.nslab),
+           "clear");
+  c->ring_fill = 0;
+  return MW3_OK;
+}
+
+int32_t mw3_cheby_begin(mw3_ctx* ctx, const double* psi0, double* stats3) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !psi0) return seterr(c, MW3_EINVAL, "mw3_cheby_begin: NULL argument");
+  size_t n = (size_t)c->t.ntot;
+  HIPCHK(c, hipMemcpyAsync(c->d_a0, psi0, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+         "mw3_cheby_begin: upload");
+  HIPCHK(c, launch_scatter_a0(c, c->d_a0, c->d_psi), "mw3_cheby_begin: scatter");
+  size_t bytes = sizeof(double) * (size_t)c->t.nc * c->t.nslab;
+  HIPCHK(c, hipMemcpyAsync(c->d_psi0, c->d_psi, bytes, hipMemcpyDeviceToDevice, c->stream),
+         "mw3_cheby_begin: copy");
+  // psi_1 = (H psi_0 - emindlt psi_0)/delta2 is written over psim1; the swap
+  // then leaves psi_k = psi_1 and psi_{k-1} = psi_0
+  HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_begin: H psi0");
+  if (stats3) {
+    HIPCHK(c, launch_update(c, 0, c->d_psi), "mw3_cheby_begin: H psi0");
+    std::vector<double> h(n), p(psi0, psi0 + n);
+    HIPCHK(c, launch_gather_a0(c, c->d_hout, c->d_a0b), "mw3_cheby_begin: gather");
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->d_a0b, n * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream),
+           "mw3_cheby_begin: download");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_cheby_begin: sync");
+    double em = 0, x1 = 0, x2 = 0;
+    for (size_t i = 0; i < n; i++) {
+      em += p[i] * h[i];
+      x1 += p[i] * p[i];
+      x2 += h[i] * h[i];
+    }
+    stats3[0] = em;
+    stats3[1] = x1;
+    stats3[2] = x2;
+  }
+  c->ring_fill = 0;
+  HIPCHK(c, launch_update(c, 1, c->d_psi), "mw3_cheby_begin: first step");
+  std::swap(c->d_psi, c->d_psim1);
+  c->k = 1;
+  c->begun = true;
+  int rc = clear_accumulators(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_cheby_begin: sync");
+  return MW3_OK;
+}
+
+static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
+  if (c->ring_fill == 0) return MW3_OK;
+  HIPCHK(c, launch_energy_accum(c), "energy accumulation");
+  std::vector<double> nb(2 * (size_t)c->ring_fill);
+  HIPCHK(c, hipMemcpyAsync(nb.data(), c->d_ring_norm, nb.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, c->stream),
+         "norms");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+  for (int i = 0; i < c->ring_fill; i++) {
+    if (norms_dst) norms_dst[i] = nb[2 * i];
+    if (auto_dst) auto_dst[i] = nb[2 * i + 1];
+  }
+  c->ring_k0 += c->ring_fill;
+  c->ring_fill = 0;
+  return MW3_OK;
+}
+
+int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* autocorr) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  if (!c->begun) return seterr(c, MW3_ESTATE, "mw3_cheby_steps: call mw3_cheby_begin first");
+  if (nsteps < 0) return seterr(c, MW3_EINVAL, "mw3_cheby_steps: nsteps < 0");
+  int done = 0;
+  while (done < nsteps) {
+    if (c->ring_fill == 0) c->ring_k0 = c->k + 1;
+    int base = done - c->ring_fill;  // output index of ring slot 0
+    // one step: H psi_k, update into psim1 (psi_{k+1}), projections, swap
+    HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
+    HIPCHK(c, launch_update(c, 2, c->d_psi), "mw3_cheby_steps: update");
+    HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
+    c->k += 1;
+    HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
+    std::swap(c->d_psi, c->d_psim1);
+    c->ring_fill += 1;
+    done += 1;
+    if (c->ring_fill == kRing || done == nsteps) {
+      int rc = flush_ring(c, norms ? norms + base : nullptr, autocorr ? autocorr + base : nullptr);
+      if (rc) return rc;
+    }
+    if (c->timing) timer_collect(c);
+  }
+  return MW3_OK;
+}
+
+int32_t mw3_flux_result(mw3_ctx* ctx, double* S2prodtot, double* S2prodelec) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  int netot = c->s.netot, ne = c->t.ne;
+  if (netot <= 0) return MW3_OK;
+  std::vector<double> sums((size_t)(1 + ne) * netot, 0.0);
+  if (c->t.nflux > 0) {
+    double* d_out = nullptr;
+    HIPCHK(c, hipMalloc(&d_out, sizeof(double) * sums.size()), "mw3_flux_result: alloc");
+    hipError_t e = launch_flux_result(c, d_out);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sums.data(), d_out, sizeof(double) * sums.size(), hipMemcpyDeviceToHost,
+                         c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d_out);
+    HIPCHK(c, e, "mw3_flux_result");
+  }
+  for (int ie = 0; ie < netot; ie++) {
+    double f = c->reacfct_host[ie] / c->s.ah1;
+    if (S2prodtot) S2prodtot[ie] = sums[ie] * f;
+    if (S2prodelec)
+      for (int q = 0; q < ne; q++)
+        S2prodelec[(size_t)q * netot + ie] = sums[(size_t)(1 + q) * netot + ie] * f;
+  }
+  return MW3_OK;
+}
+
+int32_t mw3_zs2_get(mw3_ctx* ctx, double* zS2) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !zS2) return MW3_EINVAL;
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  size_t n = 2 * (size_t)c->t.ncvj * std::max(c->s.netot, 0);
+  HIPCHK(c, hipMemcpyAsync(zS2, c->d_zs2, n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+         "mw3_zs2_get");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_zs2_get");
+  return MW3_OK;
+}
+
+int32_t mw3_flux_accum_get(mw3_ctx* ctx, double* zCR, double* zCdR) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  int netot = std::max(c->s.netot, 0);
+  size_t n = 2 * (size_t)c->t.nflux * netot;
+  if (zCR)
+    HIPCHK(c, hipMemcpyAsync(zCR, c->d_zcr, n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+           "mw3_flux_accum_get");
+  if (zCdR)
+    HIPCHK(c, hipMemcpyAsync(zCdR, c->d_zcdr, n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+           "mw3_flux_accum_get");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_flux_accum_get");
+  // the reference accumulates only where E > V(flux point)
+  for (int64_t f = 0; f < c->t.nflux; f++)
+    for (int e = 0; e < netot; e++)
+      if (!(c->etot_host[e] > c->t.potiflux[f])) {
+        if (zCR) zCR[2 * (f * netot + e)] = zCR[2 * (f * netot + e) + 1] = 0.0;
+        if (zCdR) zCdR[2 * (f * netot + e)] = zCdR[2 * (f * netot + e) + 1] = 0.0;
+      }
+  return MW3_OK;
+}
+
+int32_t mw3_flux_accum_put(mw3_ctx* ctx, const double* zCR, const double* zCdR,
+                           const double* zS2) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  int netot = std::max(c->s.netot, 0);
+  size_t n = 2 * (size_t)c->t.nflux * netot;
+  if (zCR)
+    HIPCHK(c, hipMemcpyAsync(c->d_zcr, zCR, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+           "mw3_flux_accum_put");
+  if (zCdR)
+    HIPCHK(c, hipMemcpyAsync(c->d_zcdr, zCdR, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+           "mw3_flux_accum_put");
+  if (zS2)
+    HIPCHK(c, hipMemcpyAsync(c->d_zs2, zS2, 2 * (size_t)c->t.ncvj * netot * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream),
+           "mw3_flux_accum_put");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_flux_accum_put");
+  return MW3_OK;
+}
+
+int32_t mw3_psie_get(mw3_ctx* ctx, double* zfe) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !zfe) return MW3_EINVAL;
+  if (c->s.nofe <= 0) return MW3_OK;
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  size_t nd = (size_t)c->t.nc * c->t.nslab;
+  std::vector<double> dense(2 * nd);
+  size_t n = (size_t)c->t.ntot;
+  for (int e = 0; e < c->s.nofe; e++) {
+    HIPCHK(c, hipMemcpyAsync(dense.data(), c->d_zfe + 2 * e * nd, 2 * nd * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream),
+           "mw3_psie_get");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_psie_get");
+    for (size_t i = 0; i < n; i++) {
+      zfe[2 * (e * n + i)] = dense[2 * (size_t)c->t.pt_dense[i]];
+      zfe[2 * (e * n + i) + 1] = dense[2 * (size_t)c->t.pt_dense[i] + 1];
+    }
+  }
+  return MW3_OK;
+}
+
+int32_t mw3_state_get(mw3_ctx* ctx, double* psi_km1, double* psi_k, int32_t* k) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  size_t n = (size_t)c->t.ntot;
+  if (psi_km1) {
+    HIPCHK(c, launch_gather_a0(c, c->d_psim1, c->d_a0), "mw3_state_get");
+    HIPCHK(c, hipMemcpyAsync(psi_km1, c->d_a0, n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+           "mw3_state_get");
+  }
+  if (psi_k) {
+    HIPCHK(c, launch_gather_a0(c, c->d_psi, c->d_a0b), "mw3_state_get");
+    HIPCHK(c, hipMemcpyAsync(psi_k, c->d_a0b, n * sizeof(double), hipMemcpyDeviceToHost, c->stream),
+           "mw3_state_get");
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_state_get");
+  if (k) *k = c->k;
+  return MW3_OK;
+}
+
+int32_t mw3_state_put(mw3_ctx* ctx, const double* psi_km1, const double* psi_k, int32_t k,
+                      const double* psi0) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !psi_km1 || !psi_k) return seterr(c, MW3_EINVAL, "mw3_state_put: NULL argument");
+  if (k < 1) return seterr(c, MW3_EINVAL, "mw3_state_put: k must be >= 1");
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  size_t n = (size_t)c->t.ntot;
+  HIPCHK(c, hipMemcpyAsync(c->d_a0, psi_km1, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+         "mw3_state_put");
+  HIPCHK(c, launch_scatter_a0(c, c->d_a0, c->d_psim1), "mw3_state_put");
+  HIPCHK(c, hipMemcpyAsync(c->d_a0b, psi_k, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+         "mw3_state_put");
+  HIPCHK(c, launch_scatter_a0(c, c->d_a0b, c->d_psi), "mw3_state_put");
+  if (psi0) {
+    HIPCHK(c, hipMemcpyAsync(c->d_a0, psi0, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+           "mw3_state_put");
+    HIPCHK(c, launch_scatter_a0(c, c->d_a0, c->d_psi0), "mw3_state_put");
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_state_put");
+  c->k = k;
+  c->begun = true;
+  return MW3_OK;
+}
+
+int32_t mw3_timing_enable(mw3_ctx* ctx, int32_t on) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  timer_collect(c);
+  c->timing = on != 0;
+  for (int i = 0; i < MW3_NKERNELS; i++) {
+    c->t_ms[i] = 0;
+    c->t_n[i] = 0;
+  }
+  return MW3_OK;
+}
+
+int32_t mw3_timing_get(mw3_ctx* ctx, int32_t id, double* total_ms, int64_t* launches) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || id < 0 || id >= MW3_NKERNELS) return MW3_EINVAL;
+  timer_collect(c);
+  if (total_ms) *total_ms = c->t_ms[id];
+  if (launches) *launches = c->t_n[id];
+  return MW3_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,199 @@
+// mw3_internal.h -- internal types of libmw3 (not part of the C ABI).
+//
+// Device data model (one context = one propagation on one GPU):
+//   dense slabs  X[ch][iang][ir1][ir2]   ir2 (R) fastest, iang = global angle-1
+//   mask         uint8 [iang][ir1][ir2]  1 on the kept (compressed) points
+//   V            [et][es][iang][ir1][ir2] (0 off the mask)
+//   FBR          Xt[ch][jj][ir1][ir2]    angular functions index jj
+// The compressed A0 vector of the ABI maps to the slabs through pt_dense.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mw3.h"
+
+namespace mw3 {
+
+constexpr int kRing = 64;  // Chebyshev steps buffered between energy GEMMs
+
+// Host-side operator tables built from mw3_setup (mw3_setup.cpp).
+struct HostTables {
+  // sizes
+  int n1 = 0, n2 = 0, na = 0, nc = 0, ne = 0, nJ = 0;
+  int64_t ntot = 0, nplane = 0, nslab = 0;  // nplane = n1*n2, nslab = na*nplane
+  // A0 <-> dense
+  std::vector<uint32_t> pt_dense;  // [ntot] dense index (ch,iang,ir1,ir2)
+  std::vector<uint8_t> mask;       // [na][n1][n2]
+  std::vector<double> vdense;      // [ne][ne][na][n1][n2]  (et, es)
+  std::vector<int32_t> ch_omega, ch_elec;
+  // Tradial: per global angle DST size in r and kinetic tables
+  std::vector<int32_t> nn1;     // [na]
+  std::vector<double> p2r1;     // [na][n1]
+  std::vector<double> p2r2;     // [n2]
+  std::vector<double> divi;     // [na]
+  double radcut = 0;
+  std::vector<int32_t> dst_sizes;        // distinct DST lengths (r sizes + n2)
+  std::vector<int64_t> dst_table_off;    // offset of each size's table
+  std::vector<double> dst_tables;        // concatenated n x n tables
+  // angular FBR
+  std::vector<int32_t> jlist;   // [nJ] j values of the FBR axis
+  std::vector<double> Dmat;     // [nc][na][nJ]   D_c(iang, j)
+  std::vector<double> acoef;    // [nc][nJ][n1]   j^2 term (angkinj2 FBR)
+  std::vector<double> bcoef;    // [nc][3][nJ][n2] l^2 term, q = Omega_s-Omega_t+1
+  std::vector<int32_t> nbr;     // [nc][3] source channel for q, -1 if none
+  // flux line
+  int ir4flux = 0;
+  std::vector<int32_t> ir1flux;          // [na]
+  std::vector<double> zar1flux;          // [n1][2]
+  int64_t nflux = 0;
+  std::vector<uint32_t> flux_dense;      // [nflux] dense index of the point
+  std::vector<uint32_t> flux_col;        // [nflux] dense index of (ch,iang,0,ir2)
+  std::vector<double> potiflux;          // [nflux]
+  std::vector<int32_t> flux_elec;        // [nflux] electronic state (0-based)
+  // reactant-state projection (Cvjflux_k)
+  int ncvj = 0;                          // nc * nj * nv
+  std::vector<double> fdv;               // [nc][nj][nv][n1] fd of the channel's state, 0 if unused
+  std::vector<double> dcvj;              // [nc][nj][na] Djmmp(iang, j, e, Omega)
+  std::vector<uint8_t> cvj_valid;        // [nc][nj][nv]
+  std::vector<uint8_t> cvj_isref;        // [nc][nj][nv]
+};
+
+// batched GEMM pass: C[m][n] = sum_k A[m][k] B[k][n] per descriptor
+struct GemmDesc {
+  int64_t a_off, b_off, c_off;
+  int M, N, K, lda_m;
+};
+
+struct GemmStrides {
+  int64_t lda_k;  // A element (m,k) at a_off + m*lda_m + k*lda_k
+  int64_t ldb_k, ldb_n;
+  int64_t ldc_m, ldc_n;
+};
+
+struct GemmPlan {
+  GemmDesc* d_descs = nullptr;
+  int nb = 0, maxM = 0, maxN = 0;
+  GemmStrides st{};
+};
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+struct Ctx {
+  mw3_setup s{};  // scalar copy (pointers not kept)
+  int precision = MW3_FP64;
+  HostTables t;
+  std::string err;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // device buffers
+  uint32_t* d_pt_dense = nullptr;
+  uint8_t* d_mask = nullptr;
+  void* d_v = nullptr;  // double or float dense V
+  double* d_absfr1 = nullptr;
+  double* d_absfr2 = nullptr;
+  double* d_psi = nullptr;    // psi_k dense
+  double* d_psim1 = nullptr;  // psi_{k-1} dense
+  double* d_psi0 = nullptr;   // rpaq0 dense
+  double* d_a0 = nullptr;     // A0 staging [ntot]
+  double* d_a0b = nullptr;
+  void* d_t1 = nullptr;       // radial work slab (T or float)
+  void* d_t2 = nullptr;       // radial work slab
+  void* d_fbr = nullptr;      // FBR slab [nc][nJ][n1][n2]
+  void* d_ang = nullptr;      // angular result slab [nc][na][n1][n2]
+  double* d_hout = nullptr;   // H psi dense (hphi mode)
+  double* d_scratch = nullptr; // input slab of mw3_hphi
+  std::vector<double> reacfct_host, etot_host;
+  // tables
+  int32_t* d_nn1 = nullptr;
+  double* d_p2r1 = nullptr;
+  double* d_p2r2 = nullptr;
+  double* d_divi = nullptr;
+  void* d_dst_tables = nullptr;
+  void* d_Dmat = nullptr;
+  void* d_acoef = nullptr;
+  void* d_bcoef = nullptr;
+  int32_t* d_nbr = nullptr;
+  // projections
+  uint32_t* d_flux_dense = nullptr;
+  uint32_t* d_flux_col = nullptr;
+  double* d_zar1flux = nullptr;
+  double* d_ring_zr = nullptr;   // [kRing][nflux]
+  double* d_ring_zdr = nullptr;  // [kRing][nflux][2]
+  double* d_ring_cvj = nullptr;  // [kRing][ncvj]
+  double* d_zcr = nullptr;       // [nflux][netot][2]
+  double* d_zcdr = nullptr;
+  double* d_zs2 = nullptr;       // [ncvj][netot][2]
+  double* d_etot = nullptr;
+  double* d_fdv = nullptr;
+  double* d_dcvj = nullptr;
+  uint8_t* d_cvj_skip = nullptr; // [ncvj] 1 if the reference state (kminelastic)
+  double* d_norm_part = nullptr; // partial sums [nblocks][2]
+  double* d_ring_norm = nullptr; // [kRing][2] (norm, autocorrelation)
+  double* d_efe = nullptr;
+  double* d_zfe = nullptr;       // [nofe][nc*nslab][2] dense
+  int nnorm_blocks = 0;
+  int ring_fill = 0;             // steps buffered in the ring
+  int ring_k0 = 0;               // k of ring slot 0
+  std::vector<double> host_norms;
+
+  // GEMM passes: R rows, r columns, angular forward, angular back
+  GemmPlan p_dstR, p_dstr, p_angF, p_angB;
+  void* d_phi = nullptr;        // FBR after the Omega stencil
+  double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors
+  uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)
+  int32_t* d_ch_elec = nullptr;
+  int32_t* d_ch_omega = nullptr;
+  double* d_potiflux = nullptr;
+  int32_t* d_flux_elec = nullptr;
+
+  // recursion state
+  int k = 0;
+  bool begun = false;
+
+  // timing
+  bool timing = false;
+  double t_ms[MW3_NKERNELS] = {};
+  int64_t t_n[MW3_NKERNELS] = {};
+  std::vector<std::pair<int, Timer>> pending;
+  std::vector<Timer> pool;
+};
+
+// mw3_setup.cpp
+int build_tables(const mw3_setup* s, HostTables* t, std::string* err);
+
+// mw3_kernels.hip: launchers, return hipError_t
+hipError_t launch_scatter_a0(Ctx* c, const double* a0, double* dense);
+hipError_t launch_gather_a0(Ctx* c, const double* dense, double* a0);
+hipError_t launch_hpsi(Ctx* c, const double* psi);  // T, V and angular terms into work slabs
+// 0: H psi into d_hout, 1: first step, 2: damped step (psi_new into d_psim1)
+hipError_t launch_update(Ctx* c, int mode, const double* psi);
+hipError_t launch_projections(Ctx* c, int slot);
+hipError_t launch_energy_accum(Ctx* c);
+hipError_t launch_psie(Ctx* c, int k);
+hipError_t launch_zero(Ctx* c, void* p, size_t bytes);
+hipError_t build_plans(Ctx* c);
+void free_plans(Ctx* c);
+hipError_t launch_flux_result(Ctx* c, double* d_out /* [(1+ne)][netot] */);
+
+enum KernelId {
+  K_ANG_FWD = 0,
+  K_ANG_BWD = 1,
+  K_DST_R = 2,
+  K_DST_r = 3,
+  K_DST_RINV = 4,
+  K_UPDATE = 5,
+  K_PROJ = 6,
+  K_ENERGY = 7,
+};
+
+void timer_begin(Ctx* c, int id);
+void timer_end(Ctx* c, int id);
+
+}  // namespace mw3

@@ -1,0 +1,717 @@
+// mw3_kernels.hip -- gfx950 kernels of libmw3.
+//
+// One Chebyshev step of SRC/main_madwave3.f:324-353 is:
+//   radial kinetic  (SRC/mod_Hphi_01y2.f:762-832): 2-D DST-I, scale, 2-D DST-I
+//                   as three passes over the dense slabs (R rows, r columns
+//                   with scaling, R rows);
+//   angular term    (SRC/mod_Hphi_01y2.f:859-902) in factor form:
+//                   FBR = D^T psi, stencil over Omega with the j^2 / l^2
+//                   coefficients, back with D;
+//   fused update    V psi (SRC/mod_Hphi_01y2.f:838-853), compressed-grid
+//                   projection, damped three-term update
+//                   (SRC/main_madwave3.f:332-339), norm partial sums;
+//   projections     flux-line samples (SRC/mod_flux_01y2.f:598-628) and
+//                   reactant-state amplitudes (SRC/mod_flux_01y2.f:672-702)
+//                   into a ring of kRing steps; the energy sums
+//                   sum_k C_k(E) x_k (CoefEcheby, SRC/main_madwave3.f:462-490)
+//                   run once per ring as a complex GEMM.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "mw3_internal.h"
+
+namespace mw3 {
+
+// ---------------------------------------------------------------------------
+// generic batched GEMM  C[m][n] = sum_k A[m][k] B[k][n]  (+ beta C)
+// ---------------------------------------------------------------------------
+constexpr int BM = 64, BN = 64, BK = 16;
+
+template <typename TA, typename TB, typename TC, typename TACC>
+__global__ __launch_bounds__(256) void k_gemm(const TA* __restrict__ A, const TB* __restrict__ B,
+                                              TC* __restrict__ C, const GemmDesc* __restrict__ descs,
+                                              GemmStrides st) {
+  const GemmDesc d = descs[blockIdx.z];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  if (m0 >= d.M || n0 >= d.N) return;
+  __shared__ TACC As[BK][BM + 1];
+  __shared__ TACC Bs[BK][BN];
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  TACC acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0;
+  const TA* Ab = A + d.a_off;
+  const TB* Bb = B + d.b_off;
+  for (int k0 = 0; k0 < d.K; k0 += BK) {
+    // A tile BM x BK: 1024 elements, 4 per thread
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      int e = threadIdx.x + l * 256;
+      int mm = e / BK, kk = e % BK;
+      int m = m0 + mm, k = k0 + kk;
+      TACC v = 0;
+      if (m < d.M && k < d.K) v = (TACC)Ab[(int64_t)m * d.lda_m + (int64_t)k * st.lda_k];
+      As[kk][mm] = v;
+    }
+    // B tile BK x BN: n fastest
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      int e = threadIdx.x + l * 256;
+      int kk = e / BN, nn = e % BN;
+      int k = k0 + kk, n = n0 + nn;
+      TACC v = 0;
+      if (k < d.K && n < d.N) v = (TACC)Bb[(int64_t)k * st.ldb_k + (int64_t)n * st.ldb_n];
+      Bs[kk][nn] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk++) {
+      TACC a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+  TC* Cb = C + d.c_off;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int m = m0 + ty + 16 * i;
+    if (m >= d.M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      int n = n0 + tx + 16 * j;
+      if (n < d.N) Cb[(int64_t)m * st.ldc_m + (int64_t)n * st.ldc_n] = (TC)acc[i][j];
+    }
+  }
+}
+
+template <typename TA, typename TB, typename TC, typename TACC>
+static hipError_t run_gemm(hipStream_t s, const TA* A, const TB* B, TC* C, const GemmPlan& p) {
+  dim3 grid((p.maxN + BN - 1) / BN, (p.maxM + BM - 1) / BM, p.nb);
+  hipLaunchKernelGGL((k_gemm<TA, TB, TC, TACC>), grid, dim3(256), 0, s, A, B, C, p.d_descs, p.st);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// A0 <-> dense
+// ---------------------------------------------------------------------------
+__global__ void k_scatter(const double* __restrict__ a0, const uint32_t* __restrict__ pt,
+                          double* __restrict__ dense, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dense[pt[i]] = a0[i];
+}
+
+__global__ void k_gather(const double* __restrict__ dense, const uint32_t* __restrict__ pt,
+                         double* __restrict__ a0, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a0[i] = dense[pt[i]];
+}
+
+__global__ void k_zero(uint32_t* p, int64_t nwords) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nwords; i += stride) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// kinetic scaling between the forward and inverse r transforms
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_kin_scale(T* __restrict__ x, const int32_t* __restrict__ nn1,
+                            const double* __restrict__ p2r1, const double* __restrict__ p2r2,
+                            const double* __restrict__ divi, double radcut, int nc, int na, int n1,
+                            int n2) {
+  // one block row per (c, ia, k1)
+  int64_t row = blockIdx.x;
+  int k1 = row % n1;
+  int ia = (row / n1) % na;
+  if (k1 >= nn1[ia]) return;
+  T* xr = x + row * n2;
+  double p1 = p2r1[(int64_t)ia * n1 + k1];
+  double dv = divi[ia];
+  for (int k2 = threadIdx.x; k2 < n2; k2 += blockDim.x) {
+    double e = p1 + p2r2[k2];
+    if (e > radcut) e = radcut;
+    xr[k2] = (T)((double)xr[k2] * e * dv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// angular stencil in the FBR: phi_t = (a_t(r) + b_t0(R)) f_t + b_t-(R) f_s- + b_t+(R) f_s+
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fbr_coef(const T* __restrict__ fbr, T* __restrict__ phi,
+                           const double* __restrict__ acoef, const double* __restrict__ bcoef,
+                           const int32_t* __restrict__ nbr, int nc, int nJ, int n1, int n2) {
+  int64_t row = blockIdx.x;  // (t, jj, r)
+  int r = row % n1;
+  int jj = (row / n1) % nJ;
+  int t = row / ((int64_t)n1 * nJ);
+  int64_t plane = (int64_t)n1 * n2;
+  double a = acoef[((int64_t)t * nJ + jj) * n1 + r];
+  const double* b0 = bcoef + (((int64_t)t * 3 + 1) * nJ + jj) * n2;
+  const double* bm = bcoef + (((int64_t)t * 3 + 0) * nJ + jj) * n2;
+  const double* bp = bcoef + (((int64_t)t * 3 + 2) * nJ + jj) * n2;
+  int sm = nbr[t * 3 + 0], sp = nbr[t * 3 + 2];
+  const T* ft = fbr + ((int64_t)t * nJ + jj) * plane + (int64_t)r * n2;
+  const T* fm = sm >= 0 ? fbr + ((int64_t)sm * nJ + jj) * plane + (int64_t)r * n2 : nullptr;
+  const T* fp = sp >= 0 ? fbr + ((int64_t)sp * nJ + jj) * plane + (int64_t)r * n2 : nullptr;
+  T* out = phi + ((int64_t)t * nJ + jj) * plane + (int64_t)r * n2;
+  for (int R = threadIdx.x; R < n2; R += blockDim.x) {
+    double v = (a + b0[R]) * (double)ft[R];
+    if (fm) v += bm[R] * (double)fm[R];
+    if (fp) v += bp[R] * (double)fp[R];
+    out[R] = (T)v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused V, projection and Chebyshev update
+// mode 0: hout = H psi on the mask
+// mode 1: psi_new = (H psi - emin psi)/delta           (first step, no damping)
+// mode 2: psi_new = f (2 (H psi - emin psi)/delta - f psi_m1)
+// psi_new overwrites psim1.  Partial sums of psi_new^2 and psi0.psi_new.
+// ---------------------------------------------------------------------------
+template <typename TT, typename TV>
+__global__ __launch_bounds__(256) void k_update(
+    const double* __restrict__ psi, double* __restrict__ psim1, double* __restrict__ hout,
+    const TT* __restrict__ tkin, const TT* __restrict__ ang, const TV* __restrict__ vd,
+    const uint8_t* __restrict__ mask, const int32_t* __restrict__ nn1,
+    const double* __restrict__ absfr1, const double* __restrict__ absfr2,
+    const int32_t* __restrict__ ch_elec, const int32_t* __restrict__ ch_omega,
+    const double* __restrict__ psi0, double* __restrict__ part, int nc, int ne, int na, int n1,
+    int n2, double emindlt, double delta2, int mode) {
+  const int64_t plane = (int64_t)n1 * n2, slab = (int64_t)na * plane;
+  const int64_t total = (int64_t)nc * slab;
+  double s_norm = 0.0, s_auto = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t pl = i % slab;
+    int c = (int)(i / slab);
+    if (!mask[pl]) {
+      if (mode == 0)
+        hout[i] = 0.0;
+      else
+        psim1[i] = 0.0;
+      continue;
+    }
+    int ia = (int)(pl / plane);
+    int64_t rr = pl % plane;
+    int r = (int)(rr / n2), R = (int)(rr % n2);
+    double p = psi[i];
+    double h = (r < nn1[ia]) ? (double)tkin[i] : p;
+    // V: sum over source channels of the same Omega
+    int et = ch_elec[c], om = ch_omega[c];
+    for (int s = 0; s < nc; s++) {
+      if (ch_omega[s] != om) continue;
+      int es = ch_elec[s];
+      double v = (double)vd[((int64_t)et * ne + es) * slab + pl];
+      double ps = (s == c) ? p : psi[(int64_t)s * slab + pl];
+      h += ps * v;
+    }
+    h += (double)ang[i];
+    if (mode == 0) {
+      hout[i] = h;
+      continue;
+    }
+    double out;
+    if (mode == 1) {
+      out = (h - emindlt * p) / delta2;
+    } else {
+      double f = absfr1[r] * absfr2[R];
+      double hn = (h - emindlt * p) / delta2;
+      out = (2.0 * hn - f * psim1[i]) * f;
+    }
+    psim1[i] = out;
+    s_norm += out * out;
+    if (psi0) s_auto += psi0[i] * out;
+  }
+  // block reduction
+  __shared__ double sh[2][256];
+  sh[0][threadIdx.x] = s_norm;
+  sh[1][threadIdx.x] = s_auto;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && part) {
+    part[2 * blockIdx.x] = sh[0][0];
+    part[2 * blockIdx.x + 1] = sh[1][0];
+  }
+}
+
+__global__ void k_reduce_parts(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  __shared__ double sh[2][256];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    a += part[2 * i];
+    b += part[2 * i + 1];
+  }
+  sh[0][threadIdx.x] = a;
+  sh[1][threadIdx.x] = b;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = sh[0][0];
+    out[1] = sh[1][0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// per-step projections into the ring
+// ---------------------------------------------------------------------------
+__global__ void k_flux_extract(const double* __restrict__ psi, const uint32_t* __restrict__ fdense,
+                               const uint32_t* __restrict__ fcol, const double* __restrict__ zar,
+                               double* __restrict__ zr, double* __restrict__ zdr, int64_t nflux,
+                               int n1, int n2) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nflux) return;
+  zr[f] = psi[fdense[f]];
+  const double* col = psi + fcol[f];
+  double re = 0, im = 0;
+  for (int r = 0; r < n1; r++) {
+    double v = col[(int64_t)r * n2];
+    re += zar[2 * r] * v;
+    im += zar[2 * r + 1] * v;
+  }
+  zdr[2 * f] = re;
+  zdr[2 * f + 1] = im;
+}
+
+// Cvj: one block per (c, jj); xj(r) = sum_ia psi(c,ia,r,ir2abs) D(ia) / sqrt(ah2)
+__global__ void k_cvj(const double* __restrict__ psi, const double* __restrict__ dcvj,
+                      const double* __restrict__ fdv, double* __restrict__ cvj, int nc, int nj,
+                      int nv, int na, int n1, int n2, int ir2abs, double rsah2) {
+  int c = blockIdx.x / nj, jj = blockIdx.x % nj;
+  extern __shared__ double xj[];
+  const int64_t plane = (int64_t)n1 * n2, slab = (int64_t)na * plane;
+  const double* dj = dcvj + ((int64_t)c * nj + jj) * na;
+  for (int r = threadIdx.x; r < n1; r += blockDim.x) {
+    double s = 0;
+    for (int ia = 0; ia < na; ia++)
+      s += psi[(int64_t)c * slab + ia * plane + (int64_t)r * n2 + (ir2abs - 1)] * rsah2 * dj[ia];
+    xj[r] = s;
+  }
+  __syncthreads();
+  __shared__ double red[256];
+  for (int v = 0; v < nv; v++) {
+    const double* f = fdv + (((int64_t)c * nj + jj) * nv + v) * n1;
+    double s = 0;
+    for (int r = threadIdx.x; r < n1; r += blockDim.x) s += xj[r] * f[r];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) cvj[((int64_t)c * nj + jj) * nv + v] = red[0];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// energy accumulation over a ring of steps k0 .. k0+ns-1
+//   acc[x][E] += sum_s C_{k0+s}(E) z[s][x],  C_k(E) = 2 hbr/delta e^{-i k acos Es}/sqrt(1-Es^2)
+// ---------------------------------------------------------------------------
+__global__ void k_wmat(double* __restrict__ w, const double* __restrict__ etot, int netot, int k0,
+                       int ns, double hbr, double emindlt, double delta2) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = blockIdx.y;
+  if (e >= netot || s >= ns) return;
+  int k = k0 + s;
+  double d = (k == 0) ? 1.0 : 2.0;
+  double zf = d * hbr / delta2;
+  double Es = (etot[e] - emindlt) / delta2;
+  double expo = -(double)k * acos(Es);
+  double deno = sqrt(1.0 - Es * Es);
+  w[2 * ((int64_t)s * netot + e)] = zf * cos(expo) / deno;
+  w[2 * ((int64_t)s * netot + e) + 1] = zf * sin(expo) / deno;
+}
+
+// real samples z[s][x] (zr, cvj) or complex (zdr); skip: per-x flag of the
+// reference state whose steps k < kmin are left out (Cvjflux_k)
+__global__ void k_eacc(const double* __restrict__ w, const double* __restrict__ z, int cplx,
+                       double* __restrict__ acc, int64_t nx, int netot, int ns, int k0,
+                       const uint8_t* __restrict__ skip, int kmin) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t x = blockIdx.y;
+  if (e >= netot || x >= nx) return;
+  double ar = 0, ai = 0;
+  bool sk = skip && skip[x];
+  for (int s = 0; s < ns; s++) {
+    if (sk && k0 + s < kmin) continue;
+    double wr = w[2 * ((int64_t)s * netot + e)], wi = w[2 * ((int64_t)s * netot + e) + 1];
+    if (cplx) {
+      double zr = z[2 * ((int64_t)s * nx + x)], zi = z[2 * ((int64_t)s * nx + x) + 1];
+      ar += wr * zr - wi * zi;
+      ai += wr * zi + wi * zr;
+    } else {
+      double zr = z[(int64_t)s * nx + x];
+      ar += wr * zr;
+      ai += wi * zr;
+    }
+  }
+  acc[2 * (x * netot + e)] += ar;
+  acc[2 * (x * netot + e) + 1] += ai;
+}
+
+// psi(E): zfe[e][i] += C_k(E_e) psi[i] on the dense slab (mask implied by psi)
+__global__ void k_psie(double* __restrict__ zfe, const double* __restrict__ psi,
+                       const double* __restrict__ efe, int nofe, int64_t total, int k, double hbr,
+                       double emindlt, double delta2) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  double p = psi[i];
+  double d = (k == 0) ? 1.0 : 2.0;
+  for (int e = 0; e < nofe; e++) {
+    double Es = (efe[e] - emindlt) / delta2;
+    double expo = -(double)k * acos(Es);
+    double zf = d * hbr / delta2 / sqrt(1.0 - Es * Es);
+    zfe[2 * ((int64_t)e * total + i)] += zf * cos(expo) * p;
+    zfe[2 * ((int64_t)e * total + i) + 1] += zf * sin(expo) * p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_zero(Ctx* c, void* p, size_t bytes) {
+  if (!bytes) return hipSuccess;
+  int64_t words = (int64_t)(bytes / 4);
+  unsigned g = (unsigned)std::min<int64_t>((words + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_zero, dim3(g), dim3(256), 0, c->stream, (uint32_t*)p, words);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_a0(Ctx* c, const double* a0, double* dense) {
+  hipError_t e = launch_zero(c, dense, sizeof(double) * (size_t)c->t.nc * c->t.nslab);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scatter, dim3(nblk(c->t.ntot, 256)), dim3(256), 0, c->stream, a0,
+                     c->d_pt_dense, dense, c->t.ntot);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_a0(Ctx* c, const double* dense, double* a0) {
+  hipLaunchKernelGGL(k_gather, dim3(nblk(c->t.ntot, 256)), dim3(256), 0, c->stream, dense,
+                     c->d_pt_dense, a0, c->t.ntot);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// GEMM plans (shapes are fixed per context)
+// ---------------------------------------------------------------------------
+static hipError_t upload_plan(GemmPlan* p, const std::vector<GemmDesc>& d) {
+  p->nb = (int)d.size();
+  p->maxM = p->maxN = 0;
+  for (auto& x : d) {
+    p->maxM = std::max(p->maxM, x.M);
+    p->maxN = std::max(p->maxN, x.N);
+  }
+  hipError_t e = hipMalloc(&p->d_descs, sizeof(GemmDesc) * d.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpy(p->d_descs, d.data(), sizeof(GemmDesc) * d.size(), hipMemcpyHostToDevice);
+}
+
+hipError_t build_plans(Ctx* c) {
+  const HostTables& t = c->t;
+  const int64_t plane = t.nplane, slab = t.nslab;
+  hipError_t e;
+  // R rows: every (c, ia, r) row of n2 values times the n2 sine table
+  {
+    int64_t rows = (int64_t)t.nc * t.na * t.n1;
+    int ir2tab = -1;
+    for (size_t q = 0; q < t.dst_sizes.size(); q++)
+      if (t.dst_sizes[q] == t.n2) ir2tab = (int)q;
+    std::vector<GemmDesc> d;
+    const int64_t chunk = 65535LL * BM;  // grid.y limit
+    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+      GemmDesc g{};
+      g.a_off = r0 * t.n2;
+      g.b_off = t.dst_table_off[ir2tab];
+      g.c_off = r0 * t.n2;
+      g.M = (int)std::min<int64_t>(chunk, rows - r0);
+      g.N = t.n2;
+      g.K = t.n2;
+      g.lda_m = t.n2;
+      d.push_back(g);
+    }
+    c->p_dstR.st = GemmStrides{1, t.n2, 1, t.n2, 1};
+    if ((e = upload_plan(&c->p_dstR, d)) != hipSuccess) return e;
+  }
+  // r columns: per (c, ia) an nn1 x nn1 table times the nn1 x n2 block
+  {
+    std::vector<GemmDesc> d;
+    for (int ch = 0; ch < t.nc; ch++)
+      for (int ia = 0; ia < t.na; ia++) {
+        int n = t.nn1[ia];
+        int q = 0;
+        while (t.dst_sizes[q] != n) q++;
+        GemmDesc g{};
+        g.a_off = t.dst_table_off[q];
+        g.b_off = (int64_t)ch * slab + (int64_t)ia * plane;
+        g.c_off = g.b_off;
+        g.M = n;
+        g.N = t.n2;
+        g.K = n;
+        g.lda_m = n;
+        d.push_back(g);
+      }
+    c->p_dstr.st = GemmStrides{1, t.n2, 1, t.n2, 1};
+    if ((e = upload_plan(&c->p_dstr, d)) != hipSuccess) return e;
+  }
+  // angular forward: fbr[c][jj][p] = sum_ia D_c(ia, jj) psi[c][ia][p]
+  {
+    std::vector<GemmDesc> d;
+    for (int ch = 0; ch < t.nc; ch++) {
+      GemmDesc g{};
+      g.a_off = (int64_t)ch * t.na * t.nJ;
+      g.b_off = (int64_t)ch * slab;
+      g.c_off = (int64_t)ch * t.nJ * plane;
+      g.M = t.nJ;
+      g.N = (int)plane;
+      g.K = t.na;
+      g.lda_m = 1;
+      d.push_back(g);
+    }
+    c->p_angF.st = GemmStrides{t.nJ, plane, 1, plane, 1};
+    if ((e = upload_plan(&c->p_angF, d)) != hipSuccess) return e;
+  }
+  // angular back: ang[c][ia][p] = sum_jj D_c(ia, jj) phi[c][jj][p]
+  {
+    std::vector<GemmDesc> d;
+    for (int ch = 0; ch < t.nc; ch++) {
+      GemmDesc g{};
+      g.a_off = (int64_t)ch * t.na * t.nJ;
+      g.b_off = (int64_t)ch * t.nJ * plane;
+      g.c_off = (int64_t)ch * slab;
+      g.M = t.na;
+      g.N = (int)plane;
+      g.K = t.nJ;
+      g.lda_m = t.nJ;
+      d.push_back(g);
+    }
+    c->p_angB.st = GemmStrides{1, plane, 1, plane, 1};
+    if ((e = upload_plan(&c->p_angB, d)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+void free_plans(Ctx* c) {
+  GemmPlan* ps[4] = {&c->p_dstR, &c->p_dstr, &c->p_angF, &c->p_angB};
+  for (auto* p : ps)
+    if (p->d_descs) {
+      (void)hipFree(p->d_descs);
+      p->d_descs = nullptr;
+    }
+}
+
+// H psi terms into work slabs: d_t2 = T psi (rows r < nn1), d_ang = angular
+template <typename T>
+static hipError_t hpsi_impl(Ctx* c, const double* psi) {
+  const HostTables& t = c->t;
+  hipStream_t s = c->stream;
+  T* t1 = (T*)c->d_t1;
+  T* t2 = (T*)c->d_t2;
+  const T* tab = (const T*)c->d_dst_tables;
+  hipError_t e;
+  // angular term
+  timer_begin(c, K_ANG_FWD);
+  e = run_gemm<T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr, c->p_angF);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fbr_coef<T>, dim3((unsigned)((int64_t)t.nc * t.nJ * t.n1)), dim3(256), 0,
+                     s, (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,
+                     (const double*)c->d_bcoef, c->d_nbr, t.nc, t.nJ, t.n1, t.n2);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  timer_end(c, K_ANG_FWD);
+  timer_begin(c, K_ANG_BWD);
+  e = run_gemm<T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
+  if (e != hipSuccess) return e;
+  timer_end(c, K_ANG_BWD);
+  // radial kinetic: R rows, r columns + scale + r columns, R rows
+  timer_begin(c, K_DST_R);
+  e = run_gemm<double, T, T, T>(s, psi, tab, t1, c->p_dstR);
+  if (e != hipSuccess) return e;
+  timer_end(c, K_DST_R);
+  timer_begin(c, K_DST_r);
+  e = run_gemm<T, T, T, T>(s, tab, t1, t2, c->p_dstr);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_kin_scale<T>, dim3((unsigned)((int64_t)t.nc * t.na * t.n1)), dim3(256), 0,
+                     s, t2, c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.nc, t.na, t.n1,
+                     t.n2);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = run_gemm<T, T, T, T>(s, tab, t2, t1, c->p_dstr);
+  if (e != hipSuccess) return e;
+  timer_end(c, K_DST_r);
+  timer_begin(c, K_DST_RINV);
+  e = run_gemm<T, T, T, T>(s, t1, tab, t2, c->p_dstR);
+  if (e != hipSuccess) return e;
+  timer_end(c, K_DST_RINV);
+  return hipSuccess;
+}
+
+hipError_t launch_hpsi(Ctx* c, const double* psi) {
+  if (c->precision == MW3_MIXED) return hpsi_impl<float>(c, psi);
+  return hpsi_impl<double>(c, psi);
+}
+
+template <typename T>
+static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi) {
+  const HostTables& t = c->t;
+  int nb = c->nnorm_blocks;
+  timer_begin(c, K_UPDATE);
+  hipLaunchKernelGGL((k_update<T, T>), dim3(nb), dim3(256), 0, c->stream, psi, c->d_psim1,
+                     c->d_hout, (const T*)c->d_t2, (const T*)c->d_ang, (const T*)c->d_v, c->d_mask,
+                     c->d_nn1, c->d_absfr1, c->d_absfr2, c->d_ch_elec, c->d_ch_omega,
+                     (c->s.iphoto >= 1) ? c->d_psi0 : nullptr, c->d_norm_part, t.nc, t.ne, t.na,
+                     t.n1, t.n2, c->s.emindlt, c->s.delta2, mode);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (mode != 0) {
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, c->d_norm_part, nb,
+                       c->d_ring_norm + 2 * slot);
+    e = hipGetLastError();
+  }
+  timer_end(c, K_UPDATE);
+  return e;
+}
+
+hipError_t launch_update(Ctx* c, int mode, const double* psi) {
+  int slot = c->ring_fill;
+  if (c->precision == MW3_MIXED) return update_impl<float>(c, mode, slot, psi);
+  return update_impl<double>(c, mode, slot, psi);
+}
+
+// projections of the new psi (in d_psim1 before the swap) into ring slot
+hipError_t launch_projections(Ctx* c, int slot) {
+  const HostTables& t = c->t;
+  hipError_t e = hipSuccess;
+  timer_begin(c, K_PROJ);
+  const double* psi = c->d_psim1;
+  if (t.nflux > 0) {
+    hipLaunchKernelGGL(k_flux_extract, dim3(nblk(t.nflux, 128)), dim3(128), 0, c->stream, psi,
+                       c->d_flux_dense, c->d_flux_col, c->d_zar1flux,
+                       c->d_ring_zr + (int64_t)slot * t.nflux,
+                       c->d_ring_zdr + (int64_t)2 * slot * t.nflux, t.nflux, t.n1, t.n2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  if (t.ncvj > 0) {
+    int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
+    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(256), sizeof(double) * t.n1, c->stream, psi,
+                       c->d_dcvj, c->d_fdv, c->d_ring_cvj + (int64_t)slot * t.ncvj, t.nc, nj, nv,
+                       t.na, t.n1, t.n2, c->s.ir2abs, 1.0 / std::sqrt(c->s.ah2));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  timer_end(c, K_PROJ);
+  return e;
+}
+
+hipError_t launch_energy_accum(Ctx* c) {
+  const HostTables& t = c->t;
+  int ns = c->ring_fill;
+  if (ns == 0 || c->s.netot <= 0) return hipSuccess;
+  timer_begin(c, K_ENERGY);
+  int netot = c->s.netot;
+  hipLaunchKernelGGL(k_wmat, dim3(nblk(netot, 128), ns), dim3(128), 0, c->stream, c->d_wmat,
+                     c->d_etot, netot, c->ring_k0, ns, c->s.hbr, c->s.emindlt, c->s.delta2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // totalflux_k zeroes C_k for k < kminelastic when iphoto = 0 and iprod = 1
+  int kmin_flux = (c->s.iphoto == 0 && c->s.iprod == 1) ? c->s.kminelastic : -1;
+  const int64_t maxy = 65535;
+  if (t.nflux > 0) {
+    // x runs over grid.y in chunks of 65535
+    for (int64_t x0 = 0; x0 < t.nflux; x0 += maxy) {
+      int64_t nx = std::min<int64_t>(maxy, t.nflux - x0);
+      dim3 g(nblk(netot, 128), (unsigned)nx);
+      hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_zr + x0, 0,
+                         c->d_zcr + 2 * x0 * netot, t.nflux, netot, ns, c->ring_k0,
+                         kmin_flux >= 0 ? c->d_zero_skip : nullptr, kmin_flux);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_zdr + 2 * x0, 1,
+                         c->d_zcdr + 2 * x0 * netot, t.nflux, netot, ns, c->ring_k0,
+                         kmin_flux >= 0 ? c->d_zero_skip : nullptr, kmin_flux);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+  }
+  if (t.ncvj > 0) {
+    int kmin = (c->s.iprod != 1 && c->s.iphoto == 0) ? c->s.kminelastic : -1;
+    dim3 g(nblk(netot, 128), (unsigned)t.ncvj);
+    hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_cvj, 0, c->d_zs2,
+                       (int64_t)t.ncvj, netot, ns, c->ring_k0, kmin >= 0 ? c->d_cvj_skip : nullptr,
+                       kmin);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  timer_end(c, K_ENERGY);
+  return hipSuccess;
+}
+
+hipError_t launch_psie(Ctx* c, int k) {
+  if (c->s.nofe <= 0) return hipSuccess;
+  int64_t total = (int64_t)c->t.nc * c->t.nslab;
+  hipLaunchKernelGGL(k_psie, dim3(nblk(total, 256)), dim3(256), 0, c->stream, c->d_zfe,
+                     c->d_psim1, c->d_efe, c->s.nofe, total, k, c->s.hbr, c->s.emindlt,
+                     c->s.delta2);
+  return hipGetLastError();
+}
+
+// P(E) and per-state sums (totalflux, SRC/mod_flux_01y2.f:778-816):
+// out[0][E] = sum_f [E > V_f] Im(conj(zCR) zCdR), out[1+e][E] per state e.
+__global__ void k_flux_sum(const double* __restrict__ zcr, const double* __restrict__ zcdr,
+                           const double* __restrict__ potf, const int32_t* __restrict__ felec,
+                           const double* __restrict__ etot, double* __restrict__ out,
+                           int64_t nflux, int netot, int ne) {
+  int e = blockIdx.x;
+  extern __shared__ double sh[];  // [(1+ne)][256]
+  double acc[9];
+  for (int q = 0; q <= ne && q < 9; q++) acc[q] = 0;
+  double E = etot[e];
+  for (int64_t f = threadIdx.x; f < nflux; f += blockDim.x) {
+    if (!(E > potf[f])) continue;
+    double ar = zcr[2 * (f * netot + e)], ai = zcr[2 * (f * netot + e) + 1];
+    double br = zcdr[2 * (f * netot + e)], bi = zcdr[2 * (f * netot + e) + 1];
+    double v = ar * bi - ai * br;
+    acc[0] += v;
+    acc[1 + felec[f]] += v;
+  }
+  for (int q = 0; q <= ne; q++) sh[q * 256 + threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int q = 0; q <= ne; q++) sh[q * 256 + threadIdx.x] += sh[q * 256 + threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int q = 0; q <= ne; q++) out[(int64_t)q * netot + e] = sh[q * 256];
+}
+
+hipError_t launch_flux_result(Ctx* c, double* d_out) {
+  int ne = c->t.ne;
+  hipLaunchKernelGGL(k_flux_sum, dim3(c->s.netot), dim3(256), sizeof(double) * 256 * (1 + ne),
+                     c->stream, c->d_zcr, c->d_zcdr, c->d_potiflux, c->d_flux_elec, c->d_etot,
+                     d_out, c->t.nflux, c->s.netot, ne);
+  return hipGetLastError();
+}
+
+}  // namespace mw3

@@ -1,0 +1,247 @@
+"""Synthetic MADWAVE3 inputs: a LEPS triatomic surface and its diatomic states.
+
+Plays the part of the reference's preparation program (SRC/main_potini.f:1-108
+with pot1, SRC/mod_pot_01y2.f:147-341, and radial_functions01_write,
+SRC/mod_baseYfunciones_01y2.f:553-835) for an analytic surface, and produces
+the same pot/ and func/ content (see refio).
+
+Surface: London-Eyring-Polanyi-Sato form with the Sato parameter s, built from
+a Morse singlet E1 = D (e^{-2a x} - 2 e^{-a x}) and an anti-Morse triplet
+E3 = D/2 (e^{-2a x} + 2 e^{-a x}) (x = r - re) for each of the three pairs:
+    Q_i = D/4 [(3 + s) e^{-2 a x} - (2 + 6 s) e^{-a x}]
+    J_i = D/4 [(1 + 3 s) e^{-2 a x} - (6 + 2 s) e^{-a x}]
+    V = [Q_1 + Q_2 + Q_3 - sqrt(J_1^2 + J_2^2 + J_3^2 - J_1 J_2 - J_2 J_3
+         - J_3 J_1)] / (1 + s)
+which tends to the Morse curve -D at re when the third atom is far away.
+
+Diatomic states: eigenvectors of the grid Hamiltonian of the 0-1 diatom on the
+r grid with the sine-transform kinetic operator of the propagator
+(p_k = pi k/((npun1+1) ah1)), normalised to sum fd^2 = 1 as the reference
+normalises them (SRC/mod_baseYfunciones_01y2.f:733-747).
+"""
+import math
+
+import numpy as np
+
+from . import consts as C
+from . import namelist as NL
+from .refio import FuncData, PotData
+
+__all__ = ["LEPS", "diatomic_states", "make_inputs", "config"]
+
+
+class LEPS:
+    """Symmetric LEPS surface of three identical Morse pairs (hartree, bohr)."""
+
+    def __init__(self, De_eV=4.75, a_invA=1.94, re_A=0.742, sato=0.17):
+        self.D = De_eV / C.au2eV
+        self.a = a_invA * C.convl          # 1/bohr
+        self.re = re_A / C.convl           # bohr
+        self.s = sato
+
+    def morse(self, r):
+        e = np.exp(-self.a * (r - self.re))
+        return self.D * (e * e - 2.0 * e)
+
+    def _qj(self, r):
+        e = np.exp(-self.a * (r - self.re))
+        D, s = self.D, self.s
+        q = 0.25 * D * ((3.0 + s) * e * e - (2.0 + 6.0 * s) * e)
+        j = 0.25 * D * ((1.0 + 3.0 * s) * e * e - (6.0 + 2.0 * s) * e)
+        return q, j
+
+    def __call__(self, r01, r02, r12):
+        q1, j1 = self._qj(r01)
+        q2, j2 = self._qj(r02)
+        q3, j3 = self._qj(r12)
+        x = j1 * j1 + j2 * j2 + j3 * j3 - j1 * j2 - j2 * j3 - j3 * j1
+        return (q1 + q2 + q3 - np.sqrt(np.maximum(x, 0.0))) / (1.0 + self.s)
+
+
+def jacobi_distances(r_bohr, R_bohr, cosg, xm0, xm1):
+    """Atom-atom distances for reactant Jacobi (r = 0-1 bond, R from the 0-1
+    centre of mass to atom 2), as in pot1 (SRC/mod_pot_01y2.f:196-209)."""
+    gam = xm1 / (xm0 + xm1)
+    r02 = np.sqrt(R_bohr ** 2 + gam ** 2 * r_bohr ** 2 + 2.0 * gam * R_bohr * r_bohr * cosg)
+    g1 = 1.0 - gam
+    r12 = np.sqrt(R_bohr ** 2 + g1 ** 2 * r_bohr ** 2 - 2.0 * g1 * R_bohr * r_bohr * cosg)
+    return r_bohr, r02, r12
+
+
+def diatomic_states(vdiat_zot, npun1, ah1, rmis1, mu, j, nstates):
+    """Lowest nstates eigenpairs (zots, grid vectors) of
+    -hbar^2/2mu d2/dr2 + V(r) + hbar^2 j(j+1)/(2 mu r^2) on the r grid."""
+    n = npun1
+    k = np.arange(1, n + 1)
+    U = math.sqrt(2.0 / (n + 1)) * np.sin(np.pi * np.outer(k, k) / (n + 1))
+    p = np.pi * k / ((n + 1) * ah1)
+    T = U @ np.diag(0.5 * C.hbr * C.hbr * p * p / mu) @ U
+    r = rmis1 + np.arange(n) * ah1
+    H = T + np.diag(vdiat_zot + 0.5 * C.hbr * C.hbr * j * (j + 1) / (mu * r * r))
+    e, v = np.linalg.eigh(H)
+    e = e[:nstates]
+    v = v[:, :nstates].copy()
+    for i in range(v.shape[1]):
+        if v[np.argmax(np.abs(v[:, i])), i] < 0:
+            v[:, i] = -v[:, i]
+        v[:, i] /= math.sqrt(np.sum(v[:, i] ** 2))
+    return e, v
+
+
+def config(cfg_id, scale=1.0):
+    """Parameters of the benchmark configurations (grid sizes may be divided
+    by `scale` for small test cases)."""
+    base = dict(
+        xm0=2.01410, xm1=1.00783, xm2=1.00783,   # D, H, H (amu)
+        De_eV=4.75, a_invA=1.94, re_A=0.742, sato=0.17,
+        vcut_eV=2.5, radcut_eV=2.5, rotcut_eV=5.0,
+        R0_bohr=9.0, width_bohr=0.3, ecol_eV=0.8,
+        absorb_bohr=3.0, absalp=0.5, nexpo=3,
+        ekinmin_eV=0.3, ekinmax_eV=1.5, netot=500,
+        nvini=0, nvmax=3, jini=0, jmax=10, nvref=0, jref=0, iomref=0,
+        Jtot=0, iparity=1, iommin=0, iommax=0, nelec=1,
+        npun1min=32, kminelastic=100, rmis1_bohr=0.1, rmis2_bohr=0.1,
+    )
+    if cfg_id == 1:
+        base.update(npun1=128, rfin1_bohr=8.0, npun2=256, rfin2_bohr=15.0,
+                    nangu=60, ntimes=500, nloop=39, r1flux_bohr=4.0)
+    elif cfg_id == 2:
+        base.update(npun1=256, rfin1_bohr=8.0, npun2=512, rfin2_bohr=15.0,
+                    nangu=120, ntimes=500, nloop=79, r1flux_bohr=4.0)
+    elif cfg_id == 5:
+        base.update(npun1=256, rfin1_bohr=8.0, npun2=384, rfin2_bohr=15.0,
+                    nangu=96, ntimes=500, nloop=119, r1flux_bohr=4.0, nelec=3,
+                    state_offsets_eV=[0.0, 0.3, 0.6], coupling_eV=0.05,
+                    coupling_r_bohr=1.4, coupling_width_bohr=0.3)
+    elif cfg_id == 3:
+        base.update(npun1=192, rfin1_bohr=8.0, npun2=384, rfin2_bohr=15.0,
+                    nangu=96, ntimes=500, nloop=59, r1flux_bohr=4.0,
+                    Jtot=12, iparity=1, iommin=0, iommax=6, iomref=-1)
+    else:
+        raise ValueError("configuration %r has no generator yet" % (cfg_id,))
+    if scale != 1.0:
+        for k in ("npun1", "npun2", "nangu"):
+            base[k] = max(8, int(round(base[k] / scale)))
+        base["npun1min"] = min(base["npun1min"], base["npun1"] // 2)
+        base["jmax"] = min(base["jmax"], base["nangu"] - 1)
+    return base
+
+
+def make_inputs(p):
+    """(input.dat text, PotData, FuncData) for parameter dict p (see config)."""
+    conv = C.convl
+    npun1, npun2, nangu = p["npun1"], p["npun2"], p["nangu"]
+    rmis1, rfin1 = p["rmis1_bohr"] * conv, p["rfin1_bohr"] * conv
+    rmis2, rfin2 = p["rmis2_bohr"] * conv, p["rfin2_bohr"] * conv
+    ah1 = (rfin1 - rmis1) / (npun1 - 1)
+    ah2 = (rfin2 - rmis2) / (npun2 - 1)
+    xm0, xm1, xm2 = p["xm0"], p["xm1"], p["xm2"]
+    mu1 = xm0 * xm1 / (xm0 + xm1)
+    mu2 = xm2 * (xm0 + xm1) / (xm0 + xm1 + xm2)
+    pes = LEPS(p["De_eV"], p["a_invA"], p["re_A"], p["sato"])
+    h2z = 1.0 / C.conve * C.conve1          # hartree -> zot
+    vcut = C.ev_to_zot(p["vcut_eV"])
+
+    # diatomic functions and reference energy
+    r = rmis1 + np.arange(npun1) * ah1
+    vd = pes.morse(r / conv) * h2z
+    nj = p["jmax"] - p["jini"] + 1
+    nel = p["nelec"]
+    offs_e = [C.ev_to_zot(x) for x in p.get("state_offsets_eV", [0.0] * nel)]
+    nobc = [[0] * nj for _ in range(nel)]
+    ediat, fd = {}, {}
+    for e in range(1, nel + 1):
+        for j in range(p["jini"], p["jmax"] + 1):
+            ev, v = diatomic_states(vd + offs_e[e - 1], npun1, ah1, rmis1, mu1, j, p["nvmax"] + 1)
+            nb = -1
+            for iv in range(p["nvini"], p["nvmax"] + 1):
+                if ev[iv] < offs_e[e - 1]:    # bound below the state's dissociation limit
+                    ediat[(e, j, iv)] = ev[iv]
+                    fd[(e, j, iv)] = v[:, iv]
+                    nb = iv
+            nobc[e - 1][j - p["jini"]] = nb
+    eref = ediat[(1, p["jref"], p["nvref"])]
+    fdat = FuncData(nel, p["jini"], p["jmax"], p["nvini"], npun1, nobc, ediat, fd, eref)
+
+    # potential on the grid, compressed as pot1 does (SRC/mod_pot_01y2.f:175-279)
+    from .numerics import gauleg
+    ne = p["nelec"]
+    offs = [C.ev_to_zot(x) for x in p.get("state_offsets_eV", [0.0] * ne)]
+    vc = C.ev_to_zot(p.get("coupling_eV", 0.0))
+    rc = p.get("coupling_r_bohr", 1.4) * conv
+    wc = p.get("coupling_width_bohr", 0.3) * conv
+    _, cg = gauleg(nangu)
+    R = rmis2 + np.arange(npun2) * ah2
+    indr1, indr2, vmat = [], [], []
+    vmin = 1e10
+    vmaxtot = vcut
+    for ia in range(nangu):
+        rr, RR = np.meshgrid(r / conv, R / conv)            # [ir2][ir1]
+        d01, d02, d12 = jacobi_distances(rr, RR, cg[ia], xm0, xm1)
+        v0 = pes(d01, d02, d12) * h2z
+        if ne == 1:
+            v = v0 - eref
+            v = np.where(v > vcut, vcut, v)
+            vmin = min(vmin, float(v.min()))
+            keep = v < vcut
+            i2, i1 = np.nonzero(keep)                      # R outer, r inner
+            vm = v[i2, i1].reshape(-1, 1, 1)
+        else:
+            # V[ir2, ir1, je, ie]: diabatic matrix, reference energy on the diagonal
+            V = np.zeros(v0.shape + (ne, ne))
+            cpl = vc * np.exp(-((rr * conv - rc) / wc) ** 2)
+            for e in range(ne):
+                V[..., e, e] = v0 + offs[e] - eref
+                for f in range(ne):
+                    if f != e:
+                        V[..., e, f] = cpl
+            # cut as pot1: a diagonal above vcut loses its couplings and is capped
+            for e in range(ne):
+                hi = V[..., e, e] > vcut
+                for f in range(ne):
+                    V[..., e, f] = np.where(hi, 0.0, V[..., e, f])
+                    V[..., f, e] = np.where(hi, 0.0, V[..., f, e])
+                V[..., e, e] = np.where(hi, vcut, V[..., e, e])
+            ev = np.linalg.eigvalsh(V)
+            vmin = min(vmin, float(ev.min()))
+            vmaxtot = max(vmaxtot, float(ev.max()))
+            keep = (ev < vcut).any(axis=-1)
+            i2, i1 = np.nonzero(keep)
+            vm = V[i2, i1]                                 # [n][je][ie] (symmetric)
+        indr1.append((i1 + 1).astype(np.int32))
+        indr2.append((i2 + 1).astype(np.int32))
+        vmat.append(vm)
+    pot = PotData(ne, [(0, 0, 1.0, 1.0)] * ne, vmin, vmaxtot, (rmis1, rfin1, npun1),
+                  (rmis2, rfin2, npun2), nangu, 1, indr1, indr2, vmat)
+
+    # namelists
+    ecol = p["ecol_eV"]
+    alpha = p["width_bohr"] * conv
+    ecol_z = C.ev_to_zot(ecol)
+    deltaE = math.sqrt(2.0 * ecol_z / mu2) * C.hbr / alpha / (C.eV2cm * C.conve1)
+    absr1 = rfin1 - p["absorb_bohr"] * conv
+    absr2 = rfin2 - p["absorb_bohr"] * conv
+    text = ""
+    text += NL.dump_group("INPUTGRIDBASE", dict(
+        NPUN1=npun1, RMIS1=rmis1, RFIN1=rfin1, NPUN1MIN=p["npun1min"],
+        NPUN2=npun2, RMIS2=rmis2, RFIN2=rfin2, NANGU=nangu,
+        NELECMAX=p["nelec"], JTOT=p["Jtot"], IPARITY=p["iparity"],
+        IOMMIN=p["iommin"], IOMMAX=p["iommax"], J0=0, INC=1,
+        JINI=p["jini"], JMAX=p["jmax"], NVINI=p["nvini"], NVMAX=p["nvmax"],
+        NVREF=p["nvref"], JREF=p["jref"], IOMREF=p["iomref"], IELECREF=1))
+    text += NL.dump_group("inputprocess", dict(iphoto=0))
+    text += NL.dump_group("inputabsorcion", dict(
+        absalp1=p["absalp"], absr1=absr1, n1expo=p["nexpo"], ninterv1=1,
+        absalp2=p["absalp"], absr2=absr2, n2expo=p["nexpo"], ninterv2=1))
+    text += NL.dump_group("inputprod", dict(iprod=0))
+    text += NL.dump_group("inputtime", dict(ntimes=p["ntimes"], nloop=p["nloop"],
+                                            kminelastic=p["kminelastic"]))
+    text += NL.dump_group("inputpotmass", dict(
+        system="LEPS", xm0=xm0, xm1=xm1, xm2=xm2,
+        VcutmaxeV=p["vcut_eV"], radcutmaxeV=p["radcut_eV"], rotcutmaxeV=p["rotcut_eV"]))
+    text += NL.dump_group("inputcol", dict(
+        Rcolini_ang=p["R0_bohr"] * conv, ecol_ev=ecol, deltaE_ev=deltaE))
+    text += NL.dump_group("inputflux", dict(
+        r1flux_ang=p["r1flux_bohr"] * conv, netot=p["netot"],
+        ekinmin_eV=p["ekinmin_eV"], ekinmax_eV=p["ekinmax_eV"], ncontfile=0))
+    return text, pot, fdat

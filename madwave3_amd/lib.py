@@ -1,0 +1,278 @@
+"""ctypes binding of libmw3 (include/mw3.h).
+
+The Python host hands the module state built by :mod:`madwave3_amd.model` to
+the C ABI exactly as a Fortran caller would: arrays in the reference's Fortran
+memory order, 1-based index values.  Every failing call raises Mw3Error with
+the library's message; a missing library raises at load time (there is no
+fallback path).
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+__all__ = ["Mw3Setup", "make_setup", "load_library", "Propagator", "Mw3Error",
+           "MW3_FP64", "MW3_MIXED", "LIB_PATH"]
+
+MW3_ABI_VERSION = 1
+MW3_FP64 = 0
+MW3_MIXED = 1
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmw3.so")
+
+_pi = ct.POINTER(ct.c_int32)
+_pd = ct.POINTER(ct.c_double)
+
+
+class Mw3Setup(ct.Structure):
+    """Mirror of struct mw3_setup (include/mw3.h)."""
+    _fields_ = [
+        ("abi_version", ct.c_int32),
+        ("npun1", ct.c_int32), ("npun1min", ct.c_int32), ("npun2", ct.c_int32),
+        ("nangu", ct.c_int32), ("inc", ct.c_int32),
+        ("Jtot", ct.c_int32), ("iparity", ct.c_int32), ("iommin", ct.c_int32),
+        ("iommax", ct.c_int32), ("j0", ct.c_int32),
+        ("nelecmax", ct.c_int32),
+        ("rmis1", ct.c_double), ("ah1", ct.c_double),
+        ("rmis2", ct.c_double), ("ah2", ct.c_double),
+        ("hbr", ct.c_double),
+        ("xm1red", ct.c_double), ("xm2red", ct.c_double),
+        ("radcutmax", ct.c_double), ("rotcutmax", ct.c_double),
+        ("emindlt", ct.c_double), ("delta2", ct.c_double),
+        ("proj_j2_up", ct.c_int32),
+        ("iomdiat", _pi), ("iomatom", _pi),
+        ("sigdiat", _pd), ("sigatom", _pd),
+        ("npuntot", ct.c_int32),
+        ("indangreal", _pi), ("npunreal", _pi), ("indr1", _pi), ("indr2", _pi),
+        ("VVV", _pd),
+        ("ncan", ct.c_int32),
+        ("iombas", _pi), ("nelebas", _pi), ("nojbas", _pi), ("j00", _pi),
+        ("jbas", _pi),
+        ("ndj", ct.c_int32),
+        ("Djmmp", _pd),
+        ("absfr1", _pd), ("absfr2", _pd),
+        ("ir2abs", ct.c_int32),
+        ("netot", ct.c_int32),
+        ("etots2", _pd), ("reacfct", _pd),
+        ("r1flux", ct.c_double),
+        ("nvini", ct.c_int32), ("nvmax", ct.c_int32), ("jini", ct.c_int32),
+        ("jmax", ct.c_int32),
+        ("noBCstates", _pi),
+        ("fd", _pd),
+        ("iphoto", ct.c_int32), ("iprod", ct.c_int32), ("kminelastic", ct.c_int32),
+        ("icanref", ct.c_int32), ("nvref", ct.c_int32), ("jref", ct.c_int32),
+        ("iomref", ct.c_int32),
+        ("nofe", ct.c_int32),
+        ("efe", _pd),
+    ]
+
+
+def _arr(keep, a, dt):
+    a = np.ascontiguousarray(a, dtype=dt)
+    keep.append(a)
+    return a.ctypes.data_as(_pi if dt == np.int32 else _pd)
+
+
+def make_setup(c):
+    """mw3_setup for a model.RefCase; returns (struct, keep-alive list)."""
+    keep = []
+    s = Mw3Setup()
+    s.abi_version = MW3_ABI_VERSION
+    for k in ("npun1", "npun1min", "npun2", "nangu", "inc", "Jtot", "iparity",
+              "iommin", "iommax", "j0", "nelecmax", "npuntot", "ncan", "ndj",
+              "ir2abs", "netot", "nvini", "nvmax", "jini", "jmax", "iphoto",
+              "iprod", "kminelastic", "icanref", "nvref", "jref", "iomref", "nofe"):
+        setattr(s, k, int(getattr(c, k)))
+    from . import consts
+    s.rmis1, s.ah1, s.rmis2, s.ah2 = c.rmis1, c.ah1, c.rmis2, c.ah2
+    s.hbr = consts.hbr
+    s.xm1red, s.xm2red = c.xm1red, c.xm2red
+    s.radcutmax, s.rotcutmax = c.radcutmax, c.rotcutmax
+    s.emindlt, s.delta2 = c.emindlt, c.delta2
+    s.proj_j2_up = int(c.Proj_j2_Up)
+    s.r1flux = c.r1flux
+    s.iomdiat = _arr(keep, c.iomdiat, np.int32)
+    s.iomatom = _arr(keep, c.iomatom, np.int32)
+    s.sigdiat = _arr(keep, c.sigdiat, np.float64)
+    s.sigatom = _arr(keep, c.sigatom, np.float64)
+    s.indangreal = _arr(keep, c.indangreal, np.int32)
+    s.npunreal = _arr(keep, c.npunreal, np.int32)
+    s.indr1 = _arr(keep, c.indr1, np.int32)
+    s.indr2 = _arr(keep, c.indr2, np.int32)
+    s.VVV = _arr(keep, c.VVV, np.float64)
+    s.iombas = _arr(keep, c.iombas, np.int32)
+    s.nelebas = _arr(keep, c.nelebas, np.int32)
+    s.nojbas = _arr(keep, c.nojbas, np.int32)
+    s.j00 = _arr(keep, c.j00, np.int32)
+    s.jbas = _arr(keep, c.jbas, np.int32)
+    s.Djmmp = _arr(keep, c.Djmmp, np.float64)
+    s.absfr1 = _arr(keep, c.absfr1, np.float64)
+    s.absfr2 = _arr(keep, c.absfr2, np.float64)
+    s.etots2 = _arr(keep, c.etotS2, np.float64)
+    s.reacfct = _arr(keep, c.reacfct, np.float64)
+    s.noBCstates = _arr(keep, c.noBCstates, np.int32)
+    s.fd = _arr(keep, c.fd, np.float64)
+    s.efe = _arr(keep, c.efe if len(c.efe) else np.zeros(1), np.float64)
+    return s, keep
+
+
+class Mw3Error(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libmw3.so; raises OSError when it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OSError("libmw3.so is not built (%s); run __graft_entry__.build()" % p)
+    lib = ct.CDLL(p)
+    lib.mw3_abi_version.restype = ct.c_int32
+    lib.mw3_last_error.restype = ct.c_char_p
+    lib.mw3_last_error.argtypes = [ct.c_void_p]
+    lib.mw3_create.argtypes = [ct.POINTER(Mw3Setup), ct.c_int32, ct.POINTER(ct.c_void_p)]
+    lib.mw3_destroy.argtypes = [ct.c_void_p]
+    lib.mw3_ntot.restype = ct.c_int64
+    lib.mw3_ntot.argtypes = [ct.c_void_p]
+    lib.mw3_nflux.restype = ct.c_int64
+    lib.mw3_nflux.argtypes = [ct.c_void_p]
+    lib.mw3_hphi.argtypes = [ct.c_void_p, _pd, _pd]
+    lib.mw3_cheby_begin.argtypes = [ct.c_void_p, _pd, _pd]
+    lib.mw3_cheby_steps.argtypes = [ct.c_void_p, ct.c_int32, _pd, _pd]
+    lib.mw3_cheby_k.argtypes = [ct.c_void_p]
+    lib.mw3_flux_result.argtypes = [ct.c_void_p, _pd, _pd]
+    lib.mw3_zs2_get.argtypes = [ct.c_void_p, _pd]
+    lib.mw3_flux_accum_get.argtypes = [ct.c_void_p, _pd, _pd]
+    lib.mw3_flux_accum_put.argtypes = [ct.c_void_p, _pd, _pd, _pd]
+    lib.mw3_psie_get.argtypes = [ct.c_void_p, _pd]
+    lib.mw3_state_get.argtypes = [ct.c_void_p, _pd, _pd, _pi]
+    lib.mw3_state_put.argtypes = [ct.c_void_p, _pd, _pd, ct.c_int32, _pd]
+    lib.mw3_timing_enable.argtypes = [ct.c_void_p, ct.c_int32]
+    lib.mw3_timing_get.argtypes = [ct.c_void_p, ct.c_int32, _pd, ct.POINTER(ct.c_int64)]
+    lib.mw3_kernel_name.restype = ct.c_char_p
+    lib.mw3_kernel_name.argtypes = [ct.c_int32]
+    if lib.mw3_abi_version() != MW3_ABI_VERSION:
+        raise OSError("libmw3 ABI %d != %d" % (lib.mw3_abi_version(), MW3_ABI_VERSION))
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_pd) if a is not None else None
+
+
+class Propagator:
+    """One libmw3 context: the drop-in for mod_Hphi_01y2 + the recursion."""
+
+    def __init__(self, case, precision=MW3_FP64, lib=None):
+        self.lib = lib or load_library()
+        self.case = case
+        self._setup, self._keep = make_setup(case)
+        h = ct.c_void_p()
+        rc = self.lib.mw3_create(ct.byref(self._setup), precision, ct.byref(h))
+        if rc != 0:
+            raise Mw3Error("mw3_create: %s" % self.lib.mw3_last_error(None).decode())
+        self.h = h
+        self.ntot = int(self.lib.mw3_ntot(h))
+        self.nflux = int(self.lib.mw3_nflux(h))
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise Mw3Error("%s: %s" % (what, self.lib.mw3_last_error(self.h).decode()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mw3_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def hphi(self, psi):
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        out = np.empty(self.ntot)
+        self._check(self.lib.mw3_hphi(self.h, _dp(psi), _dp(out)), "mw3_hphi")
+        return out
+
+    def begin(self, psi0):
+        psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+        st = np.zeros(3)
+        self._check(self.lib.mw3_cheby_begin(self.h, _dp(psi0), _dp(st)), "mw3_cheby_begin")
+        return st
+
+    def steps(self, n, want_norms=True, want_auto=False):
+        norms = np.zeros(n) if want_norms else None
+        au = np.zeros(n) if want_auto else None
+        self._check(self.lib.mw3_cheby_steps(self.h, n, _dp(norms), _dp(au)), "mw3_cheby_steps")
+        return norms, au
+
+    @property
+    def k(self):
+        return int(self.lib.mw3_cheby_k(self.h))
+
+    def flux_result(self):
+        c = self.case
+        tot = np.zeros(c.netot)
+        el = np.zeros((c.nelecmax, c.netot))
+        self._check(self.lib.mw3_flux_result(self.h, _dp(tot), _dp(el)), "mw3_flux_result")
+        return tot, el
+
+    def zs2(self):
+        c = self.case
+        z = np.zeros((c.ncan, c.jmax - c.jini + 1, c.nvmax - c.nvini + 1, c.netot, 2))
+        self._check(self.lib.mw3_zs2_get(self.h, _dp(z)), "mw3_zs2_get")
+        return z[..., 0] + 1j * z[..., 1]
+
+    def flux_accum(self):
+        a = np.zeros((self.nflux, self.case.netot, 2))
+        b = np.zeros((self.nflux, self.case.netot, 2))
+        self._check(self.lib.mw3_flux_accum_get(self.h, _dp(a), _dp(b)), "mw3_flux_accum_get")
+        return a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1]
+
+    def put_accum(self, zcr, zcdr, zs2):
+        def inter(z):
+            return None if z is None else np.ascontiguousarray(
+                np.stack([np.real(z), np.imag(z)], -1), dtype=np.float64)
+        a, b, s = inter(zcr), inter(zcdr), inter(zs2)
+        self._check(self.lib.mw3_flux_accum_put(self.h, _dp(a), _dp(b), _dp(s)),
+                    "mw3_flux_accum_put")
+
+    def psie(self):
+        z = np.zeros((self.case.nofe, self.ntot, 2))
+        self._check(self.lib.mw3_psie_get(self.h, _dp(z)), "mw3_psie_get")
+        return z[..., 0] + 1j * z[..., 1]
+
+    def state(self):
+        a = np.zeros(self.ntot)
+        b = np.zeros(self.ntot)
+        k = ct.c_int32()
+        self._check(self.lib.mw3_state_get(self.h, _dp(a), _dp(b), ct.byref(k)), "mw3_state_get")
+        return a, b, k.value
+
+    def put_state(self, psim1, psi, k, psi0=None):
+        psim1 = np.ascontiguousarray(psim1, dtype=np.float64)
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        p0 = None if psi0 is None else np.ascontiguousarray(psi0, dtype=np.float64)
+        self._check(self.lib.mw3_state_put(self.h, _dp(psim1), _dp(psi), k, _dp(p0)),
+                    "mw3_state_put")
+
+    def timing(self, on=True):
+        self._check(self.lib.mw3_timing_enable(self.h, 1 if on else 0), "mw3_timing_enable")
+
+    def kernel_times(self):
+        out = {}
+        for i in range(8):
+            t = ct.c_double()
+            n = ct.c_int64()
+            self._check(self.lib.mw3_timing_get(self.h, i, ct.byref(t), ct.byref(n)), "mw3_timing_get")
+            if n.value:
+                out[self.lib.mw3_kernel_name(i).decode()] = (t.value, n.value)
+        return out

@@ -1,0 +1,1122 @@
+/*
+ * cpu_ref.c -- FP64 CPU oracle for the MADWAVE3 Chebyshev H.psi path.
+ *
+ * TEST INFRASTRUCTURE.  This file restates, in plain C with OpenMP, the
+ * reference algorithm of the path so that the GPU library can be checked
+ * against it and timed beside it.  Only tests, the smoke check and the bench's
+ * CPU-baseline leg load it; the product (madwave3_amd/, libmw3) never does.
+ *
+ * Restated reference code (file:line under the reference tree):
+ *   Tradial          SRC/mod_Hphi_01y2.f:487-620   (per-angle DST sizes, momenta)
+ *   noptFFT          SRC/liboctdyn.f:1033-1117
+ *   sinmom           SRC/liboctdyn.f:1004-1031
+ *   fftmom           SRC/liboctdyn.f:1119-1151
+ *   angkinj2         SRC/mod_Hphi_01y2.f:166-311   (xj2 table, DVR form)
+ *   bfgridl2mat      SRC/mod_Hphi_01y2.f:315-484   (xl2p table, DVR form)
+ *   l2mat            SRC/liboctdyn.f:1944-1989
+ *   difs             SRC/mod_Hphi_01y2.f:623-943   (H.psi, nproc = 1)
+ *   initial step     SRC/main_madwave3.f:231-275
+ *   damped update    SRC/main_madwave3.f:324-339
+ *   check            SRC/main_madwave3.f:414-458
+ *   CoefEcheby       SRC/main_madwave3.f:462-490
+ *   ini_flux         SRC/mod_flux_01y2.f:99-257    (flux line, d/dr vector)
+ *   totalflux_k      SRC/mod_flux_01y2.f:585-660
+ *   Cvjflux_k        SRC/mod_flux_01y2.f:663-775
+ *   totalflux        SRC/mod_flux_01y2.f:778-816
+ *   funE             SRC/mod_PsiE_01y2.f:92-123
+ *
+ * The FFTW RODFT00 transforms of difs (SRC/mod_Hphi_01y2.f:780-810) are
+ * evaluated from FFTW's documented definition
+ *   Y_k = 2 sum_{j=0}^{n-1} X_j sin(pi (j+1)(k+1)/(n+1))
+ * as dense sine-matrix products.  Sums over angles and channels run in the
+ * reference's loop order for each output value; the parallel loops only
+ * partition disjoint outputs, so results do not depend on the thread count.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/mw3.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORF_EXPORT __attribute__((visibility("default")))
+
+typedef struct {
+  double re, im;
+} cplx;
+
+typedef struct oref_ctx {
+  /* copies of the setup scalars */
+  int npun1, npun2, nangu, ncan, nelec, npuntot, Jtot, inc;
+  double rmis1, ah1, rmis2, ah2, hbr, xm1red, xm2red, radcut, rotcut;
+  double emindlt, delta2;
+  int iphoto, iprod, kminelastic, icanref, nvref, jref, iomref;
+  int nvini, nvmax, jini, jmax, nv, nj;
+  int netot, nofe, ir2abs;
+  /* copies of the setup arrays */
+  int *indangreal, *npunreal, *indr1, *indr2;
+  int *iombas, *nelebas, *j00;
+  double *VVV, *absfr1, *absfr2, *etots2, *reacfct, *fd, *efe;
+  int *noBCstates;
+  double *Djmmp;
+  int ndj;
+  /* A0 layout */
+  int64_t ntot;
+  int64_t *off; /* [nangu slot][ncan] offset of the block */
+  /* Tradial */
+  int *nopr1;     /* [global angle] DST size in r */
+  double *p2r1;   /* [global angle][npun1] */
+  double *p2r2;   /* [npun2] */
+  /* DVR angular tables */
+  double *xj2;  /* xj2(npun1,nangu,nangu,ncan): [ncan][slot][iang][ir1] */
+  double *xl2p; /* xl2p(npun2,nangu,nangu,ncan,-1:1): [3][ncan][slot][iang][ir2] */
+  /* flux */
+  int ir4flux;
+  int *ir1flux; /* [global angle] */
+  cplx *zAR1flux;
+  int nflux;
+  int *fl_slot, *fl_can, *fl_ir2, *fl_ir;
+  double *potiflux;
+  cplx *zCR, *zCdR; /* [nflux][netot] */
+  cplx *zS2;        /* [ncan][nj][nv][netot] */
+  cplx *zfe;        /* [nofe][ntot] */
+  cplx *zCk;        /* [netot] */
+  /* state */
+  double *psi, *psim1, *hpsi, *psi0;
+  int k;
+  int begun;
+  char err[256];
+} oref_ctx;
+
+static char g_err[256];
+
+static void *xcalloc(size_t n, size_t sz) {
+  void *p = calloc(n ? n : 1, sz ? sz : 1);
+  return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* liboctdyn restatements                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* noptFFT (SRC/liboctdyn.f:1033-1117): smallest 2^a 3^b 5^c 7^d 11^e >= nin
+   with every exponent at most floor(log_p(ntot)) and distance below ntot,
+   capped at ntot; 1 if none. */
+ORF_EXPORT int oref_noptfft(int nin, int ntot) {
+  int primes[5] = {2, 3, 5, 7, 11};
+  int nexp[5];
+  for (int q = 0; q < 5; q++) {
+    long v = 1;
+    int n = 0;
+    for (n = 0; n <= 100; n++) {
+      v *= primes[q];
+      if (v > ntot) break;
+    }
+    nexp[q] = n;
+  }
+  int nout = 1, imin = ntot;
+  long p11 = 1;
+  for (int a11 = 0; a11 <= nexp[4]; a11++, p11 *= 11) {
+    long p7 = 1;
+    for (int a7 = 0; a7 <= nexp[3]; a7++, p7 *= 7) {
+      long p5 = 1;
+      for (int a5 = 0; a5 <= nexp[2]; a5++, p5 *= 5) {
+        long p3 = 1;
+        for (int a3 = 0; a3 <= nexp[1]; a3++, p3 *= 3) {
+          long p2 = 1;
+          for (int a2 = 0; a2 <= nexp[0]; a2++, p2 *= 2) {
+            long i = p2 * p3 * p5 * p7 * p11;
+            long idis = i - nin;
+            if (idis >= 0 && idis < imin) {
+              nout = (int)i;
+              imin = (int)idis;
+            }
+          }
+        }
+      }
+    }
+  }
+  if (nout > ntot) nout = ntot;
+  return nout;
+}
+
+/* sinmom (SRC/liboctdyn.f:1004-1031): p_k^2 hbar^2/(2 m), k = 1..npun, on a
+   box of (npun+1) steps; p2r has npun entries. */
+ORF_EXPORT void oref_sinmom(double box, int npun, double xmred, double hbr,
+                            double *pr, double *p2r) {
+  double pi = acos(-1.0);
+  double dpi = 2.0 * pi;
+  double hbrxm = 0.5 * hbr * hbr / xmred;
+  double ah = box / (double)(npun - 1);
+  double b2 = 2.0 * ((double)npun + 1.0) * ah;
+  for (int ir = 1; ir <= npun; ir++) {
+    double p = dpi * (double)ir / b2;
+    if (pr) pr[ir - 1] = p;
+    p2r[ir - 1] = p * p * hbrxm;
+  }
+}
+
+/* fftmom (SRC/liboctdyn.f:1119-1151): FFT momenta of an npun-point grid. */
+ORF_EXPORT void oref_fftmom(double box, int npun, double xmred, double hbr,
+                            double *pr, double *p2r) {
+  double pi = acos(-1.0);
+  double dpi = 2.0 * pi;
+  double hbrxm = 0.5 * hbr * hbr / xmred;
+  double ah = box / (double)(npun - 1);
+  double b2 = ah * (double)npun;
+  for (int ir = 1; ir <= npun; ir++) {
+    int iii = (ir <= npun / 2) ? ir - 1 : ir - npun - 1;
+    double p = dpi * (double)iii / b2;
+    pr[ir - 1] = p;
+    if (p2r) p2r[ir - 1] = p * p * hbrxm;
+  }
+}
+
+/* l2mat (SRC/liboctdyn.f:1944-1989): body-fixed l^2 block over Omega in
+   [om0, om1] for (j, Jtot); mat is (om1-om0+1)^2 row-major. */
+ORF_EXPORT void oref_l2mat(double *mat, double facmass, int om0, int om1,
+                           int j, int Jtot) {
+  int n = om1 - om0 + 1;
+  double xj = (double)j;
+  for (int a = 0; a < n; a++)
+    for (int b = 0; b < n; b++) {
+      int iom = om0 + a, jom = om0 + b;
+      double xk = 0.0;
+      if (iom == jom) {
+        double x1 = xj * (xj + 1.0);
+        double x2 = (double)(Jtot * (Jtot + 1) - 2 * iom * jom);
+        xk = (x1 + x2) * facmass;
+      } else if (abs(iom - jom) == 1) {
+        double x1 = (double)(Jtot * (Jtot + 1) - iom * jom);
+        double x2 = xj * (xj + 1.0) - (double)(iom * jom);
+        double x12 = sqrt(x1 * x2);
+        xk = -x12 * facmass;
+        if (iom * jom == 0) xk = xk * sqrt(2.0);
+      }
+      mat[a * n + b] = xk;
+    }
+}
+
+/* Smallest and largest eigenvalue of a small symmetric matrix (the reference
+   takes eigval(1) and eigval(n) of DIAGON, SRC/mod_Hphi_01y2.f:391-395).
+   Cyclic Jacobi rotations until the off-diagonal part vanishes. */
+ORF_EXPORT void oref_sym_eig_minmax(const double *a_in, int n, double *emin,
+                                    double *emax) {
+  double a[64 * 64];
+  memcpy(a, a_in, sizeof(double) * n * n);
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) off += fabs(a[p * n + q]);
+    if (off == 0.0) break;
+    for (int p = 0; p < n; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = a[p * n + q];
+        if (apq == 0.0) continue;
+        double theta = 0.5 * (a[q * n + q] - a[p * n + p]) / apq;
+        double t = 1.0 / (fabs(theta) + sqrt(1.0 + theta * theta));
+        if (theta < 0.0) t = -t;
+        double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        for (int r = 0; r < n; r++) {
+          double arp = a[r * n + p], arq = a[r * n + q];
+          a[r * n + p] = c * arp - s * arq;
+          a[r * n + q] = s * arp + c * arq;
+        }
+        for (int r = 0; r < n; r++) {
+          double apr = a[p * n + r], aqr = a[q * n + r];
+          a[p * n + r] = c * apr - s * aqr;
+          a[q * n + r] = s * apr + c * aqr;
+        }
+      }
+  }
+  double lo = a[0], hi = a[0];
+  for (int p = 1; p < n; p++) {
+    if (a[p * n + p] < lo) lo = a[p * n + p];
+    if (a[p * n + p] > hi) hi = a[p * n + p];
+  }
+  *emin = lo;
+  *emax = hi;
+}
+
+/* FFTW RODFT00 of length n (unnormalised), y may not alias x. */
+ORF_EXPORT void oref_dst1(int n, const double *x, double *y) {
+  double pi = acos(-1.0);
+  for (int k = 0; k < n; k++) {
+    double s = 0.0;
+    for (int j = 0; j < n; j++)
+      s += x[j] * sin(pi * (double)((long)(j + 1) * (k + 1)) / (double)(n + 1));
+    y[k] = 2.0 * s;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* sine tables for the 2-D transforms                                       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int n;
+  double *s; /* s[j*n+k] = 2 sin(pi (j+1)(k+1)/(n+1)) */
+} sintab;
+
+static sintab *g_tabs = NULL;
+static int g_ntabs = 0;
+
+static const double *get_sintab(int n) {
+  const double *res = NULL;
+#pragma omp critical(orf_sintab)
+  {
+    for (int i = 0; i < g_ntabs; i++)
+      if (g_tabs[i].n == n) res = g_tabs[i].s;
+    if (!res) {
+      g_tabs = (sintab *)realloc(g_tabs, sizeof(sintab) * (g_ntabs + 1));
+      double *s = (double *)malloc(sizeof(double) * (size_t)n * n);
+      double pi = acos(-1.0);
+      long m2 = 2L * (n + 1);
+      for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++) {
+          long m = ((long)(j + 1) * (k + 1)) % m2;
+          s[(size_t)j * n + k] = 2.0 * sin(pi * (double)m / (double)(n + 1));
+        }
+      g_tabs[g_ntabs].n = n;
+      g_tabs[g_ntabs].s = s;
+      g_ntabs++;
+      res = s;
+    }
+  }
+  return res;
+}
+
+/* ------------------------------------------------------------------------ */
+/* context set-up                                                           */
+/* ------------------------------------------------------------------------ */
+
+#define DJ(c, iom, ielec, j, iang)                                            \
+  ((c)->Djmmp[((((size_t)(iom)) * (c)->nelec + ((ielec)-1)) * (c)->ndj +      \
+               (j)) *                                                         \
+                  (c)->nangu +                                                \
+              ((iang)-1)])
+
+static int set_error(oref_ctx *c, const char *msg) {
+  if (c)
+    snprintf(c->err, sizeof c->err, "%s", msg);
+  else
+    snprintf(g_err, sizeof g_err, "%s", msg);
+  return MW3_EINVAL;
+}
+
+ORF_EXPORT const char *oref_last_error(const oref_ctx *c) {
+  return c ? c->err : g_err;
+}
+
+ORF_EXPORT void oref_destroy(oref_ctx *c);
+
+#define DUPI(dst, src, n)                                     \
+  do {                                                        \
+    (dst) = (int *)xcalloc((size_t)(n), sizeof(int));         \
+    if (src) memcpy((dst), (src), sizeof(int) * (size_t)(n)); \
+  } while (0)
+#define DUPD(dst, src, n)                                        \
+  do {                                                           \
+    (dst) = (double *)xcalloc((size_t)(n), sizeof(double));      \
+    if (src) memcpy((dst), (src), sizeof(double) * (size_t)(n)); \
+  } while (0)
+
+ORF_EXPORT int oref_create(const mw3_setup *s, oref_ctx **out) {
+  *out = NULL;
+  if (!s || s->abi_version != MW3_ABI_VERSION)
+    return set_error(NULL, "oref_create: bad setup or ABI version");
+  if (s->proj_j2_up != 0)
+    return set_error(NULL, "oref_create: Proj_j2_Up != 0 is not restated");
+  if (s->npun1 < 2 || s->npun2 < 2 || s->nangu < 2)
+    return set_error(NULL, "oref_create: needs npun1, npun2, nangu > 1");
+  oref_ctx *c = (oref_ctx *)xcalloc(1, sizeof(oref_ctx));
+  c->npun1 = s->npun1;
+  c->npun2 = s->npun2;
+  c->nangu = s->nangu;
+  c->ncan = s->ncan;
+  c->nelec = s->nelecmax;
+  c->npuntot = s->npuntot;
+  c->Jtot = s->Jtot;
+  c->inc = s->inc;
+  c->rmis1 = s->rmis1;
+  c->ah1 = s->ah1;
+  c->rmis2 = s->rmis2;
+  c->ah2 = s->ah2;
+  c->hbr = s->hbr;
+  c->xm1red = s->xm1red;
+  c->xm2red = s->xm2red;
+  c->radcut = s->radcutmax;
+  c->rotcut = s->rotcutmax;
+  c->emindlt = s->emindlt;
+  c->delta2 = s->delta2;
+  c->iphoto = s->iphoto;
+  c->iprod = s->iprod;
+  c->kminelastic = s->kminelastic;
+  c->icanref = s->icanref;
+  c->nvref = s->nvref;
+  c->jref = s->jref;
+  c->iomref = s->iomref;
+  c->nvini = s->nvini;
+  c->nvmax = s->nvmax;
+  c->jini = s->jini;
+  c->jmax = s->jmax;
+  c->nv = s->nvmax - s->nvini + 1;
+  c->nj = s->jmax - s->jini + 1;
+  c->netot = s->netot;
+  c->nofe = s->nofe;
+  c->ir2abs = s->ir2abs;
+  c->ndj = s->ndj;
+  int na = c->nangu, np = c->npuntot, ne = c->nelec;
+  DUPI(c->indangreal, s->indangreal, na);
+  DUPI(c->npunreal, s->npunreal, na);
+  DUPI(c->indr1, s->indr1, (size_t)na * np);
+  DUPI(c->indr2, s->indr2, (size_t)na * np);
+  DUPI(c->iombas, s->iombas, c->ncan);
+  DUPI(c->nelebas, s->nelebas, c->ncan);
+  DUPI(c->j00, s->j00, c->ncan);
+  DUPD(c->VVV, s->VVV, (size_t)ne * ne * na * np);
+  DUPD(c->absfr1, s->absfr1, c->npun1);
+  DUPD(c->absfr2, s->absfr2, c->npun2);
+  DUPD(c->etots2, s->etots2, c->netot);
+  DUPD(c->reacfct, s->reacfct, c->netot);
+  DUPD(c->efe, s->efe, c->nofe);
+  DUPI(c->noBCstates, s->noBCstates, (size_t)ne * c->nj);
+  DUPD(c->fd, s->fd, (size_t)ne * c->nj * c->nv * c->npun1);
+  DUPD(c->Djmmp, s->Djmmp, (size_t)(c->Jtot + 1) * ne * c->ndj * na);
+
+  /* A0 layout offsets (SRC/mod_pot_01y2.f:435-455) */
+  c->off = (int64_t *)xcalloc((size_t)na * c->ncan, sizeof(int64_t));
+  int64_t t = 0;
+  for (int sl = 0; sl < na; sl++) {
+    int iang = c->indangreal[sl];
+    for (int ic = 0; ic < c->ncan; ic++) {
+      c->off[(size_t)sl * c->ncan + ic] = t;
+      t += c->npunreal[iang - 1];
+    }
+  }
+  c->ntot = t;
+
+  /* Tradial (SRC/mod_Hphi_01y2.f:508-599) */
+  c->nopr1 = (int *)xcalloc(na, sizeof(int));
+  c->p2r1 = (double *)xcalloc((size_t)na * c->npun1, sizeof(double));
+  c->p2r2 = (double *)xcalloc(c->npun2, sizeof(double));
+  for (int iang = 1; iang <= na; iang++) {
+    int ir1min = c->npun1, ir1max = 0;
+    for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+      int ir1 = c->indr1[(size_t)(iang - 1) * np + ir];
+      if (ir1 < ir1min) ir1min = ir1;
+      if (ir1 > ir1max) ir1max = ir1;
+    }
+    int n0 = ir1max - ir1min + 1;
+    int n1 = oref_noptfft(n0, c->npun1);
+    if (n1 < s->npun1min) n1 = s->npun1min;
+    if (n1 > c->npun1 || n1 < 2) {
+      oref_destroy(c);
+      return set_error(NULL, "oref_create: r DST size outside [2, npun1]");
+    }
+    c->nopr1[iang - 1] = n1;
+    double box = (double)(n1 - 1) * c->ah1;
+    oref_sinmom(box, n1, c->xm1red, c->hbr, NULL,
+                c->p2r1 + (size_t)(iang - 1) * c->npun1);
+  }
+  {
+    int n2 = c->npun2;
+    double box = (double)(n2 - 1) * c->ah2;
+    oref_sinmom(box, n2, c->xm2red, c->hbr, NULL, c->p2r2);
+  }
+
+  /* angkinj2 (SRC/mod_Hphi_01y2.f:221-305) */
+  size_t nxj = (size_t)c->ncan * na * na * c->npun1;
+  c->xj2 = (double *)xcalloc(nxj, sizeof(double));
+  int jjjmax = (na - 1) * c->inc;
+  for (int ic = 0; ic < c->ncan; ic++) {
+    int ielec = c->nelebas[ic], iom = c->iombas[ic];
+    int iomdi = s->iomdiat[ielec - 1];
+    for (int ir1 = 1; ir1 <= c->npun1; ir1++) {
+      double r1 = c->rmis1 + (double)(ir1 - 1) * c->ah1;
+      for (int k = 0; k < s->nojbas[ic]; k++) {
+        int jk = s->jbas[(size_t)ic * na + k];
+        if (jk > jjjmax) continue;
+        double xj = (double)jk, xom = (double)iomdi;
+        double xkin = (xj * (xj + 1.0) - xom * xom) * c->hbr * c->hbr * 0.5;
+        double yyy = xkin / (c->xm1red * r1 * r1);
+        if (yyy > c->rotcut) yyy = c->rotcut;
+        for (int i = 1; i <= na; i++)
+          for (int jp = 1; jp <= na; jp++) {
+            int j = c->indangreal[jp - 1];
+            double x1 = DJ(c, iom, ielec, jk, i);
+            double x2 = DJ(c, iom, ielec, jk, j);
+            size_t idx =
+                (((size_t)ic * na + (jp - 1)) * na + (i - 1)) * c->npun1 +
+                (ir1 - 1);
+            c->xj2[idx] += yyy * x1 * x2;
+          }
+      }
+    }
+  }
+
+  /* bfgridl2mat (SRC/mod_Hphi_01y2.f:343-475) */
+  size_t nxl = (size_t)3 * c->ncan * na * na * c->npun2;
+  c->xl2p = (double *)xcalloc(nxl, sizeof(double));
+  {
+    int iommin = s->iommin, iommax = s->iommax;
+    double facmass = c->hbr * c->hbr * 0.5 / c->xm2red;
+    int parJ = (c->Jtot % 2 == 0) ? 1 : -1;
+    for (int ielec = 1; ielec <= ne; ielec++) {
+      int sigelec = 1;
+      if (s->sigdiat[ielec - 1] == -1.0 || s->sigatom[ielec - 1] == -1.0)
+        sigelec = -1;
+      for (int j = s->j0; j <= (na - 1) * c->inc; j += c->inc) {
+        int iom0 = iommin;
+        if (s->iparity != sigelec * parJ && iommin == 0) iom0 = 1;
+        if (iom0 > j) continue;
+        int iom1 = j < iommax ? j : iommax;
+        int nnn = iom1 - iom0 + 1;
+        if (nnn <= 0) continue;
+        double bfl[64 * 64];
+        if (nnn > 64) {
+          oref_destroy(c);
+          return set_error(NULL, "oref_create: more than 64 Omega values");
+        }
+        oref_l2mat(bfl, facmass, iom0, iom1, j, c->Jtot);
+        double yylmin, yylmax, yylmean;
+        if (nnn > 1) {
+          oref_sym_eig_minmax(bfl, nnn, &yylmin, &yylmax);
+        } else {
+          yylmin = bfl[0];
+          yylmax = yylmin;
+        }
+        yylmean = 0.5 * (yylmin + yylmax);
+        if (nnn == 1) yylmean = yylmin;
+        for (int ir2 = 1; ir2 <= c->npun2; ir2++) {
+          double r2 = c->rmis2 + (double)(ir2 - 1) * c->ah2;
+          double r2eff = r2;
+          double emeaneff = yylmean / (r2 * r2);
+          if (emeaneff > c->rotcut) {
+            double eee = c->rotcut;
+            r2eff = sqrt(fabs(yylmean / eee));
+          }
+          for (int a = 0; a < nnn; a++)
+            for (int b = 0; b < nnn; b++) {
+              int iom = iom0 + a, jom = iom0 + b;
+              if (abs(jom - iom) > 1) continue;
+              double xl = 0.0;
+              double bv = bfl[a * nnn + b];
+              if (fabs(bv) > 1e-8 && r2eff > 1e-3) xl = bv / (r2eff * r2eff);
+              for (int jc = 0; jc < c->ncan; jc++) {
+                if (c->nelebas[jc] != ielec || c->iombas[jc] != jom) continue;
+                int iq = jom - iom; /* -1, 0, +1 */
+                for (int iang = 1; iang <= na; iang++)
+                  for (int jangp = 1; jangp <= na; jangp++) {
+                    int jang = c->indangreal[jangp - 1];
+                    double x11 = DJ(c, iom, ielec, j, iang);
+                    double x22 = DJ(c, jom, ielec, j, jang);
+                    size_t idx =
+                        ((((size_t)(iq + 1) * c->ncan + jc) * na + (jangp - 1)) *
+                             na +
+                         (iang - 1)) *
+                            c->npun2 +
+                        (ir2 - 1);
+                    c->xl2p[idx] += x11 * x22 * xl;
+                  }
+              }
+            }
+        }
+      }
+    }
+  }
+
+  /* ini_flux: flux line and d/dr vector (SRC/mod_flux_01y2.f:108-257) */
+  c->ir1flux = (int *)xcalloc(na, sizeof(int));
+  c->zAR1flux = (cplx *)xcalloc(c->npun1, sizeof(cplx));
+  c->ir4flux = 0;
+  for (int ir1 = 1; ir1 <= c->npun1; ir1++) {
+    double r1 = c->rmis1 + (double)(ir1 - 1) * c->ah1;
+    if (r1 <= s->r1flux && ir1 > c->ir4flux) c->ir4flux = ir1;
+  }
+  {
+    double *xk = (double *)xcalloc(c->npun1, sizeof(double));
+    double box1 = (double)(c->npun1 - 1) * c->ah1;
+    oref_fftmom(box1, c->npun1, c->xm1red, c->hbr, xk, NULL);
+    for (int m = 1; m <= c->npun1; m++) {
+      double d2r = (double)(c->ir4flux - m) * c->ah1;
+      double zr = 0.0, zi = 0.0;
+      for (int k = 1; k <= c->npun1; k++) {
+        double x = xk[k - 1];
+        /* (0, x) * exp(i x d2r) */
+        double er = cos(x * d2r), ei = sin(x * d2r);
+        zr += -x * ei;
+        zi += x * er;
+      }
+      c->zAR1flux[m - 1].re = zr / (double)(2 * c->npun1);
+      c->zAR1flux[m - 1].im = zi / (double)(2 * c->npun1);
+    }
+    free(xk);
+  }
+  for (int iang = 1; iang <= na; iang++) {
+    for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+      int ir1 = c->indr1[(size_t)(iang - 1) * np + ir];
+      double r1 = (double)(ir1 - 1) * c->ah1 + c->rmis1;
+      if (r1 <= s->r1flux && ir1 > c->ir1flux[iang - 1])
+        c->ir1flux[iang - 1] = ir1;
+    }
+  }
+  int nfl = 0;
+  for (int sl = 0; sl < na; sl++) {
+    int iang = c->indangreal[sl];
+    for (int ic = 0; ic < c->ncan; ic++)
+      for (int ir = 0; ir < c->npunreal[iang - 1]; ir++)
+        if (c->indr1[(size_t)(iang - 1) * np + ir] == c->ir1flux[iang - 1])
+          nfl++;
+  }
+  c->nflux = nfl;
+  c->fl_slot = (int *)xcalloc(nfl, sizeof(int));
+  c->fl_can = (int *)xcalloc(nfl, sizeof(int));
+  c->fl_ir2 = (int *)xcalloc(nfl, sizeof(int));
+  c->fl_ir = (int *)xcalloc(nfl, sizeof(int));
+  c->potiflux = (double *)xcalloc(nfl, sizeof(double));
+  nfl = 0;
+  for (int sl = 0; sl < na; sl++) {
+    int iang = c->indangreal[sl];
+    for (int ic = 0; ic < c->ncan; ic++) {
+      int ielec = c->nelebas[ic];
+      for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+        if (c->indr1[(size_t)(iang - 1) * np + ir] != c->ir1flux[iang - 1])
+          continue;
+        c->fl_slot[nfl] = sl;
+        c->fl_can[nfl] = ic;
+        c->fl_ir2[nfl] = c->indr2[(size_t)(iang - 1) * np + ir];
+        c->fl_ir[nfl] = ir;
+        c->potiflux[nfl] =
+            c->VVV[((((size_t)(ielec - 1)) * ne + (ielec - 1)) * na + sl) * np +
+                   ir];
+        nfl++;
+      }
+    }
+  }
+  c->zCR = (cplx *)xcalloc((size_t)c->nflux * c->netot, sizeof(cplx));
+  c->zCdR = (cplx *)xcalloc((size_t)c->nflux * c->netot, sizeof(cplx));
+  c->zS2 = (cplx *)xcalloc((size_t)c->ncan * c->nj * c->nv * c->netot,
+                           sizeof(cplx));
+  c->zfe = (cplx *)xcalloc((size_t)c->nofe * c->ntot, sizeof(cplx));
+  c->zCk = (cplx *)xcalloc(c->netot, sizeof(cplx));
+  c->psi = (double *)xcalloc(c->ntot, sizeof(double));
+  c->psim1 = (double *)xcalloc(c->ntot, sizeof(double));
+  c->hpsi = (double *)xcalloc(c->ntot, sizeof(double));
+  c->psi0 = (double *)xcalloc(c->ntot, sizeof(double));
+  *out = c;
+  return MW3_OK;
+}
+
+ORF_EXPORT void oref_destroy(oref_ctx *c) {
+  if (!c) return;
+  free(c->indangreal);
+  free(c->npunreal);
+  free(c->indr1);
+  free(c->indr2);
+  free(c->iombas);
+  free(c->nelebas);
+  free(c->j00);
+  free(c->VVV);
+  free(c->absfr1);
+  free(c->absfr2);
+  free(c->etots2);
+  free(c->reacfct);
+  free(c->efe);
+  free(c->noBCstates);
+  free(c->fd);
+  free(c->Djmmp);
+  free(c->off);
+  free(c->nopr1);
+  free(c->p2r1);
+  free(c->p2r2);
+  free(c->xj2);
+  free(c->xl2p);
+  free(c->ir1flux);
+  free(c->zAR1flux);
+  free(c->fl_slot);
+  free(c->fl_can);
+  free(c->fl_ir2);
+  free(c->fl_ir);
+  free(c->potiflux);
+  free(c->zCR);
+  free(c->zCdR);
+  free(c->zS2);
+  free(c->zfe);
+  free(c->zCk);
+  free(c->psi);
+  free(c->psim1);
+  free(c->hpsi);
+  free(c->psi0);
+  free(c);
+}
+
+ORF_EXPORT int64_t oref_ntot(const oref_ctx *c) { return c->ntot; }
+ORF_EXPORT int64_t oref_nflux(const oref_ctx *c) { return c->nflux; }
+ORF_EXPORT int oref_nopr1(const oref_ctx *c, int iang) {
+  return c->nopr1[iang - 1];
+}
+ORF_EXPORT int oref_ir4flux(const oref_ctx *c) { return c->ir4flux; }
+ORF_EXPORT int oref_ir1flux(const oref_ctx *c, int iang) {
+  return c->ir1flux[iang - 1];
+}
+ORF_EXPORT const double *oref_xj2(const oref_ctx *c) { return c->xj2; }
+ORF_EXPORT const double *oref_xl2p(const oref_ctx *c) { return c->xl2p; }
+ORF_EXPORT const double *oref_p2r1(const oref_ctx *c) { return c->p2r1; }
+ORF_EXPORT const double *oref_p2r2(const oref_ctx *c) { return c->p2r2; }
+ORF_EXPORT const double *oref_zar1flux(const oref_ctx *c) {
+  return (const double *)c->zAR1flux;
+}
+
+/* ------------------------------------------------------------------------ */
+/* difs (SRC/mod_Hphi_01y2.f:623-943), nproc = 1                            */
+/* ------------------------------------------------------------------------ */
+
+ORF_EXPORT int oref_hphi(oref_ctx *c, const double *psi, double *hpsi) {
+  int na = c->nangu, np = c->npuntot, n1 = c->npun1, n2 = c->npun2;
+  int nc = c->ncan, ne = c->nelec;
+  size_t slab = (size_t)n1 * n2;
+  /* rphi0: dense copy of every (channel, angle) block, [ic][slot][ir1][ir2] */
+  double *dense = (double *)xcalloc((size_t)nc * na * slab, sizeof(double));
+  if (!dense) return MW3_ENOMEM;
+#pragma omp parallel for schedule(dynamic) collapse(2)
+  for (int ic = 0; ic < nc; ic++)
+    for (int sl = 0; sl < na; sl++) {
+      int iang = c->indangreal[sl];
+      double *d = dense + ((size_t)ic * na + sl) * slab;
+      const double *p = psi + c->off[(size_t)sl * nc + ic];
+      for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+        int ir1 = c->indr1[(size_t)(iang - 1) * np + ir];
+        int ir2 = c->indr2[(size_t)(iang - 1) * np + ir];
+        d[(size_t)(ir1 - 1) * n2 + (ir2 - 1)] = p[ir];
+      }
+    }
+
+  /* 1) T phi: 2-D RODFT00, scale, RODFT00 (SRC/mod_Hphi_01y2.f:762-832) */
+#pragma omp parallel
+  {
+    double *a = (double *)malloc(sizeof(double) * slab);
+    double *b = (double *)malloc(sizeof(double) * slab);
+#pragma omp for schedule(dynamic) collapse(2)
+    for (int ic = 0; ic < nc; ic++)
+      for (int sl = 0; sl < na; sl++) {
+        int iang = c->indangreal[sl];
+        int nn1 = c->nopr1[iang - 1], nn2 = n2;
+        const double *S1 = get_sintab(nn1);
+        const double *S2 = get_sintab(nn2);
+        const double *d = dense + ((size_t)ic * na + sl) * slab;
+        double divi = 1.0 / (4.0 * (double)(nn1 + 1) * (double)(nn2 + 1));
+        /* forward: a = X S2 (rows), b = S1^T a (columns) */
+        for (int pass = 0; pass < 2; pass++) {
+          const double *src = pass == 0 ? d : b;
+          for (int i = 0; i < nn1; i++) {
+            double *ar = a + (size_t)i * nn2;
+            for (int k = 0; k < nn2; k++) ar[k] = 0.0;
+            for (int j = 0; j < nn2; j++) {
+              double x = src[(size_t)i * nn2 + j];
+              if (x == 0.0) continue;
+              const double *sr = S2 + (size_t)j * nn2;
+              for (int k = 0; k < nn2; k++) ar[k] += x * sr[k];
+            }
+          }
+          for (int k1 = 0; k1 < nn1; k1++) {
+            double *br = b + (size_t)k1 * nn2;
+            for (int k = 0; k < nn2; k++) br[k] = 0.0;
+            for (int j1 = 0; j1 < nn1; j1++) {
+              double sv = S1[(size_t)j1 * nn1 + k1];
+              const double *ar = a + (size_t)j1 * nn2;
+              for (int k = 0; k < nn2; k++) br[k] += sv * ar[k];
+            }
+          }
+          if (pass == 0) {
+            const double *p2r1 = c->p2r1 + (size_t)(iang - 1) * n1;
+            for (int k1 = 0; k1 < nn1; k1++)
+              for (int k2 = 0; k2 < nn2; k2++) {
+                double ekin = p2r1[k1] + c->p2r2[k2];
+                if (ekin > c->radcut) ekin = c->radcut;
+                b[(size_t)k1 * nn2 + k2] =
+                    b[(size_t)k1 * nn2 + k2] * ekin * divi;
+              }
+          }
+        }
+        /* scatter: raux(ir2,ir1) for ir1 <= nn1, psi itself beyond (Q1) */
+        double *h = hpsi + c->off[(size_t)sl * nc + ic];
+        for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+          int ir1 = c->indr1[(size_t)(iang - 1) * np + ir];
+          int ir2 = c->indr2[(size_t)(iang - 1) * np + ir];
+          double v;
+          if (ir1 <= nn1)
+            v = b[(size_t)(ir1 - 1) * nn2 + (ir2 - 1)];
+          else
+            v = d[(size_t)(ir1 - 1) * n2 + (ir2 - 1)];
+          h[ir] = 0.0 + v;
+        }
+      }
+    free(a);
+    free(b);
+  }
+
+  /* 2) V phi (SRC/mod_Hphi_01y2.f:838-853) and 3) j^2 + l^2 (859-902),
+     accumulated per target block in the reference's source order. */
+#pragma omp parallel
+  {
+    double *acc = (double *)malloc(sizeof(double) * (size_t)np);
+#pragma omp for schedule(dynamic) collapse(2)
+    for (int it = 0; it < nc; it++)
+      for (int tsl = 0; tsl < na; tsl++) {
+        int iang = c->indangreal[tsl];
+        int npr = c->npunreal[iang - 1];
+        const int *i1 = c->indr1 + (size_t)(iang - 1) * np;
+        const int *i2 = c->indr2 + (size_t)(iang - 1) * np;
+        double *h = hpsi + c->off[(size_t)tsl * nc + it];
+        int et = c->nelebas[it], omt = c->iombas[it];
+        /* V: sources s with Omega_s == Omega_t, in source channel order */
+        for (int is = 0; is < nc; is++) {
+          if (c->iombas[is] != omt) continue;
+          int es = c->nelebas[is];
+          const double *vv =
+              c->VVV + ((((size_t)(et - 1)) * ne + (es - 1)) * na + tsl) * np;
+          const double *p = psi + c->off[(size_t)tsl * nc + is];
+          for (int ir = 0; ir < npr; ir++) h[ir] += p[ir] * vv[ir];
+        }
+        /* angular: sum over source channels and source angles */
+        for (int ir = 0; ir < npr; ir++) acc[ir] = 0.0;
+        for (int is = 0; is < nc; is++) {
+          if (c->nelebas[is] != et) continue;
+          int q = c->iombas[is] - omt; /* Omega_s - Omega_t */
+          if (q < -1 || q > 1) continue;
+          for (int jsl = 0; jsl < na; jsl++) {
+            const double *d = dense + ((size_t)is * na + jsl) * slab;
+            const double *xl =
+                c->xl2p +
+                ((((size_t)(q + 1) * nc + is) * na + jsl) * na + (iang - 1)) *
+                    n2;
+            if (q == 0) {
+              const double *xj =
+                  c->xj2 + (((size_t)is * na + jsl) * na + (iang - 1)) * n1;
+              for (int ir = 0; ir < npr; ir++) {
+                int r1 = i1[ir] - 1, r2 = i2[ir] - 1;
+                acc[ir] += d[(size_t)r1 * n2 + r2] * (xj[r1] + xl[r2]);
+              }
+            } else {
+              for (int ir = 0; ir < npr; ir++) {
+                int r1 = i1[ir] - 1, r2 = i2[ir] - 1;
+                acc[ir] += d[(size_t)r1 * n2 + r2] * xl[r2];
+              }
+            }
+          }
+        }
+        for (int ir = 0; ir < npr; ir++) h[ir] += acc[ir];
+      }
+    free(acc);
+  }
+  free(dense);
+  return MW3_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* recursion and projections                                                */
+/* ------------------------------------------------------------------------ */
+
+/* CoefEcheby (SRC/main_madwave3.f:462-490) on an energy list. */
+static void coef_echeby(const oref_ctx *c, int k, const double *E, int n,
+                        cplx *out) {
+  double d = (k == 0) ? 1.0 : 2.0;
+  double zf = d * c->hbr / c->delta2;
+  for (int ie = 0; ie < n; ie++) {
+    double Es = (E[ie] - c->emindlt) / c->delta2;
+    double expo = -(double)k * acos(Es);
+    double deno = sqrt(1.0 - Es * Es);
+    out[ie].re = zf * cos(expo) / deno;
+    out[ie].im = zf * sin(expo) / deno;
+  }
+}
+
+ORF_EXPORT void oref_coef_echeby(const oref_ctx *c, int k, const double *E,
+                                 int n, double *out) {
+  coef_echeby(c, k, E, n, (cplx *)out);
+}
+
+/* totalflux_k (SRC/mod_flux_01y2.f:585-660) */
+static void totalflux_k(oref_ctx *c, int k) {
+  int na = c->nangu, np = c->npuntot, nc = c->ncan, n2 = c->npun2;
+  cplx zero = {0, 0};
+  int zeroall = (c->iphoto == 0 && c->iprod == 1 && k < c->kminelastic);
+  /* zfR1flux(ir2,slot,can) = sum over kept points of zAR1flux(ir1) psi */
+  cplx *zf = (cplx *)xcalloc((size_t)n2 * na * nc, sizeof(cplx));
+  for (int sl = 0; sl < na; sl++) {
+    int iang = c->indangreal[sl];
+    for (int ic = 0; ic < nc; ic++) {
+      const double *p = c->psi + c->off[(size_t)sl * nc + ic];
+      cplx *z = zf + ((size_t)ic * na + sl) * n2;
+      for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+        int ir1 = c->indr1[(size_t)(iang - 1) * np + ir];
+        int ir2 = c->indr2[(size_t)(iang - 1) * np + ir];
+        z[ir2 - 1].re += c->zAR1flux[ir1 - 1].re * p[ir];
+        z[ir2 - 1].im += c->zAR1flux[ir1 - 1].im * p[ir];
+      }
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int f = 0; f < c->nflux; f++) {
+    int sl = c->fl_slot[f], ic = c->fl_can[f];
+    double zR = c->psi[c->off[(size_t)sl * nc + ic] + c->fl_ir[f]];
+    cplx zdR = zf[((size_t)ic * na + sl) * n2 + (c->fl_ir2[f] - 1)];
+    for (int ie = 0; ie < c->netot; ie++) {
+      cplx ze = zeroall ? zero : c->zCk[ie];
+      if (c->etots2[ie] > c->potiflux[f]) {
+        cplx *a = &c->zCR[(size_t)f * c->netot + ie];
+        cplx *b = &c->zCdR[(size_t)f * c->netot + ie];
+        a->re += ze.re * zR;
+        a->im += ze.im * zR;
+        b->re += ze.re * zdR.re - ze.im * zdR.im;
+        b->im += ze.re * zdR.im + ze.im * zdR.re;
+      }
+    }
+  }
+  free(zf);
+}
+
+/* Cvjflux_k (SRC/mod_flux_01y2.f:663-775) */
+static void cvjflux_k(oref_ctx *c, int k, double *cvj_out) {
+  int na = c->nangu, np = c->npuntot, nc = c->ncan, n1 = c->npun1;
+  double *funreac = (double *)xcalloc((size_t)nc * n1 * na, sizeof(double));
+  double sah2 = sqrt(c->ah2);
+  for (int sl = 0; sl < na; sl++) {
+    int iang = c->indangreal[sl];
+    for (int ic = 0; ic < nc; ic++) {
+      const double *p = c->psi + c->off[(size_t)sl * nc + ic];
+      for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+        int ir1 = c->indr1[(size_t)(iang - 1) * np + ir];
+        int ir2 = c->indr2[(size_t)(iang - 1) * np + ir];
+        if (ir2 == c->ir2abs)
+          funreac[((size_t)ic * n1 + (ir1 - 1)) * na + sl] = p[ir] / sah2;
+      }
+    }
+  }
+  size_t ncvj = (size_t)nc * c->nj * c->nv;
+  double *rpopi = (double *)xcalloc(ncvj, sizeof(double));
+  for (int ic = 0; ic < nc; ic++) {
+    int ielec = c->nelebas[ic], iom = c->iombas[ic];
+    for (int ir1 = 1; ir1 <= n1; ir1++)
+      for (int j = c->j00[ic]; j <= c->jmax; j += c->inc) {
+        double xj = 0.0;
+        for (int sl = 0; sl < na; sl++) {
+          int iang = c->indangreal[sl];
+          double rori = funreac[((size_t)ic * n1 + (ir1 - 1)) * na + sl];
+          xj += rori * DJ(c, iom, ielec, j, iang);
+        }
+        if (j < c->jini) continue;
+        int nbc = c->noBCstates[(size_t)(ielec - 1) * c->nj + (j - c->jini)];
+        int maxnv = c->nvmax < nbc ? c->nvmax : nbc;
+        for (int iv = c->nvini; iv <= maxnv; iv++) {
+          double fdv =
+              c->fd[((((size_t)(ielec - 1)) * c->nj + (j - c->jini)) * c->nv +
+                     (iv - c->nvini)) *
+                        n1 +
+                    (ir1 - 1)];
+          rpopi[((size_t)ic * c->nj + (j - c->jini)) * c->nv + (iv - c->nvini)] +=
+              xj * fdv;
+        }
+      }
+  }
+  for (int ic = 0; ic < nc; ic++) {
+    int ielec = c->nelebas[ic];
+    for (int j = c->j00[ic]; j <= c->jmax; j += c->inc) {
+      if (j < c->jini) continue;
+      int nbc = c->noBCstates[(size_t)(ielec - 1) * c->nj + (j - c->jini)];
+      int maxnv = c->nvmax < nbc ? c->nvmax : nbc;
+      for (int iv = c->nvini; iv <= maxnv; iv++) {
+        int ifail = 0;
+        int isref;
+        if (c->iomref >= 0)
+          isref = (ic + 1 == c->icanref && iv == c->nvref && j == c->jref);
+        else
+          isref = (iv == c->nvref && j == c->jref);
+        if (isref && c->iprod != 1 && c->iphoto == 0 && k < c->kminelastic)
+          ifail = 1;
+        if (ifail) continue;
+        double cv =
+            rpopi[((size_t)ic * c->nj + (j - c->jini)) * c->nv + (iv - c->nvini)];
+        cplx *z = c->zS2 +
+                  (((size_t)ic * c->nj + (j - c->jini)) * c->nv + (iv - c->nvini)) *
+                      c->netot;
+        for (int ie = 0; ie < c->netot; ie++) {
+          z[ie].re += cv * c->zCk[ie].re;
+          z[ie].im += cv * c->zCk[ie].im;
+        }
+      }
+    }
+  }
+  if (cvj_out) memcpy(cvj_out, rpopi, sizeof(double) * ncvj);
+  free(rpopi);
+  free(funreac);
+}
+
+/* funE (SRC/mod_PsiE_01y2.f:92-123) */
+static void fun_e(oref_ctx *c, int k) {
+  if (c->nofe <= 0) return;
+  cplx *ck = (cplx *)xcalloc(c->nofe, sizeof(cplx));
+  coef_echeby(c, k, c->efe, c->nofe, ck);
+  for (int ie = 0; ie < c->nofe; ie++) {
+    cplx *z = c->zfe + (size_t)ie * c->ntot;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < c->ntot; i++) {
+      z[i].re += ck[ie].re * c->psi[i];
+      z[i].im += ck[ie].im * c->psi[i];
+    }
+  }
+  free(ck);
+}
+
+static void clear_accumulators(oref_ctx *c) {
+  memset(c->zCR, 0, sizeof(cplx) * (size_t)c->nflux * c->netot);
+  memset(c->zCdR, 0, sizeof(cplx) * (size_t)c->nflux * c->netot);
+  memset(c->zS2, 0, sizeof(cplx) * (size_t)c->ncan * c->nj * c->nv * c->netot);
+  memset(c->zfe, 0, sizeof(cplx) * (size_t)c->nofe * c->ntot);
+}
+
+/* Initial Chebyshev step (SRC/main_madwave3.f:231-275). */
+ORF_EXPORT int oref_cheby_begin(oref_ctx *c, const double *psi0,
+                                double *stats3) {
+  int64_t n = c->ntot;
+  memcpy(c->psi0, psi0, sizeof(double) * n);
+  memcpy(c->psim1, psi0, sizeof(double) * n);
+  int rc = oref_hphi(c, psi0, c->hpsi);
+  if (rc) return rc;
+  double emean = 0, x1 = 0, x2 = 0;
+  for (int64_t i = 0; i < n; i++) {
+    c->psi[i] = (c->hpsi[i] - c->emindlt * c->psim1[i]) / c->delta2;
+    emean += c->psim1[i] * c->hpsi[i];
+    x1 += c->psim1[i] * c->psim1[i];
+    x2 += c->hpsi[i] * c->hpsi[i];
+  }
+  if (stats3) {
+    stats3[0] = emean;
+    stats3[1] = x1;
+    stats3[2] = x2;
+  }
+  c->k = 1;
+  c->begun = 1;
+  clear_accumulators(c);
+  return MW3_OK;
+}
+
+/* Damped recursion and the per-step projections (SRC/main_madwave3.f:324-353).
+   cvj (may be NULL) receives Cvj of every step: [nsteps][ncan][nj][nv]. */
+ORF_EXPORT int oref_cheby_steps(oref_ctx *c, int nsteps, double *norms,
+                                double *autocorr, double *cvj) {
+  if (!c->begun) {
+    snprintf(c->err, sizeof c->err, "oref_cheby_steps before begin");
+    return MW3_ESTATE;
+  }
+  int na = c->nangu, np = c->npuntot, nc = c->ncan;
+  size_t ncvj = (size_t)nc * c->nj * c->nv;
+  for (int s = 0; s < nsteps; s++) {
+    int rc = oref_hphi(c, c->psi, c->hpsi);
+    if (rc) return rc;
+#pragma omp parallel for schedule(dynamic)
+    for (int sl = 0; sl < na; sl++) {
+      int iang = c->indangreal[sl];
+      for (int ic = 0; ic < nc; ic++) {
+        int64_t o = c->off[(size_t)sl * nc + ic];
+        for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+          int ir1 = c->indr1[(size_t)(iang - 1) * np + ir];
+          int ir2 = c->indr2[(size_t)(iang - 1) * np + ir];
+          int64_t i = o + ir;
+          double fabs_ = c->absfr1[ir1 - 1] * c->absfr2[ir2 - 1];
+          double rHnorm = (c->hpsi[i] - c->emindlt * c->psi[i]) / c->delta2;
+          double xxx = (2.0 * rHnorm - fabs_ * c->psim1[i]) * fabs_;
+          c->psim1[i] = c->psi[i];
+          c->psi[i] = xxx;
+        }
+      }
+    }
+    c->k += 1;
+    int k = c->k;
+    coef_echeby(c, k, c->etots2, c->netot, c->zCk);
+    totalflux_k(c, k);
+    cvjflux_k(c, k, cvj ? cvj + (size_t)s * ncvj : NULL);
+    double xn = 0.0, au = 0.0;
+    for (int64_t i = 0; i < c->ntot; i++) xn += c->psi[i] * c->psi[i];
+    if (c->iphoto >= 1)
+      for (int64_t i = 0; i < c->ntot; i++) au += c->psi0[i] * c->psi[i];
+    if (norms) norms[s] = xn;
+    if (autocorr) autocorr[s] = au;
+    fun_e(c, k);
+  }
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_cheby_k(const oref_ctx *c) { return c->k; }
+
+/* totalflux (SRC/mod_flux_01y2.f:778-816) */
+ORF_EXPORT int oref_flux_result(oref_ctx *c, double *S2prodtot,
+                                double *S2prodelec) {
+  for (int ie = 0; ie < c->netot; ie++) {
+    double tot = 0.0;
+    double el[64];
+    for (int e = 0; e < c->nelec && e < 64; e++) el[e] = 0.0;
+    for (int f = 0; f < c->nflux; f++) {
+      cplx a = c->zCR[(size_t)f * c->netot + ie];
+      cplx b = c->zCdR[(size_t)f * c->netot + ie];
+      double v = a.re * b.im - a.im * b.re; /* Im(conj(a) b) */
+      tot += v;
+      el[c->nelebas[c->fl_can[f]] - 1] += v;
+    }
+    if (S2prodtot) S2prodtot[ie] = tot * c->reacfct[ie] / c->ah1;
+    if (S2prodelec)
+      for (int e = 0; e < c->nelec; e++)
+        S2prodelec[(size_t)e * c->netot + ie] = el[e] * c->reacfct[ie] / c->ah1;
+  }
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_zs2_get(oref_ctx *c, double *zS2) {
+  memcpy(zS2, c->zS2, sizeof(cplx) * (size_t)c->ncan * c->nj * c->nv * c->netot);
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_flux_accum_get(oref_ctx *c, double *zCR, double *zCdR) {
+  if (zCR) memcpy(zCR, c->zCR, sizeof(cplx) * (size_t)c->nflux * c->netot);
+  if (zCdR) memcpy(zCdR, c->zCdR, sizeof(cplx) * (size_t)c->nflux * c->netot);
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_psie_get(oref_ctx *c, double *zfe) {
+  memcpy(zfe, c->zfe, sizeof(cplx) * (size_t)c->nofe * c->ntot);
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_state_get(oref_ctx *c, double *psim1, double *psi,
+                              int *k) {
+  if (psim1) memcpy(psim1, c->psim1, sizeof(double) * c->ntot);
+  if (psi) memcpy(psi, c->psi, sizeof(double) * c->ntot);
+  if (k) *k = c->k;
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_state_put(oref_ctx *c, const double *psim1,
+                              const double *psi, int k, const double *psi0) {
+  memcpy(c->psim1, psim1, sizeof(double) * c->ntot);
+  memcpy(c->psi, psi, sizeof(double) * c->ntot);
+  if (psi0) memcpy(c->psi0, psi0, sizeof(double) * c->ntot);
+  c->k = k;
+  c->begun = 1;
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}

@@ -1,0 +1,167 @@
+"""ctypes wrapper of oracle/libmw3_oracle.so (cpu_ref.c) -- test infrastructure.
+
+Same set-up struct as libmw3 (include/mw3.h), so the oracle and the GPU
+library read identical inputs.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+from madwave3_amd.lib import Mw3Setup, make_setup
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libmw3_oracle.so")
+_pd = ct.POINTER(ct.c_double)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise OSError("oracle not built: make -C oracle")
+        L = ct.CDLL(LIB)
+        L.oref_create.argtypes = [ct.POINTER(Mw3Setup), ct.POINTER(ct.c_void_p)]
+        L.oref_last_error.restype = ct.c_char_p
+        L.oref_last_error.argtypes = [ct.c_void_p]
+        L.oref_destroy.argtypes = [ct.c_void_p]
+        for f in ("oref_ntot", "oref_nflux"):
+            getattr(L, f).restype = ct.c_int64
+            getattr(L, f).argtypes = [ct.c_void_p]
+        L.oref_hphi.argtypes = [ct.c_void_p, _pd, _pd]
+        L.oref_cheby_begin.argtypes = [ct.c_void_p, _pd, _pd]
+        L.oref_cheby_steps.argtypes = [ct.c_void_p, ct.c_int, _pd, _pd, _pd]
+        L.oref_flux_result.argtypes = [ct.c_void_p, _pd, _pd]
+        L.oref_zs2_get.argtypes = [ct.c_void_p, _pd]
+        L.oref_flux_accum_get.argtypes = [ct.c_void_p, _pd, _pd]
+        L.oref_psie_get.argtypes = [ct.c_void_p, _pd]
+        L.oref_state_get.argtypes = [ct.c_void_p, _pd, _pd, ct.POINTER(ct.c_int)]
+        L.oref_state_put.argtypes = [ct.c_void_p, _pd, _pd, ct.c_int, _pd]
+        L.oref_nopr1.argtypes = [ct.c_void_p, ct.c_int]
+        L.oref_ir1flux.argtypes = [ct.c_void_p, ct.c_int]
+        L.oref_ir4flux.argtypes = [ct.c_void_p]
+        for f in ("oref_xj2", "oref_xl2p", "oref_p2r1", "oref_p2r2", "oref_zar1flux"):
+            getattr(L, f).restype = _pd
+            getattr(L, f).argtypes = [ct.c_void_p]
+        L.oref_noptfft.argtypes = [ct.c_int, ct.c_int]
+        L.oref_sinmom.argtypes = [ct.c_double, ct.c_int, ct.c_double, ct.c_double, _pd, _pd]
+        L.oref_fftmom.argtypes = [ct.c_double, ct.c_int, ct.c_double, ct.c_double, _pd, _pd]
+        L.oref_l2mat.argtypes = [_pd, ct.c_double, ct.c_int, ct.c_int, ct.c_int, ct.c_int]
+        L.oref_sym_eig_minmax.argtypes = [_pd, ct.c_int, _pd, _pd]
+        L.oref_dst1.argtypes = [ct.c_int, _pd, _pd]
+        L.oref_coef_echeby.argtypes = [ct.c_void_p, ct.c_int, _pd, ct.c_int, _pd]
+        L.oref_set_threads.argtypes = [ct.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_pd)
+
+
+class Oracle:
+    def __init__(self, case):
+        L = lib()
+        self.L = L
+        self.case = case
+        self._s, self._keep = make_setup(case)
+        h = ct.c_void_p()
+        rc = L.oref_create(ct.byref(self._s), ct.byref(h))
+        if rc != 0:
+            raise RuntimeError("oref_create: " + L.oref_last_error(None).decode())
+        self.h = h
+        self.ntot = int(L.oref_ntot(h))
+        self.nflux = int(L.oref_nflux(h))
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.oref_destroy(self.h)
+        except Exception:
+            pass
+
+    def hphi(self, psi):
+        psi = np.ascontiguousarray(psi, dtype=float)
+        out = np.zeros(self.ntot)
+        assert self.L.oref_hphi(self.h, _dp(psi), _dp(out)) == 0
+        return out
+
+    def begin(self, psi0):
+        psi0 = np.ascontiguousarray(psi0, dtype=float)
+        st = np.zeros(3)
+        assert self.L.oref_cheby_begin(self.h, _dp(psi0), _dp(st)) == 0
+        return st
+
+    def steps(self, n, want_cvj=False):
+        c = self.case
+        norms = np.zeros(n)
+        au = np.zeros(n)
+        cvj = None
+        if want_cvj:
+            cvj = np.zeros((n, c.ncan, c.jmax - c.jini + 1, c.nvmax - c.nvini + 1))
+        assert self.L.oref_cheby_steps(self.h, n, _dp(norms), _dp(au), _dp(cvj)) == 0
+        return norms, au, cvj
+
+    def flux_result(self):
+        c = self.case
+        tot = np.zeros(c.netot)
+        el = np.zeros((c.nelecmax, c.netot))
+        self.L.oref_flux_result(self.h, _dp(tot), _dp(el))
+        return tot, el
+
+    def zs2(self):
+        c = self.case
+        z = np.zeros((c.ncan, c.jmax - c.jini + 1, c.nvmax - c.nvini + 1, c.netot, 2))
+        self.L.oref_zs2_get(self.h, _dp(z))
+        return z[..., 0] + 1j * z[..., 1]
+
+    def flux_accum(self):
+        a = np.zeros((self.nflux, self.case.netot, 2))
+        b = np.zeros((self.nflux, self.case.netot, 2))
+        self.L.oref_flux_accum_get(self.h, _dp(a), _dp(b))
+        return a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1]
+
+    def psie(self):
+        z = np.zeros((self.case.nofe, self.ntot, 2))
+        self.L.oref_psie_get(self.h, _dp(z))
+        return z[..., 0] + 1j * z[..., 1]
+
+    def state(self):
+        a = np.zeros(self.ntot)
+        b = np.zeros(self.ntot)
+        k = ct.c_int()
+        self.L.oref_state_get(self.h, _dp(a), _dp(b), ct.byref(k))
+        return a, b, k.value
+
+    def put_state(self, psim1, psi, k, psi0=None):
+        self.L.oref_state_put(self.h, _dp(np.ascontiguousarray(psim1, dtype=float)),
+                              _dp(np.ascontiguousarray(psi, dtype=float)), k,
+                              _dp(None if psi0 is None else np.ascontiguousarray(psi0, dtype=float)))
+
+    def tables(self):
+        c = self.case
+        na, n1, n2, nc = c.nangu, c.npun1, c.npun2, c.ncan
+        xj2 = np.ctypeslib.as_array(self.L.oref_xj2(self.h), shape=(nc * na * na * n1,))
+        xl2p = np.ctypeslib.as_array(self.L.oref_xl2p(self.h), shape=(3 * nc * na * na * n2,))
+        p2r1 = np.ctypeslib.as_array(self.L.oref_p2r1(self.h), shape=(na * n1,))
+        p2r2 = np.ctypeslib.as_array(self.L.oref_p2r2(self.h), shape=(n2,))
+        za = np.ctypeslib.as_array(self.L.oref_zar1flux(self.h), shape=(n1 * 2,))
+        return dict(
+            xj2=xj2.reshape(nc, na, na, n1).copy(),
+            xl2p=xl2p.reshape(3, nc, na, na, n2).copy(),
+            p2r1=p2r1.reshape(na, n1).copy(), p2r2=p2r2.copy(),
+            zAR1flux=(za[0::2] + 1j * za[1::2]).copy(),
+            nopr1=np.array([self.L.oref_nopr1(self.h, ia) for ia in range(1, na + 1)]),
+            ir1flux=np.array([self.L.oref_ir1flux(self.h, ia) for ia in range(1, na + 1)]),
+            ir4flux=int(self.L.oref_ir4flux(self.h)))
+
+    def coef_echeby(self, k, E):
+        E = np.ascontiguousarray(E, dtype=float)
+        out = np.zeros((len(E), 2))
+        self.L.oref_coef_echeby(self.h, k, _dp(E), len(E), _dp(out))
+        return out[:, 0] + 1j * out[:, 1]
+
+
+def set_threads(n):
+    return lib().oref_set_threads(n)
