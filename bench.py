@@ -1,0 +1,181 @@
+"""Chebyshev H.psi propagation benchmark on one MI355X.
+
+Workload: the J = 0 collision of configuration 2 -- nr = 256 r points on
+[0.1, 8] bohr, nR = 512 R points on [0.1, 15] bohr, 120 Gauss-Legendre gamma
+points (15.7M grid values), LEPS surface, v=0 j=0 packet at R = 9 bohr --
+built in memory by the synthetic input generator.  A "step" is one full
+Chebyshev iteration (H psi, damped update, per-step projections; the energy
+sums run once per 64 steps and are included).
+
+Prints one JSON line: value = steps/s of the mixed-precision propagator (FP32
+operator terms, FP64 Chebyshev state and accumulators); the FP64 propagator's
+steps/s and max |dP(E)| between the two over the timed run are reported
+beside it, with the roofline of the dominant kernel group and the CPU oracle
+timed on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TF = 78.6
+FP32_PEAK_TF = 157.3
+
+
+def build_case(cfg):
+    from madwave3_amd import leps, model
+    from madwave3_amd import namelist as NL
+
+    p = leps.config(cfg)
+    text, pot, fdat = leps.make_inputs(p)
+    return model.build_case(NL.parse(text), pot, fdat)
+
+
+def kernel_groups(times):
+    g = {"radial_dst": 0.0, "angular": 0.0, "update_fused": 0.0, "projections": 0.0}
+    n = {}
+    for name, (ms, cnt) in times.items():
+        if name.startswith("dst"):
+            g["radial_dst"] += ms
+            n["radial_dst"] = max(n.get("radial_dst", 0), cnt)
+        elif name.startswith("angular"):
+            g["angular"] += ms
+            n["angular"] = max(n.get("angular", 0), cnt)
+        elif name == "update_fused":
+            g["update_fused"] += ms
+            n["update_fused"] = cnt
+        else:
+            g["projections"] += ms
+            n["projections"] = max(n.get("projections", 0), cnt)
+    return g, n
+
+
+def run_gpu(case, precision, steps, warmup, timing):
+    from madwave3_amd.lib import Propagator
+
+    g = Propagator(case, precision)
+    g.begin(case.rpaq0)
+    if warmup:
+        g.steps(warmup)
+    if timing:
+        g.timing(True)
+    t0 = time.perf_counter()
+    g.steps(steps)
+    dt = time.perf_counter() - t0
+    times = g.kernel_times() if timing else {}
+    P = g.flux_result()[0]
+    g.close()
+    return dt, times, P
+
+
+def cpu_baseline(case, nsteps):
+    from oracle.oracle import Oracle, set_threads
+
+    nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    nthr = set_threads(nthr)
+    o = Oracle(case)
+    o.begin(case.rpaq0)
+    t0 = time.perf_counter()
+    o.steps(nsteps)
+    dt = time.perf_counter() - t0
+    return {"value": nsteps / dt, "unit": "steps/s", "cores": nthr, "kind": "port",
+            "sample": "%d FP64 Chebyshev steps of the same workload by the C/OpenMP "
+                      "oracle (dense sine-matrix DST-I, DVR angular sums)" % nsteps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU; this build has no multi-GPU path (--gpus %d)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    from madwave3_amd.lib import MW3_FP64, MW3_MIXED
+
+    t0 = time.perf_counter()
+    case = build_case(a.config)
+    print("# case %s built in %.1fs" % (case, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    N = case.ncan * case.nangu * case.npun1 * case.npun2
+    dt64, tim64, P64 = run_gpu(case, MW3_FP64, a.steps, a.warmup, not a.no_timing)
+    print("# fp64: %.3f ms/step" % (1e3 * dt64 / a.steps), file=sys.stderr, flush=True)
+    dtmx, timmx, Pmx = run_gpu(case, MW3_MIXED, a.steps, a.warmup, not a.no_timing)
+    print("# mixed: %.3f ms/step" % (1e3 * dtmx / a.steps), file=sys.stderr, flush=True)
+    dP = float(np.max(np.abs(Pmx - P64)))
+
+    # roofline of the dominant kernel group of the headline (mixed) run
+    groups, counts = kernel_groups(timmx)
+    roof = None
+    kern = {}
+    if groups and sum(groups.values()) > 0:
+        s_op = 4  # FP32 operator I/O
+        work = {
+            # (a) radial kinetic: 2 s N bytes per step
+            "radial_dst": ("hbm", 2.0 * s_op * N, "GB/s"),
+            # (b) angular: 4 n_gamma N flops per step
+            "angular": ("mfma", 4.0 * case.nangu * N, "TFLOP/s"),
+            # (c) fused V + absorption + update: N (8 + 8 + s + s_V + 8) bytes
+            "update_fused": ("hbm", N * (8 + 8 + s_op + 4 + 8.0), "GB/s"),
+            "projections": ("hbm", 16.0 * max(case.npun2 * case.nangu, 1), "GB/s"),
+        }
+        for k, ms in groups.items():
+            cnt = counts.get(k, 0)
+            if cnt == 0:
+                continue
+            per = ms / cnt / 1e3  # s per step
+            bound, w, unit = work[k]
+            ach = w / per / (1e9 if unit == "GB/s" else 1e12)
+            peak = HBM_PEAK_GBS if unit == "GB/s" else FP32_PEAK_TF
+            kern[k] = {"ms_per_step": per * 1e3, "achieved": ach, "unit": unit, "peak": peak,
+                       "frac": ach / peak}
+        dom = max(groups, key=lambda k: groups[k])
+        d = kern[dom]
+        roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],
+                "unit": d["unit"], "frac": d["frac"], "traffic": None, "kernel": dom}
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(case, a.cpu_steps)
+    steps_s = a.steps / dtmx
+    line = {
+        "metric": "Chebyshev H.psi steps/s (mixed vs FP64 GPU) at <=1e-4 P(E) err; HBM GB/s",
+        "value": steps_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * dtmx / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 operator terms, f64 Chebyshev state/update/accumulators",
+        "data": "synthetic (LEPS surface, generated in memory)",
+        "config": {"workload": "configuration 2: J=0 collision, 256 r x 512 R x 120 gamma "
+                               "(15.7M grid values), one Omega block",
+                   "grid_values": N, "active_fraction": float(case.ntot) / N,
+                   "parallelism": "single GPU"},
+        "fp64_steps_per_s": a.steps / dt64,
+        "fp64_ms_per_step": 1e3 * dt64 / a.steps,
+        "mixed_vs_fp64_max_abs_dP": dP,
+        "dP_steps": a.warmup + a.steps,
+        "roofline": roof,
+        "kernels": kern,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

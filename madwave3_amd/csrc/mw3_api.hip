@@ -356,6 +356,7 @@ static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
   }
   c->ring_k0 += c->ring_fill;
   c->ring_fill = 0;
+  if (c->timing) timer_collect(c);
   return MW3_OK;
 }
 
@@ -381,7 +382,6 @@ int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* aut
       int rc = flush_ring(c, norms ? norms + base : nullptr, autocorr ? autocorr + base : nullptr);
       if (rc) return rc;
     }
-    if (c->timing) timer_collect(c);
   }
   return MW3_OK;
 }
