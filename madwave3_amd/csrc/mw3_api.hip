@@ -108,7 +108,7 @@ static void free_all(Ctx* c) {
                   c->d_ring_cvj, c->d_zcr, c->d_zcdr, c->d_zs2, c->d_etot, c->d_fdv, c->d_dcvj,
                   c->d_cvj_skip, c->d_norm_part, c->d_ring_norm, c->d_efe, c->d_zfe, c->d_phi,
                   c->d_wmat, c->d_zero_skip, c->d_ch_elec, c->d_ch_omega, c->d_potiflux,
-                  c->d_flux_elec, c->d_scratch};
+                  c->d_flux_elec, c->d_scratch, c->d_fmap, c->d_ir1flux, c->d_slice};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   free_plans(c);
@@ -238,7 +238,23 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(upload_raw(&c->d_fdv, t.fdv.data(), t.fdv.size()), "mw3_create: fd");
   CK(upload_raw(&c->d_dcvj, t.dcvj.data(), t.dcvj.size()), "mw3_create: D");
   CK(upload_raw(&c->d_cvj_skip, t.cvj_isref.data(), t.cvj_isref.size()), "mw3_create: flags");
-  c->nnorm_blocks = 1024;
+  c->nnorm_blocks = (int)std::min<int64_t>(4096, (int64_t)t.nc * t.na * t.n1);
+  {
+    std::vector<int32_t> fmap((size_t)t.nc * t.na * t.n2, -1);
+    for (int64_t f = 0; f < t.nflux; f++) {
+      // flux_col is the dense index of (c, ia, 0, R): c*slab + ia*plane + R
+      uint32_t col = t.flux_col[f];
+      int64_t cia = col / t.nplane;
+      int64_t R = col % t.n2;
+      fmap[(size_t)cia * t.n2 + R] = (int32_t)f;
+    }
+    CK(upload_raw(&c->d_fmap, fmap.data(), fmap.size()), "mw3_create: flux map");
+    std::vector<int32_t> i1f(t.ir1flux.begin(), t.ir1flux.end());
+    if (i1f.empty()) i1f.assign(t.na, 1);
+    CK(upload_raw(&c->d_ir1flux, i1f.data(), i1f.size()), "mw3_create: flux map");
+    CK(alloc_zero((void**)&c->d_slice, sizeof(double) * (size_t)t.nc * t.n1 * t.na),
+       "mw3_create: slice");
+  }
   CK(alloc_zero((void**)&c->d_norm_part, sizeof(double) * 2 * c->nnorm_blocks), "mw3_create: norms");
   CK(alloc_zero((void**)&c->d_ring_norm, sizeof(double) * 2 * kRing), "mw3_create: norms");
   if (setup->nofe > 0) {

@@ -153,6 +153,10 @@ struct Ctx {
   double* d_potiflux = nullptr;
   int32_t* d_flux_elec = nullptr;
 
+  int32_t* d_fmap = nullptr;     // [nc][na][n2] flux point index or -1
+  int32_t* d_ir1flux = nullptr;  // [na]
+  double* d_slice = nullptr;     // [nc][n1][na] psi at R = ir2abs
+
   // recursion state
   int k = 0;
   bool begun = false;

@@ -192,49 +192,57 @@ __global__ __launch_bounds__(256) void k_update(
     const double* __restrict__ psi0, double* __restrict__ part, int nc, int ne, int na, int n1,
     int n2, double emindlt, double delta2, int mode) {
   const int64_t plane = (int64_t)n1 * n2, slab = (int64_t)na * plane;
-  const int64_t total = (int64_t)nc * slab;
+  const int nrows = nc * na * n1;
   double s_norm = 0.0, s_auto = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t pl = i % slab;
-    int c = (int)(i / slab);
-    if (!mask[pl]) {
-      if (mode == 0)
-        hout[i] = 0.0;
-      else
-        psim1[i] = 0.0;
-      continue;
+  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
+    const int r = row % n1;
+    const int cia = row / n1;
+    const int ia = cia % na, c = cia / na;
+    const bool tk = r < nn1[ia];
+    const int et = ch_elec[c], om = ch_omega[c];
+    const double f1 = absfr1[r];
+    const int64_t rowoff = (int64_t)row * n2;           // (c, ia, r, 0)
+    const int64_t ploff = (int64_t)ia * plane + (int64_t)r * n2;
+    for (int R = threadIdx.x; R < n2; R += blockDim.x) {
+      const int64_t i = rowoff + R;
+      const int64_t pl = ploff + R;
+      if (!mask[pl]) {
+        if (mode == 0)
+          hout[i] = 0.0;
+        else
+          psim1[i] = 0.0;
+        continue;
+      }
+      double p = psi[i];
+      double h = tk ? (double)tkin[i] : p;
+      if (ne == 1 && nc == 1) {
+        h += p * (double)vd[pl];
+      } else {
+        for (int s = 0; s < nc; s++) {
+          if (ch_omega[s] != om) continue;
+          int es = ch_elec[s];
+          double v = (double)vd[((int64_t)et * ne + es) * slab + pl];
+          double ps = (s == c) ? p : psi[(int64_t)s * slab + pl];
+          h += ps * v;
+        }
+      }
+      h += (double)ang[i];
+      if (mode == 0) {
+        hout[i] = h;
+        continue;
+      }
+      double out;
+      if (mode == 1) {
+        out = (h - emindlt * p) / delta2;
+      } else {
+        double f = f1 * absfr2[R];
+        double hn = (h - emindlt * p) / delta2;
+        out = (2.0 * hn - f * psim1[i]) * f;
+      }
+      psim1[i] = out;
+      s_norm += out * out;
+      if (psi0) s_auto += psi0[i] * out;
     }
-    int ia = (int)(pl / plane);
-    int64_t rr = pl % plane;
-    int r = (int)(rr / n2), R = (int)(rr % n2);
-    double p = psi[i];
-    double h = (r < nn1[ia]) ? (double)tkin[i] : p;
-    // V: sum over source channels of the same Omega
-    int et = ch_elec[c], om = ch_omega[c];
-    for (int s = 0; s < nc; s++) {
-      if (ch_omega[s] != om) continue;
-      int es = ch_elec[s];
-      double v = (double)vd[((int64_t)et * ne + es) * slab + pl];
-      double ps = (s == c) ? p : psi[(int64_t)s * slab + pl];
-      h += ps * v;
-    }
-    h += (double)ang[i];
-    if (mode == 0) {
-      hout[i] = h;
-      continue;
-    }
-    double out;
-    if (mode == 1) {
-      out = (h - emindlt * p) / delta2;
-    } else {
-      double f = absfr1[r] * absfr2[R];
-      double hn = (h - emindlt * p) / delta2;
-      out = (2.0 * hn - f * psim1[i]) * f;
-    }
-    psim1[i] = out;
-    s_norm += out * out;
-    if (psi0) s_auto += psi0[i] * out;
   }
   // block reduction
   __shared__ double sh[2][256];
@@ -280,36 +288,65 @@ __global__ void k_reduce_parts(const double* __restrict__ part, int nb, double* 
 // ---------------------------------------------------------------------------
 // per-step projections into the ring
 // ---------------------------------------------------------------------------
-__global__ void k_flux_extract(const double* __restrict__ psi, const uint32_t* __restrict__ fdense,
-                               const uint32_t* __restrict__ fcol, const double* __restrict__ zar,
-                               double* __restrict__ zr, double* __restrict__ zdr, int64_t nflux,
-                               int n1, int n2) {
-  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nflux) return;
-  zr[f] = psi[fdense[f]];
-  const double* col = psi + fcol[f];
+// flux line: for every (c, ia, R) column, zdR = sum_r zAR1flux(r) psi(c,ia,r,R)
+// (the full-state derivative of totalflux_k, SRC/mod_flux_01y2.f:598-609);
+// flux points f = fmap(c,ia,R) >= 0 store zR = psi at r = ir1flux(ia) and zdR.
+// Block = (64 R values, c*na + ia), 4 lanes over r.
+__global__ __launch_bounds__(256) void k_flux_cols(const double* __restrict__ psi,
+                                                   const int32_t* __restrict__ fmap,
+                                                   const int32_t* __restrict__ ir1flux,
+                                                   const double* __restrict__ zar,
+                                                   double* __restrict__ zr, double* __restrict__ zdr,
+                                                   int na, int n1, int n2) {
+  const int cia = blockIdx.y;
+  const int ia = cia % na;
+  const int lane = threadIdx.x >> 6;
+  const int R = blockIdx.x * 64 + (threadIdx.x & 63);
+  __shared__ double sre[4][64], sim[4][64];
   double re = 0, im = 0;
-  for (int r = 0; r < n1; r++) {
-    double v = col[(int64_t)r * n2];
-    re += zar[2 * r] * v;
-    im += zar[2 * r + 1] * v;
+  const double* col = psi + (int64_t)cia * n1 * n2 + R;
+  if (R < n2)
+    for (int r = lane; r < n1; r += 4) {
+      double v = col[(int64_t)r * n2];
+      re += zar[2 * r] * v;
+      im += zar[2 * r + 1] * v;
+    }
+  sre[lane][threadIdx.x & 63] = re;
+  sim[lane][threadIdx.x & 63] = im;
+  __syncthreads();
+  if (lane == 0 && R < n2) {
+    int f = fmap[(int64_t)cia * n2 + R];
+    if (f >= 0) {
+      int t = threadIdx.x & 63;
+      zdr[2 * (int64_t)f] = (sre[0][t] + sre[1][t]) + (sre[2][t] + sre[3][t]);
+      zdr[2 * (int64_t)f + 1] = (sim[0][t] + sim[1][t]) + (sim[2][t] + sim[3][t]);
+      zr[f] = col[(int64_t)(ir1flux[ia] - 1) * n2];
+    }
   }
-  zdr[2 * f] = re;
-  zdr[2 * f + 1] = im;
 }
 
-// Cvj: one block per (c, jj); xj(r) = sum_ia psi(c,ia,r,ir2abs) D(ia) / sqrt(ah2)
-__global__ void k_cvj(const double* __restrict__ psi, const double* __restrict__ dcvj,
-                      const double* __restrict__ fdv, double* __restrict__ cvj, int nc, int nj,
-                      int nv, int na, int n1, int n2, int ir2abs, double rsah2) {
+// reactant-state projection (Cvjflux_k, SRC/mod_flux_01y2.f:672-702)
+// slice[c][r][ia] = psi(c, ia, r, ir2abs) / sqrt(ah2)
+__global__ void k_cvj_slice(const double* __restrict__ psi, double* __restrict__ slice, int nc,
+                            int na, int n1, int n2, int ir2abs, double rsah2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc * na * n1) return;
+  int ia = i % na, r = (i / na) % n1, c = i / (na * n1);
+  slice[i] = psi[(((int64_t)c * na + ia) * n1 + r) * n2 + (ir2abs - 1)] * rsah2;
+}
+
+// Cvj(c, j, v) = sum_r fd(r, v, j) sum_ia slice(c, r, ia) D(ia, j); block per (c, jj)
+__global__ __launch_bounds__(256) void k_cvj(const double* __restrict__ slice,
+                                             const double* __restrict__ dcvj,
+                                             const double* __restrict__ fdv, double* __restrict__ cvj,
+                                             int nj, int nv, int na, int n1) {
   int c = blockIdx.x / nj, jj = blockIdx.x % nj;
   extern __shared__ double xj[];
-  const int64_t plane = (int64_t)n1 * n2, slab = (int64_t)na * plane;
   const double* dj = dcvj + ((int64_t)c * nj + jj) * na;
   for (int r = threadIdx.x; r < n1; r += blockDim.x) {
+    const double* sl = slice + ((int64_t)c * n1 + r) * na;
     double s = 0;
-    for (int ia = 0; ia < na; ia++)
-      s += psi[(int64_t)c * slab + ia * plane + (int64_t)r * n2 + (ir2abs - 1)] * rsah2 * dj[ia];
+    for (int ia = 0; ia < na; ia++) s += sl[ia] * dj[ia];
     xj[r] = s;
   }
   __syncthreads();
@@ -436,6 +473,7 @@ static hipError_t upload_plan(GemmPlan* p, const std::vector<GemmDesc>& d) {
 
 hipError_t build_plans(Ctx* c) {
   const HostTables& t = c->t;
+
   const int64_t plane = t.nplane, slab = t.nslab;
   hipError_t e;
   // R rows: every (c, ia, r) row of n2 values times the n2 sine table
@@ -519,6 +557,7 @@ hipError_t build_plans(Ctx* c) {
 }
 
 void free_plans(Ctx* c) {
+
   GemmPlan* ps[4] = {&c->p_dstR, &c->p_dstr, &c->p_angF, &c->p_angB};
   for (auto* p : ps)
     if (p->d_descs) {
@@ -610,17 +649,21 @@ hipError_t launch_projections(Ctx* c, int slot) {
   timer_begin(c, K_PROJ);
   const double* psi = c->d_psim1;
   if (t.nflux > 0) {
-    hipLaunchKernelGGL(k_flux_extract, dim3(nblk(t.nflux, 128)), dim3(128), 0, c->stream, psi,
-                       c->d_flux_dense, c->d_flux_col, c->d_zar1flux,
-                       c->d_ring_zr + (int64_t)slot * t.nflux,
-                       c->d_ring_zdr + (int64_t)2 * slot * t.nflux, t.nflux, t.n1, t.n2);
+    dim3 g((unsigned)((t.n2 + 63) / 64), (unsigned)(t.nc * t.na));
+    hipLaunchKernelGGL(k_flux_cols, g, dim3(256), 0, c->stream, psi, c->d_fmap, c->d_ir1flux,
+                       c->d_zar1flux, c->d_ring_zr + (int64_t)slot * t.nflux,
+                       c->d_ring_zdr + (int64_t)2 * slot * t.nflux, t.na, t.n1, t.n2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   if (t.ncvj > 0) {
     int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
-    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(256), sizeof(double) * t.n1, c->stream, psi,
-                       c->d_dcvj, c->d_fdv, c->d_ring_cvj + (int64_t)slot * t.ncvj, t.nc, nj, nv,
-                       t.na, t.n1, t.n2, c->s.ir2abs, 1.0 / std::sqrt(c->s.ah2));
+    int ns = t.nc * t.na * t.n1;
+    hipLaunchKernelGGL(k_cvj_slice, dim3(nblk(ns, 256)), dim3(256), 0, c->stream, psi,
+                       c->d_slice, t.nc, t.na, t.n1, t.n2, c->s.ir2abs, 1.0 / std::sqrt(c->s.ah2));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(256), sizeof(double) * t.n1, c->stream,
+                       c->d_slice, c->d_dcvj, c->d_fdv, c->d_ring_cvj + (int64_t)slot * t.ncvj,
+                       nj, nv, t.na, t.n1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   timer_end(c, K_PROJ);
