@@ -311,8 +311,9 @@ def point_index(c):
     return sl, ch, ir, c.indr1[ia, ir], c.indr2[ia, ir]
 
 
-def _colini(c, nl):
-    """set_colini + colini_reacJacobi (SRC/mod_colini_01y2.f:25-177)."""
+def colini_params(c, nl):
+    """Packet parameters of set_colini (SRC/mod_colini_01y2.f:36-84): k0,
+    alpha, normalisation factor, closest l, and the norm of the R packet."""
     g = NL.group(nl, "inputcol")
     c.Rcolini = float(g["rcolini_ang"])
     c.ecol = C.ev_to_zot(float(g["ecol_ev"]))
@@ -326,6 +327,12 @@ def _colini(c, nl):
     r2 = c.rmis2 + np.arange(c.npun2) * c.ah2
     xr2 = rgauscolini(r2, c.Rcolini, c.alpha0, c.xk0, c.factor0)
     c.norm_gauss_R2 = float(np.sum(xr2 * xr2) * c.ah2)
+    return xr2
+
+
+def _colini(c, nl):
+    """set_colini + colini_reacJacobi (SRC/mod_colini_01y2.f:25-177)."""
+    xr2 = colini_params(c, nl)
     if c.iprod == 1:
         raise InputError("iprod=1 (product Jacobi propagation) is not on this path")
     sl, ch, ir, ir1, ir2 = point_index(c)
@@ -349,8 +356,8 @@ def _colini(c, nl):
     c.norm_initial = float(np.sum(psi * psi))
 
 
-def _flux(c, nl):
-    """ini_flux inputs and Smatini (SRC/mod_flux_01y2.f:54-334, 338-581)."""
+def flux_params(c, nl):
+    """inputflux / inputtime values and the energy grid of Smatini."""
     g = NL.group(nl, "inputflux", {"ncontfile": 0})
     c.r1flux = float(g.get("r1flux_ang", c.absr1 - 1.0))
     c.netot = int(g["netot"])
@@ -360,6 +367,40 @@ def _flux(c, nl):
     tm = NL.group(nl, "inputtime", {"kminelastic": 0})
     c.ntimes, c.nloop = int(tm["ntimes"]), int(tm["nloop"])
     c.kminelastic = int(tm["kminelastic"])
+    estep = (c.ekinmax - c.ekinmin) / float(c.netot - 1)
+    c.etotS2 = c.ekinmin + np.arange(c.netot) * estep
+    c.estep = estep
+
+
+def smatini_norm(c):
+    """Energy distribution of the incident packet in Smatini_col
+    (SRC/mod_flux_01y2.f:427-503): per energy |<packet|l-wave>|^2 and its
+    integral ("Norm ini. wvp in energy"); returns (paqini, pini, norm)."""
+    npunt = 1000
+    ahg = (c.rfin2 - c.rmis2) / float(npunt - 1)
+    rg = c.rmis2 + np.arange(npunt) * ahg
+    rgaussr = rgauscolini(rg, c.Rcolini, c.alpha0, c.xk0, c.factor0) / math.sqrt(2.0)
+    pepe = float(c.il0 * (c.il0 + 1))
+    r = c.Rcolini + 1.0
+    erot = C.hbr * C.hbr * pepe * 0.5 / (r * r * c.xmasa0)
+    paq = np.zeros(c.netot)
+    pin = np.zeros(c.netot)
+    for ie in range(c.netot):
+        ekinini = c.etotS2[ie]
+        if ekinini > 0.0 and ekinini > erot:
+            pini = math.sqrt(ekinini * c.xmasa0 * 2.0 / C.hbr / C.hbr)
+            F, _, G, _ = besph2(c.il0, rg * pini)
+            keep = np.abs(F) > 1e-20
+            zr = np.sum(np.where(keep, rgaussr * (-G), 0.0)) * ahg / 2.0 / C.pi
+            zi = np.sum(np.where(keep, rgaussr * F, 0.0)) * ahg / 2.0 / C.pi
+            paq[ie] = zr * zr + zi * zi
+            pin[ie] = pini
+    return paq, pin, float(np.sum(paq) * c.estep)
+
+
+def _flux(c, nl):
+    """ini_flux inputs and Smatini (SRC/mod_flux_01y2.f:54-334, 338-581)."""
+    flux_params(c, nl)
     # reference channel (SRC/mod_flux_01y2.f:348-370)
     c.icanref = 0
     if c.No_ref_energy == 1:
@@ -368,9 +409,6 @@ def _flux(c, nl):
                 c.icanref = ic + 1
         if c.icanref == 0:
             raise InputError("icanref can not be set")
-    etotmin, etotmax = c.ekinmin, c.ekinmax
-    estep = (etotmax - etotmin) / float(c.netot - 1)
-    c.etotS2 = etotmin + np.arange(c.netot) * estep
     nj = c.jmax - c.jini + 1
     nv = c.nvmax - c.nvini + 1
     c.S2factor = np.zeros((c.nelecmax, nj, nv, c.netot))
@@ -378,28 +416,10 @@ def _flux(c, nl):
     c.apaqini = np.zeros(c.netot)
     if c.iphoto == 0:
         c.photonorm = 1.0
-        npunt = 1000
-        ahg = (c.rfin2 - c.rmis2) / float(npunt - 1)
-        rg = c.rmis2 + np.arange(npunt) * ahg
-        rgaussr = rgauscolini(rg, c.Rcolini, c.alpha0, c.xk0, c.factor0) / math.sqrt(2.0)
-        pepe = float(c.il0 * (c.il0 + 1))
-        r = c.Rcolini + 1.0
-        erot = C.hbr * C.hbr * pepe * 0.5 / (r * r * c.xmasa0)
-        xnorm = 0.0
+        paq, pin, c.norm_energy = smatini_norm(c)
+        c.apaqini = paq
         for ie in range(c.netot):
-            ekinini = c.etotS2[ie]
-            if ekinini > 0.0 and ekinini > erot:
-                pini = math.sqrt(ekinini * c.xmasa0 * 2.0 / C.hbr / C.hbr)
-                F, _, G, _ = besph2(c.il0, rg * pini)
-                keep = np.abs(F) > 1e-20
-                zr = np.sum(np.where(keep, rgaussr * (-G), 0.0))
-                zi = np.sum(np.where(keep, rgaussr * F, 0.0))
-                zr *= ahg / 2.0 / C.pi
-                zi *= ahg / 2.0 / C.pi
-                paqini = zr * zr + zi * zi
-            else:
-                pini, paqini = 0.0, 0.0
-            c.apaqini[ie] = paqini
+            paqini, pini = paq[ie], pin[ie]
             for iele in range(c.nelec):
                 for j in range(c.jini, c.jmax + 1):
                     for iv in range(c.nvini, c.nvmax + 1):
@@ -408,10 +428,8 @@ def _flux(c, nl):
                         if paqini >= 1e-10:
                             c.S2factor[iele, j - c.jini, iv - c.nvini, ie] = (
                                 C.hbr * C.hbr * pfin * pini / (2.0 * paqini) / c.xm2reac / c.xmasa0)
-            xnorm += paqini
             if paqini >= 1e-10:
                 c.reacfct[ie] = C.hbr * C.hbr * pini / paqini / c.xmasa0 / c.xm1reac / 4.0 / C.pi / C.pi
-        c.norm_energy = xnorm * estep
     else:
         c.photonorm = 1.0
         for ie in range(c.netot):

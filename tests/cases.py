@@ -38,3 +38,32 @@ def case(name):
         raise KeyError(name)
     p.update(kminelastic=5)
     return _build(p)
+
+
+def box_case(n1=16, n2=24, na=6, Jtot=0, iommax=0, damp=True, vzero=True):
+    """Every grid point kept (Vcut above the whole surface) and, with vzero,
+    V = 0: the operator is the bare kinetic + angular part."""
+    import numpy as np
+    from madwave3_amd.refio import PotData
+
+    p = leps.config(1, scale=1)
+    p.update(npun1=n1, npun2=n2, nangu=na, npun1min=4, jmax=min(3, na - 1), netot=20,
+             rfin1_bohr=6.0, rfin2_bohr=10.0, R0_bohr=6.0, width_bohr=0.8, Jtot=Jtot,
+             iommax=iommax, iomref=0, nvmax=1, kminelastic=3)
+    text, pot, fdat = leps.make_inputs(p)
+    i2, i1 = np.meshgrid(np.arange(1, n2 + 1), np.arange(1, n1 + 1), indexing="ij")
+    ii1 = i1.ravel().astype(np.int32)
+    ii2 = i2.ravel().astype(np.int32)
+    if vzero:
+        vm = np.zeros((len(ii1), 1, 1))
+    else:   # smooth well in r and R, inside [0, Vcut]
+        x1 = (ii1 - 1) / (n1 - 1.0) - 0.3
+        x2 = (ii2 - 1) / (n2 - 1.0) - 0.5
+        vm = (0.5 * pot.vmaxtot * (x1 * x1 + x2 * x2)).reshape(-1, 1, 1)
+    full = PotData(1, pot.elec, 0.0, pot.vmaxtot, pot.grid1, pot.grid2, na, 1,
+                   [ii1] * na, [ii2] * na, [vm] * na)
+    c = model.build_case(NL.parse(text), full, fdat)
+    if not damp:
+        c.absfr1[:] = 1.0
+        c.absfr2[:] = 1.0
+    return c

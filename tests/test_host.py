@@ -1,0 +1,141 @@
+"""Host logic and the C ABI boundary (CPU only; no device calls)."""
+import ctypes as ct
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from madwave3_amd import leps, model, refio
+from madwave3_amd import namelist as NL
+from madwave3_amd.lib import LIB_PATH, Mw3Setup, load_library
+from tests.cases import case
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mw3.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    return re.findall(r"MW3_API[^;]*?\b(mw3_\w+)\s*\(", txt)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = load_library()
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.mw3_abi_version() == 1
+    assert lib.mw3_kernel_name(0).decode() == "angular_fwd"
+    assert isinstance(lib.mw3_last_error(None), bytes)
+
+
+def test_setup_struct_layout_matches_header():
+    fields = [f[0] for f in Mw3Setup._fields_]
+    src = "#include <stddef.h>\n#include <stdio.h>\n#include \"%s\"\nint main(void){\n" % HEADER
+    for f in fields:
+        src += 'printf("%s %%zu\\n", offsetof(mw3_setup, %s));\n' % (f, f)
+    src += 'printf("sizeof %zu\\n", sizeof(mw3_setup));\nreturn 0;}\n'
+    with tempfile.TemporaryDirectory() as d:
+        cfile = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(cfile, "w").write(src)
+        subprocess.run(["gcc", "-o", exe, cfile], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(l.split() for l in out if l)
+    for f in fields:
+        assert int(got[f]) == getattr(Mw3Setup, f).offset, f
+    assert int(got["sizeof"]) == ct.sizeof(Mw3Setup)
+
+
+def test_create_rejects_bad_setup_without_device_work():
+    """Argument checks run before any device call; with no device in the
+    container mw3_create fails cleanly either way."""
+    lib = load_library()
+    h = ct.c_void_p()
+    rc = lib.mw3_create(None, 0, ct.byref(h))
+    assert rc < 0 and not h.value
+    assert lib.mw3_last_error(None)
+
+
+def test_namelist_parse_and_roundtrip():
+    txt = """
+&inputflux
+!----> data for total flux
+r1flux_ang=4.d0
+netot=500
+ekinmin_eV=0.1d0,ekinmax_eV=1.5d0      ! comment
+/
+&inputprod iprod=2, jiniprod=0, jmaxprod=20  iomminprod=0!  , iommaxprod=0
+ Rbalinprod=6.d0 /
+&inputpotmass system='H+DH' xm0=2.014101779d0 /
+"""
+    g = NL.parse(txt)
+    assert g["inputflux"] == {"r1flux_ang": 4.0, "netot": 500, "ekinmin_ev": 0.1, "ekinmax_ev": 1.5}
+    assert g["inputprod"]["iprod"] == 2 and g["inputprod"]["rbalinprod"] == 6.0
+    assert "iommaxprod" not in g["inputprod"]
+    assert g["inputpotmass"]["system"] == "H+DH"
+    back = NL.parse(NL.dump_group("inputflux", g["inputflux"]))
+    assert back["inputflux"] == g["inputflux"]
+    assert NL.group(g, "inputflux", {"ncontfile": 0})["ncontfile"] == 0
+
+
+def test_reference_formats_roundtrip(tmp_path):
+    p = leps.config(5, scale=12)
+    text, pot, fdat = leps.make_inputs(p)
+    model.write_case(str(tmp_path), text, pot, fdat)
+    c1 = model.build_case(NL.parse(text), pot, fdat)
+    c2 = model.load_case(str(tmp_path))
+    for a in ("VVV", "indr1", "indr2", "npunreal", "Djmmp", "fd", "rpaq0", "etotS2", "reacfct"):
+        x, y = getattr(c1, a), getattr(c2, a)
+        assert np.array_equal(x, y) or np.max(np.abs(x - y)) <= 1e-15 * max(np.max(np.abs(x)), 1), a
+
+
+def test_a0_layout_and_point_decode():
+    c = case("coriol")
+    sl, ch, ir, i1, i2 = model.point_index(c)
+    assert len(sl) == c.ntot
+    # indiproc order: slots outer, channels, points (SRC/mod_pot_01y2.f:435-455)
+    assert np.all(np.diff(sl) >= 0)
+    assert c.indangreal[0] == c.nangu and c.indangreal[-1] == 1
+    for s in range(c.nangu):
+        ia = c.indangreal[s] - 1
+        for q in range(c.ncan):
+            o = c.off[s, q]
+            n = c.npunreal[ia]
+            assert np.all(ch[o:o + n] == q) and np.all(ir[o:o + n] == np.arange(n))
+            # R outer / r inner within an angle
+            key = (i2[o:o + n].astype(np.int64) << 20) + i1[o:o + n]
+            assert np.all(np.diff(key) > 0)
+
+
+def test_leps_surface_limits_and_diatomic_states():
+    pes = leps.LEPS()
+    r = np.linspace(0.8, 3.0, 9)
+    far = pes(r, np.full_like(r, 60.0), np.full_like(r, 61.0))
+    assert np.max(np.abs(far - pes.morse(r))) < 1e-12
+    assert abs(pes.morse(np.array([pes.re]))[0] + pes.D) < 1e-15
+    a, b, cc = 1.3, 2.1, 2.9
+    assert abs(pes(np.array(a), np.array(b), np.array(cc)) - pes(np.array(cc), np.array(a), np.array(b))) < 1e-15
+    c = case("j0")
+    f = c.fd[0, 0, :, :]
+    nb = c.noBCstates[0, 0] - c.nvini + 1
+    G = f[:nb] @ f[:nb].T
+    assert np.max(np.abs(G - np.eye(nb))) < 1e-12
+    assert abs(c.ediat[0, c.jref - c.jini, c.nvref - c.nvini]) < 1e-12   # eref subtracted
+
+
+def test_initial_packet_matches_colini_form():
+    """rpaq0 = -g(R) D(gamma, jref) fd(r) sqrt(ah2) on the kept points
+    (SRC/mod_colini_01y2.f:143-170)."""
+    c = case("j0")
+    sl, ch, ir, i1, i2 = model.point_index(c)
+    from madwave3_amd.numerics import rgauscolini
+    R = c.rmis2 + (i2 - 1) * c.ah2
+    g = rgauscolini(R, c.Rcolini, c.alpha0, c.xk0, c.factor0)
+    D = c.Djmmp[0, 0, c.jref, c.indangreal[sl] - 1]
+    fd = c.fd[0, c.jref - c.jini, c.nvref - c.nvini, i1 - 1]
+    assert np.max(np.abs(c.rpaq0 - (-g * D * fd * np.sqrt(c.ah2)))) < 1e-15
