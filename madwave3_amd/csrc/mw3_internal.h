@@ -36,9 +36,12 @@ struct HostTables {
   std::vector<double> p2r2;     // [n2]
   std::vector<double> divi;     // [na]
   double radcut = 0;
+  // RODFT00 tables for the folded DST: for size n, h = n/2, ke = n - h,
+  // S_e[j][k'] = S[j][2k'] (ke x ke) then S_o[j][k'] = S[j][2k'+1] (h x h,
+  // stored in a ke x ke block), S[j][k] = 2 sin(pi (j+1)(k+1)/(n+1))
   std::vector<int32_t> dst_sizes;        // distinct DST lengths (r sizes + n2)
-  std::vector<int64_t> dst_table_off;    // offset of each size's table
-  std::vector<double> dst_tables;        // concatenated n x n tables
+  std::vector<int64_t> dst_table_off;    // offset of each size's S_e block
+  std::vector<double> dst_tables;
   // angular FBR
   std::vector<int32_t> jlist;   // [nJ] j values of the FBR axis
   std::vector<double> Dmat;     // [nc][na][nJ]   D_c(iang, j)
@@ -62,14 +65,19 @@ struct HostTables {
   std::vector<uint8_t> cvj_isref;        // [nc][nj][nv]
 };
 
-// batched GEMM pass: C[m][n] = sum_k A[m][k] B[k][n] per descriptor
+// batched GEMM pass: C[m][n] = sum_k A[m][k] B[k][n] per descriptor.
+// Folded DST-I: with fold_op = 1 (A) or 2 (B) the operand's k-th element is
+// x(k) + fold_sgn x(fold_n-1-k) for k < fold_n/2 (x(k) at the middle of an
+// odd fold_n).  kin_ia >= 0 scales output row m (DST index rmul*m + radd) by
+// the radial kinetic factor of angle kin_ia.
 struct GemmDesc {
   int64_t a_off, b_off, c_off;
-  int M, N, K, lda_m;
+  int M, N, K, lda_m;  // A element (m,k) at a_off + m*lda_m + k*lda_k
+  int lda_k, fold_op, fold_n, fold_sgn;
+  int kin_ia, rmul, radd, pad0;
 };
 
 struct GemmStrides {
-  int64_t lda_k;  // A element (m,k) at a_off + m*lda_m + k*lda_k
   int64_t ldb_k, ldb_n;
   int64_t ldc_m, ldc_n;
 };
@@ -144,7 +152,7 @@ struct Ctx {
   std::vector<double> host_norms;
 
   // GEMM passes: R rows, r columns, angular forward, angular back
-  GemmPlan p_dstR, p_dstr, p_angF, p_angB;
+  GemmPlan p_dstR, p_dstr, p_dstr2, p_angF, p_angB;
   void* d_phi = nullptr;        // FBR after the Omega stencil
   double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors
   uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)

@@ -29,10 +29,42 @@ namespace mw3 {
 // ---------------------------------------------------------------------------
 constexpr int BM = 64, BN = 64, BK = 16;
 
+// operand element with the DST fold (see GemmDesc)
+template <typename TACC, typename T>
+__device__ __forceinline__ TACC ld_fold(const T* base, int64_t idx, int64_t kstride, int k,
+                                        int fold_n, int fold_sgn) {
+  TACC v = (TACC)base[idx];
+  if (fold_n > 0) {
+    int mk = fold_n - 1 - k;
+    if (mk != k) {
+      TACC w = (TACC)base[idx + (int64_t)(mk - k) * kstride];
+      v = fold_sgn > 0 ? v + w : v - w;
+    }
+  }
+  return v;
+}
+
+struct KinParams {
+  const int32_t* nn1;
+  const double* p2r1;
+  const double* p2r2;
+  const double* divi;
+  double radcut;
+  int n1;
+};
+
+__device__ __forceinline__ double kin_factor(const GemmDesc& d, const KinParams& kp, int m, int n) {
+  int k1 = d.rmul * m + d.radd;
+  double e = kp.p2r1[(int64_t)d.kin_ia * kp.n1 + k1] + kp.p2r2[n];
+  if (e > kp.radcut) e = kp.radcut;
+  return e * kp.divi[d.kin_ia];
+}
+
+
 template <typename TA, typename TB, typename TC, typename TACC>
 __global__ __launch_bounds__(256) void k_gemm(const TA* __restrict__ A, const TB* __restrict__ B,
                                               TC* __restrict__ C, const GemmDesc* __restrict__ descs,
-                                              GemmStrides st) {
+                                              GemmStrides st, KinParams kp) {
   const GemmDesc d = descs[blockIdx.z];
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   if (m0 >= d.M || n0 >= d.N) return;
@@ -54,7 +86,9 @@ __global__ __launch_bounds__(256) void k_gemm(const TA* __restrict__ A, const TB
       int mm = e / BK, kk = e % BK;
       int m = m0 + mm, k = k0 + kk;
       TACC v = 0;
-      if (m < d.M && k < d.K) v = (TACC)Ab[(int64_t)m * d.lda_m + (int64_t)k * st.lda_k];
+      if (m < d.M && k < d.K)
+        v = ld_fold<TACC>(Ab, (int64_t)m * d.lda_m + (int64_t)k * d.lda_k, d.lda_k, k,
+                          d.fold_op == 1 ? d.fold_n : 0, d.fold_sgn);
       As[kk][mm] = v;
     }
     // B tile BK x BN: n fastest
@@ -64,7 +98,9 @@ __global__ __launch_bounds__(256) void k_gemm(const TA* __restrict__ A, const TB
       int kk = e / BN, nn = e % BN;
       int k = k0 + kk, n = n0 + nn;
       TACC v = 0;
-      if (k < d.K && n < d.N) v = (TACC)Bb[(int64_t)k * st.ldb_k + (int64_t)n * st.ldb_n];
+      if (k < d.K && n < d.N)
+        v = ld_fold<TACC>(Bb, (int64_t)k * st.ldb_k + (int64_t)n * st.ldb_n, st.ldb_k, k,
+                          d.fold_op == 2 ? d.fold_n : 0, d.fold_sgn);
       Bs[kk][nn] = v;
     }
     __syncthreads();
@@ -90,15 +126,143 @@ __global__ __launch_bounds__(256) void k_gemm(const TA* __restrict__ A, const TB
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       int n = n0 + tx + 16 * j;
-      if (n < d.N) Cb[(int64_t)m * st.ldc_m + (int64_t)n * st.ldc_n] = (TC)acc[i][j];
+      if (n < d.N) {
+        TACC v = acc[i][j];
+        if (d.kin_ia >= 0) v = (TACC)((double)v * kin_factor(d, kp, m, n));
+        Cb[(int64_t)m * st.ldc_m + (int64_t)n * st.ldc_n] = (TC)v;
+      }
     }
   }
 }
 
+// ---------------------------------------------------------------------------
+// MFMA GEMM, FP64 (v_mfma_f64_16x16x4f64) or FP32 (v_mfma_f32_16x16x4f32),
+// same descriptors as k_gemm.  256 threads = 4 waves, block tile 64 x 64,
+// each wave a 32 x 32 quadrant of 2 x 2 MFMA tiles; K staged through LDS 16 at
+// a time with the next K tile prefetched into registers during the MFMAs.
+// Lane layout (wave64): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k;
+// D[i][j] in lane j + 16 (i % 4), register i / 4 (FP64) and in lane
+// j + 16 (i / 4), register i % 4 (FP32).
+// ---------------------------------------------------------------------------
+typedef double mw3_d4 __attribute__((ext_vector_type(4)));
+typedef float mw3_f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaAcc;
+template <>
+struct MfmaAcc<double> {
+  typedef mw3_d4 V;
+  static __device__ __forceinline__ V mma(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct MfmaAcc<float> {
+  typedef mw3_f4 V;
+  static __device__ __forceinline__ V mma(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T, typename TA, typename TB>
+__device__ __forceinline__ void mfma_load_tiles(const GemmDesc& d, const GemmStrides& st,
+                                                const TA* Ab, const TB* Bb, int m0, int n0, int k0,
+                                                T (&ra)[4], T (&rb)[4]) {
+  const bool fa = d.fold_op == 1, fb = d.fold_op == 2;
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    int e = threadIdx.x + l * 256;
+    int mm = e / BK, kk = e % BK;
+    int m = m0 + mm, k = k0 + kk;
+    T v = 0;
+    if (m < d.M && k < d.K) {
+      int64_t idx = (int64_t)m * d.lda_m + (int64_t)k * d.lda_k;
+      v = fa ? ld_fold<T>(Ab, idx, d.lda_k, k, d.fold_n, d.fold_sgn) : (T)Ab[idx];
+    }
+    ra[l] = v;
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    int e = threadIdx.x + l * 256;
+    int kk = e / BN, nn = e % BN;
+    int k = k0 + kk, n = n0 + nn;
+    T v = 0;
+    if (k < d.K && n < d.N) {
+      int64_t idx = (int64_t)k * st.ldb_k + (int64_t)n * st.ldb_n;
+      v = fb ? ld_fold<T>(Bb, idx, st.ldb_k, k, d.fold_n, d.fold_sgn) : (T)Bb[idx];
+    }
+    rb[l] = v;
+  }
+}
+
+template <typename T, typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const TA* __restrict__ A, const TB* __restrict__ B,
+                                                   TC* __restrict__ C,
+                                                   const GemmDesc* __restrict__ descs,
+                                                   GemmStrides st, KinParams kp) {
+  typedef MfmaAcc<T> M;
+  const GemmDesc d = descs[blockIdx.z];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  if (m0 >= d.M || n0 >= d.N) return;
+  __shared__ T As[BK][BM + 4];
+  __shared__ T Bs[BK][BN + 4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  typename M::V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = (typename M::V){0, 0, 0, 0};
+  const TA* Ab = A + d.a_off;
+  const TB* Bb = B + d.b_off;
+  T ra[4], rb[4];
+  mfma_load_tiles<T>(d, st, Ab, Bb, m0, n0, 0, ra, rb);
+  for (int k0 = 0; k0 < d.K; k0 += BK) {
+#pragma unroll
+    for (int l = 0; l < 4; l++) {
+      int e = threadIdx.x + l * 256;
+      As[e % BK][e / BK] = ra[l];
+      Bs[e / BN][e % BN] = rb[l];
+    }
+    __syncthreads();
+    if (k0 + BK < d.K) mfma_load_tiles<T>(d, st, Ab, Bb, m0, n0, k0 + BK, ra, rb);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4), c16 = lane & 15;
+      T a0 = As[kr][wm * 32 + c16], a1 = As[kr][wm * 32 + 16 + c16];
+      T b0 = Bs[kr][wn * 32 + c16], b1 = Bs[kr][wn * 32 + 16 + c16];
+      acc[0][0] = M::mma(a0, b0, acc[0][0]);
+      acc[0][1] = M::mma(a0, b1, acc[0][1]);
+      acc[1][0] = M::mma(a1, b0, acc[1][0]);
+      acc[1][1] = M::mma(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  TC* Cb = C + d.c_off;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int m = m0 + wm * 32 + a * 16 + M::row(lane, r);
+        int n = n0 + wn * 32 + b * 16 + (lane & 15);
+        if (m < d.M && n < d.N) {
+          double v = (double)acc[a][b][r];
+          if (d.kin_ia >= 0) v *= kin_factor(d, kp, m, n);
+          Cb[(int64_t)m * st.ldc_m + (int64_t)n * st.ldc_n] = (TC)v;
+        }
+      }
+}
+
 template <typename TA, typename TB, typename TC, typename TACC>
-static hipError_t run_gemm(hipStream_t s, const TA* A, const TB* B, TC* C, const GemmPlan& p) {
+static hipError_t run_gemm(hipStream_t s, const TA* A, const TB* B, TC* C, const GemmPlan& p,
+                           KinParams kp = KinParams{}) {
   dim3 grid((p.maxN + BN - 1) / BN, (p.maxM + BM - 1) / BM, p.nb);
-  hipLaunchKernelGGL((k_gemm<TA, TB, TC, TACC>), grid, dim3(256), 0, s, A, B, C, p.d_descs, p.st);
+  hipLaunchKernelGGL((k_gemm_mfma<TACC, TA, TB, TC>), grid, dim3(256), 0, s, A, B, C, p.d_descs,
+                     p.st, kp);
   return hipGetLastError();
 }
 
@@ -471,59 +635,84 @@ static hipError_t upload_plan(GemmPlan* p, const std::vector<GemmDesc>& d) {
   return hipMemcpy(p->d_descs, d.data(), sizeof(GemmDesc) * d.size(), hipMemcpyHostToDevice);
 }
 
+static GemmDesc gd() {
+  GemmDesc g{};
+  g.kin_ia = -1;
+  return g;
+}
+
 hipError_t build_plans(Ctx* c) {
   const HostTables& t = c->t;
-
   const int64_t plane = t.nplane, slab = t.nslab;
   hipError_t e;
-  // R rows: every (c, ia, r) row of n2 values times the n2 sine table
+  auto tab_of = [&](int n) {
+    int q = 0;
+    while (t.dst_sizes[q] != n) q++;
+    return t.dst_table_off[q];
+  };
+  // R rows (folded): per (c, ia), rows r < nn1, even and odd output columns
   {
-    int64_t rows = (int64_t)t.nc * t.na * t.n1;
-    int ir2tab = -1;
-    for (size_t q = 0; q < t.dst_sizes.size(); q++)
-      if (t.dst_sizes[q] == t.n2) ir2tab = (int)q;
+    const int n = t.n2, h = n / 2, ke = n - h;
+    const int64_t toff = tab_of(n);
     std::vector<GemmDesc> d;
-    const int64_t chunk = 65535LL * BM;  // grid.y limit
-    for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
-      GemmDesc g{};
-      g.a_off = r0 * t.n2;
-      g.b_off = t.dst_table_off[ir2tab];
-      g.c_off = r0 * t.n2;
-      g.M = (int)std::min<int64_t>(chunk, rows - r0);
-      g.N = t.n2;
-      g.K = t.n2;
-      g.lda_m = t.n2;
-      d.push_back(g);
-    }
-    c->p_dstR.st = GemmStrides{1, t.n2, 1, t.n2, 1};
+    for (int ch = 0; ch < t.nc; ch++)
+      for (int ia = 0; ia < t.na; ia++)
+        for (int par = 0; par < 2; par++) {
+          GemmDesc g = gd();
+          g.a_off = (int64_t)ch * slab + (int64_t)ia * plane;
+          g.b_off = toff + (par ? (int64_t)ke * ke : 0);
+          g.c_off = g.a_off + par;
+          g.M = t.nn1[ia];
+          g.N = par ? h : ke;
+          g.K = par ? h : ke;
+          g.lda_m = n;
+          g.lda_k = 1;
+          g.fold_op = 1;
+          g.fold_n = n;
+          g.fold_sgn = par ? -1 : 1;
+          d.push_back(g);
+        }
+    c->p_dstR.st = GemmStrides{ke, 1, n, 2};
     if ((e = upload_plan(&c->p_dstR, d)) != hipSuccess) return e;
   }
-  // r columns: per (c, ia) an nn1 x nn1 table times the nn1 x n2 block
-  {
+  // r columns (folded, scaled): per (c, ia) rows k1 of the nn1 transform
+  for (int pass = 0; pass < 2; pass++) {
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++)
       for (int ia = 0; ia < t.na; ia++) {
-        int n = t.nn1[ia];
-        int q = 0;
-        while (t.dst_sizes[q] != n) q++;
-        GemmDesc g{};
-        g.a_off = t.dst_table_off[q];
-        g.b_off = (int64_t)ch * slab + (int64_t)ia * plane;
-        g.c_off = g.b_off;
-        g.M = n;
-        g.N = t.n2;
-        g.K = n;
-        g.lda_m = n;
-        d.push_back(g);
+        const int n = t.nn1[ia], h = n / 2, ke = n - h;
+        const int64_t toff = tab_of(n);
+        for (int par = 0; par < 2; par++) {
+          GemmDesc g = gd();
+          // A[k1'][j1] = S[2k1'+par][j1] = S_par[j1][k1']
+          g.a_off = toff + (par ? (int64_t)ke * ke : 0);
+          g.lda_m = 1;
+          g.lda_k = ke;
+          g.b_off = (int64_t)ch * slab + (int64_t)ia * plane;
+          g.c_off = g.b_off + (int64_t)par * t.n2;
+          g.M = par ? h : ke;
+          g.N = t.n2;
+          g.K = par ? h : ke;
+          g.fold_op = 2;
+          g.fold_n = n;
+          g.fold_sgn = par ? -1 : 1;
+          if (pass == 0) {
+            g.kin_ia = ia;
+            g.rmul = 2;
+            g.radd = par;
+          }
+          d.push_back(g);
+        }
       }
-    c->p_dstr.st = GemmStrides{1, t.n2, 1, t.n2, 1};
-    if ((e = upload_plan(&c->p_dstr, d)) != hipSuccess) return e;
+    GemmPlan* pl = pass == 0 ? &c->p_dstr : &c->p_dstr2;
+    pl->st = GemmStrides{t.n2, 1, 2 * (int64_t)t.n2, 1};
+    if ((e = upload_plan(pl, d)) != hipSuccess) return e;
   }
   // angular forward: fbr[c][jj][p] = sum_ia D_c(ia, jj) psi[c][ia][p]
   {
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++) {
-      GemmDesc g{};
+      GemmDesc g = gd();
       g.a_off = (int64_t)ch * t.na * t.nJ;
       g.b_off = (int64_t)ch * slab;
       g.c_off = (int64_t)ch * t.nJ * plane;
@@ -531,16 +720,17 @@ hipError_t build_plans(Ctx* c) {
       g.N = (int)plane;
       g.K = t.na;
       g.lda_m = 1;
+      g.lda_k = t.nJ;
       d.push_back(g);
     }
-    c->p_angF.st = GemmStrides{t.nJ, plane, 1, plane, 1};
+    c->p_angF.st = GemmStrides{plane, 1, plane, 1};
     if ((e = upload_plan(&c->p_angF, d)) != hipSuccess) return e;
   }
   // angular back: ang[c][ia][p] = sum_jj D_c(ia, jj) phi[c][jj][p]
   {
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++) {
-      GemmDesc g{};
+      GemmDesc g = gd();
       g.a_off = (int64_t)ch * t.na * t.nJ;
       g.b_off = (int64_t)ch * t.nJ * plane;
       g.c_off = (int64_t)ch * slab;
@@ -548,9 +738,10 @@ hipError_t build_plans(Ctx* c) {
       g.N = (int)plane;
       g.K = t.nJ;
       g.lda_m = t.nJ;
+      g.lda_k = 1;
       d.push_back(g);
     }
-    c->p_angB.st = GemmStrides{1, plane, 1, plane, 1};
+    c->p_angB.st = GemmStrides{plane, 1, plane, 1};
     if ((e = upload_plan(&c->p_angB, d)) != hipSuccess) return e;
   }
   return hipSuccess;
@@ -558,7 +749,7 @@ hipError_t build_plans(Ctx* c) {
 
 void free_plans(Ctx* c) {
 
-  GemmPlan* ps[4] = {&c->p_dstR, &c->p_dstr, &c->p_angF, &c->p_angB};
+  GemmPlan* ps[5] = {&c->p_dstR, &c->p_dstr, &c->p_dstr2, &c->p_angF, &c->p_angB};
   for (auto* p : ps)
     if (p->d_descs) {
       (void)hipFree(p->d_descs);
@@ -588,19 +779,17 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   e = run_gemm<T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
   if (e != hipSuccess) return e;
   timer_end(c, K_ANG_BWD);
-  // radial kinetic: R rows, r columns + scale + r columns, R rows
+  // radial kinetic: folded DST over R, folded DST over r with the kinetic
+  // factor in the epilogue, folded DST over r, folded DST over R
+  KinParams kp{c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.n1};
   timer_begin(c, K_DST_R);
   e = run_gemm<double, T, T, T>(s, psi, tab, t1, c->p_dstR);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_R);
   timer_begin(c, K_DST_r);
-  e = run_gemm<T, T, T, T>(s, tab, t1, t2, c->p_dstr);
+  e = run_gemm<T, T, T, T>(s, tab, t1, t2, c->p_dstr, kp);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_kin_scale<T>, dim3((unsigned)((int64_t)t.nc * t.na * t.n1)), dim3(256), 0,
-                     s, t2, c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.nc, t.na, t.n1,
-                     t.n2);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  e = run_gemm<T, T, T, T>(s, tab, t2, t1, c->p_dstr);
+  e = run_gemm<T, T, T, T>(s, tab, t2, t1, c->p_dstr2);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_r);
   timer_begin(c, K_DST_RINV);

@@ -231,13 +231,19 @@ int build_tables(const mw3_setup* s, HostTables* t, std::string* err) {
     for (int n : sizes) {
       t->dst_sizes.push_back(n);
       t->dst_table_off.push_back(off);
+      int h = n / 2, ke = n - h;
       long m2 = 2L * (n + 1);
-      for (int j = 0; j < n; j++)
-        for (int k = 0; k < n; k++) {
-          long m = ((long)(j + 1) * (k + 1)) % m2;
-          t->dst_tables.push_back(2.0 * std::sin(pi * (double)m / (double)(n + 1)));
-        }
-      off += (int64_t)n * n;
+      auto S = [&](int j, int k) {
+        long m = ((long)(j + 1) * (k + 1)) % m2;
+        if (m == 0 || m == n + 1) return 0.0;  // exact zeros of sin(pi m/(n+1))
+        return 2.0 * std::sin(pi * (double)m / (double)(n + 1));
+      };
+      for (int j = 0; j < ke; j++)
+        for (int k = 0; k < ke; k++) t->dst_tables.push_back(S(j, 2 * k));
+      for (int j = 0; j < ke; j++)
+        for (int k = 0; k < ke; k++)
+          t->dst_tables.push_back((j < h && k < h) ? S(j, 2 * k + 1) : 0.0);
+      off += 2 * (int64_t)ke * ke;
     }
   }
 
