@@ -172,6 +172,8 @@ def main():
         "dP_steps": a.warmup + a.steps,
         "roofline": roof,
         "kernels": kern,
+        "kernel_ms_per_step": {k: ms / max(cnt, 1) for k, (ms, cnt) in timmx.items()},
+        "fp64_kernel_ms_per_step": {k: ms / max(cnt, 1) for k, (ms, cnt) in tim64.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -23,7 +23,7 @@ constexpr int kRing = 64;  // Chebyshev steps buffered between energy GEMMs
 // Host-side operator tables built from mw3_setup (mw3_setup.cpp).
 struct HostTables {
   // sizes
-  int n1 = 0, n2 = 0, na = 0, nc = 0, ne = 0, nJ = 0;
+  int n1 = 0, n2 = 0, na = 0, nc = 0, ne = 0, nJ = 0, nJp = 0;
   int64_t ntot = 0, nplane = 0, nslab = 0;  // nplane = n1*n2, nslab = na*nplane
   // A0 <-> dense
   std::vector<uint32_t> pt_dense;  // [ntot] dense index (ch,iang,ir1,ir2)
@@ -41,10 +41,11 @@ struct HostTables {
   // stored in a ke x ke block), S[j][k] = 2 sin(pi (j+1)(k+1)/(n+1))
   std::vector<int32_t> dst_sizes;        // distinct DST lengths (r sizes + n2)
   std::vector<int64_t> dst_table_off;    // offset of each size's S_e block
+  std::vector<int> dst_ld;               // leading dimension of each size's blocks
   std::vector<double> dst_tables;
   // angular FBR
   std::vector<int32_t> jlist;   // [nJ] j values of the FBR axis
-  std::vector<double> Dmat;     // [nc][na][nJ]   D_c(iang, j)
+  std::vector<double> Dmat;     // [nc][na][nJp]  D_c(iang, j), j stride nJp = nJ rounded to 4
   std::vector<double> acoef;    // [nc][nJ][n1]   j^2 term (angkinj2 FBR)
   std::vector<double> bcoef;    // [nc][3][nJ][n2] l^2 term, q = Omega_s-Omega_t+1
   std::vector<int32_t> nbr;     // [nc][3] source channel for q, -1 if none
@@ -82,9 +83,12 @@ struct GemmStrides {
   int64_t ldc_m, ldc_n;
 };
 
+// one pass of the step as a batch of GEMMs; tile = 10 TM + TN selects the
+// MFMA tile layout (wave tile TM x TN blocks of 16 x 16); vec: the operands
+// meet the alignment rules of the vectorised kernel
 struct GemmPlan {
   GemmDesc* d_descs = nullptr;
-  int nb = 0, maxM = 0, maxN = 0;
+  int nb = 0, maxM = 0, maxN = 0, tile = 22, vec = 0;
   GemmStrides st{};
 };
 
@@ -193,6 +197,9 @@ hipError_t launch_zero(Ctx* c, void* p, size_t bytes);
 hipError_t build_plans(Ctx* c);
 void free_plans(Ctx* c);
 hipError_t launch_flux_result(Ctx* c, double* d_out /* [(1+ne)][netot] */);
+
+// GEMM passes of the step (template tag: one kernel symbol per pass)
+enum GemmRole { G_ANG_FWD = 0, G_ANG_BWD, G_DST_R, G_DST_r_KIN, G_DST_r_INV, G_DST_R_INV };
 
 enum KernelId {
   K_ANG_FWD = 0,

@@ -19,16 +19,15 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 
 #include "mw3_internal.h"
 
 namespace mw3 {
 
 // ---------------------------------------------------------------------------
-// generic batched GEMM  C[m][n] = sum_k A[m][k] B[k][n]  (+ beta C)
+// batched GEMM operands: C[m][n] = sum_k A[m][k] B[k][n] per descriptor
 // ---------------------------------------------------------------------------
-constexpr int BM = 64, BN = 64, BK = 16;
-
 // operand element with the DST fold (see GemmDesc)
 template <typename TACC, typename T>
 __device__ __forceinline__ TACC ld_fold(const T* base, int64_t idx, int64_t kstride, int k,
@@ -61,89 +60,19 @@ __device__ __forceinline__ double kin_factor(const GemmDesc& d, const KinParams&
 }
 
 
-template <typename TA, typename TB, typename TC, typename TACC>
-__global__ __launch_bounds__(256) void k_gemm(const TA* __restrict__ A, const TB* __restrict__ B,
-                                              TC* __restrict__ C, const GemmDesc* __restrict__ descs,
-                                              GemmStrides st, KinParams kp) {
-  const GemmDesc d = descs[blockIdx.z];
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  if (m0 >= d.M || n0 >= d.N) return;
-  __shared__ TACC As[BK][BM + 1];
-  __shared__ TACC Bs[BK][BN];
-  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
-  TACC acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0;
-  const TA* Ab = A + d.a_off;
-  const TB* Bb = B + d.b_off;
-  for (int k0 = 0; k0 < d.K; k0 += BK) {
-    // A tile BM x BK: 1024 elements, 4 per thread
-#pragma unroll
-    for (int l = 0; l < 4; l++) {
-      int e = threadIdx.x + l * 256;
-      int mm = e / BK, kk = e % BK;
-      int m = m0 + mm, k = k0 + kk;
-      TACC v = 0;
-      if (m < d.M && k < d.K)
-        v = ld_fold<TACC>(Ab, (int64_t)m * d.lda_m + (int64_t)k * d.lda_k, d.lda_k, k,
-                          d.fold_op == 1 ? d.fold_n : 0, d.fold_sgn);
-      As[kk][mm] = v;
-    }
-    // B tile BK x BN: n fastest
-#pragma unroll
-    for (int l = 0; l < 4; l++) {
-      int e = threadIdx.x + l * 256;
-      int kk = e / BN, nn = e % BN;
-      int k = k0 + kk, n = n0 + nn;
-      TACC v = 0;
-      if (k < d.K && n < d.N)
-        v = ld_fold<TACC>(Bb, (int64_t)k * st.ldb_k + (int64_t)n * st.ldb_n, st.ldb_k, k,
-                          d.fold_op == 2 ? d.fold_n : 0, d.fold_sgn);
-      Bs[kk][nn] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk++) {
-      TACC a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) a[i] = As[kk][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) b[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] += a[i] * b[j];
-    }
-    __syncthreads();
-  }
-  TC* Cb = C + d.c_off;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    int m = m0 + ty + 16 * i;
-    if (m >= d.M) continue;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      int n = n0 + tx + 16 * j;
-      if (n < d.N) {
-        TACC v = acc[i][j];
-        if (d.kin_ia >= 0) v = (TACC)((double)v * kin_factor(d, kp, m, n));
-        Cb[(int64_t)m * st.ldc_m + (int64_t)n * st.ldc_n] = (TC)v;
-      }
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// MFMA GEMM, FP64 (v_mfma_f64_16x16x4f64) or FP32 (v_mfma_f32_16x16x4f32),
-// same descriptors as k_gemm.  256 threads = 4 waves, block tile 64 x 64,
-// each wave a 32 x 32 quadrant of 2 x 2 MFMA tiles; K staged through LDS 16 at
-// a time with the next K tile prefetched into registers during the MFMAs.
+// MFMA GEMM, FP64 (v_mfma_f64_16x16x4f64) or FP32 (v_mfma_f32_16x16x4f32).
+// 256 threads = 2 x 2 waves; each wave owns TM x TN MFMA tiles of 16 x 16, so
+// the block tile is (32 TM) x (32 TN).  K is staged through LDS BK at a time;
+// the next K tile is loaded into registers while the current one is in the
+// MFMAs.  Operand loads run along whichever of the operand's two axes is
+// contiguous in memory.  ROLE only names the instantiation (one symbol per
+// pass of the step, so kernel traces separate them).
 // Lane layout (wave64): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k;
 // D[i][j] in lane j + 16 (i % 4), register i / 4 (FP64) and in lane
 // j + 16 (i / 4), register i % 4 (FP32).
 // ---------------------------------------------------------------------------
+constexpr int BK = 16;
 typedef double mw3_d4 __attribute__((ext_vector_type(4)));
 typedef float mw3_f4 __attribute__((ext_vector_type(4)));
 
@@ -166,15 +95,29 @@ struct MfmaAcc<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
-template <typename T, typename TA, typename TB>
+// element e of a (rows x BK) A tile or (BK x cols) B tile -> (row/col, k)
+template <int ROWS>
+__device__ __forceinline__ void tile_pos(int e, bool rowfast, int& rc, int& kk) {
+  if (rowfast) {
+    rc = e % ROWS;
+    kk = e / ROWS;
+  } else {
+    rc = e / BK;
+    kk = e % BK;
+  }
+}
+
+template <typename T, int TM, int TN, typename TA, typename TB>
 __device__ __forceinline__ void mfma_load_tiles(const GemmDesc& d, const GemmStrides& st,
                                                 const TA* Ab, const TB* Bb, int m0, int n0, int k0,
-                                                T (&ra)[4], T (&rb)[4]) {
+                                                bool amf, bool bnf, T (&ra)[2 * TM],
+                                                T (&rb)[2 * TN]) {
+  constexpr int BMt = 32 * TM, BNt = 32 * TN;
   const bool fa = d.fold_op == 1, fb = d.fold_op == 2;
 #pragma unroll
-  for (int l = 0; l < 4; l++) {
-    int e = threadIdx.x + l * 256;
-    int mm = e / BK, kk = e % BK;
+  for (int l = 0; l < 2 * TM; l++) {
+    int mm, kk;
+    tile_pos<BMt>(threadIdx.x + l * 256, amf, mm, kk);
     int m = m0 + mm, k = k0 + kk;
     T v = 0;
     if (m < d.M && k < d.K) {
@@ -184,9 +127,9 @@ __device__ __forceinline__ void mfma_load_tiles(const GemmDesc& d, const GemmStr
     ra[l] = v;
   }
 #pragma unroll
-  for (int l = 0; l < 4; l++) {
-    int e = threadIdx.x + l * 256;
-    int kk = e / BN, nn = e % BN;
+  for (int l = 0; l < 2 * TN; l++) {
+    int nn, kk;
+    tile_pos<BNt>(threadIdx.x + l * 256, bnf, nn, kk);
     int k = k0 + kk, n = n0 + nn;
     T v = 0;
     if (k < d.K && n < d.N) {
@@ -197,58 +140,69 @@ __device__ __forceinline__ void mfma_load_tiles(const GemmDesc& d, const GemmStr
   }
 }
 
-template <typename T, typename TA, typename TB, typename TC>
+template <int ROLE, typename T, int TM, int TN, typename TA, typename TB, typename TC>
 __global__ __launch_bounds__(256) void k_gemm_mfma(const TA* __restrict__ A, const TB* __restrict__ B,
                                                    TC* __restrict__ C,
                                                    const GemmDesc* __restrict__ descs,
                                                    GemmStrides st, KinParams kp) {
   typedef MfmaAcc<T> M;
+  constexpr int BMt = 32 * TM, BNt = 32 * TN;
   const GemmDesc d = descs[blockIdx.z];
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = blockIdx.y * BMt, n0 = blockIdx.x * BNt;
   if (m0 >= d.M || n0 >= d.N) return;
-  __shared__ T As[BK][BM + 4];
-  __shared__ T Bs[BK][BN + 4];
+  __shared__ T As[BK][BMt + 4];
+  __shared__ T Bs[BK][BNt + 4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  typename M::V acc[2][2];
+  const bool amf = d.lda_m == 1, bnf = st.ldb_n == 1;
+  typename M::V acc[TM][TN];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = (typename M::V){0, 0, 0, 0};
+    for (int b = 0; b < TN; b++) acc[a][b] = (typename M::V){0, 0, 0, 0};
   const TA* Ab = A + d.a_off;
   const TB* Bb = B + d.b_off;
-  T ra[4], rb[4];
-  mfma_load_tiles<T>(d, st, Ab, Bb, m0, n0, 0, ra, rb);
+  T ra[2 * TM], rb[2 * TN];
+  mfma_load_tiles<T, TM, TN>(d, st, Ab, Bb, m0, n0, 0, amf, bnf, ra, rb);
   for (int k0 = 0; k0 < d.K; k0 += BK) {
 #pragma unroll
-    for (int l = 0; l < 4; l++) {
-      int e = threadIdx.x + l * 256;
-      As[e % BK][e / BK] = ra[l];
-      Bs[e / BN][e % BN] = rb[l];
+    for (int l = 0; l < 2 * TM; l++) {
+      int mm, kk;
+      tile_pos<BMt>(threadIdx.x + l * 256, amf, mm, kk);
+      As[kk][mm] = ra[l];
+    }
+#pragma unroll
+    for (int l = 0; l < 2 * TN; l++) {
+      int nn, kk;
+      tile_pos<BNt>(threadIdx.x + l * 256, bnf, nn, kk);
+      Bs[kk][nn] = rb[l];
     }
     __syncthreads();
-    if (k0 + BK < d.K) mfma_load_tiles<T>(d, st, Ab, Bb, m0, n0, k0 + BK, ra, rb);
+    if (k0 + BK < d.K) mfma_load_tiles<T, TM, TN>(d, st, Ab, Bb, m0, n0, k0 + BK, amf, bnf, ra, rb);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kr = kk + (lane >> 4), c16 = lane & 15;
-      T a0 = As[kr][wm * 32 + c16], a1 = As[kr][wm * 32 + 16 + c16];
-      T b0 = Bs[kr][wn * 32 + c16], b1 = Bs[kr][wn * 32 + 16 + c16];
-      acc[0][0] = M::mma(a0, b0, acc[0][0]);
-      acc[0][1] = M::mma(a0, b1, acc[0][1]);
-      acc[1][0] = M::mma(a1, b0, acc[1][0]);
-      acc[1][1] = M::mma(a1, b1, acc[1][1]);
+      T av[TM], bv[TN];
+#pragma unroll
+      for (int a = 0; a < TM; a++) av[a] = As[kr][wm * 16 * TM + 16 * a + c16];
+#pragma unroll
+      for (int b = 0; b < TN; b++) bv[b] = Bs[kr][wn * 16 * TN + 16 * b + c16];
+#pragma unroll
+      for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = M::mma(av[a], bv[b], acc[a][b]);
     }
     __syncthreads();
   }
   TC* Cb = C + d.c_off;
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++)
+    for (int b = 0; b < TN; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int m = m0 + wm * 32 + a * 16 + M::row(lane, r);
-        int n = n0 + wn * 32 + b * 16 + (lane & 15);
+        int m = m0 + wm * 16 * TM + a * 16 + M::row(lane, r);
+        int n = n0 + wn * 16 * TN + b * 16 + (lane & 15);
         if (m < d.M && n < d.N) {
           double v = (double)acc[a][b][r];
           if (d.kin_ia >= 0) v *= kin_factor(d, kp, m, n);
@@ -257,12 +211,220 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const TA* __restrict__ A, con
       }
 }
 
-template <typename TA, typename TB, typename TC, typename TACC>
+// ---------------------------------------------------------------------------
+// Vectorised MFMA GEMM for plans whose operand rows are 16-byte aligned
+// (offsets, strides and fold length multiples of 4; checked on the host):
+// every operand access is a 4-element vector along the operand's contiguous
+// axis.  A is K-contiguous for the R passes and the angular back transform
+// (rows of psi / of D) and M-contiguous for the r passes and the angular
+// forward transform (columns of the tables); B is always N-contiguous.
+// The MFMA k index of lane group g = lane / 16 at step s is 4 g + s, so a
+// K-contiguous A row hands each lane its four k values in one LDS read.
+// ---------------------------------------------------------------------------
+template <typename T, typename TS>
+__device__ __forceinline__ void ld4(const TS* p, T (&v)[4]) {
+  typedef TS TS4 __attribute__((ext_vector_type(4)));
+  TS4 x = *(const TS4*)p;
+  v[0] = (T)x.x;
+  v[1] = (T)x.y;
+  v[2] = (T)x.z;
+  v[3] = (T)x.w;
+}
+
+template <typename T>
+__device__ __forceinline__ void st4(T* p, const T (&v)[4]) {
+  typedef T T4 __attribute__((ext_vector_type(4)));
+  *(T4*)p = (T4){v[0], v[1], v[2], v[3]};
+}
+
+// A row m, elements k .. k+3 (K-contiguous), folded along K when fold_op == 1
+template <typename T, typename TA>
+__device__ __forceinline__ void ld_a_kc(const GemmDesc& d, const TA* Ab, int m, int k, T (&v)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) v[e] = 0;
+  if (m < d.M && k < d.K) {
+    const TA* row = Ab + (int64_t)m * d.lda_m;
+    ld4(row + k, v);
+    if (d.fold_op == 1) {
+      // element k+e pairs with fold_n-1-k-e = (fold_n-4-k) + (3-e)
+      T w[4];
+      ld4(row + (d.fold_n - 4 - k), w);
+#pragma unroll
+      for (int e = 0; e < 4; e++) v[e] = d.fold_sgn > 0 ? v[e] + w[3 - e] : v[e] - w[3 - e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      if (k + e >= d.K) v[e] = 0;
+  }
+}
+
+// operand row k (stride ldk), elements i .. i+3 of the contiguous axis (extent
+// MN), folded along the rows when fold is set
+template <typename T, typename TS>
+__device__ __forceinline__ void ld_mn(const TS* base, int64_t ldk, int k, int i, int K, int MN,
+                                      bool fold, int fold_n, int sgn, T (&v)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; e++) v[e] = 0;
+  if (k < K && i < MN) {
+    const TS* row = base + (int64_t)k * ldk + i;
+    const int mk = fold_n - 1 - k;
+    const bool f = fold && mk != k;
+    const int64_t dp = (int64_t)(mk - k) * ldk;
+    if (i + 3 < MN) {
+      ld4(row, v);
+      if (f) {
+        T w[4];
+        ld4(row + dp, w);
+#pragma unroll
+        for (int e = 0; e < 4; e++) v[e] = sgn > 0 ? v[e] + w[e] : v[e] - w[e];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (i + e < MN) {
+          T x = (T)row[e];
+          if (f) {
+            T y = (T)row[dp + e];
+            x = sgn > 0 ? x + y : x - y;
+          }
+          v[e] = x;
+        }
+    }
+  }
+}
+
+template <int ROLE>
+struct RoleAKC {
+  static constexpr bool value = ROLE == G_DST_R || ROLE == G_DST_R_INV || ROLE == G_ANG_BWD;
+};
+
+template <int ROLE, typename T, int TM, int TN, typename TA, typename TB, typename TC>
+__global__ __launch_bounds__(256) void k_gemm_vec(const TA* __restrict__ A, const TB* __restrict__ B,
+                                                  TC* __restrict__ C,
+                                                  const GemmDesc* __restrict__ descs,
+                                                  GemmStrides st, KinParams kp) {
+  typedef MfmaAcc<T> M;
+  constexpr bool AKC = RoleAKC<ROLE>::value;
+  constexpr int BMt = 32 * TM, BNt = 32 * TN;
+  constexpr int LA = AKC ? BK + 4 : BMt + 4;  // LDS row length of A (K-major or M-major)
+  constexpr int LB = BNt + 4;
+  const GemmDesc d = descs[blockIdx.z];
+  const int m0 = blockIdx.y * BMt, n0 = blockIdx.x * BNt;
+  if (m0 >= d.M || n0 >= d.N) return;
+  __shared__ __align__(16) T As[AKC ? BMt * LA : BK * LA];
+  __shared__ __align__(16) T Bs[BK * LB];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int g = lane >> 4, c16 = lane & 15;
+  const bool fb = d.fold_op == 2;
+  typename M::V acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; a++)
+#pragma unroll
+    for (int b = 0; b < TN; b++) acc[a][b] = (typename M::V){0, 0, 0, 0};
+  const TA* Ab = A + d.a_off;
+  const TB* Bb = B + d.b_off;
+  T ra[TM / 2][4], rb[TN / 2][4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int l = 0; l < TM / 2; l++) {
+      const int v = threadIdx.x + 256 * l;
+      if (AKC)
+        ld_a_kc<T>(d, Ab, m0 + (v >> 2), k0 + 4 * (v & 3), ra[l]);
+      else
+        ld_mn<T>(Ab, d.lda_k, k0 + v / (8 * TM), m0 + 4 * (v % (8 * TM)), d.K, d.M, false, 0, 1,
+                 ra[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < TN / 2; l++) {
+      const int v = threadIdx.x + 256 * l;
+      ld_mn<T>(Bb, st.ldb_k, k0 + v / (8 * TN), n0 + 4 * (v % (8 * TN)), d.K, d.N, fb, d.fold_n,
+               d.fold_sgn, rb[l]);
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < d.K; k0 += BK) {
+#pragma unroll
+    for (int l = 0; l < TM / 2; l++) {
+      const int v = threadIdx.x + 256 * l;
+      if (AKC)
+        st4(&As[(v >> 2) * LA + 4 * (v & 3)], ra[l]);
+      else
+        st4(&As[(v / (8 * TM)) * LA + 4 * (v % (8 * TM))], ra[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < TN / 2; l++) {
+      const int v = threadIdx.x + 256 * l;
+      st4(&Bs[(v / (8 * TN)) * LB + 4 * (v % (8 * TN))], rb[l]);
+    }
+    __syncthreads();
+    if (k0 + BK < d.K) load(k0 + BK);
+    T av[TM][4], bv[TN][4];
+#pragma unroll
+    for (int a = 0; a < TM; a++) {
+      const int row = wm * 16 * TM + 16 * a + c16;
+      if (AKC) {
+        ld4(&As[row * LA + 4 * g], av[a]);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 4; s++) av[a][s] = As[(4 * g + s) * LA + row];
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < TN; b++) {
+      const int col = wn * 16 * TN + 16 * b + c16;
+#pragma unroll
+      for (int s = 0; s < 4; s++) bv[b][s] = Bs[(4 * g + s) * LB + col];
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int a = 0; a < TM; a++)
+#pragma unroll
+        for (int b = 0; b < TN; b++) acc[a][b] = M::mma(av[a][s], bv[b][s], acc[a][b]);
+    __syncthreads();
+  }
+  TC* Cb = C + d.c_off;
+#pragma unroll
+  for (int a = 0; a < TM; a++)
+#pragma unroll
+    for (int b = 0; b < TN; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int m = m0 + wm * 16 * TM + a * 16 + M::row(lane, r);
+        int n = n0 + wn * 16 * TN + b * 16 + (lane & 15);
+        if (m < d.M && n < d.N) {
+          double v = (double)acc[a][b][r];
+          if (d.kin_ia >= 0) v *= kin_factor(d, kp, m, n);
+          Cb[(int64_t)m * st.ldc_m + (int64_t)n * st.ldc_n] = (TC)v;
+        }
+      }
+}
+
+template <int ROLE, int TM, int TN, typename TACC, typename TA, typename TB, typename TC>
+static void launch_gemm_vec(hipStream_t s, const TA* A, const TB* B, TC* C, const GemmPlan& p,
+                            KinParams kp) {
+  dim3 grid((p.maxN + 32 * TN - 1) / (32 * TN), (p.maxM + 32 * TM - 1) / (32 * TM), p.nb);
+  hipLaunchKernelGGL((k_gemm_vec<ROLE, TACC, TM, TN, TA, TB, TC>), grid, dim3(256), 0, s, A, B, C,
+                     p.d_descs, p.st, kp);
+}
+
+template <int ROLE, typename TA, typename TB, typename TC, typename TACC>
 static hipError_t run_gemm(hipStream_t s, const TA* A, const TB* B, TC* C, const GemmPlan& p,
                            KinParams kp = KinParams{}) {
-  dim3 grid((p.maxN + BN - 1) / BN, (p.maxM + BM - 1) / BM, p.nb);
-  hipLaunchKernelGGL((k_gemm_mfma<TACC, TA, TB, TC>), grid, dim3(256), 0, s, A, B, C, p.d_descs,
-                     p.st, kp);
+  if (!p.vec) {
+    dim3 grid((p.maxN + 63) / 64, (p.maxM + 63) / 64, p.nb);
+    hipLaunchKernelGGL((k_gemm_mfma<ROLE, TACC, 2, 2, TA, TB, TC>), grid, dim3(256), 0, s, A, B, C,
+                       p.d_descs, p.st, kp);
+    return hipGetLastError();
+  }
+  switch (p.tile) {
+    case 22: launch_gemm_vec<ROLE, 2, 2, TACC>(s, A, B, C, p, kp); break;
+    case 42: launch_gemm_vec<ROLE, 4, 2, TACC>(s, A, B, C, p, kp); break;
+    case 24: launch_gemm_vec<ROLE, 2, 4, TACC>(s, A, B, C, p, kp); break;
+    case 44: launch_gemm_vec<ROLE, 4, 4, TACC>(s, A, B, C, p, kp); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -623,7 +785,10 @@ hipError_t launch_gather_a0(Ctx* c, const double* dense, double* a0) {
 // ---------------------------------------------------------------------------
 // GEMM plans (shapes are fixed per context)
 // ---------------------------------------------------------------------------
-static hipError_t upload_plan(GemmPlan* p, const std::vector<GemmDesc>& d) {
+static bool plan_vec_ok(const std::vector<GemmDesc>& ds, const GemmStrides& st, bool akc);
+
+static hipError_t upload_plan(GemmPlan* p, const std::vector<GemmDesc>& d, bool akc) {
+  p->vec = plan_vec_ok(d, p->st, akc) ? 1 : 0;
   p->nb = (int)d.size();
   p->maxM = p->maxN = 0;
   for (auto& x : d) {
@@ -633,6 +798,41 @@ static hipError_t upload_plan(GemmPlan* p, const std::vector<GemmDesc>& d) {
   hipError_t e = hipMalloc(&p->d_descs, sizeof(GemmDesc) * d.size());
   if (e != hipSuccess) return e;
   return hipMemcpy(p->d_descs, d.data(), sizeof(GemmDesc) * d.size(), hipMemcpyHostToDevice);
+}
+
+// whether every descriptor of a plan meets the vector kernel's alignment
+// rules (see k_gemm_vec); akc: A is K-contiguous
+static bool plan_vec_ok(const std::vector<GemmDesc>& ds, const GemmStrides& st, bool akc) {
+  if (st.ldb_n != 1 || st.ldb_k % 4) return false;
+  for (const GemmDesc& d : ds) {
+    if (d.a_off % 4 || d.b_off % 4) return false;
+    if (akc) {
+      if (d.lda_k != 1 || d.lda_m % 4) return false;
+      if (d.fold_op == 1 ? (d.fold_n % 4 != 0 || 2 * d.K > d.fold_n)
+                         : (d.lda_m < ((d.K + 3) & ~3)))
+        return false;
+    } else {
+      if (d.lda_m != 1 || d.lda_k % 4 || d.fold_op == 1) return false;
+    }
+  }
+  return true;
+}
+
+// MFMA tile per plan (angular fwd, angular back, R passes, r pass with the
+// kinetic factor, r pass back); MW3_TILES="t0,t1,t2,t3,t4" overrides
+static void plan_tiles(Ctx* c) {
+  int t[5] = {22, 22, 22, 22, 22};
+  if (const char* env = getenv("MW3_TILES")) {
+    int v[5];
+    if (sscanf(env, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == 5)
+      for (int i = 0; i < 5; i++)
+        if (v[i] == 22 || v[i] == 24 || v[i] == 42 || v[i] == 44) t[i] = v[i];
+  }
+  c->p_angF.tile = t[0];
+  c->p_angB.tile = t[1];
+  c->p_dstR.tile = t[2];
+  c->p_dstr.tile = t[3];
+  c->p_dstr2.tile = t[4];
 }
 
 static GemmDesc gd() {
@@ -645,22 +845,23 @@ hipError_t build_plans(Ctx* c) {
   const HostTables& t = c->t;
   const int64_t plane = t.nplane, slab = t.nslab;
   hipError_t e;
-  auto tab_of = [&](int n) {
+  auto tab_q = [&](int n) {
     int q = 0;
     while (t.dst_sizes[q] != n) q++;
-    return t.dst_table_off[q];
+    return q;
   };
   // R rows (folded): per (c, ia), rows r < nn1, even and odd output columns
   {
     const int n = t.n2, h = n / 2, ke = n - h;
-    const int64_t toff = tab_of(n);
+    const int q = tab_q(n), kp = t.dst_ld[q];
+    const int64_t toff = t.dst_table_off[q];
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++)
       for (int ia = 0; ia < t.na; ia++)
         for (int par = 0; par < 2; par++) {
           GemmDesc g = gd();
           g.a_off = (int64_t)ch * slab + (int64_t)ia * plane;
-          g.b_off = toff + (par ? (int64_t)ke * ke : 0);
+          g.b_off = toff + (par ? (int64_t)kp * kp : 0);
           g.c_off = g.a_off + par;
           g.M = t.nn1[ia];
           g.N = par ? h : ke;
@@ -672,8 +873,8 @@ hipError_t build_plans(Ctx* c) {
           g.fold_sgn = par ? -1 : 1;
           d.push_back(g);
         }
-    c->p_dstR.st = GemmStrides{ke, 1, n, 2};
-    if ((e = upload_plan(&c->p_dstR, d)) != hipSuccess) return e;
+    c->p_dstR.st = GemmStrides{kp, 1, n, 2};
+    if ((e = upload_plan(&c->p_dstR, d, true)) != hipSuccess) return e;
   }
   // r columns (folded, scaled): per (c, ia) rows k1 of the nn1 transform
   for (int pass = 0; pass < 2; pass++) {
@@ -681,13 +882,14 @@ hipError_t build_plans(Ctx* c) {
     for (int ch = 0; ch < t.nc; ch++)
       for (int ia = 0; ia < t.na; ia++) {
         const int n = t.nn1[ia], h = n / 2, ke = n - h;
-        const int64_t toff = tab_of(n);
+        const int q = tab_q(n), kp = t.dst_ld[q];
+        const int64_t toff = t.dst_table_off[q];
         for (int par = 0; par < 2; par++) {
           GemmDesc g = gd();
           // A[k1'][j1] = S[2k1'+par][j1] = S_par[j1][k1']
-          g.a_off = toff + (par ? (int64_t)ke * ke : 0);
+          g.a_off = toff + (par ? (int64_t)kp * kp : 0);
           g.lda_m = 1;
-          g.lda_k = ke;
+          g.lda_k = kp;
           g.b_off = (int64_t)ch * slab + (int64_t)ia * plane;
           g.c_off = g.b_off + (int64_t)par * t.n2;
           g.M = par ? h : ke;
@@ -706,44 +908,45 @@ hipError_t build_plans(Ctx* c) {
       }
     GemmPlan* pl = pass == 0 ? &c->p_dstr : &c->p_dstr2;
     pl->st = GemmStrides{t.n2, 1, 2 * (int64_t)t.n2, 1};
-    if ((e = upload_plan(pl, d)) != hipSuccess) return e;
+    if ((e = upload_plan(pl, d, false)) != hipSuccess) return e;
   }
   // angular forward: fbr[c][jj][p] = sum_ia D_c(ia, jj) psi[c][ia][p]
   {
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++) {
       GemmDesc g = gd();
-      g.a_off = (int64_t)ch * t.na * t.nJ;
+      g.a_off = (int64_t)ch * t.na * t.nJp;
       g.b_off = (int64_t)ch * slab;
       g.c_off = (int64_t)ch * t.nJ * plane;
       g.M = t.nJ;
       g.N = (int)plane;
       g.K = t.na;
       g.lda_m = 1;
-      g.lda_k = t.nJ;
+      g.lda_k = t.nJp;
       d.push_back(g);
     }
     c->p_angF.st = GemmStrides{plane, 1, plane, 1};
-    if ((e = upload_plan(&c->p_angF, d)) != hipSuccess) return e;
+    if ((e = upload_plan(&c->p_angF, d, false)) != hipSuccess) return e;
   }
   // angular back: ang[c][ia][p] = sum_jj D_c(ia, jj) phi[c][jj][p]
   {
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++) {
       GemmDesc g = gd();
-      g.a_off = (int64_t)ch * t.na * t.nJ;
+      g.a_off = (int64_t)ch * t.na * t.nJp;
       g.b_off = (int64_t)ch * t.nJ * plane;
       g.c_off = (int64_t)ch * slab;
       g.M = t.na;
       g.N = (int)plane;
       g.K = t.nJ;
-      g.lda_m = t.nJ;
+      g.lda_m = t.nJp;
       g.lda_k = 1;
       d.push_back(g);
     }
     c->p_angB.st = GemmStrides{plane, 1, plane, 1};
-    if ((e = upload_plan(&c->p_angB, d)) != hipSuccess) return e;
+    if ((e = upload_plan(&c->p_angB, d, true)) != hipSuccess) return e;
   }
+  plan_tiles(c);
   return hipSuccess;
 }
 
@@ -768,7 +971,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   hipError_t e;
   // angular term
   timer_begin(c, K_ANG_FWD);
-  e = run_gemm<T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr, c->p_angF);
+  e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr, c->p_angF);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_fbr_coef<T>, dim3((unsigned)((int64_t)t.nc * t.nJ * t.n1)), dim3(256), 0,
                      s, (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,
@@ -776,24 +979,24 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   if ((e = hipGetLastError()) != hipSuccess) return e;
   timer_end(c, K_ANG_FWD);
   timer_begin(c, K_ANG_BWD);
-  e = run_gemm<T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
+  e = run_gemm<G_ANG_BWD, T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
   if (e != hipSuccess) return e;
   timer_end(c, K_ANG_BWD);
   // radial kinetic: folded DST over R, folded DST over r with the kinetic
   // factor in the epilogue, folded DST over r, folded DST over R
   KinParams kp{c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.n1};
   timer_begin(c, K_DST_R);
-  e = run_gemm<double, T, T, T>(s, psi, tab, t1, c->p_dstR);
+  e = run_gemm<G_DST_R, double, T, T, T>(s, psi, tab, t1, c->p_dstR);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_R);
   timer_begin(c, K_DST_r);
-  e = run_gemm<T, T, T, T>(s, tab, t1, t2, c->p_dstr, kp);
+  e = run_gemm<G_DST_r_KIN, T, T, T, T>(s, tab, t1, t2, c->p_dstr, kp);
   if (e != hipSuccess) return e;
-  e = run_gemm<T, T, T, T>(s, tab, t2, t1, c->p_dstr2);
+  e = run_gemm<G_DST_r_INV, T, T, T, T>(s, tab, t2, t1, c->p_dstr2);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_r);
   timer_begin(c, K_DST_RINV);
-  e = run_gemm<T, T, T, T>(s, t1, tab, t2, c->p_dstR);
+  e = run_gemm<G_DST_R_INV, T, T, T, T>(s, t1, tab, t2, c->p_dstR);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_RINV);
   return hipSuccess;

@@ -231,19 +231,23 @@ int build_tables(const mw3_setup* s, HostTables* t, std::string* err) {
     for (int n : sizes) {
       t->dst_sizes.push_back(n);
       t->dst_table_off.push_back(off);
-      int h = n / 2, ke = n - h;
+      int h = n / 2, ke = n - h, kp = (ke + 3) & ~3;
+      t->dst_ld.push_back(kp);
       long m2 = 2L * (n + 1);
       auto S = [&](int j, int k) {
         long m = ((long)(j + 1) * (k + 1)) % m2;
         if (m == 0 || m == n + 1) return 0.0;  // exact zeros of sin(pi m/(n+1))
         return 2.0 * std::sin(pi * (double)m / (double)(n + 1));
       };
-      for (int j = 0; j < ke; j++)
-        for (int k = 0; k < ke; k++) t->dst_tables.push_back(S(j, 2 * k));
-      for (int j = 0; j < ke; j++)
-        for (int k = 0; k < ke; k++)
+      // S_e[j][k] = S(j, 2k) and S_o[j][k] = S(j, 2k+1), each a kp x kp block
+      // (leading dimension kp, a multiple of 4, zero padded)
+      for (int j = 0; j < kp; j++)
+        for (int k = 0; k < kp; k++)
+          t->dst_tables.push_back((j < ke && k < ke) ? S(j, 2 * k) : 0.0);
+      for (int j = 0; j < kp; j++)
+        for (int k = 0; k < kp; k++)
           t->dst_tables.push_back((j < h && k < h) ? S(j, 2 * k + 1) : 0.0);
-      off += 2 * (int64_t)ke * ke;
+      off += 2 * (int64_t)kp * kp;
     }
   }
 
@@ -257,11 +261,13 @@ int build_tables(const mw3_setup* s, HostTables* t, std::string* err) {
   auto DJ = [&](int iom, int e, int j, int iang) {
     return s->Djmmp[(((size_t)iom * ne + e) * s->ndj + j) * na + (iang - 1)];
   };
-  t->Dmat.assign((size_t)nc * na * nJ, 0.0);
+  t->nJp = (nJ + 3) & ~3;
+  t->Dmat.assign((size_t)nc * na * t->nJp, 0.0);
   for (int c = 0; c < nc; c++)
     for (int ia = 1; ia <= na; ia++)
       for (int jj = 0; jj < nJ; jj++)
-        t->Dmat[((size_t)c * na + (ia - 1)) * nJ + jj] = DJ(t->ch_omega[c], t->ch_elec[c], jj, ia);
+        t->Dmat[((size_t)c * na + (ia - 1)) * t->nJp + jj] =
+            DJ(t->ch_omega[c], t->ch_elec[c], jj, ia);
   // j^2 (SRC/mod_Hphi_01y2.f:235-272)
   t->acoef.assign((size_t)nc * nJ * n1, 0.0);
   for (int c = 0; c < nc; c++) {
