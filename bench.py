@@ -75,6 +75,32 @@ def run_gpu(case, precision, steps, warmup, timing):
     return dt, times, P
 
 
+def copy_calibration(mib):
+    """A plain device copy of known size (torch.neg over float32, reads and
+    writes mib MiB each) run three times, so that hardware byte counters
+    collected over the same process can be scaled against known traffic."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    del x, y
+
+
+def load_traffic(kernel):
+    """HBM bytes per step of a kernel group from the committed byte-counter
+    summary (profiles/traffic_cfg2.json), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_cfg2.json")
+    try:
+        with open(path) as f:
+            return json.load(f)["mixed"][kernel]["hbm_bytes_per_step"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(case, nsteps):
     from oracle.oracle import Oracle, set_threads
 
@@ -99,6 +125,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run a copy of this many MiB first (byte-counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (--gpus %d)" % a.gpus,
@@ -106,6 +134,8 @@ def main():
         sys.exit(2)
     from madwave3_amd.lib import MW3_FP64, MW3_MIXED
 
+    if a.copy_mib:
+        copy_calibration(a.copy_mib)
     t0 = time.perf_counter()
     case = build_case(a.config)
     print("# case %s built in %.1fs" % (case, time.perf_counter() - t0), file=sys.stderr, flush=True)
@@ -144,7 +174,8 @@ def main():
         dom = max(groups, key=lambda k: groups[k])
         d = kern[dom]
         roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],
-                "unit": d["unit"], "frac": d["frac"], "traffic": None, "kernel": dom}
+                "unit": d["unit"], "frac": d["frac"], "traffic": load_traffic(dom),
+                "kernel": dom}
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(case, a.cpu_steps)

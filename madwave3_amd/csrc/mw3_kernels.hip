@@ -298,18 +298,50 @@ struct RoleAKC {
   static constexpr bool value = ROLE == G_DST_R || ROLE == G_DST_R_INV || ROLE == G_ANG_BWD;
 };
 
+// Block -> tile order.  Workgroups go to the 8 XCDs round robin by linear
+// id, each XCD with its own L2; hardware id hw is mapped to a logical id so
+// that each XCD runs one contiguous run of logical ids, and the logical order
+// puts the blocks that read the same operand panel next to each other:
+//   share_b = 0 (A panel shared: same descriptor group, same M tile)
+//           order [z / group][y][z % group][x];
+//   share_b = 1 (B panel shared: same descriptor, same N tile)
+//           order [z][x][y].
+struct GemmGrid {
+  int gx, gy, group, share_b;
+};
+
+__device__ __forceinline__ void gemm_block(const GemmGrid& g, int& x, int& y, int& z) {
+  const int hw = blockIdx.x, q = (int)(gridDim.x >> 3);
+  const int l = hw < 8 * q ? (hw & 7) * q + (hw >> 3) : hw;
+  if (g.share_b) {
+    y = l % g.gy;
+    const int t = l / g.gy;
+    x = t % g.gx;
+    z = t / g.gx;
+  } else {
+    x = l % g.gx;
+    int t = l / g.gx;
+    const int zg = t % g.group;
+    t /= g.group;
+    y = t % g.gy;
+    z = (t / g.gy) * g.group + zg;
+  }
+}
+
 template <int ROLE, typename T, int TM, int TN, typename TA, typename TB, typename TC>
 __global__ __launch_bounds__(256) void k_gemm_vec(const TA* __restrict__ A, const TB* __restrict__ B,
                                                   TC* __restrict__ C,
                                                   const GemmDesc* __restrict__ descs,
-                                                  GemmStrides st, KinParams kp) {
+                                                  GemmStrides st, KinParams kp, GemmGrid gg) {
   typedef MfmaAcc<T> M;
   constexpr bool AKC = RoleAKC<ROLE>::value;
   constexpr int BMt = 32 * TM, BNt = 32 * TN;
   constexpr int LA = AKC ? BK + 4 : BMt + 4;  // LDS row length of A (K-major or M-major)
   constexpr int LB = BNt + 4;
-  const GemmDesc d = descs[blockIdx.z];
-  const int m0 = blockIdx.y * BMt, n0 = blockIdx.x * BNt;
+  int bx, by, bz;
+  gemm_block(gg, bx, by, bz);
+  const GemmDesc d = descs[bz];
+  const int m0 = by * BMt, n0 = bx * BNt;
   if (m0 >= d.M || n0 >= d.N) return;
   __shared__ __align__(16) T As[AKC ? BMt * LA : BK * LA];
   __shared__ __align__(16) T Bs[BK * LB];
@@ -404,9 +436,11 @@ __global__ __launch_bounds__(256) void k_gemm_vec(const TA* __restrict__ A, cons
 template <int ROLE, int TM, int TN, typename TACC, typename TA, typename TB, typename TC>
 static void launch_gemm_vec(hipStream_t s, const TA* A, const TB* B, TC* C, const GemmPlan& p,
                             KinParams kp) {
-  dim3 grid((p.maxN + 32 * TN - 1) / (32 * TN), (p.maxM + 32 * TM - 1) / (32 * TM), p.nb);
-  hipLaunchKernelGGL((k_gemm_vec<ROLE, TACC, TM, TN, TA, TB, TC>), grid, dim3(256), 0, s, A, B, C,
-                     p.d_descs, p.st, kp);
+  GemmGrid gg{(p.maxN + 32 * TN - 1) / (32 * TN), (p.maxM + 32 * TM - 1) / (32 * TM), p.group,
+              p.share_b};
+  const unsigned nblk = (unsigned)gg.gx * (unsigned)gg.gy * (unsigned)p.nb;
+  hipLaunchKernelGGL((k_gemm_vec<ROLE, TACC, TM, TN, TA, TB, TC>), dim3(nblk), dim3(256), 0, s, A,
+                     B, C, p.d_descs, p.st, kp, gg);
 }
 
 template <int ROLE, typename TA, typename TB, typename TC, typename TACC>
@@ -828,6 +862,10 @@ static void plan_tiles(Ctx* c) {
       for (int i = 0; i < 5; i++)
         if (v[i] == 22 || v[i] == 24 || v[i] == 42 || v[i] == 44) t[i] = v[i];
   }
+  // R passes: the two parities of one (channel, angle) fold the same rows
+  c->p_dstR.group = 2;
+  c->p_dstR.share_b = 0;
+  c->p_dstr.share_b = c->p_dstr2.share_b = c->p_angF.share_b = c->p_angB.share_b = 1;
   c->p_angF.tile = t[0];
   c->p_angB.tile = t[1];
   c->p_dstR.tile = t[2];
