@@ -100,7 +100,7 @@ static hipError_t alloc_zero(void** p, size_t bytes) {
 }
 
 static void free_all(Ctx* c) {
-  void* ptrs[] = {c->d_pt_dense, c->d_mask, c->d_v, c->d_absfr1, c->d_absfr2, c->d_psi,
+  void* ptrs[] = {c->d_pt_dense, c->d_mask, c->d_mbits, c->d_v, c->d_absfr1, c->d_absfr2, c->d_psi,
                   c->d_psim1, c->d_psi0, c->d_a0, c->d_a0b, c->d_t1, c->d_t2, c->d_fbr,
                   c->d_ang, c->d_hout, c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi,
                   c->d_dst_tables, c->d_Dmat, c->d_acoef, c->d_bcoef, c->d_nbr,
@@ -177,6 +177,17 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "mw3_create: stream");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
+  {
+    // mask as 64-bit words: word (ia, R chunk rc, r) holds R = 64 rc + l in bit l
+    const int nrc = (t.n2 + 63) / 64;
+    std::vector<uint64_t> bits((size_t)t.na * nrc * t.n1, 0);
+    for (int ia = 0; ia < t.na; ia++)
+      for (int r = 0; r < t.n1; r++)
+        for (int R = 0; R < t.n2; R++)
+          if (t.mask[(size_t)ia * t.nplane + (size_t)r * t.n2 + R])
+            bits[((size_t)ia * nrc + R / 64) * t.n1 + r] |= 1ull << (R % 64);
+    CK(upload_raw(&c->d_mbits, bits.data(), bits.size()), "mw3_create: mask words");
+  }
   if (precision == MW3_MIXED)
     CK(upload<float>(&c->d_v, t.vdense), "mw3_create: V");
   else
@@ -238,7 +249,7 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(upload_raw(&c->d_fdv, t.fdv.data(), t.fdv.size()), "mw3_create: fd");
   CK(upload_raw(&c->d_dcvj, t.dcvj.data(), t.dcvj.size()), "mw3_create: D");
   CK(upload_raw(&c->d_cvj_skip, t.cvj_isref.data(), t.cvj_isref.size()), "mw3_create: flags");
-  c->nnorm_blocks = (int)std::min<int64_t>(4096, (int64_t)t.nc * t.na * t.n1);
+  c->nnorm_blocks = t.nc * t.na * ((t.n2 + 63) / 64);  // one per k_update block
   {
     std::vector<int32_t> fmap((size_t)t.nc * t.na * t.n2, -1);
     for (int64_t f = 0; f < t.nflux; f++) {
@@ -387,7 +398,7 @@ int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* aut
     int base = done - c->ring_fill;  // output index of ring slot 0
     // one step: H psi_k, update into psim1 (psi_{k+1}), projections, swap
     HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
-    HIPCHK(c, launch_update(c, 2, c->d_psi), "mw3_cheby_steps: update");
+    HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
     HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
     c->k += 1;
     HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");

@@ -109,6 +109,7 @@ struct Ctx {
   // device buffers
   uint32_t* d_pt_dense = nullptr;
   uint8_t* d_mask = nullptr;
+  uint64_t* d_mbits = nullptr;  // [na][ceil(n2/64)][n1] mask words (bit l: R = 64 chunk + l)
   void* d_v = nullptr;  // double or float dense V
   double* d_absfr1 = nullptr;
   double* d_absfr2 = nullptr;
@@ -169,7 +170,7 @@ struct Ctx {
 
   int32_t* d_fmap = nullptr;     // [nc][na][n2] flux point index or -1
   int32_t* d_ir1flux = nullptr;  // [na]
-  double* d_slice = nullptr;     // [nc][n1][na] psi at R = ir2abs
+  double* d_slice = nullptr;     // [nc][na][n1] psi at R = ir2abs / sqrt(ah2)
 
   // recursion state
   int k = 0;
@@ -191,7 +192,7 @@ hipError_t launch_scatter_a0(Ctx* c, const double* a0, double* dense);
 hipError_t launch_gather_a0(Ctx* c, const double* dense, double* a0);
 hipError_t launch_hpsi(Ctx* c, const double* psi);  // T, V and angular terms into work slabs
 // 0: H psi into d_hout, 1: first step, 2: damped step (psi_new into d_psim1)
-hipError_t launch_update(Ctx* c, int mode, const double* psi);
+hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj = false);
 hipError_t launch_projections(Ctx* c, int slot);
 hipError_t launch_energy_accum(Ctx* c);
 hipError_t launch_psie(Ctx* c, int k);

@@ -542,83 +542,172 @@ __global__ void k_fbr_coef(const T* __restrict__ fbr, T* __restrict__ phi,
 // mode 2: psi_new = f (2 (H psi - emin psi)/delta - f psi_m1)
 // psi_new overwrites psim1.  Partial sums of psi_new^2 and psi0.psi_new.
 // ---------------------------------------------------------------------------
+// Fused V psi + compressed-grid projection + damped update
+// (SRC/mod_Hphi_01y2.f:838-853, SRC/main_madwave3.f:245-248, 332-339) with the
+// per-step projections of the new psi (SRC/mod_flux_01y2.f:598-628, 672-702).
+// Block = one (channel, angle) and 64 consecutive R; its 8 waves take the r
+// rows r = w, w+8, ...; lane = R, so every row access is one contiguous run.
+// Points outside the compressed grid are neither read nor written: every
+// state buffer is zero there and stays zero.  mode 0: hout = H psi;
+// mode 1: out = H_norm psi (first step, undamped); mode 2: damped update.
+// proj: also sum zAR1flux(r) psi_new(r, R) and take psi_new(ir1flux, R) for the
+// flux columns, and psi_new(r, ir2abs) / sqrt(ah2) for the reactant slice.
+struct UpdArgs {
+  const uint64_t* mbits;
+  const int32_t* nn1;
+  const double* absfr1;
+  const double* absfr2;
+  const int32_t* ch_elec;
+  const int32_t* ch_omega;
+  const double* psi0;  // autocorrelation reference (null: none)
+  double* part;        // per-block (norm, autocorrelation) partials
+  int nc, ne, na, n1, n2;
+  double emindlt, rdelta2;
+  int mode, pflux;  // pflux: flux-column projections on
+  // projections (slice non-null: reactant slice on)
+  const int32_t* fmap;
+  const int32_t* ir1flux;
+  const double* zar;
+  double* zr;
+  double* zdr;
+  double* slice;
+  int ir2abs;
+  double rsah2;
+};
+
+constexpr int UPD_RL = 8;  // r lanes (waves) per block
+
 template <typename TT, typename TV>
-__global__ __launch_bounds__(256) void k_update(
-    const double* __restrict__ psi, double* __restrict__ psim1, double* __restrict__ hout,
-    const TT* __restrict__ tkin, const TT* __restrict__ ang, const TV* __restrict__ vd,
-    const uint8_t* __restrict__ mask, const int32_t* __restrict__ nn1,
-    const double* __restrict__ absfr1, const double* __restrict__ absfr2,
-    const int32_t* __restrict__ ch_elec, const int32_t* __restrict__ ch_omega,
-    const double* __restrict__ psi0, double* __restrict__ part, int nc, int ne, int na, int n1,
-    int n2, double emindlt, double delta2, int mode) {
-  const int64_t plane = (int64_t)n1 * n2, slab = (int64_t)na * plane;
-  const int nrows = nc * na * n1;
-  double s_norm = 0.0, s_auto = 0.0;
-  for (int row = blockIdx.x; row < nrows; row += gridDim.x) {
-    const int r = row % n1;
-    const int cia = row / n1;
-    const int ia = cia % na, c = cia / na;
-    const bool tk = r < nn1[ia];
-    const int et = ch_elec[c], om = ch_omega[c];
-    const double f1 = absfr1[r];
-    const int64_t rowoff = (int64_t)row * n2;           // (c, ia, r, 0)
-    const int64_t ploff = (int64_t)ia * plane + (int64_t)r * n2;
-    for (int R = threadIdx.x; R < n2; R += blockDim.x) {
-      const int64_t i = rowoff + R;
-      const int64_t pl = ploff + R;
-      if (!mask[pl]) {
-        if (mode == 0)
-          hout[i] = 0.0;
-        else
-          psim1[i] = 0.0;
-        continue;
-      }
-      double p = psi[i];
-      double h = tk ? (double)tkin[i] : p;
-      if (ne == 1 && nc == 1) {
-        h += p * (double)vd[pl];
-      } else {
-        for (int s = 0; s < nc; s++) {
-          if (ch_omega[s] != om) continue;
-          int es = ch_elec[s];
-          double v = (double)vd[((int64_t)et * ne + es) * slab + pl];
-          double ps = (s == c) ? p : psi[(int64_t)s * slab + pl];
-          h += ps * v;
-        }
-      }
-      h += (double)ang[i];
-      if (mode == 0) {
-        hout[i] = h;
-        continue;
-      }
-      double out;
-      if (mode == 1) {
-        out = (h - emindlt * p) / delta2;
-      } else {
-        double f = f1 * absfr2[R];
-        double hn = (h - emindlt * p) / delta2;
-        out = (2.0 * hn - f * psim1[i]) * f;
-      }
-      psim1[i] = out;
-      s_norm += out * out;
-      if (psi0) s_auto += psi0[i] * out;
-    }
+__global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict__ psi,
+                                                        double* __restrict__ out_buf,
+                                                        const TT* __restrict__ tkin,
+                                                        const TT* __restrict__ ang,
+                                                        const TV* __restrict__ vd, UpdArgs a) {
+  const int64_t plane = (int64_t)a.n1 * a.n2, slab = (int64_t)a.na * plane;
+  const int nrc = (a.n2 + 63) / 64;
+  const int cia = blockIdx.x / nrc, rc = blockIdx.x % nrc;
+  const int ia = cia % a.na, c = cia / a.na;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int R = rc * 64 + lane;
+  const bool inR = R < a.n2;
+  const int nn = a.nn1[ia];
+  const int et = a.ch_elec[c], om = a.ch_omega[c];
+  const double f2 = inR ? a.absfr2[R] : 0.0;
+  const int rflux = a.pflux ? a.ir1flux[ia] - 1 : -1;
+  const bool cvjcol = a.slice && R == a.ir2abs - 1;
+  double s_norm = 0.0, s_auto = 0.0, zre = 0.0, zim = 0.0, zval = 0.0;
+  const bool single = a.ne == 1 && a.nc == 1;
+  // this block's mask words, one per r row
+  extern __shared__ uint64_t smask[];
+  {
+    const uint64_t* mb = a.mbits + ((int64_t)ia * nrc + rc) * a.n1;
+    for (int r = threadIdx.x; r < a.n1; r += blockDim.x) smask[r] = mb[r];
   }
-  // block reduction
-  __shared__ double sh[2][256];
-  sh[0][threadIdx.x] = s_norm;
-  sh[1][threadIdx.x] = s_auto;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      sh[0][threadIdx.x] += sh[0][threadIdx.x + w];
-      sh[1][threadIdx.x] += sh[1][threadIdx.x + w];
+  if (inR) {
+    // rows r = w + UPD_RL (4 b + u): the loads of four rows are issued
+    // before any of them is used.  Lanes outside the compressed grid load
+    // element 0 instead (one shared cache line), which keeps the loads free of
+    // branches; their values are never used.
+    for (int rb = w; rb < a.n1; rb += 4 * UPD_RL) {
+      bool act[4];
+      int64_t pl[4];
+      double p[4], ob[4], tk[4], an[4], vv[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int r = min(rb + u * UPD_RL, a.n1 - 1);
+        pl[u] = (int64_t)ia * plane + (int64_t)r * a.n2 + R;
+        act[u] = rb + u * UPD_RL < a.n1 && ((smask[r] >> lane) & 1ull);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t ple = act[u] ? pl[u] : 0;
+        const int64_t ie = act[u] ? (int64_t)c * slab + pl[u] : 0;
+        p[u] = psi[ie];
+        tk[u] = (double)tkin[ie];
+        an[u] = (double)ang[ie];
+        vv[u] = single ? (double)vd[ple] : 0.0;
+        ob[u] = a.mode == 2 ? out_buf[ie] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (!act[u]) continue;
+        const int r = rb + u * UPD_RL;
+        const int64_t i = (int64_t)c * slab + pl[u];
+        double h = (r < nn ? tk[u] : p[u]) + an[u];
+        if (single) {
+          h += p[u] * vv[u];
+        } else {
+          for (int s2 = 0; s2 < a.nc; s2++) {
+            if (a.ch_omega[s2] != om) continue;
+            const int es = a.ch_elec[s2];
+            const double v = (double)vd[((int64_t)et * a.ne + es) * slab + pl[u]];
+            const double ps = (s2 == c) ? p[u] : psi[(int64_t)s2 * slab + pl[u]];
+            h += ps * v;
+          }
+        }
+        if (a.mode == 0) {
+          out_buf[i] = h;
+          continue;
+        }
+        const double hn = (h - a.emindlt * p[u]) * a.rdelta2;
+        double o;
+        if (a.mode == 1) {
+          o = hn;
+        } else {
+          const double f = a.absfr1[r] * f2;
+          o = (2.0 * hn - f * ob[u]) * f;
+        }
+        out_buf[i] = o;
+        s_norm += o * o;
+        if (a.psi0) s_auto += a.psi0[i] * o;
+        if (a.pflux) {
+          zre += a.zar[2 * r] * o;
+          zim += a.zar[2 * r + 1] * o;
+          if (r == rflux) zval = o;
+        }
+        if (cvjcol) a.slice[((int64_t)c * a.na + ia) * a.n1 + r] = o * a.rsah2;
+      }
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0 && part) {
-    part[2 * blockIdx.x] = sh[0][0];
-    part[2 * blockIdx.x + 1] = sh[1][0];
+  __shared__ double sh[5][UPD_RL][64];
+  sh[0][w][lane] = s_norm;
+  sh[1][w][lane] = s_auto;
+  if (a.pflux) {
+    sh[2][w][lane] = zre;
+    sh[3][w][lane] = zim;
+    sh[4][w][lane] = zval;
+  }
+  __syncthreads();
+  if (w == 0) {
+    double n0 = 0, a0 = 0, zr0 = 0, zi0 = 0, zv = 0;
+#pragma unroll
+    for (int q = 0; q < UPD_RL; q++) {
+      n0 += sh[0][q][lane];
+      a0 += sh[1][q][lane];
+      if (a.pflux) {
+        zr0 += sh[2][q][lane];
+        zi0 += sh[3][q][lane];
+        zv += sh[4][q][lane];
+      }
+    }
+    if (a.pflux && inR) {
+      const int f = a.fmap[(int64_t)cia * a.n2 + R];
+      if (f >= 0) {
+        a.zdr[2 * (int64_t)f] = zr0;
+        a.zdr[2 * (int64_t)f + 1] = zi0;
+        a.zr[f] = zv;
+      }
+    }
+    // wave reduction of the norm partials
+    for (int off = 32; off > 0; off >>= 1) {
+      n0 += __shfl_xor(n0, off);
+      a0 += __shfl_xor(a0, off);
+    }
+    if (lane == 0 && a.part) {
+      a.part[2 * blockIdx.x] = n0;
+      a.part[2 * blockIdx.x + 1] = a0;
+    }
   }
 }
 
@@ -646,83 +735,48 @@ __global__ void k_reduce_parts(const double* __restrict__ part, int nb, double* 
 }
 
 // ---------------------------------------------------------------------------
-// per-step projections into the ring
+// per-step projections into the ring.  The flux line (every (c, ia, R)
+// column: zdR = sum_r zAR1flux(r) psi(c,ia,r,R), the full-state derivative of
+// totalflux_k, SRC/mod_flux_01y2.f:598-609, and zR = psi at r = ir1flux(ia),
+// stored at flux points f = fmap(c,ia,R) >= 0) and the reactant slice at
+// R = ir2abs are taken inside k_update.
 // ---------------------------------------------------------------------------
-// flux line: for every (c, ia, R) column, zdR = sum_r zAR1flux(r) psi(c,ia,r,R)
-// (the full-state derivative of totalflux_k, SRC/mod_flux_01y2.f:598-609);
-// flux points f = fmap(c,ia,R) >= 0 store zR = psi at r = ir1flux(ia) and zdR.
-// Block = (64 R values, c*na + ia), 4 lanes over r.
-__global__ __launch_bounds__(256) void k_flux_cols(const double* __restrict__ psi,
-                                                   const int32_t* __restrict__ fmap,
-                                                   const int32_t* __restrict__ ir1flux,
-                                                   const double* __restrict__ zar,
-                                                   double* __restrict__ zr, double* __restrict__ zdr,
-                                                   int na, int n1, int n2) {
-  const int cia = blockIdx.y;
-  const int ia = cia % na;
-  const int lane = threadIdx.x >> 6;
-  const int R = blockIdx.x * 64 + (threadIdx.x & 63);
-  __shared__ double sre[4][64], sim[4][64];
-  double re = 0, im = 0;
-  const double* col = psi + (int64_t)cia * n1 * n2 + R;
-  if (R < n2)
-    for (int r = lane; r < n1; r += 4) {
-      double v = col[(int64_t)r * n2];
-      re += zar[2 * r] * v;
-      im += zar[2 * r + 1] * v;
-    }
-  sre[lane][threadIdx.x & 63] = re;
-  sim[lane][threadIdx.x & 63] = im;
-  __syncthreads();
-  if (lane == 0 && R < n2) {
-    int f = fmap[(int64_t)cia * n2 + R];
-    if (f >= 0) {
-      int t = threadIdx.x & 63;
-      zdr[2 * (int64_t)f] = (sre[0][t] + sre[1][t]) + (sre[2][t] + sre[3][t]);
-      zdr[2 * (int64_t)f + 1] = (sim[0][t] + sim[1][t]) + (sim[2][t] + sim[3][t]);
-      zr[f] = col[(int64_t)(ir1flux[ia] - 1) * n2];
-    }
-  }
-}
-
-// reactant-state projection (Cvjflux_k, SRC/mod_flux_01y2.f:672-702)
-// slice[c][r][ia] = psi(c, ia, r, ir2abs) / sqrt(ah2)
-__global__ void k_cvj_slice(const double* __restrict__ psi, double* __restrict__ slice, int nc,
-                            int na, int n1, int n2, int ir2abs, double rsah2) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc * na * n1) return;
-  int ia = i % na, r = (i / na) % n1, c = i / (na * n1);
-  slice[i] = psi[(((int64_t)c * na + ia) * n1 + r) * n2 + (ir2abs - 1)] * rsah2;
-}
-
-// Cvj(c, j, v) = sum_r fd(r, v, j) sum_ia slice(c, r, ia) D(ia, j); block per (c, jj)
-__global__ __launch_bounds__(256) void k_cvj(const double* __restrict__ slice,
-                                             const double* __restrict__ dcvj,
-                                             const double* __restrict__ fdv, double* __restrict__ cvj,
-                                             int nj, int nv, int na, int n1) {
-  int c = blockIdx.x / nj, jj = blockIdx.x % nj;
-  extern __shared__ double xj[];
+// Cvj(c, j, v) = sum_r fd(r, v, j) sum_ia slice(c, ia, r) D(ia, j); block per (c, jj)
+// (Cvjflux_k, SRC/mod_flux_01y2.f:672-702; slice = psi at R = ir2abs / sqrt(ah2)).
+// 16 waves: the angle sum is split over the waves (lanes over r), then each
+// wave takes some of the v sums.
+constexpr int CVJ_WAVES = 16;
+__global__ __launch_bounds__(64 * CVJ_WAVES) void k_cvj(const double* __restrict__ slice,
+                                                        const double* __restrict__ dcvj,
+                                                        const double* __restrict__ fdv,
+                                                        double* __restrict__ cvj, int nj, int nv,
+                                                        int na, int n1) {
+  const int c = blockIdx.x / nj, jj = blockIdx.x % nj;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  extern __shared__ double xj[];  // [n1]
+  __shared__ double part[CVJ_WAVES][64];
   const double* dj = dcvj + ((int64_t)c * nj + jj) * na;
-  for (int r = threadIdx.x; r < n1; r += blockDim.x) {
-    const double* sl = slice + ((int64_t)c * n1 + r) * na;
+  const double* sl = slice + (int64_t)c * na * n1;
+  for (int r0 = 0; r0 < n1; r0 += 64) {
+    const int r = r0 + lane;
     double s = 0;
-    for (int ia = 0; ia < na; ia++) s += sl[ia] * dj[ia];
-    xj[r] = s;
+    if (r < n1)
+      for (int ia = wv; ia < na; ia += CVJ_WAVES) s += sl[(int64_t)ia * n1 + r] * dj[ia];
+    part[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && r < n1) {
+      double t = 0;
+      for (int q = 0; q < CVJ_WAVES; q++) t += part[q][lane];
+      xj[r] = t;
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  __shared__ double red[256];
-  for (int v = 0; v < nv; v++) {
+  for (int v = wv; v < nv; v += CVJ_WAVES) {
     const double* f = fdv + (((int64_t)c * nj + jj) * nv + v) * n1;
     double s = 0;
-    for (int r = threadIdx.x; r < n1; r += blockDim.x) s += xj[r] * f[r];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) cvj[((int64_t)c * nj + jj) * nv + v] = red[0];
-    __syncthreads();
+    for (int r = lane; r < n1; r += 64) s += xj[r] * f[r];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) cvj[((int64_t)c * nj + jj) * nv + v] = s;
   }
 }
 
@@ -1046,15 +1100,42 @@ hipError_t launch_hpsi(Ctx* c, const double* psi) {
 }
 
 template <typename T>
-static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi) {
+static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, bool proj) {
+  // psi_new rows that are never written (outside the compressed grid) keep
+  // zeros, so the reactant slice must start cleared only once (at create)
   const HostTables& t = c->t;
-  int nb = c->nnorm_blocks;
+  const int nb = c->nnorm_blocks;
+  UpdArgs a{};
+  a.mbits = c->d_mbits;
+  a.nn1 = c->d_nn1;
+  a.absfr1 = c->d_absfr1;
+  a.absfr2 = c->d_absfr2;
+  a.ch_elec = c->d_ch_elec;
+  a.ch_omega = c->d_ch_omega;
+  a.psi0 = (c->s.iphoto >= 1) ? c->d_psi0 : nullptr;
+  a.part = mode != 0 ? c->d_norm_part : nullptr;
+  a.nc = t.nc;
+  a.ne = t.ne;
+  a.na = t.na;
+  a.n1 = t.n1;
+  a.n2 = t.n2;
+  a.emindlt = c->s.emindlt;
+  a.rdelta2 = 1.0 / c->s.delta2;
+  a.mode = mode;
+  a.pflux = (proj && t.nflux > 0 && c->d_fmap) ? 1 : 0;
+  a.fmap = c->d_fmap;
+  a.ir1flux = c->d_ir1flux;
+  a.zar = c->d_zar1flux;
+  a.zr = c->d_ring_zr + (int64_t)slot * t.nflux;
+  a.zdr = c->d_ring_zdr + (int64_t)2 * slot * t.nflux;
+  a.slice = (proj && t.ncvj > 0) ? c->d_slice : nullptr;
+  a.ir2abs = c->s.ir2abs;
+  a.rsah2 = 1.0 / std::sqrt(c->s.ah2);
   timer_begin(c, K_UPDATE);
-  hipLaunchKernelGGL((k_update<T, T>), dim3(nb), dim3(256), 0, c->stream, psi, c->d_psim1,
-                     c->d_hout, (const T*)c->d_t2, (const T*)c->d_ang, (const T*)c->d_v, c->d_mask,
-                     c->d_nn1, c->d_absfr1, c->d_absfr2, c->d_ch_elec, c->d_ch_omega,
-                     (c->s.iphoto >= 1) ? c->d_psi0 : nullptr, c->d_norm_part, t.nc, t.ne, t.na,
-                     t.n1, t.n2, c->s.emindlt, c->s.delta2, mode);
+  double* out = mode == 0 ? c->d_hout : c->d_psim1;
+  hipLaunchKernelGGL((k_update<T, T>), dim3(nb), dim3(64 * UPD_RL), sizeof(uint64_t) * t.n1,
+                     c->stream, psi, out,
+                     (const T*)c->d_t2, (const T*)c->d_ang, (const T*)c->d_v, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (mode != 0) {
@@ -1066,37 +1147,25 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi) {
   return e;
 }
 
-hipError_t launch_update(Ctx* c, int mode, const double* psi) {
+hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj) {
   int slot = c->ring_fill;
-  if (c->precision == MW3_MIXED) return update_impl<float>(c, mode, slot, psi);
-  return update_impl<double>(c, mode, slot, psi);
+  if (c->precision == MW3_MIXED) return update_impl<float>(c, mode, slot, psi, proj);
+  return update_impl<double>(c, mode, slot, psi, proj);
 }
 
-// projections of the new psi (in d_psim1 before the swap) into ring slot
+// reactant-state amplitudes of the slice the update wrote, into ring slot
 hipError_t launch_projections(Ctx* c, int slot) {
   const HostTables& t = c->t;
   hipError_t e = hipSuccess;
-  timer_begin(c, K_PROJ);
-  const double* psi = c->d_psim1;
-  if (t.nflux > 0) {
-    dim3 g((unsigned)((t.n2 + 63) / 64), (unsigned)(t.nc * t.na));
-    hipLaunchKernelGGL(k_flux_cols, g, dim3(256), 0, c->stream, psi, c->d_fmap, c->d_ir1flux,
-                       c->d_zar1flux, c->d_ring_zr + (int64_t)slot * t.nflux,
-                       c->d_ring_zdr + (int64_t)2 * slot * t.nflux, t.na, t.n1, t.n2);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
   if (t.ncvj > 0) {
+    timer_begin(c, K_PROJ);
     int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
-    int ns = t.nc * t.na * t.n1;
-    hipLaunchKernelGGL(k_cvj_slice, dim3(nblk(ns, 256)), dim3(256), 0, c->stream, psi,
-                       c->d_slice, t.nc, t.na, t.n1, t.n2, c->s.ir2abs, 1.0 / std::sqrt(c->s.ah2));
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(256), sizeof(double) * t.n1, c->stream,
+    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(64 * CVJ_WAVES), sizeof(double) * t.n1, c->stream,
                        c->d_slice, c->d_dcvj, c->d_fdv, c->d_ring_cvj + (int64_t)slot * t.ncvj,
                        nj, nv, t.na, t.n1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    timer_end(c, K_PROJ);
   }
-  timer_end(c, K_PROJ);
   return e;
 }
 
