@@ -38,6 +38,10 @@ def build_case(cfg):
     return model.build_case(NL.parse(text), pot, fdat)
 
 
+# kernel launches per step in each group (mixed and FP64 alike)
+GROUP_LAUNCHES = {"radial_dst": 4, "angular": 3, "update_fused": 2, "projections": 1}
+
+
 def kernel_groups(times):
     g = {"radial_dst": 0.0, "angular": 0.0, "update_fused": 0.0, "projections": 0.0}
     n = {}
@@ -122,7 +126,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -175,7 +179,10 @@ def main():
         d = kern[dom]
         roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],
                 "unit": d["unit"], "frac": d["frac"], "traffic": load_traffic(dom),
-                "kernel": dom}
+                "kernel": dom, "launches_per_unit": GROUP_LAUNCHES[dom],
+                "unit_of_work": "one Chebyshev step of the group (the scope table's per-step "
+                                "formula); traffic = HBM bytes per step from the committed "
+                                "byte-counter summary"}
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(case, a.cpu_steps)
