@@ -53,6 +53,38 @@ def build(verbose=False, extra_flags=()):
     return OUT
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FDRIVE = os.path.join(FDIR, "mw3_fdrive")
+
+
+def build_fortran(verbose=False):
+    """Compile the ISO_C_BINDING module mw3_capi and the Fortran host program
+    mw3_fdrive against libmw3.so (amdflang)."""
+    objdir = os.path.join(HERE, "_build", "fortran")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in ("mw3_capi.f90", "mw3_fdrive.f90"):
+        obj = os.path.join(objdir, src.replace(".f90", ".o"))
+        cmd = [FLANG, "-O2", "-fPIC", "-module-dir", objdir, "-c", os.path.join(FDIR, src),
+               "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("amdflang failed for %s:\n%s%s" % (src, r.stdout, r.stderr))
+        objs.append(obj)
+    tmp = FDRIVE + ".tmp"
+    cmd = [FLANG, "-o", tmp] + objs + ["-L" + HERE, "-lmw3", "-L/opt/rocm/lib",
+                                       "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran link failed:\n%s%s" % (r.stdout, r.stderr))
+    os.replace(tmp, FDRIVE)
+    if verbose:
+        print("built", FDRIVE)
+    return FDRIVE
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_fortran(verbose=True)
     sys.exit(0)

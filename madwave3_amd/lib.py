@@ -115,6 +115,51 @@ def make_setup(c):
     return s, keep
 
 
+def write_case_file(path, case, psi0, nsteps, precision):
+    """Write the case file read by the Fortran host madwave3_amd/fortran/mw3_fdrive
+    (format in its header): the mw3_setup members in declaration order, pointer
+    members as an int64 count and their elements, then psi0, nsteps and the
+    precision."""
+    s, keep = make_setup(case)
+    by_addr = {a.ctypes.data: a for a in keep}
+    out = [b"MW3D", np.int32(1).tobytes()]
+    for name, typ in Mw3Setup._fields_:
+        v = getattr(s, name)
+        if typ is ct.c_int32:
+            out.append(np.int32(v).tobytes())
+        elif typ is ct.c_double:
+            out.append(np.float64(v).tobytes())
+        else:
+            addr = ct.cast(v, ct.c_void_p).value
+            a = by_addr.get(addr) if addr else None
+            if a is None:
+                out.append(np.int64(0).tobytes())
+            else:
+                a = np.ascontiguousarray(a).ravel()
+                out.append(np.int64(a.size).tobytes())
+                out.append(a.astype(np.int32 if typ is _pi else np.float64).tobytes())
+    psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+    out.append(np.int64(psi0.size).tobytes())
+    out.append(psi0.tobytes())
+    out.append(np.int32(nsteps).tobytes())
+    out.append(np.int32(precision).tobytes())
+    with open(path, "wb") as f:
+        f.write(b"".join(out))
+
+
+def read_fdrive_result(path):
+    """(norms, P, zS2 interleaved) from a mw3_fdrive result file."""
+    b = open(path, "rb").read()
+    o = 0
+    ns = int(np.frombuffer(b, np.int32, 1, o)[0]); o += 4
+    norms = np.frombuffer(b, np.float64, ns, o).copy(); o += 8 * ns
+    ne = int(np.frombuffer(b, np.int32, 1, o)[0]); o += 4
+    P = np.frombuffer(b, np.float64, ne, o).copy(); o += 8 * ne
+    m = int(np.frombuffer(b, np.int64, 1, o)[0]); o += 8
+    z = np.frombuffer(b, np.float64, m, o).copy()
+    return norms, P, z
+
+
 class Mw3Error(RuntimeError):
     pass
 

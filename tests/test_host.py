@@ -139,3 +139,54 @@ def test_initial_packet_matches_colini_form():
     D = c.Djmmp[0, 0, c.jref, c.indangreal[sl] - 1]
     fd = c.fd[0, c.jref - c.jini, c.nvref - c.nvini, i1 - 1]
     assert np.max(np.abs(c.rpaq0 - (-g * D * fd * np.sqrt(c.ah2)))) < 1e-15
+
+
+FLANG = "/opt/rocm/llvm/bin/amdflang"
+FMODDIR = os.path.join(ROOT, "madwave3_amd", "_build", "fortran")
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(FMODDIR, "mw3_capi.mod")),
+                    reason="Fortran module not built (run __graft_entry__.build())")
+def test_fortran_setup_type_layout_matches_header():
+    """type(mw3_setup), bind(C) of module mw3_capi has the C struct's member
+    offsets and size (the ctypes mirror is checked against gcc above)."""
+    fields = [f[0] for f in Mw3Setup._fields_]
+    src = ["program layout", "  use, intrinsic :: iso_c_binding", "  use mw3_capi",
+           "  implicit none", "  type(mw3_setup), target :: s", "  integer(c_intptr_t) :: b",
+           "  b = transfer(c_loc(s), b)"]
+    for f in fields:
+        src.append("  print '(a,1x,i0)', '%s', transfer(c_loc(s%%%s), b) - b" % (f, f))
+    src.append("  print '(a,1x,i0)', 'sizeof', c_sizeof(s)")
+    src.append("end program layout")
+    with tempfile.TemporaryDirectory() as d:
+        f90 = os.path.join(d, "layout.f90")
+        exe = os.path.join(d, "layout")
+        open(f90, "w").write("\n".join(src) + "\n")
+        # the program uses only the derived type: no library symbols to link
+        subprocess.run([FLANG, "-I", FMODDIR, "-o", exe, f90], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(l.split() for l in out if l.strip())
+    for f in fields:
+        assert int(got[f]) == getattr(Mw3Setup, f).offset, f
+    assert int(got["sizeof"]) == ct.sizeof(Mw3Setup)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "madwave3_amd", "fortran", "mw3_fdrive")),
+                    reason="Fortran host not built (run __graft_entry__.build())")
+def test_fortran_host_reads_case_file_and_reports_library_status(tmp_path):
+    """The Fortran host reads a case file written by lib.write_case_file and
+    calls mw3_create; without a device it stops with the library's message
+    (the reference's write-and-stop convention)."""
+    from madwave3_amd.lib import MW3_FP64, write_case_file
+
+    c = case("j0")
+    cf = str(tmp_path / "case.bin")
+    rf = str(tmp_path / "res.bin")
+    write_case_file(cf, c, c.rpaq0, 3, MW3_FP64)
+    r = subprocess.run([os.path.join(ROOT, "madwave3_amd", "fortran", "mw3_fdrive"), cf, rf],
+                       capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        assert "mw3_create: libmw3 status -3" in r.stdout, r.stdout + r.stderr
+        assert "no HIP device" in r.stdout
+    else:  # a device is visible: the run completed
+        assert os.path.exists(rf)
