@@ -190,3 +190,16 @@ def test_fortran_host_reads_case_file_and_reports_library_status(tmp_path):
         assert "no HIP device" in r.stdout
     else:  # a device is visible: the run completed
         assert os.path.exists(rf)
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/SRC/main_madwave3.f"),
+                    reason="reference tree not present")
+def test_fortran_dropin_compiles_with_the_reference_main():
+    """The replacement mod_Hphi_01y2 and mod_mw3_cheby compile in the reference's
+    module chain, and the reference's unmodified main_madwave3.f and the modules
+    that use mod_Hphi_01y2 compile against them (compile only; oracle/Makefile
+    target shimcheck)."""
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "shimcheck"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "main_madwave3.f compiled" in r.stdout
