@@ -42,6 +42,9 @@ struct HostTables {
   std::vector<int32_t> dst_sizes;        // distinct DST lengths (r sizes + n2)
   std::vector<int64_t> dst_table_off;    // offset of each size's S_e block
   std::vector<int> dst_ld;               // leading dimension of each size's blocks
+  std::vector<int64_t> low_off;          // offset of each size's low-mode table L[k][j] = S(j,k)
+  std::vector<int> low_ld;               // its row stride
+  std::vector<int32_t> k1lo, k2lo;       // [na] r / R modes with p2r1 + p2r2 < radcut possible
   std::vector<double> dst_tables;
   // angular FBR
   std::vector<int32_t> jlist;   // [nJ] j values of the FBR axis
@@ -75,7 +78,7 @@ struct GemmDesc {
   int64_t a_off, b_off, c_off;
   int M, N, K, lda_m;  // A element (m,k) at a_off + m*lda_m + k*lda_k
   int lda_k, fold_op, fold_n, fold_sgn;
-  int kin_ia, rmul, radd, pad0;
+  int kin_ia, rmul, radd, kin_mode;  // kin_mode 0: min(E, c) divi; 1: (c - E)_+ divi
 };
 
 struct GemmStrides {
@@ -159,7 +162,7 @@ struct Ctx {
   std::vector<double> host_norms;
 
   // GEMM passes: R rows, r columns, angular forward, angular back
-  GemmPlan p_dstR, p_dstr, p_dstr2, p_angF, p_angB;
+  GemmPlan p_dstR, p_dstr, p_dstr2, p_dstRi, p_angF, p_angB;
   void* d_phi = nullptr;        // FBR after the Omega stencil
   double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors
   uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)

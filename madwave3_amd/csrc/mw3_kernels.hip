@@ -52,10 +52,18 @@ struct KinParams {
   int n1;
 };
 
+// radial kinetic factor of mode (k1 = rmul m + radd, k2 = n) of angle kin_ia:
+// kin_mode 0: min(E, radcut) divi (Tradial, SRC/mod_Hphi_01y2.f:770-771);
+// kin_mode 1: (radcut - E)_+ divi, the low-mode correction of c psi
 __device__ __forceinline__ double kin_factor(const GemmDesc& d, const KinParams& kp, int m, int n) {
   int k1 = d.rmul * m + d.radd;
   double e = kp.p2r1[(int64_t)d.kin_ia * kp.n1 + k1] + kp.p2r2[n];
-  if (e > kp.radcut) e = kp.radcut;
+  if (d.kin_mode) {
+    e = kp.radcut - e;
+    if (e < 0.0) e = 0.0;
+  } else if (e > kp.radcut) {
+    e = kp.radcut;
+  }
   return e * kp.divi[d.kin_ia];
 }
 
@@ -446,6 +454,7 @@ static void launch_gemm_vec(hipStream_t s, const TA* A, const TB* B, TC* C, cons
 template <int ROLE, typename TA, typename TB, typename TC, typename TACC>
 static hipError_t run_gemm(hipStream_t s, const TA* A, const TB* B, TC* C, const GemmPlan& p,
                            KinParams kp = KinParams{}) {
+  if (p.nb == 0 || p.maxM == 0 || p.maxN == 0) return hipSuccess;  // nothing to write
   if (!p.vec) {
     dim3 grid((p.maxN + 63) / 64, (p.maxM + 63) / 64, p.nb);
     hipLaunchKernelGGL((k_gemm_mfma<ROLE, TACC, 2, 2, TA, TB, TC>), grid, dim3(256), 0, s, A, B, C,
@@ -562,7 +571,7 @@ struct UpdArgs {
   const double* psi0;  // autocorrelation reference (null: none)
   double* part;        // per-block (norm, autocorrelation) partials
   int nc, ne, na, n1, n2;
-  double emindlt, rdelta2;
+  double emindlt, rdelta2, radcut;
   int mode, pflux;  // pflux: flux-column projections on
   // projections (slice non-null: reactant slice on)
   const int32_t* fmap;
@@ -634,7 +643,8 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
         if (!act[u]) continue;
         const int r = rb + u * UPD_RL;
         const int64_t i = (int64_t)c * slab + pl[u];
-        double h = (r < nn ? tk[u] : p[u]) + an[u];
+        // T psi = radcut psi - low-mode correction on r < nn1; psi beyond (Q1)
+        double h = (r < nn ? a.radcut * p[u] - tk[u] : p[u]) + an[u];
         if (single) {
           h += p[u] * vv[u];
         } else {
@@ -916,10 +926,12 @@ static void plan_tiles(Ctx* c) {
       for (int i = 0; i < 5; i++)
         if (v[i] == 22 || v[i] == 24 || v[i] == 42 || v[i] == 44) t[i] = v[i];
   }
-  // R passes: the two parities of one (channel, angle) fold the same rows
+  // R pass (1): the two parities of one (channel, angle) fold the same rows
   c->p_dstR.group = 2;
   c->p_dstR.share_b = 0;
+  c->p_dstRi.share_b = 0;
   c->p_dstr.share_b = c->p_dstr2.share_b = c->p_angF.share_b = c->p_angB.share_b = 1;
+  c->p_dstRi.tile = t[2];
   c->p_angF.tile = t[0];
   c->p_angB.tile = t[1];
   c->p_dstR.tile = t[2];
@@ -942,7 +954,9 @@ hipError_t build_plans(Ctx* c) {
     while (t.dst_sizes[q] != n) q++;
     return q;
   };
-  // R rows (folded): per (c, ia), rows r < nn1, even and odd output columns
+  // Radial kinetic as c psi - S (c - E)_+ divi S psi on the low modes
+  // k1 < k1lo(ia), k2 < k2lo(ia) (see the low-mode tables in mw3_setup.cpp).
+  // (1) R rows, folded: t1[r][k2] = sum_R S(R, k2) psi[r][R], k2 < k2lo, r < nn1
   {
     const int n = t.n2, h = n / 2, ke = n - h;
     const int q = tab_q(n), kp = t.dst_ld[q];
@@ -951,12 +965,13 @@ hipError_t build_plans(Ctx* c) {
     for (int ch = 0; ch < t.nc; ch++)
       for (int ia = 0; ia < t.na; ia++)
         for (int par = 0; par < 2; par++) {
+          const int k2 = t.k2lo[ia];
           GemmDesc g = gd();
           g.a_off = (int64_t)ch * slab + (int64_t)ia * plane;
           g.b_off = toff + (par ? (int64_t)kp * kp : 0);
           g.c_off = g.a_off + par;
           g.M = t.nn1[ia];
-          g.N = par ? h : ke;
+          g.N = par ? k2 / 2 : (k2 + 1) / 2;  // modes 2k'+par < k2lo
           g.K = par ? h : ke;
           g.lda_m = n;
           g.lda_k = 1;
@@ -968,14 +983,16 @@ hipError_t build_plans(Ctx* c) {
     c->p_dstR.st = GemmStrides{kp, 1, n, 2};
     if ((e = upload_plan(&c->p_dstR, d, true)) != hipSuccess) return e;
   }
-  // r columns (folded, scaled): per (c, ia) rows k1 of the nn1 transform
-  for (int pass = 0; pass < 2; pass++) {
+  // (2) r columns, folded, times (c - E)_+ divi:
+  //     t2[k1][k2] = f(k1, k2) sum_r S(r, k1) t1[r][k2], k1 < k1lo
+  {
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++)
       for (int ia = 0; ia < t.na; ia++) {
         const int n = t.nn1[ia], h = n / 2, ke = n - h;
         const int q = tab_q(n), kp = t.dst_ld[q];
         const int64_t toff = t.dst_table_off[q];
+        const int k1 = t.k1lo[ia];
         for (int par = 0; par < 2; par++) {
           GemmDesc g = gd();
           // A[k1'][j1] = S[2k1'+par][j1] = S_par[j1][k1']
@@ -984,23 +1001,62 @@ hipError_t build_plans(Ctx* c) {
           g.lda_k = kp;
           g.b_off = (int64_t)ch * slab + (int64_t)ia * plane;
           g.c_off = g.b_off + (int64_t)par * t.n2;
-          g.M = par ? h : ke;
-          g.N = t.n2;
+          g.M = par ? k1 / 2 : (k1 + 1) / 2;
+          g.N = t.k2lo[ia];
           g.K = par ? h : ke;
           g.fold_op = 2;
           g.fold_n = n;
           g.fold_sgn = par ? -1 : 1;
-          if (pass == 0) {
-            g.kin_ia = ia;
-            g.rmul = 2;
-            g.radd = par;
-          }
+          g.kin_ia = ia;
+          g.rmul = 2;
+          g.radd = par;
+          g.kin_mode = 1;
           d.push_back(g);
         }
       }
-    GemmPlan* pl = pass == 0 ? &c->p_dstr : &c->p_dstr2;
-    pl->st = GemmStrides{t.n2, 1, 2 * (int64_t)t.n2, 1};
-    if ((e = upload_plan(pl, d, false)) != hipSuccess) return e;
+    c->p_dstr.st = GemmStrides{t.n2, 1, 2 * (int64_t)t.n2, 1};
+    if ((e = upload_plan(&c->p_dstr, d, false)) != hipSuccess) return e;
+  }
+  // (3) back over r: t1[r][k2] = sum_{k1 < k1lo} S(r, k1) t2[k1][k2], r < nn1
+  {
+    std::vector<GemmDesc> d;
+    for (int ch = 0; ch < t.nc; ch++)
+      for (int ia = 0; ia < t.na; ia++) {
+        const int q = tab_q(t.nn1[ia]);
+        GemmDesc g = gd();
+        g.a_off = t.low_off[q];  // A[r][k1] = L[k1][r]
+        g.lda_m = 1;
+        g.lda_k = t.low_ld[q];
+        g.b_off = (int64_t)ch * slab + (int64_t)ia * plane;
+        g.c_off = g.b_off;
+        g.M = t.nn1[ia];
+        g.N = t.k2lo[ia];
+        g.K = t.k1lo[ia];
+        d.push_back(g);
+      }
+    c->p_dstr2.st = GemmStrides{t.n2, 1, t.n2, 1};
+    if ((e = upload_plan(&c->p_dstr2, d, false)) != hipSuccess) return e;
+  }
+  // (4) back over R: t2[r][R] = sum_{k2 < k2lo} t1[r][k2] S(k2, R) (the
+  //     correction subtracted from c psi by the update)
+  {
+    const int q = tab_q(t.n2);
+    std::vector<GemmDesc> d;
+    for (int ch = 0; ch < t.nc; ch++)
+      for (int ia = 0; ia < t.na; ia++) {
+        GemmDesc g = gd();
+        g.a_off = (int64_t)ch * slab + (int64_t)ia * plane;
+        g.lda_m = t.n2;
+        g.lda_k = 1;
+        g.b_off = t.low_off[q];
+        g.c_off = g.a_off;
+        g.M = t.nn1[ia];
+        g.N = t.n2;
+        g.K = t.k2lo[ia];
+        d.push_back(g);
+      }
+    c->p_dstRi.st = GemmStrides{t.low_ld[q], 1, t.n2, 1};
+    if ((e = upload_plan(&c->p_dstRi, d, true)) != hipSuccess) return e;
   }
   // angular forward: fbr[c][jj][p] = sum_ia D_c(ia, jj) psi[c][ia][p]
   {
@@ -1044,7 +1100,7 @@ hipError_t build_plans(Ctx* c) {
 
 void free_plans(Ctx* c) {
 
-  GemmPlan* ps[5] = {&c->p_dstR, &c->p_dstr, &c->p_dstr2, &c->p_angF, &c->p_angB};
+  GemmPlan* ps[6] = {&c->p_dstR, &c->p_dstr, &c->p_dstr2, &c->p_dstRi, &c->p_angF, &c->p_angB};
   for (auto* p : ps)
     if (p->d_descs) {
       (void)hipFree(p->d_descs);
@@ -1074,8 +1130,9 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   e = run_gemm<G_ANG_BWD, T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
   if (e != hipSuccess) return e;
   timer_end(c, K_ANG_BWD);
-  // radial kinetic: folded DST over R, folded DST over r with the kinetic
-  // factor in the epilogue, folded DST over r, folded DST over R
+  // radial kinetic correction on the low modes: folded DST over R, folded DST
+  // over r times (c - E)_+ divi, back over r, back over R into d_t2; the
+  // update forms T psi = radcut psi - d_t2 on rows r < nn1
   KinParams kp{c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.n1};
   timer_begin(c, K_DST_R);
   e = run_gemm<G_DST_R, double, T, T, T>(s, psi, tab, t1, c->p_dstR);
@@ -1088,7 +1145,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_r);
   timer_begin(c, K_DST_RINV);
-  e = run_gemm<G_DST_R_INV, T, T, T, T>(s, t1, tab, t2, c->p_dstR);
+  e = run_gemm<G_DST_R_INV, T, T, T, T>(s, t1, tab, t2, c->p_dstRi);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_RINV);
   return hipSuccess;
@@ -1121,6 +1178,7 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   a.n2 = t.n2;
   a.emindlt = c->s.emindlt;
   a.rdelta2 = 1.0 / c->s.delta2;
+  a.radcut = t.radcut;
   a.mode = mode;
   a.pflux = (proj && t.nflux > 0 && c->d_fmap) ? 1 : 0;
   a.fmap = c->d_fmap;

@@ -252,6 +252,45 @@ int build_tables(const mw3_setup* s, HostTables* t, std::string* err) {
           t->dst_tables.push_back((j < h && k < h) ? S(j, 2 * k + 1) : 0.0);
       off += 2 * (int64_t)kp * kp;
     }
+    // Low-mode counts.  The kinetic factor is min(E, radcut) with
+    // E = p2r1(k1) + p2r2(k2); min(E, c) = c - (c - E)_+ and the unnormalised
+    // DST-I squares to 2(n+1) I, so T psi = c psi - S (c - E)_+ divi S psi,
+    // and (c - E)_+ vanishes unless E < c: only modes k1 < k1lo(ia),
+    // k2 < k2lo(ia) take part.
+    t->k1lo.assign(na, 0);
+    t->k2lo.assign(na, 0);
+    double p2min = *std::min_element(t->p2r2.begin(), t->p2r2.end());
+    for (int ia = 0; ia < na; ia++) {
+      const double* q1 = &t->p2r1[(size_t)ia * n1];
+      const int nn = t->nn1[ia];
+      double p1min = *std::min_element(q1, q1 + nn);
+      for (int k = 0; k < nn; k++)
+        if (q1[k] + p2min < t->radcut) t->k1lo[ia] = k + 1;
+      for (int k = 0; k < n2; k++)
+        if (t->p2r2[k] + p1min < t->radcut) t->k2lo[ia] = k + 1;
+    }
+    // low-mode tables L[k][j] = S(j, k), k < kcap (largest count of that size),
+    // row stride a multiple of 4
+    for (size_t q = 0; q < t->dst_sizes.size(); q++) {
+      const int n = t->dst_sizes[q];
+      int kcap = 0;
+      for (int ia = 0; ia < na; ia++) {
+        if (t->nn1[ia] == n) kcap = std::max(kcap, t->k1lo[ia]);
+        if (n == n2) kcap = std::max(kcap, t->k2lo[ia]);
+      }
+      const int ld = (n + 3) & ~3;
+      t->low_off.push_back(off);
+      t->low_ld.push_back(ld);
+      long m2 = 2L * (n + 1);
+      for (int k = 0; k < kcap; k++)
+        for (int j = 0; j < ld; j++) {
+          long m = ((long)(j + 1) * (k + 1)) % m2;
+          double v = 0.0;
+          if (j < n && m != 0 && m != n + 1) v = 2.0 * std::sin(pi * (double)m / (double)(n + 1));
+          t->dst_tables.push_back(v);
+        }
+      off += (int64_t)kcap * ld;
+    }
   }
 
   // angular FBR tables (angkinj2 / bfgridl2mat restated in factor form)
