@@ -586,7 +586,12 @@ struct UpdArgs {
 
 constexpr int UPD_RL = 8;  // r lanes (waves) per block
 
-template <typename TT, typename TV>
+// MODE 0: hout = H psi; 1: first step (undamped); 2: damped update.  SINGLE:
+// one channel and one state (V diagonal).  PROJ: flux columns and reactant
+// slice.  AUTO: <psi_0|psi_new> partials.  All four are template parameters so
+// that no load sits behind a runtime branch (the compiler waits for every
+// load at the join of such a branch, which serialises the batch).
+template <typename TT, typename TV, int MODE, bool SINGLE, bool PROJ, bool AUTO>
 __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict__ psi,
                                                         double* __restrict__ out_buf,
                                                         const TT* __restrict__ tkin,
@@ -596,16 +601,16 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
   const int nrc = (a.n2 + 63) / 64;
   const int cia = blockIdx.x / nrc, rc = blockIdx.x % nrc;
   const int ia = cia % a.na, c = cia / a.na;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
   const int R = rc * 64 + lane;
   const bool inR = R < a.n2;
   const int nn = a.nn1[ia];
   const int et = a.ch_elec[c], om = a.ch_omega[c];
   const double f2 = inR ? a.absfr2[R] : 0.0;
-  const int rflux = a.pflux ? a.ir1flux[ia] - 1 : -1;
-  const bool cvjcol = a.slice && R == a.ir2abs - 1;
+  const int rflux = PROJ ? a.ir1flux[ia] - 1 : -1;
+  const bool cvjcol = PROJ && a.slice && R == a.ir2abs - 1;
   double s_norm = 0.0, s_auto = 0.0, zre = 0.0, zim = 0.0, zval = 0.0;
-  const bool single = a.ne == 1 && a.nc == 1;
   // this block's mask words, one per r row
   extern __shared__ uint64_t smask[];
   {
@@ -614,14 +619,14 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
   }
   __syncthreads();
   if (inR) {
-    // rows r = w + UPD_RL (4 b + u): the loads of four rows are issued
-    // before any of them is used.  Lanes outside the compressed grid load
-    // element 0 instead (one shared cache line), which keeps the loads free of
-    // branches; their values are never used.
+    // rows r = w + UPD_RL (4 b + u): the loads of four rows, and their per-row
+    // tables, are issued before any of them is used.  Lanes outside the
+    // compressed grid load element 0 instead (one shared cache line), which
+    // keeps the loads free of branches; their values are never used.
     for (int rb = w; rb < a.n1; rb += 4 * UPD_RL) {
       bool act[4];
       int64_t pl[4];
-      double p[4], ob[4], tk[4], an[4], vv[4];
+      double p[4], ob[4], tk[4], an[4], vv[4], f1[4], za[4], zb[4], p0[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int r = min(rb + u * UPD_RL, a.n1 - 1);
@@ -630,13 +635,18 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
+        const int r = min(rb + u * UPD_RL, a.n1 - 1);
         const int64_t ple = act[u] ? pl[u] : 0;
         const int64_t ie = act[u] ? (int64_t)c * slab + pl[u] : 0;
         p[u] = psi[ie];
         tk[u] = (double)tkin[ie];
         an[u] = (double)ang[ie];
-        vv[u] = single ? (double)vd[ple] : 0.0;
-        ob[u] = a.mode == 2 ? out_buf[ie] : 0.0;
+        vv[u] = SINGLE ? (double)vd[ple] : 0.0;
+        ob[u] = MODE == 2 ? out_buf[ie] : 0.0;
+        f1[u] = MODE == 2 ? a.absfr1[r] : 0.0;
+        za[u] = PROJ ? a.zar[2 * r] : 0.0;
+        zb[u] = PROJ ? a.zar[2 * r + 1] : 0.0;
+        p0[u] = AUTO ? a.psi0[ie] : 0.0;
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
@@ -645,7 +655,7 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
         const int64_t i = (int64_t)c * slab + pl[u];
         // T psi = radcut psi - low-mode correction on r < nn1; psi beyond (Q1)
         double h = (r < nn ? a.radcut * p[u] - tk[u] : p[u]) + an[u];
-        if (single) {
+        if (SINGLE) {
           h += p[u] * vv[u];
         } else {
           for (int s2 = 0; s2 < a.nc; s2++) {
@@ -656,34 +666,34 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
             h += ps * v;
           }
         }
-        if (a.mode == 0) {
+        if (MODE == 0) {
           out_buf[i] = h;
           continue;
         }
         const double hn = (h - a.emindlt * p[u]) * a.rdelta2;
         double o;
-        if (a.mode == 1) {
+        if (MODE == 1) {
           o = hn;
         } else {
-          const double f = a.absfr1[r] * f2;
+          const double f = f1[u] * f2;
           o = (2.0 * hn - f * ob[u]) * f;
         }
         out_buf[i] = o;
         s_norm += o * o;
-        if (a.psi0) s_auto += a.psi0[i] * o;
-        if (a.pflux) {
-          zre += a.zar[2 * r] * o;
-          zim += a.zar[2 * r + 1] * o;
+        if (AUTO) s_auto += p0[u] * o;
+        if (PROJ) {
+          zre += za[u] * o;
+          zim += zb[u] * o;
           if (r == rflux) zval = o;
+          if (cvjcol) a.slice[((int64_t)c * a.na + ia) * a.n1 + r] = o * a.rsah2;
         }
-        if (cvjcol) a.slice[((int64_t)c * a.na + ia) * a.n1 + r] = o * a.rsah2;
       }
     }
   }
   __shared__ double sh[5][UPD_RL][64];
   sh[0][w][lane] = s_norm;
   sh[1][w][lane] = s_auto;
-  if (a.pflux) {
+  if (PROJ) {
     sh[2][w][lane] = zre;
     sh[3][w][lane] = zim;
     sh[4][w][lane] = zval;
@@ -695,13 +705,13 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
     for (int q = 0; q < UPD_RL; q++) {
       n0 += sh[0][q][lane];
       a0 += sh[1][q][lane];
-      if (a.pflux) {
+      if (PROJ) {
         zr0 += sh[2][q][lane];
         zi0 += sh[3][q][lane];
         zv += sh[4][q][lane];
       }
     }
-    if (a.pflux && inR) {
+    if (PROJ && a.pflux && inR) {
       const int f = a.fmap[(int64_t)cia * a.n2 + R];
       if (f >= 0) {
         a.zdr[2 * (int64_t)f] = zr0;
@@ -719,6 +729,30 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
       a.part[2 * blockIdx.x + 1] = a0;
     }
   }
+}
+
+template <typename T, int MODE, bool SINGLE, bool PROJ, bool AUTO>
+static void launch_update_kernel(Ctx* c, unsigned nb, const double* psi, double* out,
+                                 const UpdArgs& a) {
+  hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO>), dim3(nb), dim3(64 * UPD_RL),
+                     sizeof(uint64_t) * c->t.n1, c->stream, psi, out, (const T*)c->d_t2,
+                     (const T*)c->d_ang, (const T*)c->d_v, a);
+}
+
+template <typename T, bool SINGLE>
+static void dispatch_update(Ctx* c, unsigned nb, const double* psi, double* out, const UpdArgs& a,
+                            bool proj, bool autoc) {
+  if (a.mode == 0)
+    launch_update_kernel<T, 0, SINGLE, false, false>(c, nb, psi, out, a);
+  else if (a.mode == 1)
+    autoc ? launch_update_kernel<T, 1, SINGLE, false, true>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 1, SINGLE, false, false>(c, nb, psi, out, a);
+  else if (proj)
+    autoc ? launch_update_kernel<T, 2, SINGLE, true, true>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 2, SINGLE, true, false>(c, nb, psi, out, a);
+  else
+    autoc ? launch_update_kernel<T, 2, SINGLE, false, true>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 2, SINGLE, false, false>(c, nb, psi, out, a);
 }
 
 __global__ void k_reduce_parts(const double* __restrict__ part, int nb, double* __restrict__ out) {
@@ -1191,9 +1225,12 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   a.rsah2 = 1.0 / std::sqrt(c->s.ah2);
   timer_begin(c, K_UPDATE);
   double* out = mode == 0 ? c->d_hout : c->d_psim1;
-  hipLaunchKernelGGL((k_update<T, T>), dim3(nb), dim3(64 * UPD_RL), sizeof(uint64_t) * t.n1,
-                     c->stream, psi, out,
-                     (const T*)c->d_t2, (const T*)c->d_ang, (const T*)c->d_v, a);
+  const bool projk = a.pflux || a.slice;
+  const bool autoc = a.psi0 != nullptr;
+  if (t.ne == 1 && t.nc == 1)
+    dispatch_update<T, true>(c, (unsigned)nb, psi, out, a, projk, autoc);
+  else
+    dispatch_update<T, false>(c, (unsigned)nb, psi, out, a, projk, autoc);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (mode != 0) {
