@@ -15,8 +15,9 @@ static thread_local std::string g_create_err;
 
 namespace mw3 {
 
-void timer_begin(Ctx* c, int id) {
+void timer_begin(Ctx* c, int id, hipStream_t s) {
   if (!c->timing) return;
+  if (!s) s = c->stream;
   Timer t;
   if (!c->pool.empty()) {
     t = c->pool.back();
@@ -25,15 +26,16 @@ void timer_begin(Ctx* c, int id) {
     (void)hipEventCreate(&t.a);
     (void)hipEventCreate(&t.b);
   }
-  (void)hipEventRecord(t.a, c->stream);
+  (void)hipEventRecord(t.a, s);
   c->pending.push_back({id, t});
 }
 
-void timer_end(Ctx* c, int id) {
+void timer_end(Ctx* c, int id, hipStream_t s) {
   if (!c->timing) return;
+  if (!s) s = c->stream;
   for (auto it = c->pending.rbegin(); it != c->pending.rend(); ++it)
     if (it->first == id) {
-      (void)hipEventRecord(it->second.b, c->stream);
+      (void)hipEventRecord(it->second.b, s);
       return;
     }
 }
@@ -41,6 +43,7 @@ void timer_end(Ctx* c, int id) {
 static void timer_collect(Ctx* c) {
   if (c->pending.empty()) return;
   (void)hipStreamSynchronize(c->stream);
+  if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (auto& p : c->pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, p.second.a, p.second.b) == hipSuccess) {
@@ -119,6 +122,12 @@ static void free_all(Ctx* c) {
     (void)hipEventDestroy(t.b);
   }
   c->pool.clear();
+  if (c->aux) {
+    (void)hipStreamSynchronize(c->aux);
+    (void)hipStreamDestroy(c->aux);
+  }
+  if (c->ev_slice) (void)hipEventDestroy(c->ev_slice);
+  if (c->ev_cvj) (void)hipEventDestroy(c->ev_cvj);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -175,6 +184,9 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
     }                                                             \
   } while (0)
   CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "mw3_create: stream");
+  CK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "mw3_create: stream");
+  CK(hipEventCreateWithFlags(&c->ev_slice, hipEventDisableTiming), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_cvj, hipEventDisableTiming), "mw3_create: event");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
   {
@@ -370,6 +382,10 @@ int32_t mw3_cheby_begin(mw3_ctx* ctx, const double* psi0, double* stats3) {
 }
 
 static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
+  if (c->cvj_pending) {  // the reactant-state sums of the ring are done first
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "ring flush: wait");
+    c->cvj_pending = false;
+  }
   if (c->ring_fill == 0) return MW3_OK;
   HIPCHK(c, launch_energy_accum(c), "energy accumulation");
   std::vector<double> nb(2 * (size_t)c->ring_fill);
@@ -398,6 +414,8 @@ int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* aut
     int base = done - c->ring_fill;  // output index of ring slot 0
     // one step: H psi_k, update into psim1 (psi_{k+1}), projections, swap
     HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
+    if (c->cvj_pending)  // the update rewrites the slice the last k_cvj reads
+      HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
     HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
     HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
     c->k += 1;

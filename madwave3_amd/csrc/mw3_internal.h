@@ -107,6 +107,12 @@ struct Ctx {
   HostTables t;
   std::string err;
   hipStream_t stream = nullptr;
+  // second stream (default priority) for the reactant-state sums, which then
+  // overlap the next step's H psi: ev_slice orders them after the update that
+  // wrote the slice, ev_cvj orders the next update and the ring flush after them
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_slice = nullptr, ev_cvj = nullptr;
+  bool cvj_pending = false;
   int device = 0;
 
   // device buffers
@@ -163,6 +169,7 @@ struct Ctx {
 
   // GEMM passes: R rows, r columns, angular forward, angular back
   GemmPlan p_dstR, p_dstr, p_dstr2, p_dstRi, p_angF, p_angB;
+  bool ang_fused = false;  // FBR coefficients applied in the forward GEMM (no Coriolis)
   void* d_phi = nullptr;        // FBR after the Omega stencil
   double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors
   uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)
@@ -218,7 +225,7 @@ enum KernelId {
   K_ENERGY = 7,
 };
 
-void timer_begin(Ctx* c, int id);
-void timer_end(Ctx* c, int id);
+void timer_begin(Ctx* c, int id, hipStream_t s = nullptr);
+void timer_end(Ctx* c, int id, hipStream_t s = nullptr);
 
 }  // namespace mw3

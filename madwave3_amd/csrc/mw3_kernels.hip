@@ -50,12 +50,24 @@ struct KinParams {
   const double* divi;
   double radcut;
   int n1;
+  // angular FBR coefficients (kin_mode 2)
+  const double* acoef;
+  const double* bcoef;
+  int nJ, n2;
 };
 
 // radial kinetic factor of mode (k1 = rmul m + radd, k2 = n) of angle kin_ia:
 // kin_mode 0: min(E, radcut) divi (Tradial, SRC/mod_Hphi_01y2.f:770-771);
 // kin_mode 1: (radcut - E)_+ divi, the low-mode correction of c psi
+// kin_mode 2: the diagonal FBR coefficient a_j(r1) + b_j(R) of channel
+// kin_ia for output row m = j and column n = r n2 + R (angkinj2 / bfgridl2mat
+// in factor form, used when no channel has a Coriolis neighbour)
 __device__ __forceinline__ double kin_factor(const GemmDesc& d, const KinParams& kp, int m, int n) {
+  if (d.kin_mode == 2) {
+    const int r = n / kp.n2, R = n - r * kp.n2;
+    return kp.acoef[((int64_t)d.kin_ia * kp.nJ + m) * kp.n1 + r] +
+           kp.bcoef[(((int64_t)d.kin_ia * 3 + 1) * kp.nJ + m) * kp.n2 + R];
+  }
   int k1 = d.rmul * m + d.radd;
   double e = kp.p2r1[(int64_t)d.kin_ia * kp.n1 + k1] + kp.p2r2[n];
   if (d.kin_mode) {
@@ -544,12 +556,6 @@ __global__ void k_fbr_coef(const T* __restrict__ fbr, T* __restrict__ phi,
   }
 }
 
-// ---------------------------------------------------------------------------
-// fused V, projection and Chebyshev update
-// mode 0: hout = H psi on the mask
-// mode 1: psi_new = (H psi - emin psi)/delta           (first step, no damping)
-// mode 2: psi_new = f (2 (H psi - emin psi)/delta - f psi_m1)
-// psi_new overwrites psim1.  Partial sums of psi_new^2 and psi0.psi_new.
 // ---------------------------------------------------------------------------
 // Fused V psi + compressed-grid projection + damped update
 // (SRC/mod_Hphi_01y2.f:838-853, SRC/main_madwave3.f:245-248, 332-339) with the
@@ -1092,7 +1098,11 @@ hipError_t build_plans(Ctx* c) {
     c->p_dstRi.st = GemmStrides{t.low_ld[q], 1, t.n2, 1};
     if ((e = upload_plan(&c->p_dstRi, d, true)) != hipSuccess) return e;
   }
-  // angular forward: fbr[c][jj][p] = sum_ia D_c(ia, jj) psi[c][ia][p]
+  // angular forward: fbr[c][jj][p] = sum_ia D_c(ia, jj) psi[c][ia][p]; with no
+  // Coriolis neighbour in any channel the coefficients go into its epilogue
+  c->ang_fused = true;
+  for (int ch = 0; ch < t.nc; ch++)
+    if (t.nbr[ch * 3 + 0] >= 0 || t.nbr[ch * 3 + 2] >= 0) c->ang_fused = false;
   {
     std::vector<GemmDesc> d;
     for (int ch = 0; ch < t.nc; ch++) {
@@ -1105,6 +1115,10 @@ hipError_t build_plans(Ctx* c) {
       g.K = t.na;
       g.lda_m = 1;
       g.lda_k = t.nJp;
+      if (c->ang_fused) {  // write phi = (a_j + b_j) D^T psi directly
+        g.kin_ia = ch;
+        g.kin_mode = 2;
+      }
       d.push_back(g);
     }
     c->p_angF.st = GemmStrides{plane, 1, plane, 1};
@@ -1153,12 +1167,25 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   hipError_t e;
   // angular term
   timer_begin(c, K_ANG_FWD);
-  e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr, c->p_angF);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fbr_coef<T>, dim3((unsigned)((int64_t)t.nc * t.nJ * t.n1)), dim3(256), 0,
-                     s, (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,
-                     (const double*)c->d_bcoef, c->d_nbr, t.nc, t.nJ, t.n1, t.n2);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (c->ang_fused) {
+    KinParams ka{};
+    ka.n1 = t.n1;
+    ka.acoef = (const double*)c->d_acoef;
+    ka.bcoef = (const double*)c->d_bcoef;
+    ka.nJ = t.nJ;
+    ka.n2 = t.n2;
+    e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_phi,
+                                             c->p_angF, ka);
+    if (e != hipSuccess) return e;
+  } else {
+    e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr,
+                                             c->p_angF);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fbr_coef<T>, dim3((unsigned)((int64_t)t.nc * t.nJ * t.n1)), dim3(256),
+                       0, s, (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,
+                       (const double*)c->d_bcoef, c->d_nbr, t.nc, t.nJ, t.n1, t.n2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   timer_end(c, K_ANG_FWD);
   timer_begin(c, K_ANG_BWD);
   e = run_gemm<G_ANG_BWD, T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
@@ -1253,13 +1280,18 @@ hipError_t launch_projections(Ctx* c, int slot) {
   const HostTables& t = c->t;
   hipError_t e = hipSuccess;
   if (t.ncvj > 0) {
-    timer_begin(c, K_PROJ);
+    // on the second stream, after the update that wrote the slice
+    if ((e = hipEventRecord(c->ev_slice, c->stream)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(c->aux, c->ev_slice, 0)) != hipSuccess) return e;
+    timer_begin(c, K_PROJ, c->aux);
     int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
-    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(64 * CVJ_WAVES), sizeof(double) * t.n1, c->stream,
+    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(64 * CVJ_WAVES), sizeof(double) * t.n1, c->aux,
                        c->d_slice, c->d_dcvj, c->d_fdv, c->d_ring_cvj + (int64_t)slot * t.ncvj,
                        nj, nv, t.na, t.n1);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    timer_end(c, K_PROJ);
+    timer_end(c, K_PROJ, c->aux);
+    if ((e = hipEventRecord(c->ev_cvj, c->aux)) != hipSuccess) return e;
+    c->cvj_pending = true;
   }
   return e;
 }

@@ -43,7 +43,7 @@ def main():
     res = {
         "workload": "configuration 1: 128 r x 256 R x 60 gamma, J=0, LEPS, %d steps" % nsteps,
         "steps": nsteps, "threads": nthr,
-        "etot_zot": [float(x) for x in np.asarray(case.etots2)],
+        "etot_zot": [float(x) for x in np.asarray(case.etotS2)],
         "P": [float(x) for x in P],
         "P_elec": np.asarray(Pe).tolist(),
         "norm_every_100": [float(x) for x in norms[99::100]],
