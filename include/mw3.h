@@ -202,9 +202,13 @@ MW3_API int32_t mw3_state_get(mw3_ctx *ctx, double *psi_km1, double *psi_k,
 MW3_API int32_t mw3_state_put(mw3_ctx *ctx, const double *psi_km1, const double *psi_k,
                       int32_t k, const double *psi0);
 
-/* Per-kernel timing with HIP events on the context's stream.  When enabled,
-   every launch of a timed kernel in mw3_cheby_steps is bracketed by an event
-   pair; mw3_timing_get adds the elapsed times.  ids: 0 .. MW3_NKERNELS-1. */
+/* Per-kernel timing with HIP events on the stream each kernel runs on.  on = 0
+   disables it; on = 1 brackets every launch of a timed kernel with an event
+   pair; on = N > 1 does so on every N-th Chebyshev step only (sampling keeps
+   the event overhead out of the step time).  mw3_timing_get adds the elapsed
+   times and counts the timed launches.  ids: 0 .. MW3_NKERNELS-1.  No
+   counterpart in the reference (its timing is wall-clock per loop,
+   SRC/main_madwave3.f:370-372). */
 #define MW3_NKERNELS 8
 MW3_API int32_t mw3_timing_enable(mw3_ctx *ctx, int32_t on);
 MW3_API int32_t mw3_timing_get(mw3_ctx *ctx, int32_t id, double *total_ms,

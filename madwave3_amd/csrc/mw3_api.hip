@@ -16,7 +16,7 @@ static thread_local std::string g_create_err;
 namespace mw3 {
 
 void timer_begin(Ctx* c, int id, hipStream_t s) {
-  if (!c->timing) return;
+  if (!c->timing || !c->timing_step) return;
   if (!s) s = c->stream;
   Timer t;
   if (!c->pool.empty()) {
@@ -31,7 +31,7 @@ void timer_begin(Ctx* c, int id, hipStream_t s) {
 }
 
 void timer_end(Ctx* c, int id, hipStream_t s) {
-  if (!c->timing) return;
+  if (!c->timing || !c->timing_step) return;
   if (!s) s = c->stream;
   for (auto it = c->pending.rbegin(); it != c->pending.rend(); ++it)
     if (it->first == id) {
@@ -413,6 +413,7 @@ int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* aut
     if (c->ring_fill == 0) c->ring_k0 = c->k + 1;
     int base = done - c->ring_fill;  // output index of ring slot 0
     // one step: H psi_k, update into psim1 (psi_{k+1}), projections, swap
+    c->timing_step = (c->k % c->timing_every) == 0;
     HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
     if (c->cvj_pending)  // the update rewrites the slice the last k_cvj reads
       HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
@@ -424,10 +425,12 @@ int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* aut
     c->ring_fill += 1;
     done += 1;
     if (c->ring_fill == kRing || done == nsteps) {
+      c->timing_step = true;  // the energy sums of every ring are timed
       int rc = flush_ring(c, norms ? norms + base : nullptr, autocorr ? autocorr + base : nullptr);
       if (rc) return rc;
     }
   }
+  c->timing_step = true;
   return MW3_OK;
 }
 
@@ -588,7 +591,10 @@ int32_t mw3_timing_enable(mw3_ctx* ctx, int32_t on) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c) return MW3_EINVAL;
   timer_collect(c);
+  if (on < 0) return seterr(c, MW3_EINVAL, "mw3_timing_enable: on < 0");
   c->timing = on != 0;
+  c->timing_every = on > 1 ? on : 1;
+  c->timing_step = true;
   for (int i = 0; i < MW3_NKERNELS; i++) {
     c->t_ms[i] = 0;
     c->t_n[i] = 0;

@@ -188,6 +188,8 @@ struct Ctx {
 
   // timing
   bool timing = false;
+  int timing_every = 1;     // time every N-th step
+  bool timing_step = true;  // the current step is timed
   double t_ms[MW3_NKERNELS] = {};
   int64_t t_n[MW3_NKERNELS] = {};
   std::vector<std::pair<int, Timer>> pending;

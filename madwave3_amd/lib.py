@@ -309,8 +309,10 @@ class Propagator:
         self._check(self.lib.mw3_state_put(self.h, _dp(psim1), _dp(psi), k, _dp(p0)),
                     "mw3_state_put")
 
-    def timing(self, on=True):
-        self._check(self.lib.mw3_timing_enable(self.h, 1 if on else 0), "mw3_timing_enable")
+    def timing(self, on=True, every=1):
+        """HIP-event timing of the kernels; every > 1 times every N-th step only."""
+        n = (max(int(every), 1) if on else 0)
+        self._check(self.lib.mw3_timing_enable(self.h, n), "mw3_timing_enable")
 
     def kernel_times(self):
         out = {}
