@@ -40,26 +40,36 @@ def build_case(cfg):
 
 
 # kernel launches per step in each group (mixed and FP64 alike)
-GROUP_LAUNCHES = {"radial_dst": 4, "angular": 3, "update_fused": 2, "projections": 1}
+GROUP_LAUNCHES = {"radial_dst": 4, "angular": 2, "update_fused": 2, "projections": 1,
+                  "energy_sums": 2}
+RING = 64  # steps per energy-sum flush (kRing of the library)
 
 
 def kernel_groups(times):
-    g = {"radial_dst": 0.0, "angular": 0.0, "update_fused": 0.0, "projections": 0.0}
-    n = {}
+    """Per-step milliseconds of each kernel group from mw3_timing_get totals:
+    radial_dst (the four radial GEMM passes), angular (forward GEMM with the
+    FBR coefficients, back GEMM), update_fused (update + norm reduction),
+    projections (reactant-state sums; second stream, overlapped with the next
+    step) and energy_sums (once per 64-step ring, divided by 64)."""
+    acc = {}
     for name, (ms, cnt) in times.items():
         if name.startswith("dst"):
-            g["radial_dst"] += ms
-            n["radial_dst"] = max(n.get("radial_dst", 0), cnt)
+            g = "radial_dst"
         elif name.startswith("angular"):
-            g["angular"] += ms
-            n["angular"] = max(n.get("angular", 0), cnt)
+            g = "angular"
         elif name == "update_fused":
-            g["update_fused"] += ms
-            n["update_fused"] = cnt
+            g = "update_fused"
+        elif name == "energy_accum":
+            g = "energy_sums"
         else:
-            g["projections"] += ms
-            n["projections"] = max(n.get("projections", 0), cnt)
-    return g, n
+            g = "projections"
+        tot, n = acc.get(g, (0.0, 0))
+        acc[g] = (tot + ms, max(n, cnt))
+    per = {}
+    for g, (ms, n) in acc.items():
+        if n:
+            per[g] = ms / n / (RING if g == "energy_sums" else 1)
+    return per
 
 
 def run_gpu(case, precision, steps, warmup, timing):
@@ -78,8 +88,9 @@ def run_gpu(case, precision, steps, warmup, timing):
     dt = time.perf_counter() - t0
     times = g.kernel_times() if timing else {}
     P = g.flux_result()[0]
+    nflux = g.nflux
     g.close()
-    return dt, times, P
+    return dt, times, P, nflux
 
 
 def copy_calibration(mib):
@@ -147,17 +158,18 @@ def main():
     case = build_case(a.config)
     print("# case %s built in %.1fs" % (case, time.perf_counter() - t0), file=sys.stderr, flush=True)
     N = case.ncan * case.nangu * case.npun1 * case.npun2
-    dt64, tim64, P64 = run_gpu(case, MW3_FP64, a.steps, a.warmup, not a.no_timing)
+    dt64, tim64, P64, nflux = run_gpu(case, MW3_FP64, a.steps, a.warmup, not a.no_timing)
     print("# fp64: %.3f ms/step" % (1e3 * dt64 / a.steps), file=sys.stderr, flush=True)
-    dtmx, timmx, Pmx = run_gpu(case, MW3_MIXED, a.steps, a.warmup, not a.no_timing)
+    dtmx, timmx, Pmx, _ = run_gpu(case, MW3_MIXED, a.steps, a.warmup, not a.no_timing)
     print("# mixed: %.3f ms/step" % (1e3 * dtmx / a.steps), file=sys.stderr, flush=True)
     dP = float(np.max(np.abs(Pmx - P64)))
 
-    # roofline of the dominant kernel group of the headline (mixed) run
-    groups, counts = kernel_groups(timmx)
+    # roofline of the dominant kernel group of the headline (mixed) run: the
+    # largest per-step time among the groups on the library's main stream
+    per = kernel_groups(timmx)
     roof = None
     kern = {}
-    if groups and sum(groups.values()) > 0:
+    if per:
         s_op = 4  # FP32 operator I/O
         work = {
             # (a) radial kinetic: 2 s N bytes per step
@@ -166,19 +178,19 @@ def main():
             "angular": ("mfma", 4.0 * case.nangu * N, "TFLOP/s"),
             # (c) fused V + absorption + update: N (8 + 8 + s + s_V + 8) bytes
             "update_fused": ("hbm", N * (8 + 8 + s_op + 4 + 8.0), "GB/s"),
-            "projections": ("hbm", 16.0 * max(case.npun2 * case.nangu, 1), "GB/s"),
+            # (e) energy projection: 8 netot n_flux flops per step (time only)
+            "energy_sums": ("hbm", 16.0 * nflux, "GB/s"),
+            "projections": ("hbm", 8.0 * case.nangu * case.npun1, "GB/s"),
         }
-        for k, ms in groups.items():
-            cnt = counts.get(k, 0)
-            if cnt == 0:
-                continue
-            per = ms / cnt / 1e3  # s per step
+        for k, ms in per.items():
             bound, w, unit = work[k]
-            ach = w / per / (1e9 if unit == "GB/s" else 1e12)
+            sec = ms / 1e3
+            ach = w / sec / (1e9 if unit == "GB/s" else 1e12)
             peak = HBM_PEAK_GBS if unit == "GB/s" else FP32_PEAK_TF
-            kern[k] = {"ms_per_step": per * 1e3, "achieved": ach, "unit": unit, "peak": peak,
+            kern[k] = {"ms_per_step": ms, "achieved": ach, "unit": unit, "peak": peak,
                        "frac": ach / peak}
-        dom = max(groups, key=lambda k: groups[k])
+        main = [k for k in per if k not in ("projections", "energy_sums")]
+        dom = max(main, key=lambda k: per[k])
         d = kern[dom]
         roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],
                 "unit": d["unit"], "frac": d["frac"], "traffic": load_traffic(dom),
@@ -213,6 +225,7 @@ def main():
         "dP_steps": a.warmup + a.steps,
         "roofline": roof,
         "kernels": kern,
+        "kernels_fp64": {k: {"ms_per_step": v} for k, v in kernel_groups(tim64).items()},
         "kernel_ms_per_step": {k: ms / max(cnt, 1) for k, (ms, cnt) in timmx.items()},
         "fp64_kernel_ms_per_step": {k: ms / max(cnt, 1) for k, (ms, cnt) in tim64.items()},
         "cpu_baseline": cpu,
