@@ -41,7 +41,7 @@ def build_case(cfg):
 
 # kernel launches per step in each group (mixed and FP64 alike)
 GROUP_LAUNCHES = {"radial_dst": 4, "angular": 2, "update_fused": 2, "projections": 1,
-                  "energy_sums": 2}
+                  "energy_sums": 2, "coriolis": 1}
 RING = 64  # steps per energy-sum flush (kRing of the library)
 
 
@@ -50,7 +50,9 @@ def kernel_groups(times):
     radial_dst (the four radial GEMM passes), angular (forward GEMM with the
     FBR coefficients, back GEMM), update_fused (update + norm reduction),
     projections (reactant-state sums; second stream, overlapped with the next
-    step) and energy_sums (once per 64-step ring, divided by 64)."""
+    step), energy_sums (once per 64-step ring, divided by 64) and coriolis
+    (the FBR coefficient pass with the Omega +- 1 stencil, run instead of the
+    fused forward epilogue when channels are Coriolis-coupled)."""
     acc = {}
     for name, (ms, cnt) in times.items():
         if name.startswith("dst"):
@@ -61,6 +63,8 @@ def kernel_groups(times):
             g = "update_fused"
         elif name == "energy_accum":
             g = "energy_sums"
+        elif name == "coriolis":
+            g = "coriolis"
         else:
             g = "projections"
         tot, n = acc.get(g, (0.0, 0))
@@ -91,6 +95,14 @@ def run_gpu(case, precision, steps, warmup, timing):
     nflux = g.nflux
     g.close()
     return dt, times, P, nflux
+
+
+def workload_name(cfg, case, N):
+    kind = {1: "J=0 collision", 2: "J=0 collision", 3: "J=%d collision, Omega=0..%d coupled by "
+            "Coriolis" % (case.Jtot, case.iommax), 5: "%d coupled electronic states"
+            % case.nelecmax}.get(cfg, "collision")
+    return ("configuration %d: %s, %d r x %d R x %d gamma, %d channel(s) (%.1fM grid values)"
+            % (cfg, kind, case.npun1, case.npun2, case.nangu, case.ncan, N / 1e6))
 
 
 def copy_calibration(mib):
@@ -181,6 +193,8 @@ def main():
             # (e) energy projection: 8 netot n_flux flops per step (time only)
             "energy_sums": ("hbm", 16.0 * nflux, "GB/s"),
             "projections": ("hbm", 8.0 * case.nangu * case.npun1, "GB/s"),
+            # (d) Coriolis stencil in the FBR: 2 s N bytes over the Omega-coupled blocks
+            "coriolis": ("hbm", 2.0 * s_op * N, "GB/s"),
         }
         for k, ms in per.items():
             bound, w, unit = work[k]
@@ -215,8 +229,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 operator terms, f64 Chebyshev state/update/accumulators",
         "data": "synthetic (LEPS surface, generated in memory)",
-        "config": {"workload": "configuration 2: J=0 collision, 256 r x 512 R x 120 gamma "
-                               "(15.7M grid values), one Omega block",
+        "config": {"workload": workload_name(a.config, case, N),
                    "grid_values": N, "active_fraction": float(case.ntot) / N,
                    "parallelism": "single GPU"},
         "fp64_steps_per_s": a.steps / dt64,

@@ -171,6 +171,15 @@ MW3_API int32_t mw3_cheby_begin(mw3_ctx *ctx, const double *psi0, double *stats3
 MW3_API int32_t mw3_cheby_steps(mw3_ctx *ctx, int32_t nsteps, double *norms,
                         double *autocorr);
 
+/* Advances every context of ctxs[0..n) by nsteps Chebyshev steps, as
+   mw3_cheby_steps does for one, enqueuing step s of every context before step
+   s+1 of any so that the kernels of different contexts share the GPU.  Each
+   context keeps its own state and accumulators.  norms (may be NULL) receives
+   [n][nsteps] |psi_k|^2.  No counterpart in the reference, which propagates
+   one total J per run (SRC/main_madwave3.f): it is for batches of J / Omega
+   blocks of uneven size on one GPU.  All contexts must be on one device. */
+MW3_API int32_t mw3_cheby_steps_batch(mw3_ctx **ctxs, int32_t n, int32_t nsteps, double *norms);
+
 /* Current Chebyshev index k (1 after mw3_cheby_begin). */
 MW3_API int32_t mw3_cheby_k(const mw3_ctx *ctx);
 
@@ -209,7 +218,7 @@ MW3_API int32_t mw3_state_put(mw3_ctx *ctx, const double *psi_km1, const double 
    times and counts the timed launches.  ids: 0 .. MW3_NKERNELS-1.  No
    counterpart in the reference (its timing is wall-clock per loop,
    SRC/main_madwave3.f:370-372). */
-#define MW3_NKERNELS 8
+#define MW3_NKERNELS 9
 MW3_API int32_t mw3_timing_enable(mw3_ctx *ctx, int32_t on);
 MW3_API int32_t mw3_timing_get(mw3_ctx *ctx, int32_t id, double *total_ms,
                        int64_t *launches);

@@ -141,9 +141,9 @@ const char* mw3_last_error(const mw3_ctx* ctx) {
 }
 
 const char* mw3_kernel_name(int32_t id) {
-  static const char* names[MW3_NKERNELS] = {"angular_fwd",   "angular_bwd",     "dst_R_fwd",
-                                            "dst_r_scale",   "dst_R_inv",       "update_fused",
-                                            "projections",   "energy_accum"};
+  static const char* names[MW3_NKERNELS] = {"angular_fwd", "angular_bwd", "dst_R_fwd",
+                                            "dst_r_scale", "dst_R_inv",   "update_fused",
+                                            "projections", "energy_accum", "coriolis"};
   if (id < 0 || id >= MW3_NKERNELS) return "";
   return names[id];
 }
@@ -403,34 +403,68 @@ static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
   return MW3_OK;
 }
 
+// Enqueues one Chebyshev step of c (H psi_k, update into psim1 = psi_{k+1},
+// projections, swap) and flushes the ring when it is full or when this is step
+// nsteps of the call; *done counts the call's steps (output index).
+static int step_enqueue(Ctx* c, int32_t* done, int32_t nsteps, double* norms, double* autocorr) {
+  if (c->ring_fill == 0) c->ring_k0 = c->k + 1;
+  int base = *done - c->ring_fill;  // output index of ring slot 0
+  c->timing_step = (c->k % c->timing_every) == 0;
+  HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
+  if (c->cvj_pending)  // the update rewrites the slice the last k_cvj reads
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
+  HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
+  HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
+  c->k += 1;
+  HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
+  std::swap(c->d_psi, c->d_psim1);
+  c->ring_fill += 1;
+  *done += 1;
+  if (c->ring_fill == kRing || *done == nsteps) {
+    c->timing_step = true;  // the energy sums of every ring are timed
+    int rc = flush_ring(c, norms ? norms + base : nullptr, autocorr ? autocorr + base : nullptr);
+    if (rc) return rc;
+  }
+  c->timing_step = true;
+  return MW3_OK;
+}
+
 int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* autocorr) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c) return MW3_EINVAL;
   if (!c->begun) return seterr(c, MW3_ESTATE, "mw3_cheby_steps: call mw3_cheby_begin first");
   if (nsteps < 0) return seterr(c, MW3_EINVAL, "mw3_cheby_steps: nsteps < 0");
-  int done = 0;
+  int32_t done = 0;
   while (done < nsteps) {
-    if (c->ring_fill == 0) c->ring_k0 = c->k + 1;
-    int base = done - c->ring_fill;  // output index of ring slot 0
-    // one step: H psi_k, update into psim1 (psi_{k+1}), projections, swap
-    c->timing_step = (c->k % c->timing_every) == 0;
-    HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
-    if (c->cvj_pending)  // the update rewrites the slice the last k_cvj reads
-      HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
-    HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
-    HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
-    c->k += 1;
-    HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
-    std::swap(c->d_psi, c->d_psim1);
-    c->ring_fill += 1;
-    done += 1;
-    if (c->ring_fill == kRing || done == nsteps) {
-      c->timing_step = true;  // the energy sums of every ring are timed
-      int rc = flush_ring(c, norms ? norms + base : nullptr, autocorr ? autocorr + base : nullptr);
+    int rc = step_enqueue(c, &done, nsteps, norms, autocorr);
+    if (rc) return rc;
+  }
+  return MW3_OK;
+}
+
+int32_t mw3_cheby_steps_batch(mw3_ctx** ctxs, int32_t n, int32_t nsteps, double* norms) {
+  if (!ctxs || n < 1 || nsteps < 0) return seterr(nullptr, MW3_EINVAL, "mw3_cheby_steps_batch: bad arguments");
+  int dev = -1;
+  for (int i = 0; i < n; i++) {
+    Ctx* c = reinterpret_cast<Ctx*>(ctxs[i]);
+    if (!c) return seterr(nullptr, MW3_EINVAL, "mw3_cheby_steps_batch: NULL context");
+    if (!c->begun) return seterr(c, MW3_ESTATE, "mw3_cheby_steps_batch: call mw3_cheby_begin first");
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == ctxs[i]) return seterr(c, MW3_EINVAL, "mw3_cheby_steps_batch: repeated context");
+    if (dev < 0) dev = c->device;
+    if (c->device != dev)
+      return seterr(c, MW3_EINVAL, "mw3_cheby_steps_batch: contexts on different devices");
+  }
+  std::vector<int32_t> done(n, 0);
+  // step-major order: every context's step s is enqueued before any step s+1,
+  // so the streams of all contexts hold work at the same time
+  for (int32_t st = 0; st < nsteps; st++)
+    for (int i = 0; i < n; i++) {
+      Ctx* c = reinterpret_cast<Ctx*>(ctxs[i]);
+      int rc = step_enqueue(c, &done[i], nsteps, norms ? norms + (size_t)i * nsteps : nullptr,
+                            nullptr);
       if (rc) return rc;
     }
-  }
-  c->timing_step = true;
   return MW3_OK;
 }
 

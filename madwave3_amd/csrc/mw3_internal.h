@@ -225,6 +225,7 @@ enum KernelId {
   K_UPDATE = 5,
   K_PROJ = 6,
   K_ENERGY = 7,
+  K_CORIOLIS = 8,  // FBR coefficients with the Omega +- 1 Coriolis stencil (k_fbr_coef)
 };
 
 void timer_begin(Ctx* c, int id, hipStream_t s = nullptr);

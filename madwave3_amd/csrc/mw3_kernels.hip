@@ -1177,16 +1177,19 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
     e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_phi,
                                              c->p_angF, ka);
     if (e != hipSuccess) return e;
+    timer_end(c, K_ANG_FWD);
   } else {
     e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr,
                                              c->p_angF);
     if (e != hipSuccess) return e;
+    timer_end(c, K_ANG_FWD);
+    timer_begin(c, K_CORIOLIS);
     hipLaunchKernelGGL(k_fbr_coef<T>, dim3((unsigned)((int64_t)t.nc * t.nJ * t.n1)), dim3(256),
                        0, s, (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,
                        (const double*)c->d_bcoef, c->d_nbr, t.nc, t.nJ, t.n1, t.n2);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    timer_end(c, K_CORIOLIS);
   }
-  timer_end(c, K_ANG_FWD);
   timer_begin(c, K_ANG_BWD);
   e = run_gemm<G_ANG_BWD, T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
   if (e != hipSuccess) return e;

@@ -122,6 +122,15 @@ module mw3_capi
       integer(c_int32_t) :: rc
     end function mw3_cheby_steps
 
+    function mw3_cheby_steps_batch(ctxs, n, nsteps, norms) &
+        bind(C, name="mw3_cheby_steps_batch") result(rc)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), intent(in) :: ctxs(*)
+      integer(c_int32_t), value :: n, nsteps
+      type(c_ptr), value :: norms
+      integer(c_int32_t) :: rc
+    end function mw3_cheby_steps_batch
+
     function mw3_cheby_k(ctx) bind(C, name="mw3_cheby_k") result(k)
       import :: c_ptr, c_int32_t
       type(c_ptr), value :: ctx

@@ -88,9 +88,10 @@ def diatomic_states(vdiat_zot, npun1, ah1, rmis1, mu, j, nstates):
     return e, v
 
 
-def config(cfg_id, scale=1.0):
+def config(cfg_id, scale=1.0, J=0):
     """Parameters of the benchmark configurations (grid sizes may be divided
-    by `scale` for small test cases)."""
+    by `scale` for small test cases).  Configuration 4 is one total angular
+    momentum J of the batched J = 0..23 workload."""
     base = dict(
         xm0=2.01410, xm1=1.00783, xm2=1.00783,   # D, H, H (amu)
         De_eV=4.75, a_invA=1.94, re_A=0.742, sato=0.17,
@@ -102,6 +103,7 @@ def config(cfg_id, scale=1.0):
         Jtot=0, iparity=1, iommin=0, iommax=0, nelec=1,
         npun1min=32, kminelastic=100, rmis1_bohr=0.1, rmis2_bohr=0.1,
     )
+    base["Jtot"] = J
     if cfg_id == 1:
         base.update(npun1=128, rfin1_bohr=8.0, npun2=256, rfin2_bohr=15.0,
                     nangu=60, ntimes=500, nloop=39, r1flux_bohr=4.0)
@@ -117,6 +119,14 @@ def config(cfg_id, scale=1.0):
         base.update(npun1=192, rfin1_bohr=8.0, npun2=384, rfin2_bohr=15.0,
                     nangu=96, ntimes=500, nloop=59, r1flux_bohr=4.0,
                     Jtot=12, iparity=1, iommin=0, iommax=6, iomref=-1)
+    elif cfg_id == 4:
+        # one total J of the batched workload J = 0..23 (Jtot set by the caller;
+        # Omega = 0..min(J, 6); parity +1 has no Omega = 0 block at odd J)
+        J = int(base.get("Jtot", 0))
+        base.update(npun1=160, rfin1_bohr=8.0, npun2=320, rfin2_bohr=15.0,
+                    nangu=80, ntimes=500, nloop=59, r1flux_bohr=4.0,
+                    Jtot=J, iparity=1, iommin=J % 2, iommax=min(J, 6),
+                    iomref=-1 if J else 0)
     else:
         raise ValueError("configuration %r has no generator yet" % (cfg_id,))
     if scale != 1.0:

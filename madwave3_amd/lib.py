@@ -15,6 +15,7 @@ __all__ = ["Mw3Setup", "make_setup", "load_library", "Propagator", "Mw3Error",
            "MW3_FP64", "MW3_MIXED", "LIB_PATH"]
 
 MW3_ABI_VERSION = 1
+MW3_NKERNELS = 9  # include/mw3.h
 MW3_FP64 = 0
 MW3_MIXED = 1
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmw3.so")
@@ -164,6 +165,19 @@ class Mw3Error(RuntimeError):
     pass
 
 
+def steps_batch(props, n):
+    """mw3_cheby_steps_batch over Propagators (one GPU): n steps of each,
+    interleaved; returns norms [len(props)][n]."""
+    lib = props[0].lib
+    hs = [p.h.value if isinstance(p.h, ct.c_void_p) else p.h for p in props]
+    arr = (ct.c_void_p * len(props))(*hs)
+    norms = np.zeros((len(props), n))
+    rc = lib.mw3_cheby_steps_batch(arr, len(props), n, norms.ctypes.data_as(ct.c_void_p))
+    if rc != 0:
+        raise Mw3Error("mw3_cheby_steps_batch: %s" % lib.mw3_last_error(None).decode())
+    return norms
+
+
 _lib = None
 
 
@@ -198,6 +212,9 @@ def load_library(path=None):
     lib.mw3_state_put.argtypes = [ct.c_void_p, _pd, _pd, ct.c_int32, _pd]
     lib.mw3_timing_enable.argtypes = [ct.c_void_p, ct.c_int32]
     lib.mw3_timing_get.argtypes = [ct.c_void_p, ct.c_int32, _pd, ct.POINTER(ct.c_int64)]
+    lib.mw3_cheby_steps_batch.restype = ct.c_int32
+    lib.mw3_cheby_steps_batch.argtypes = [ct.POINTER(ct.c_void_p), ct.c_int32, ct.c_int32,
+                                          ct.c_void_p]
     lib.mw3_kernel_name.restype = ct.c_char_p
     lib.mw3_kernel_name.argtypes = [ct.c_int32]
     if lib.mw3_abi_version() != MW3_ABI_VERSION:
@@ -316,7 +333,7 @@ class Propagator:
 
     def kernel_times(self):
         out = {}
-        for i in range(8):
+        for i in range(MW3_NKERNELS):
             t = ct.c_double()
             n = ct.c_int64()
             self._check(self.lib.mw3_timing_get(self.h, i, ct.byref(t), ct.byref(n)), "mw3_timing_get")
