@@ -97,6 +97,87 @@ def run_gpu(case, precision, steps, warmup, timing):
     return dt, times, P, nflux
 
 
+def build_case_j(J):
+    from madwave3_amd import leps, model
+    from madwave3_amd import namelist as NL
+
+    p = leps.config(4, J=J)
+    text, pot, fdat = leps.make_inputs(p)
+    return model.build_case(NL.parse(text), pot, fdat)
+
+
+def run_batch(cases, precision, steps, warmup):
+    """All contexts advanced together with mw3_cheby_steps_batch; seconds for
+    `steps` batched steps."""
+    from madwave3_amd.lib import Propagator, steps_batch
+
+    props = [Propagator(c, precision) for c in cases]
+    for g, c in zip(props, cases):
+        g.begin(c.rpaq0)
+    if warmup:
+        steps_batch(props, warmup)
+    t0 = time.perf_counter()
+    steps_batch(props, steps)
+    dt = time.perf_counter() - t0
+    for g in props:
+        g.close()
+    return dt
+
+
+def run_sequential(cases, precision, steps, warmup):
+    """Each context alone, one after the other; summed seconds for `steps`
+    steps of every context."""
+    from madwave3_amd.lib import Propagator
+
+    tot = 0.0
+    for c in cases:
+        g = Propagator(c, precision)
+        g.begin(c.rpaq0)
+        if warmup:
+            g.steps(warmup)
+        t0 = time.perf_counter()
+        g.steps(steps)
+        tot += time.perf_counter() - t0
+        g.close()
+    return tot
+
+
+def main_batched(a):
+    """Configuration 4: J = 0..23, Omega = 0..min(J, 6) (odd J from 1 at
+    parity +1), 160 r x 320 R x 80 gamma per Omega block, all in one batch."""
+    from madwave3_amd.lib import MW3_FP64, MW3_MIXED
+
+    t0 = time.perf_counter()
+    cases = [build_case_j(J) for J in range(24)]
+    nblk = sum(c.ncan for c in cases)
+    N = sum(c.ncan * c.nangu * c.npun1 * c.npun2 for c in cases)
+    print("# 24 J values, %d Omega blocks built in %.1fs" % (nblk, time.perf_counter() - t0),
+          file=sys.stderr, flush=True)
+    dmx = run_batch(cases, MW3_MIXED, a.steps, a.warmup)
+    print("# batched mixed: %.3f ms/step" % (1e3 * dmx / a.steps), file=sys.stderr, flush=True)
+    d64 = run_batch(cases, MW3_FP64, a.steps, a.warmup)
+    print("# batched fp64: %.3f ms/step" % (1e3 * d64 / a.steps), file=sys.stderr, flush=True)
+    dseq = run_sequential(cases, MW3_FP64, a.steps, a.warmup)
+    print("# sequential fp64: %.3f ms/step" % (1e3 * dseq / a.steps), file=sys.stderr, flush=True)
+    line = {
+        "metric": "aggregate Chebyshev H.psi steps/s of the J=0..23 batch (every Omega block "
+                  "advanced one step)",
+        "value": a.steps / dmx, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * dmx / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 operator terms, f64 Chebyshev state/update/accumulators",
+        "data": "synthetic (LEPS surface, generated in memory)",
+        "config": {"workload": "configuration 4: J = 0..23 batched, %d Omega blocks of "
+                               "160 r x 320 R x 80 gamma (%.0fM grid values)" % (nblk, N / 1e6),
+                   "grid_values": N, "parallelism": "single GPU, %d contexts" % len(cases)},
+        "fp64_batched_steps_per_s": a.steps / d64,
+        "fp64_sequential_steps_per_s": a.steps / dseq,
+        "batched_fp64_vs_sequential_fp64": dseq / d64,
+        "batched_mixed_vs_sequential_fp64": dseq / dmx,
+    }
+    print(json.dumps(line))
+
+
 def workload_name(cfg, case, N):
     kind = {1: "J=0 collision", 2: "J=0 collision", 3: "J=%d collision, Omega=0..%d coupled by "
             "Coriolis" % (case.Jtot, case.iommax), 5: "%d coupled electronic states"
@@ -164,6 +245,9 @@ def main():
         sys.exit(2)
     from madwave3_amd.lib import MW3_FP64, MW3_MIXED
 
+    if a.config == 4:
+        main_batched(a)
+        return
     if a.copy_mib:
         copy_calibration(a.copy_mib)
     t0 = time.perf_counter()
