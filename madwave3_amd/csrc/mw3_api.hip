@@ -111,7 +111,8 @@ static void free_all(Ctx* c) {
                   c->d_ring_cvj, c->d_zcr, c->d_zcdr, c->d_zs2, c->d_etot, c->d_fdv, c->d_dcvj,
                   c->d_cvj_skip, c->d_norm_part, c->d_ring_norm, c->d_efe, c->d_zfe, c->d_phi,
                   c->d_wmat, c->d_zero_skip, c->d_ch_elec, c->d_ch_omega, c->d_potiflux,
-                  c->d_flux_elec, c->d_scratch, c->d_fmap, c->d_ir1flux, c->d_slice};
+                  c->d_flux_elec, c->d_scratch, c->d_fmap, c->d_ir1flux, c->d_slice,
+                  c->d_psi32, c->d_psim132, c->d_scr32};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   free_plans(c);
@@ -211,6 +212,11 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(alloc_zero((void**)&c->d_psi0, nd * sizeof(double)), "mw3_create: psi0");
   CK(alloc_zero((void**)&c->d_hout, nd * sizeof(double)), "mw3_create: hout");
   CK(alloc_zero((void**)&c->d_scratch, nd * sizeof(double)), "mw3_create: scratch");
+  if (precision == MW3_MIXED) {
+    CK(alloc_zero((void**)&c->d_psi32, nd * sizeof(float)), "mw3_create: psi FP32");
+    CK(alloc_zero((void**)&c->d_psim132, nd * sizeof(float)), "mw3_create: psi FP32");
+    CK(alloc_zero((void**)&c->d_scr32, nd * sizeof(float)), "mw3_create: psi FP32");
+  }
   if (setup->netot > 0) {
     c->reacfct_host.assign(setup->reacfct, setup->reacfct + setup->netot);
     c->etot_host.assign(setup->etots2, setup->etots2 + setup->netot);
@@ -317,6 +323,7 @@ int32_t mw3_hphi(mw3_ctx* ctx, const double* psi, double* hpsi) {
   HIPCHK(c, hipMemcpyAsync(c->d_a0, psi, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
          "mw3_hphi: upload");
   HIPCHK(c, launch_scatter_a0(c, c->d_a0, c->d_scratch), "mw3_hphi: scatter");
+  HIPCHK(c, launch_to_f32(c, c->d_scratch, c->d_scr32), "mw3_hphi: FP32 copy");
   HIPCHK(c, launch_hpsi(c, c->d_scratch), "mw3_hphi: H psi");
   HIPCHK(c, launch_update(c, 0, c->d_scratch), "mw3_hphi: H psi");
   HIPCHK(c, launch_gather_a0(c, c->d_hout, c->d_a0b), "mw3_hphi: gather");
@@ -346,6 +353,7 @@ int32_t mw3_cheby_begin(mw3_ctx* ctx, const double* psi0, double* stats3) {
   HIPCHK(c, hipMemcpyAsync(c->d_a0, psi0, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
          "mw3_cheby_begin: upload");
   HIPCHK(c, launch_scatter_a0(c, c->d_a0, c->d_psi), "mw3_cheby_begin: scatter");
+  HIPCHK(c, launch_to_f32(c, c->d_psi, c->d_psi32), "mw3_cheby_begin: FP32 copy");
   size_t bytes = sizeof(double) * (size_t)c->t.nc * c->t.nslab;
   HIPCHK(c, hipMemcpyAsync(c->d_psi0, c->d_psi, bytes, hipMemcpyDeviceToDevice, c->stream),
          "mw3_cheby_begin: copy");
@@ -373,6 +381,7 @@ int32_t mw3_cheby_begin(mw3_ctx* ctx, const double* psi0, double* stats3) {
   c->ring_fill = 0;
   HIPCHK(c, launch_update(c, 1, c->d_psi), "mw3_cheby_begin: first step");
   std::swap(c->d_psi, c->d_psim1);
+  std::swap(c->d_psi32, c->d_psim132);
   c->k = 1;
   c->begun = true;
   int rc = clear_accumulators(c);
@@ -418,6 +427,7 @@ static int step_enqueue(Ctx* c, int32_t* done, int32_t nsteps, double* norms, do
   c->k += 1;
   HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
   std::swap(c->d_psi, c->d_psim1);
+  std::swap(c->d_psi32, c->d_psim132);
   c->ring_fill += 1;
   *done += 1;
   if (c->ring_fill == kRing || *done == nsteps) {
@@ -610,6 +620,8 @@ int32_t mw3_state_put(mw3_ctx* ctx, const double* psi_km1, const double* psi_k, 
   HIPCHK(c, hipMemcpyAsync(c->d_a0b, psi_k, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
          "mw3_state_put");
   HIPCHK(c, launch_scatter_a0(c, c->d_a0b, c->d_psi), "mw3_state_put");
+  HIPCHK(c, launch_to_f32(c, c->d_psi, c->d_psi32), "mw3_state_put: FP32 copy");
+  HIPCHK(c, launch_to_f32(c, c->d_psim1, c->d_psim132), "mw3_state_put: FP32 copy");
   if (psi0) {
     HIPCHK(c, hipMemcpyAsync(c->d_a0, psi0, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
            "mw3_state_put");

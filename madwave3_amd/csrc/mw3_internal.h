@@ -132,6 +132,12 @@ struct Ctx {
   void* d_fbr = nullptr;      // FBR slab [nc][nJ][n1][n2]
   void* d_ang = nullptr;      // angular result slab [nc][na][n1][n2]
   double* d_hout = nullptr;   // H psi dense (hphi mode)
+  // FP32 copies of the psi slabs (mixed precision only): the operator passes
+  // read psi rounded to FP32, so the update writes that rounding once next to
+  // the FP64 state; they swap with d_psi / d_psim1
+  float* d_psi32 = nullptr;
+  float* d_psim132 = nullptr;
+  float* d_scr32 = nullptr;
   double* d_scratch = nullptr; // input slab of mw3_hphi
   std::vector<double> reacfct_host, etot_host;
   // tables
@@ -202,7 +208,9 @@ int build_tables(const mw3_setup* s, HostTables* t, std::string* err);
 // mw3_kernels.hip: launchers, return hipError_t
 hipError_t launch_scatter_a0(Ctx* c, const double* a0, double* dense);
 hipError_t launch_gather_a0(Ctx* c, const double* dense, double* a0);
-hipError_t launch_hpsi(Ctx* c, const double* psi);  // T, V and angular terms into work slabs
+hipError_t launch_hpsi(Ctx* c, const double* psi);
+float* psi_shadow(Ctx* c, const double* psi);  // FP32 copy of a psi slab (mixed), or null
+hipError_t launch_to_f32(Ctx* c, const double* src, float* dst);  // T, V and angular terms into work slabs
 // 0: H psi into d_hout, 1: first step, 2: damped step (psi_new into d_psim1)
 hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj = false);
 hipError_t launch_projections(Ctx* c, int slot);

@@ -437,6 +437,30 @@ __global__ __launch_bounds__(256) void k_gemm_vec(const TA* __restrict__ A, cons
     __syncthreads();
   }
   TC* Cb = C + d.c_off;
+  if (d.kin_mode == 2 && kp.n2 % BNt == 0) {
+    // angular coefficients with the block's columns inside one r row:
+    // r uniform per block, a_j(r) once per output row
+    const int r0 = n0 / kp.n2, Rb = n0 - r0 * kp.n2;
+    const double* ac = kp.acoef + (int64_t)d.kin_ia * kp.nJ * kp.n1 + r0;
+    const double* bc = kp.bcoef + ((int64_t)d.kin_ia * 3 + 1) * kp.nJ * kp.n2 + Rb;
+#pragma unroll
+    for (int a = 0; a < TM; a++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm * 16 * TM + a * 16 + M::row(lane, r);
+        if (m >= d.M) continue;
+        const double am = ac[(int64_t)m * kp.n1];
+        const double* bm = bc + (int64_t)m * kp.n2;
+#pragma unroll
+        for (int b = 0; b < TN; b++) {
+          const int nl = wn * 16 * TN + b * 16 + (lane & 15);
+          if (n0 + nl < d.N)
+            Cb[(int64_t)m * st.ldc_m + (int64_t)(n0 + nl) * st.ldc_n] =
+                (TC)((double)acc[a][b][r] * (am + bm[nl]));
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < TM; a++)
 #pragma unroll
@@ -586,6 +610,7 @@ struct UpdArgs {
   double* zr;
   double* zdr;
   double* slice;
+  float* out32;  // FP32 copy of the new psi (mixed precision), or null
   int ir2abs;
   double rsah2;
 };
@@ -617,31 +642,56 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
   const int rflux = PROJ ? a.ir1flux[ia] - 1 : -1;
   const bool cvjcol = PROJ && a.slice && R == a.ir2abs - 1;
   double s_norm = 0.0, s_auto = 0.0, zre = 0.0, zim = 0.0, zval = 0.0;
-  // this block's mask words, one per r row
+  // this block's mask words, one per r row, and the list of the rows with a
+  // point on the compressed grid (in increasing r)
   extern __shared__ uint64_t smask[];
+  int* rows = (int*)(smask + a.n1);
+  __shared__ int s_cnt, s_wsum[UPD_RL];
   {
     const uint64_t* mb = a.mbits + ((int64_t)ia * nrc + rc) * a.n1;
     for (int r = threadIdx.x; r < a.n1; r += blockDim.x) smask[r] = mb[r];
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    for (int base = 0; base < a.n1; base += blockDim.x) {
+      const int r = base + threadIdx.x;
+      const bool f = r < a.n1 && smask[r] != 0ull;
+      const uint64_t bal = __ballot(f);
+      const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+      if (lane == 0) s_wsum[w] = __popcll(bal);
+      __syncthreads();
+      int off = s_cnt;
+      for (int q = 0; q < w; q++) off += s_wsum[q];
+      if (f) rows[off + pos] = r;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        int tsum = 0;
+        for (int q = 0; q < UPD_RL; q++) tsum += s_wsum[q];
+        s_cnt += tsum;
+      }
+      __syncthreads();
+    }
   }
-  __syncthreads();
+  const int nact = s_cnt;
   if (inR) {
-    // rows r = w + UPD_RL (4 b + u): the loads of four rows, and their per-row
-    // tables, are issued before any of them is used.  Lanes outside the
-    // compressed grid load element 0 instead (one shared cache line), which
-    // keeps the loads free of branches; their values are never used.
-    for (int rb = w; rb < a.n1; rb += 4 * UPD_RL) {
+    // listed rows i = w + UPD_RL (4 b + u): the loads of four rows, and their
+    // per-row tables, are issued before any of them is used.  Lanes outside
+    // the compressed grid load element 0 instead (one shared cache line),
+    // which keeps the loads free of branches; their values are never used.
+    for (int ib = w; ib < nact; ib += 4 * UPD_RL) {
       bool act[4];
+      int rr[4];
       int64_t pl[4];
       double p[4], ob[4], tk[4], an[4], vv[4], f1[4], za[4], zb[4], p0[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int r = min(rb + u * UPD_RL, a.n1 - 1);
-        pl[u] = (int64_t)ia * plane + (int64_t)r * a.n2 + R;
-        act[u] = rb + u * UPD_RL < a.n1 && ((smask[r] >> lane) & 1ull);
+        const int idx = ib + u * UPD_RL;
+        rr[u] = rows[idx < nact ? idx : nact - 1];
+        pl[u] = (int64_t)ia * plane + (int64_t)rr[u] * a.n2 + R;
+        act[u] = idx < nact && ((smask[rr[u]] >> lane) & 1ull);
       }
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const int r = min(rb + u * UPD_RL, a.n1 - 1);
+        const int r = rr[u];
         const int64_t ple = act[u] ? pl[u] : 0;
         const int64_t ie = act[u] ? (int64_t)c * slab + pl[u] : 0;
         p[u] = psi[ie];
@@ -657,7 +707,7 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         if (!act[u]) continue;
-        const int r = rb + u * UPD_RL;
+        const int r = rr[u];
         const int64_t i = (int64_t)c * slab + pl[u];
         // T psi = radcut psi - low-mode correction on r < nn1; psi beyond (Q1)
         double h = (r < nn ? a.radcut * p[u] - tk[u] : p[u]) + an[u];
@@ -685,6 +735,7 @@ __global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict
           o = (2.0 * hn - f * ob[u]) * f;
         }
         out_buf[i] = o;
+        if (a.out32) a.out32[i] = (float)o;
         s_norm += o * o;
         if (AUTO) s_auto += p0[u] * o;
         if (PROJ) {
@@ -741,7 +792,8 @@ template <typename T, int MODE, bool SINGLE, bool PROJ, bool AUTO>
 static void launch_update_kernel(Ctx* c, unsigned nb, const double* psi, double* out,
                                  const UpdArgs& a) {
   hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO>), dim3(nb), dim3(64 * UPD_RL),
-                     sizeof(uint64_t) * c->t.n1, c->stream, psi, out, (const T*)c->d_t2,
+                     (sizeof(uint64_t) + sizeof(int)) * c->t.n1, c->stream, psi, out,
+                     (const T*)c->d_t2,
                      (const T*)c->d_ang, (const T*)c->d_v, a);
 }
 
@@ -1157,8 +1209,22 @@ void free_plans(Ctx* c) {
 }
 
 // H psi terms into work slabs: d_t2 = T psi (rows r < nn1), d_ang = angular
+// psi in the operator precision T: the FP32 copy in mixed mode
 template <typename T>
-static hipError_t hpsi_impl(Ctx* c, const double* psi) {
+static const T* op_psi(Ctx* c, const double* psi);
+template <>
+const double* op_psi<double>(Ctx*, const double* psi) {
+  return psi;
+}
+template <>
+const float* op_psi<float>(Ctx* c, const double* psi) {
+  return psi_shadow(c, psi);
+}
+
+template <typename T>
+static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
+  const T* psi = op_psi<T>(c, psi64);
+  if (!psi) return hipErrorInvalidValue;
   const HostTables& t = c->t;
   hipStream_t s = c->stream;
   T* t1 = (T*)c->d_t1;
@@ -1174,13 +1240,13 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
     ka.bcoef = (const double*)c->d_bcoef;
     ka.nJ = t.nJ;
     ka.n2 = t.n2;
-    e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_phi,
-                                             c->p_angF, ka);
+    e = run_gemm<G_ANG_FWD, T, T, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_phi,
+                                        c->p_angF, ka);
     if (e != hipSuccess) return e;
     timer_end(c, K_ANG_FWD);
   } else {
-    e = run_gemm<G_ANG_FWD, T, double, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr,
-                                             c->p_angF);
+    e = run_gemm<G_ANG_FWD, T, T, T, T>(s, (const T*)c->d_Dmat, psi, (T*)c->d_fbr,
+                                        c->p_angF);
     if (e != hipSuccess) return e;
     timer_end(c, K_ANG_FWD);
     timer_begin(c, K_CORIOLIS);
@@ -1199,7 +1265,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   // update forms T psi = radcut psi - d_t2 on rows r < nn1
   KinParams kp{c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.n1};
   timer_begin(c, K_DST_R);
-  e = run_gemm<G_DST_R, double, T, T, T>(s, psi, tab, t1, c->p_dstR);
+  e = run_gemm<G_DST_R, T, T, T, T>(s, psi, tab, t1, c->p_dstR);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_R);
   timer_begin(c, K_DST_r);
@@ -1213,6 +1279,27 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi) {
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_RINV);
   return hipSuccess;
+}
+
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+float* psi_shadow(Ctx* c, const double* psi) {
+  if (psi == c->d_psi) return c->d_psi32;
+  if (psi == c->d_psim1) return c->d_psim132;
+  if (psi == c->d_scratch) return c->d_scr32;
+  return nullptr;
+}
+
+hipError_t launch_to_f32(Ctx* c, const double* src, float* dst) {
+  if (!dst) return hipSuccess;
+  const int64_t n = (int64_t)c->t.nc * c->t.nslab;
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)),
+                     dim3(256), 0, c->stream, src, dst, n);
+  return hipGetLastError();
 }
 
 hipError_t launch_hpsi(Ctx* c, const double* psi) {
@@ -1253,6 +1340,7 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   a.slice = (proj && t.ncvj > 0) ? c->d_slice : nullptr;
   a.ir2abs = c->s.ir2abs;
   a.rsah2 = 1.0 / std::sqrt(c->s.ah2);
+  a.out32 = mode != 0 ? psi_shadow(c, c->d_psim1) : nullptr;
   timer_begin(c, K_UPDATE);
   double* out = mode == 0 ? c->d_hout : c->d_psim1;
   const bool projk = a.pflux || a.slice;

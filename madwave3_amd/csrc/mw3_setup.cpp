@@ -131,8 +131,8 @@ int build_tables(const mw3_setup* s, HostTables* t, std::string* err) {
   if (s->npun1 < 2 || s->npun2 < 2 || s->nangu < 2)
     return (*err = "mw3_create: npun1, npun2 and nangu must exceed 1 (Hdiatom / rigid-rotor paths are not built)",
             MW3_EUNSUP);
-  if (s->npun1 > 8192)
-    return (*err = "mw3_create: npun1 above 8192 (the update kernel keeps one mask word per r row in LDS)",
+  if (s->npun1 > 4096)
+    return (*err = "mw3_create: npun1 above 4096 (the update kernel keeps a mask word and a row index per r row in LDS)",
             MW3_EUNSUP);
   if (s->ncan < 1 || s->nelecmax < 1 || s->npuntot < 1 || s->inc < 1)
     return (*err = "mw3_create: ncan, nelecmax, npuntot and inc must be positive", MW3_EINVAL);
