@@ -103,7 +103,7 @@ static hipError_t alloc_zero(void** p, size_t bytes) {
 }
 
 static void free_all(Ctx* c) {
-  void* ptrs[] = {c->d_pt_dense, c->d_mask, c->d_mbits, c->d_v, c->d_absfr1, c->d_absfr2, c->d_psi,
+  void* ptrs[] = {c->d_pt_dense, c->d_mask, c->d_v, c->d_absfr1, c->d_absfr2, c->d_psi,
                   c->d_psim1, c->d_psi0, c->d_a0, c->d_a0b, c->d_t1, c->d_t2, c->d_fbr,
                   c->d_ang, c->d_hout, c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi,
                   c->d_dst_tables, c->d_Dmat, c->d_acoef, c->d_bcoef, c->d_nbr,
@@ -112,7 +112,8 @@ static void free_all(Ctx* c) {
                   c->d_cvj_skip, c->d_norm_part, c->d_ring_norm, c->d_efe, c->d_zfe, c->d_phi,
                   c->d_wmat, c->d_zero_skip, c->d_ch_elec, c->d_ch_omega, c->d_potiflux,
                   c->d_flux_elec, c->d_scratch, c->d_fmap, c->d_ir1flux, c->d_slice,
-                  c->d_psi32, c->d_psim132, c->d_scr32};
+                  c->d_psi32, c->d_psim132, c->d_scr32, c->d_uw_row, c->d_uw_word,
+                  c->d_uw_item, c->d_uw_gfirst, c->d_uw_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   free_plans(c);
@@ -190,17 +191,7 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(hipEventCreateWithFlags(&c->ev_cvj, hipEventDisableTiming), "mw3_create: event");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
-  {
-    // mask as 64-bit words: word (ia, R chunk rc, r) holds R = 64 rc + l in bit l
-    const int nrc = (t.n2 + 63) / 64;
-    std::vector<uint64_t> bits((size_t)t.na * nrc * t.n1, 0);
-    for (int ia = 0; ia < t.na; ia++)
-      for (int r = 0; r < t.n1; r++)
-        for (int R = 0; R < t.n2; R++)
-          if (t.mask[(size_t)ia * t.nplane + (size_t)r * t.n2 + R])
-            bits[((size_t)ia * nrc + R / 64) * t.n1 + r] |= 1ull << (R % 64);
-    CK(upload_raw(&c->d_mbits, bits.data(), bits.size()), "mw3_create: mask words");
-  }
+
   if (precision == MW3_MIXED)
     CK(upload<float>(&c->d_v, t.vdense), "mw3_create: V");
   else
@@ -248,7 +239,13 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   int netot = std::max(setup->netot, 0);
   CK(upload_raw(&c->d_flux_dense, t.flux_dense.data(), t.flux_dense.size()), "mw3_create: flux");
   CK(upload_raw(&c->d_flux_col, t.flux_col.data(), t.flux_col.size()), "mw3_create: flux");
-  CK(upload_raw(&c->d_zar1flux, t.zar1flux.data(), t.zar1flux.size()), "mw3_create: flux");
+  {
+    // zAR1flux is read by the update whenever projections run (zeros if the
+    // flux line has no point)
+    std::vector<double> zar(t.zar1flux.begin(), t.zar1flux.end());
+    if (zar.size() < 2 * (size_t)t.n1) zar.resize(2 * (size_t)t.n1, 0.0);
+    CK(upload_raw(&c->d_zar1flux, zar.data(), zar.size()), "mw3_create: flux");
+  }
   CK(upload_raw(&c->d_potiflux, t.potiflux.data(), t.potiflux.size()), "mw3_create: flux");
   CK(upload_raw(&c->d_flux_elec, t.flux_elec.data(), t.flux_elec.size()), "mw3_create: flux");
   {
@@ -267,7 +264,55 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(upload_raw(&c->d_fdv, t.fdv.data(), t.fdv.size()), "mw3_create: fd");
   CK(upload_raw(&c->d_dcvj, t.dcvj.data(), t.dcvj.size()), "mw3_create: D");
   CK(upload_raw(&c->d_cvj_skip, t.cvj_isref.data(), t.cvj_isref.size()), "mw3_create: flags");
-  c->nnorm_blocks = t.nc * t.na * ((t.n2 + 63) / 64);  // one per k_update block
+  {
+    // update work items: for each (channel, angle, 64-R chunk), the r rows with
+    // a point of the compressed grid in that chunk (static), cut into items of
+    // at most kUpdRows rows, one per wave of k_update
+    const int nrc = (t.n2 + 63) / 64;
+    std::vector<int32_t> rows, items, gfirst;
+    std::vector<uint64_t> words;
+    std::vector<int32_t> list_off((size_t)t.na * nrc), list_cnt((size_t)t.na * nrc);
+    for (int ia = 0; ia < t.na; ia++)
+      for (int rc = 0; rc < nrc; rc++) {
+        list_off[(size_t)ia * nrc + rc] = (int32_t)rows.size();
+        for (int r = 0; r < t.n1; r++) {
+          uint64_t wd = 0;
+          for (int l = 0; l < 64 && rc * 64 + l < t.n2; l++)
+            if (t.mask[(size_t)ia * t.nplane + (size_t)r * t.n2 + rc * 64 + l]) wd |= 1ull << l;
+          if (wd) {
+            rows.push_back(r);
+            words.push_back(wd);
+          }
+        }
+        list_cnt[(size_t)ia * nrc + rc] = (int32_t)rows.size() - list_off[(size_t)ia * nrc + rc];
+      }
+    for (int ch = 0; ch < t.nc; ch++)
+      for (int ia = 0; ia < t.na; ia++)
+        for (int rc = 0; rc < nrc; rc++) {
+          const int gi = (ch * t.na + ia) * nrc + rc;
+          gfirst.push_back((int32_t)(items.size() / 4));
+          const int lo = list_off[(size_t)ia * nrc + rc], n = list_cnt[(size_t)ia * nrc + rc];
+          for (int k = 0; k < n; k += kUpdRows) {
+            items.push_back(gi);
+            items.push_back(lo + k);
+            items.push_back(std::min(kUpdRows, n - k));
+            items.push_back(0);
+          }
+        }
+    gfirst.push_back((int32_t)(items.size() / 4));
+    if (rows.empty()) {  // keep the arrays non-empty
+      rows.push_back(0);
+      words.push_back(0);
+    }
+    c->nnorm_blocks = (int)(items.size() / 4);
+    CK(upload_raw(&c->d_uw_row, rows.data(), rows.size()), "mw3_create: update rows");
+    CK(upload_raw(&c->d_uw_word, words.data(), words.size()), "mw3_create: update rows");
+    if (!items.empty())
+      CK(upload_raw(&c->d_uw_item, items.data(), items.size()), "mw3_create: update items");
+    CK(upload_raw(&c->d_uw_gfirst, gfirst.data(), gfirst.size()), "mw3_create: update items");
+    CK(alloc_zero((void**)&c->d_uw_part, sizeof(double) * 3 * 64 * std::max(1, c->nnorm_blocks)),
+       "mw3_create: update partials");
+  }
   {
     std::vector<int32_t> fmap((size_t)t.nc * t.na * t.n2, -1);
     for (int64_t f = 0; f < t.nflux; f++) {
@@ -284,7 +329,8 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
     CK(alloc_zero((void**)&c->d_slice, sizeof(double) * (size_t)t.nc * t.n1 * t.na),
        "mw3_create: slice");
   }
-  CK(alloc_zero((void**)&c->d_norm_part, sizeof(double) * 2 * c->nnorm_blocks), "mw3_create: norms");
+  CK(alloc_zero((void**)&c->d_norm_part, sizeof(double) * 2 * std::max(1, c->nnorm_blocks)),
+     "mw3_create: norms");
   CK(alloc_zero((void**)&c->d_ring_norm, sizeof(double) * 2 * kRing), "mw3_create: norms");
   if (setup->nofe > 0) {
     if (!setup->efe) {

@@ -118,7 +118,14 @@ struct Ctx {
   // device buffers
   uint32_t* d_pt_dense = nullptr;
   uint8_t* d_mask = nullptr;
-  uint64_t* d_mbits = nullptr;  // [na][ceil(n2/64)][n1] mask words (bit l: R = 64 chunk + l)
+  // update work items (see mw3_create): rows and mask words of each
+  // (angle, 64-R chunk), items [nitems][4], item range per (channel, angle,
+  // chunk), flux partials [nitems][3][64]
+  int32_t* d_uw_row = nullptr;
+  uint64_t* d_uw_word = nullptr;
+  int32_t* d_uw_item = nullptr;
+  int32_t* d_uw_gfirst = nullptr;
+  double* d_uw_part = nullptr;
   void* d_v = nullptr;  // double or float dense V
   double* d_absfr1 = nullptr;
   double* d_absfr2 = nullptr;
@@ -168,7 +175,7 @@ struct Ctx {
   double* d_ring_norm = nullptr; // [kRing][2] (norm, autocorrelation)
   double* d_efe = nullptr;
   double* d_zfe = nullptr;       // [nofe][nc*nslab][2] dense
-  int nnorm_blocks = 0;
+  int nnorm_blocks = 0;  // update work items (norm partials)
   int ring_fill = 0;             // steps buffered in the ring
   int ring_k0 = 0;               // k of ring slot 0
   std::vector<double> host_norms;
@@ -220,6 +227,8 @@ hipError_t launch_zero(Ctx* c, void* p, size_t bytes);
 hipError_t build_plans(Ctx* c);
 void free_plans(Ctx* c);
 hipError_t launch_flux_result(Ctx* c, double* d_out /* [(1+ne)][netot] */);
+
+constexpr int kUpdRows = 16;  // rows per update work item
 
 // GEMM passes of the step (template tag: one kernel symbol per pass)
 enum GemmRole { G_ANG_FWD = 0, G_ANG_BWD, G_DST_R, G_DST_r_KIN, G_DST_r_INV, G_DST_R_INV };

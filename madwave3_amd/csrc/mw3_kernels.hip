@@ -592,7 +592,11 @@ __global__ void k_fbr_coef(const T* __restrict__ fbr, T* __restrict__ phi,
 // proj: also sum zAR1flux(r) psi_new(r, R) and take psi_new(ir1flux, R) for the
 // flux columns, and psi_new(r, ir2abs) / sqrt(ah2) for the reactant slice.
 struct UpdArgs {
-  const uint64_t* mbits;
+  const int32_t* items;  // [nitems][4]: group (c, ia, R chunk), first row entry, count, 0
+  const int32_t* urow;   // listed rows r
+  const uint64_t* uword; // their mask words
+  double* upart;         // [nitems][3][64] flux partials (sum zAR1flux psi: re, im; psi at ir1flux)
+  int nitems;
   const int32_t* nn1;
   const double* absfr1;
   const double* absfr2;
@@ -615,7 +619,6 @@ struct UpdArgs {
   double rsah2;
 };
 
-constexpr int UPD_RL = 8;  // r lanes (waves) per block
 
 // MODE 0: hout = H psi; 1: first step (undamped); 2: damped update.  SINGLE:
 // one channel and one state (V diagonal).  PROJ: flux columns and reactant
@@ -623,178 +626,149 @@ constexpr int UPD_RL = 8;  // r lanes (waves) per block
 // that no load sits behind a runtime branch (the compiler waits for every
 // load at the join of such a branch, which serialises the batch).
 template <typename TT, typename TV, int MODE, bool SINGLE, bool PROJ, bool AUTO>
-__global__ __launch_bounds__(64 * UPD_RL) void k_update(const double* __restrict__ psi,
-                                                        double* __restrict__ out_buf,
-                                                        const TT* __restrict__ tkin,
-                                                        const TT* __restrict__ ang,
-                                                        const TV* __restrict__ vd, UpdArgs a) {
+__global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
+                                               double* __restrict__ out_buf,
+                                               const TT* __restrict__ tkin,
+                                               const TT* __restrict__ ang,
+                                               const TV* __restrict__ vd, UpdArgs a) {
   const int64_t plane = (int64_t)a.n1 * a.n2, slab = (int64_t)a.na * plane;
   const int nrc = (a.n2 + 63) / 64;
-  const int cia = blockIdx.x / nrc, rc = blockIdx.x % nrc;
-  const int ia = cia % a.na, c = cia / a.na;
   const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  const int item = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * 4 + (threadIdx.x >> 6)));
+  if (item >= a.nitems) return;  // no block-level synchronisation in this kernel
+  const int g = a.items[4 * item], off = a.items[4 * item + 1], cnt = a.items[4 * item + 2];
+  const int rc = g % nrc, cia = g / nrc;
+  const int ia = cia % a.na, c = cia / a.na;
   const int R = rc * 64 + lane;
   const bool inR = R < a.n2;
+  const int Ra = inR ? R : a.n2 - 1;  // address of lanes past the grid (never used)
   const int nn = a.nn1[ia];
   const int et = a.ch_elec[c], om = a.ch_omega[c];
-  const double f2 = inR ? a.absfr2[R] : 0.0;
+  const double f2 = a.absfr2[Ra];
   const int rflux = PROJ ? a.ir1flux[ia] - 1 : -1;
   const bool cvjcol = PROJ && a.slice && R == a.ir2abs - 1;
   double s_norm = 0.0, s_auto = 0.0, zre = 0.0, zim = 0.0, zval = 0.0;
-  // this block's mask words, one per r row, and the list of the rows with a
-  // point on the compressed grid (in increasing r)
-  extern __shared__ uint64_t smask[];
-  int* rows = (int*)(smask + a.n1);
-  __shared__ int s_cnt, s_wsum[UPD_RL];
-  {
-    const uint64_t* mb = a.mbits + ((int64_t)ia * nrc + rc) * a.n1;
-    for (int r = threadIdx.x; r < a.n1; r += blockDim.x) smask[r] = mb[r];
-    if (threadIdx.x == 0) s_cnt = 0;
-    __syncthreads();
-    for (int base = 0; base < a.n1; base += blockDim.x) {
-      const int r = base + threadIdx.x;
-      const bool f = r < a.n1 && smask[r] != 0ull;
-      const uint64_t bal = __ballot(f);
-      const int pos = __popcll(bal & ((1ull << lane) - 1ull));
-      if (lane == 0) s_wsum[w] = __popcll(bal);
-      __syncthreads();
-      int off = s_cnt;
-      for (int q = 0; q < w; q++) off += s_wsum[q];
-      if (f) rows[off + pos] = r;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        int tsum = 0;
-        for (int q = 0; q < UPD_RL; q++) tsum += s_wsum[q];
-        s_cnt += tsum;
-      }
-      __syncthreads();
+  // the item's rows, four at a time: the loads of the four rows and their
+  // per-row tables are issued before any is used, with no branch around them
+  for (int b = 0; b < cnt; b += 4) {
+    bool act[4];
+    int rr[4];
+    int64_t pl[4];
+    double p[4], ob[4], tk[4], an[4], vv[4], f1[4], za[4], zb[4], p0[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int idx = b + u < cnt ? b + u : cnt - 1;
+      rr[u] = a.urow[off + idx];
+      act[u] = b + u < cnt && inR && ((a.uword[off + idx] >> lane) & 1ull);
+      pl[u] = (int64_t)ia * plane + (int64_t)rr[u] * a.n2 + Ra;
     }
-  }
-  const int nact = s_cnt;
-  if (inR) {
-    // listed rows i = w + UPD_RL (4 b + u): the loads of four rows, and their
-    // per-row tables, are issued before any of them is used.  Lanes outside
-    // the compressed grid load element 0 instead (one shared cache line),
-    // which keeps the loads free of branches; their values are never used.
-    for (int ib = w; ib < nact; ib += 4 * UPD_RL) {
-      bool act[4];
-      int rr[4];
-      int64_t pl[4];
-      double p[4], ob[4], tk[4], an[4], vv[4], f1[4], za[4], zb[4], p0[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int idx = ib + u * UPD_RL;
-        rr[u] = rows[idx < nact ? idx : nact - 1];
-        pl[u] = (int64_t)ia * plane + (int64_t)rr[u] * a.n2 + R;
-        act[u] = idx < nact && ((smask[rr[u]] >> lane) & 1ull);
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int r = rr[u];
-        const int64_t ple = act[u] ? pl[u] : 0;
-        const int64_t ie = act[u] ? (int64_t)c * slab + pl[u] : 0;
-        p[u] = psi[ie];
-        tk[u] = (double)tkin[ie];
-        an[u] = (double)ang[ie];
-        vv[u] = SINGLE ? (double)vd[ple] : 0.0;
-        ob[u] = MODE == 2 ? out_buf[ie] : 0.0;
-        f1[u] = MODE == 2 ? a.absfr1[r] : 0.0;
-        za[u] = PROJ ? a.zar[2 * r] : 0.0;
-        zb[u] = PROJ ? a.zar[2 * r + 1] : 0.0;
-        p0[u] = AUTO ? a.psi0[ie] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (!act[u]) continue;
-        const int r = rr[u];
-        const int64_t i = (int64_t)c * slab + pl[u];
-        // T psi = radcut psi - low-mode correction on r < nn1; psi beyond (Q1)
-        double h = (r < nn ? a.radcut * p[u] - tk[u] : p[u]) + an[u];
-        if (SINGLE) {
-          h += p[u] * vv[u];
-        } else {
-          for (int s2 = 0; s2 < a.nc; s2++) {
-            if (a.ch_omega[s2] != om) continue;
-            const int es = a.ch_elec[s2];
-            const double v = (double)vd[((int64_t)et * a.ne + es) * slab + pl[u]];
-            const double ps = (s2 == c) ? p[u] : psi[(int64_t)s2 * slab + pl[u]];
-            h += ps * v;
-          }
-        }
-        if (MODE == 0) {
-          out_buf[i] = h;
-          continue;
-        }
-        const double hn = (h - a.emindlt * p[u]) * a.rdelta2;
-        double o;
-        if (MODE == 1) {
-          o = hn;
-        } else {
-          const double f = f1[u] * f2;
-          o = (2.0 * hn - f * ob[u]) * f;
-        }
-        out_buf[i] = o;
-        if (a.out32) a.out32[i] = (float)o;
-        s_norm += o * o;
-        if (AUTO) s_auto += p0[u] * o;
-        if (PROJ) {
-          zre += za[u] * o;
-          zim += zb[u] * o;
-          if (r == rflux) zval = o;
-          if (cvjcol) a.slice[((int64_t)c * a.na + ia) * a.n1 + r] = o * a.rsah2;
-        }
-      }
+    for (int u = 0; u < 4; u++) {
+      const int r = rr[u];
+      const int64_t ie = (int64_t)c * slab + pl[u];
+      p[u] = psi[ie];
+      tk[u] = (double)tkin[ie];
+      an[u] = (double)ang[ie];
+      vv[u] = SINGLE ? (double)vd[pl[u]] : 0.0;
+      ob[u] = MODE == 2 ? out_buf[ie] : 0.0;
+      f1[u] = MODE == 2 ? a.absfr1[r] : 0.0;
+      za[u] = PROJ ? a.zar[2 * r] : 0.0;
+      zb[u] = PROJ ? a.zar[2 * r + 1] : 0.0;
+      p0[u] = AUTO ? a.psi0[ie] : 0.0;
     }
-  }
-  __shared__ double sh[5][UPD_RL][64];
-  sh[0][w][lane] = s_norm;
-  sh[1][w][lane] = s_auto;
-  if (PROJ) {
-    sh[2][w][lane] = zre;
-    sh[3][w][lane] = zim;
-    sh[4][w][lane] = zval;
-  }
-  __syncthreads();
-  if (w == 0) {
-    double n0 = 0, a0 = 0, zr0 = 0, zi0 = 0, zv = 0;
 #pragma unroll
-    for (int q = 0; q < UPD_RL; q++) {
-      n0 += sh[0][q][lane];
-      a0 += sh[1][q][lane];
+    for (int u = 0; u < 4; u++) {
+      if (!act[u]) continue;
+      const int r = rr[u];
+      const int64_t i = (int64_t)c * slab + pl[u];
+      // T psi = radcut psi - low-mode correction on r < nn1; psi beyond (Q1)
+      double h = (r < nn ? a.radcut * p[u] - tk[u] : p[u]) + an[u];
+      if (SINGLE) {
+        h += p[u] * vv[u];
+      } else {
+        for (int s2 = 0; s2 < a.nc; s2++) {
+          if (a.ch_omega[s2] != om) continue;
+          const int es = a.ch_elec[s2];
+          const double v = (double)vd[((int64_t)et * a.ne + es) * slab + pl[u]];
+          const double ps = (s2 == c) ? p[u] : psi[(int64_t)s2 * slab + pl[u]];
+          h += ps * v;
+        }
+      }
+      if (MODE == 0) {
+        out_buf[i] = h;
+        continue;
+      }
+      const double hn = (h - a.emindlt * p[u]) * a.rdelta2;
+      double o;
+      if (MODE == 1) {
+        o = hn;
+      } else {
+        const double f = f1[u] * f2;
+        o = (2.0 * hn - f * ob[u]) * f;
+      }
+      out_buf[i] = o;
+      if (a.out32) a.out32[i] = (float)o;
+      s_norm += o * o;
+      if (AUTO) s_auto += p0[u] * o;
       if (PROJ) {
-        zr0 += sh[2][q][lane];
-        zi0 += sh[3][q][lane];
-        zv += sh[4][q][lane];
+        zre += za[u] * o;
+        zim += zb[u] * o;
+        if (r == rflux) zval = o;
+        if (cvjcol) a.slice[((int64_t)c * a.na + ia) * a.n1 + r] = o * a.rsah2;
       }
     }
-    if (PROJ && a.pflux && inR) {
-      const int f = a.fmap[(int64_t)cia * a.n2 + R];
-      if (f >= 0) {
-        a.zdr[2 * (int64_t)f] = zr0;
-        a.zdr[2 * (int64_t)f + 1] = zi0;
-        a.zr[f] = zv;
-      }
+  }
+  if (PROJ) {
+    double* up = a.upart + (int64_t)item * 3 * 64;
+    up[lane] = zre;
+    up[64 + lane] = zim;
+    up[128 + lane] = zval;
+  }
+  if (a.part) {
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+      s_norm += __shfl_xor(s_norm, o2);
+      s_auto += __shfl_xor(s_auto, o2);
     }
-    // wave reduction of the norm partials
-    for (int off = 32; off > 0; off >>= 1) {
-      n0 += __shfl_xor(n0, off);
-      a0 += __shfl_xor(a0, off);
+    if (lane == 0) {
+      a.part[2 * item] = s_norm;
+      a.part[2 * item + 1] = s_auto;
     }
-    if (lane == 0 && a.part) {
-      a.part[2 * blockIdx.x] = n0;
-      a.part[2 * blockIdx.x + 1] = a0;
-    }
+  }
+}
+
+// flux columns of each (channel, angle, 64-R chunk) from the partials of its
+// work items (in item order); chunks with no listed rows write zeros
+__global__ __launch_bounds__(64) void k_update_flux(const double* __restrict__ upart,
+                                                    const int32_t* __restrict__ gfirst,
+                                                    const int32_t* __restrict__ fmap,
+                                                    double* __restrict__ zr,
+                                                    double* __restrict__ zdr, int n2) {
+  const int g = blockIdx.x, lane = threadIdx.x;
+  const int nrc = (n2 + 63) / 64;
+  const int rc = g % nrc, cia = g / nrc;
+  const int R = rc * 64 + lane;
+  if (R >= n2) return;
+  double re = 0.0, im = 0.0, zv = 0.0;
+  for (int it = gfirst[g]; it < gfirst[g + 1]; it++) {
+    const double* up = upart + (int64_t)it * 3 * 64;
+    re += up[lane];
+    im += up[64 + lane];
+    zv += up[128 + lane];
+  }
+  const int f = fmap[(int64_t)cia * n2 + R];
+  if (f >= 0) {
+    zdr[2 * (int64_t)f] = re;
+    zdr[2 * (int64_t)f + 1] = im;
+    zr[f] = zv;
   }
 }
 
 template <typename T, int MODE, bool SINGLE, bool PROJ, bool AUTO>
 static void launch_update_kernel(Ctx* c, unsigned nb, const double* psi, double* out,
                                  const UpdArgs& a) {
-  hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO>), dim3(nb), dim3(64 * UPD_RL),
-                     (sizeof(uint64_t) + sizeof(int)) * c->t.n1, c->stream, psi, out,
-                     (const T*)c->d_t2,
-                     (const T*)c->d_ang, (const T*)c->d_v, a);
+  hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO>), dim3(nb), dim3(256), 0,
+                     c->stream, psi, out, (const T*)c->d_t2, (const T*)c->d_ang,
+                     (const T*)c->d_v, a);
 }
 
 template <typename T, bool SINGLE>
@@ -1312,9 +1286,13 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   // psi_new rows that are never written (outside the compressed grid) keep
   // zeros, so the reactant slice must start cleared only once (at create)
   const HostTables& t = c->t;
-  const int nb = c->nnorm_blocks;
+  const int nb = c->nnorm_blocks;  // work items
   UpdArgs a{};
-  a.mbits = c->d_mbits;
+  a.items = c->d_uw_item;
+  a.urow = c->d_uw_row;
+  a.uword = c->d_uw_word;
+  a.upart = c->d_uw_part;
+  a.nitems = nb;
   a.nn1 = c->d_nn1;
   a.absfr1 = c->d_absfr1;
   a.absfr2 = c->d_absfr2;
@@ -1345,12 +1323,21 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   double* out = mode == 0 ? c->d_hout : c->d_psim1;
   const bool projk = a.pflux || a.slice;
   const bool autoc = a.psi0 != nullptr;
-  if (t.ne == 1 && t.nc == 1)
-    dispatch_update<T, true>(c, (unsigned)nb, psi, out, a, projk, autoc);
-  else
-    dispatch_update<T, false>(c, (unsigned)nb, psi, out, a, projk, autoc);
+  const unsigned nblk4 = (unsigned)((nb + 3) / 4);  // one work item per wave
+  if (nblk4 > 0) {
+    if (t.ne == 1 && t.nc == 1)
+      dispatch_update<T, true>(c, nblk4, psi, out, a, projk, autoc);
+    else
+      dispatch_update<T, false>(c, nblk4, psi, out, a, projk, autoc);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (mode == 2 && a.pflux) {  // flux columns from the work-item partials
+    const int ng = t.nc * t.na * ((t.n2 + 63) / 64);
+    hipLaunchKernelGGL(k_update_flux, dim3(ng), dim3(64), 0, c->stream, c->d_uw_part,
+                       c->d_uw_gfirst, c->d_fmap, a.zr, a.zdr, t.n2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   if (mode != 0) {
     hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, c->d_norm_part, nb,
                        c->d_ring_norm + 2 * slot);
