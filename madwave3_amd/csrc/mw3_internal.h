@@ -228,7 +228,7 @@ hipError_t build_plans(Ctx* c);
 void free_plans(Ctx* c);
 hipError_t launch_flux_result(Ctx* c, double* d_out /* [(1+ne)][netot] */);
 
-constexpr int kUpdRows = 16;  // rows per update work item
+constexpr int kUpdRows = 32;  // rows per update work item
 
 // GEMM passes of the step (template tag: one kernel symbol per pass)
 enum GemmRole { G_ANG_FWD = 0, G_ANG_BWD, G_DST_R, G_DST_r_KIN, G_DST_r_INV, G_DST_R_INV };
