@@ -211,6 +211,17 @@ MW3_API int32_t mw3_state_get(mw3_ctx *ctx, double *psi_km1, double *psi_k,
 MW3_API int32_t mw3_state_put(mw3_ctx *ctx, const double *psi_km1, const double *psi_k,
                       int32_t k, const double *psi0);
 
+/* <- cheby-spectra (SRC/photo/cheby-spectra.f:58-97): the spectrum from the
+   autocorrelation auto[0..kmax] (auto[0] = 1, auto[k] = <psi_0|psi_k>):
+     spectrum[i] = [auto[0] hbr + sum_{k=1..kmax} 2 hbr cos(k acos Es)
+                    e^{-k xkgam} auto[k]] / (sqrt(1 - Es^2) delta2),
+     Es = (e[i] - emindlt)/delta2, e in zots;
+   a NaN (|Es| >= 1) gives 0, as in the reference.  Stateless; runs on the
+   current HIP device.  Errors: mw3_last_error(NULL). */
+MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emindlt,
+                                   double delta2, double hbr, double xkgam, const double *e,
+                                   int32_t ne, double *spectrum);
+
 /* Per-kernel timing with HIP events on the stream each kernel runs on.  on = 0
    disables it; on = 1 brackets every launch of a timed kernel with an event
    pair; on = N > 1 does so on every N-th Chebyshev step only (sampling keeps

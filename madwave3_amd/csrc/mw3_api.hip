@@ -704,3 +704,41 @@ int32_t mw3_timing_get(mw3_ctx* ctx, int32_t id, double* total_ms, int64_t* laun
 }
 
 }  // extern "C"
+
+int32_t mw3_cheby_spectrum(const double* autoc, int32_t kmax, double emindlt, double delta2,
+                           double hbr, double xkgam, const double* e, int32_t ne,
+                           double* spectrum) {
+  if (!autoc || !e || !spectrum || kmax < 0 || ne < 0)
+    return seterr(nullptr, MW3_EINVAL, "mw3_cheby_spectrum: bad arguments");
+  if (!(delta2 > 0.0)) return seterr(nullptr, MW3_EINVAL, "mw3_cheby_spectrum: delta2 must be positive");
+  if (ne == 0) return MW3_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return seterr(nullptr, MW3_EDEVICE, "mw3_cheby_spectrum: no HIP device");
+  double *d_auto = nullptr, *d_a = nullptr, *d_e = nullptr, *d_s = nullptr;
+  hipStream_t st = nullptr;
+  const size_t nk = (size_t)kmax + 1;
+  hipError_t err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (err == hipSuccess) err = hipMalloc((void**)&d_auto, nk * sizeof(double));
+  if (err == hipSuccess) err = hipMalloc((void**)&d_a, nk * sizeof(double));
+  if (err == hipSuccess) err = hipMalloc((void**)&d_e, (size_t)ne * sizeof(double));
+  if (err == hipSuccess) err = hipMalloc((void**)&d_s, (size_t)ne * sizeof(double));
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(d_auto, autoc, nk * sizeof(double), hipMemcpyHostToDevice, st);
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(d_e, e, (size_t)ne * sizeof(double), hipMemcpyHostToDevice, st);
+  if (err == hipSuccess)
+    err = mw3::launch_spectrum(st, d_auto, d_a, kmax, hbr, xkgam, d_e, ne, emindlt, delta2, d_s);
+  if (err == hipSuccess)
+    err = hipMemcpyAsync(spectrum, d_s, (size_t)ne * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (err == hipSuccess) err = hipStreamSynchronize(st);
+  if (d_auto) (void)hipFree(d_auto);
+  if (d_a) (void)hipFree(d_a);
+  if (d_e) (void)hipFree(d_e);
+  if (d_s) (void)hipFree(d_s);
+  if (st) (void)hipStreamDestroy(st);
+  if (err != hipSuccess)
+    return seterr(nullptr, err == hipErrorOutOfMemory ? MW3_ENOMEM : MW3_EDEVICE,
+                  "mw3_cheby_spectrum", err);
+  return MW3_OK;
+}

@@ -217,7 +217,10 @@ hipError_t launch_scatter_a0(Ctx* c, const double* a0, double* dense);
 hipError_t launch_gather_a0(Ctx* c, const double* dense, double* a0);
 hipError_t launch_hpsi(Ctx* c, const double* psi);
 float* psi_shadow(Ctx* c, const double* psi);  // FP32 copy of a psi slab (mixed), or null
-hipError_t launch_to_f32(Ctx* c, const double* src, float* dst);  // T, V and angular terms into work slabs
+hipError_t launch_to_f32(Ctx* c, const double* src, float* dst);
+hipError_t launch_spectrum(hipStream_t s, const double* d_auto, double* d_a, int kmax,
+                           double hbr, double xkgam, const double* d_e, int ne, double emindlt,
+                           double delta2, double* d_spec);  // T, V and angular terms into work slabs
 // 0: H psi into d_hout, 1: first step, 2: damped step (psi_new into d_psim1)
 hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj = false);
 hipError_t launch_projections(Ctx* c, int slot);

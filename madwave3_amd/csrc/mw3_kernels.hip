@@ -920,6 +920,54 @@ __global__ void k_psie(double* __restrict__ zfe, const double* __restrict__ psi,
 }
 
 // ---------------------------------------------------------------------------
+// spectrum from the autocorrelation (cheby-spectra, SRC/photo/cheby-spectra.f:
+// 71-90): spec(E) = [a_0 + sum_k a_k cos(-k acos Es)] / (sqrt(1 - Es^2) delta2)
+// with a_0 = auto_0 hbr, a_k = 2 hbr e^{-k xkgam} auto_k; NaN -> 0.  One
+// thread per energy; the a_k pass through LDS in chunks shared by the block.
+// ---------------------------------------------------------------------------
+__global__ void k_spec_coef(const double* __restrict__ autoc, double* __restrict__ a, int kmax,
+                            double hbr, double xkgam) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k <= kmax; k += gridDim.x * blockDim.x)
+    a[k] = k == 0 ? autoc[0] * hbr : 2.0 * hbr * exp(-(double)k * xkgam) * autoc[k];
+}
+
+constexpr int SPEC_CHUNK = 2048;
+
+__global__ __launch_bounds__(256) void k_spectrum(const double* __restrict__ a, int kmax,
+                                                  const double* __restrict__ e, int ne,
+                                                  double emindlt, double delta2,
+                                                  double* __restrict__ spec) {
+  __shared__ double sa[SPEC_CHUNK];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double Es = i < ne ? (e[i] - emindlt) / delta2 : 0.0;
+  const double th = acos(Es);
+  double sp = a[0];
+  for (int k0 = 1; k0 <= kmax; k0 += SPEC_CHUNK) {
+    const int nk = min(SPEC_CHUNK, kmax - k0 + 1);
+    __syncthreads();
+    for (int q = threadIdx.x; q < nk; q += blockDim.x) sa[q] = a[k0 + q];
+    __syncthreads();
+    for (int q = 0; q < nk; q++) sp += sa[q] * cos(-(double)(k0 + q) * th);
+  }
+  if (i < ne) {
+    sp = sp / (sqrt(1.0 - Es * Es) * delta2);
+    spec[i] = isnan(sp) ? 0.0 : sp;
+  }
+}
+
+hipError_t launch_spectrum(hipStream_t s, const double* d_auto, double* d_a, int kmax,
+                           double hbr, double xkgam, const double* d_e, int ne, double emindlt,
+                           double delta2, double* d_spec) {
+  hipLaunchKernelGGL(k_spec_coef, dim3((unsigned)std::min(1024, (kmax + 256) / 256)), dim3(256),
+                     0, s, d_auto, d_a, kmax, hbr, xkgam);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_spectrum, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, d_a, kmax,
+                     d_e, ne, emindlt, delta2, d_spec);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }

@@ -131,6 +131,18 @@ module mw3_capi
       integer(c_int32_t) :: rc
     end function mw3_cheby_steps_batch
 
+    function mw3_cheby_spectrum(autoc, kmax, emindlt, delta2, hbr, xkgam, e, ne, spectrum) &
+        bind(C, name="mw3_cheby_spectrum") result(rc)
+      import :: c_int32_t, c_double
+      real(c_double), intent(in) :: autoc(*)
+      integer(c_int32_t), value :: kmax
+      real(c_double), value :: emindlt, delta2, hbr, xkgam
+      real(c_double), intent(in) :: e(*)
+      integer(c_int32_t), value :: ne
+      real(c_double), intent(out) :: spectrum(*)
+      integer(c_int32_t) :: rc
+    end function mw3_cheby_spectrum
+
     function mw3_cheby_k(ctx) bind(C, name="mw3_cheby_k") result(k)
       import :: c_ptr, c_int32_t
       type(c_ptr), value :: ctx

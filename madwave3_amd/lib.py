@@ -165,6 +165,21 @@ class Mw3Error(RuntimeError):
     pass
 
 
+def cheby_spectrum(autoc, emindlt, delta2, hbr, xkgam, e_zot):
+    """mw3_cheby_spectrum: the Chebyshev-sum spectrum of cheby-spectra on the GPU
+    (autoc[0..kmax], energies in zots)."""
+    lib = load_library()
+    a = np.ascontiguousarray(autoc, dtype=np.float64)
+    e = np.ascontiguousarray(e_zot, dtype=np.float64)
+    out = np.zeros(e.size)
+    rc = lib.mw3_cheby_spectrum(a.ctypes.data_as(ct.c_void_p), a.size - 1, emindlt, delta2, hbr,
+                                xkgam, e.ctypes.data_as(ct.c_void_p), e.size,
+                                out.ctypes.data_as(ct.c_void_p))
+    if rc != 0:
+        raise Mw3Error("mw3_cheby_spectrum: %s" % lib.mw3_last_error(None).decode())
+    return out
+
+
 def steps_batch(props, n):
     """mw3_cheby_steps_batch over Propagators (one GPU): n steps of each,
     interleaved; returns norms [len(props)][n]."""
@@ -212,6 +227,10 @@ def load_library(path=None):
     lib.mw3_state_put.argtypes = [ct.c_void_p, _pd, _pd, ct.c_int32, _pd]
     lib.mw3_timing_enable.argtypes = [ct.c_void_p, ct.c_int32]
     lib.mw3_timing_get.argtypes = [ct.c_void_p, ct.c_int32, _pd, ct.POINTER(ct.c_int64)]
+    lib.mw3_cheby_spectrum.restype = ct.c_int32
+    lib.mw3_cheby_spectrum.argtypes = [ct.c_void_p, ct.c_int32, ct.c_double, ct.c_double,
+                                       ct.c_double, ct.c_double, ct.c_void_p, ct.c_int32,
+                                       ct.c_void_p]
     lib.mw3_cheby_steps_batch.restype = ct.c_int32
     lib.mw3_cheby_steps_batch.argtypes = [ct.POINTER(ct.c_void_p), ct.c_int32, ct.c_int32,
                                           ct.c_void_p]

@@ -24,6 +24,8 @@
  *   Cvjflux_k        SRC/mod_flux_01y2.f:663-775
  *   totalflux        SRC/mod_flux_01y2.f:778-816
  *   funE             SRC/mod_PsiE_01y2.f:92-123
+ *   cheby-spectra    SRC/photo/cheby-spectra.f:58-97 (spectrum from the
+ *                    autocorrelation, oref_cheby_spectrum)
  *
  * The FFTW RODFT00 transforms of difs (SRC/mod_Hphi_01y2.f:780-810) are
  * evaluated from FFTW's documented definition
@@ -1119,4 +1121,31 @@ ORF_EXPORT int oref_set_threads(int n) {
   (void)n;
   return 1;
 #endif
+}
+
+
+/* Spectrum from the autocorrelation by the modified Chebyshev sum
+ * (SRC/photo/cheby-spectra.f:71-89): for each energy e[i] (zots),
+ *   Es = (e - emindlt)/delta2,
+ *   spec = [auto[0] hbr + sum_{k=1..kmax} 2 hbr cos(-k acos Es) e^{-k xkgam} auto[k]]
+ *          / (sqrt(1 - Es^2) delta2),
+ * and a NaN result is replaced by 0 (SRC/photo/cheby-spectra.f:90). */
+ORF_EXPORT void oref_cheby_spectrum(const double* autoc, int kmax, double emindlt, double delta2,
+                                    double hbr, double xkgam, const double* e, int ne,
+                                    double* spec) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < ne; i++) {
+    double Es = (e[i] - emindlt) / delta2;
+    double deno = sqrt(1.0 - Es * Es);
+    double sp = autoc[0] * hbr;
+    double factor = 2.0 * hbr;
+    for (int k = 1; k <= kmax; k++) {
+      double expo = -(double)k * acos(Es);
+      double ck = factor * cos(expo) * exp(-(double)k * xkgam);
+      sp += ck * autoc[k];
+    }
+    sp = sp / (deno * delta2);
+    if (isnan(sp)) sp = 0.0;
+    spec[i] = sp;
+  }
 }

@@ -36,6 +36,9 @@ def lib():
         L.oref_zs2_get.argtypes = [ct.c_void_p, _pd]
         L.oref_flux_accum_get.argtypes = [ct.c_void_p, _pd, _pd]
         L.oref_psie_get.argtypes = [ct.c_void_p, _pd]
+        L.oref_cheby_spectrum.restype = None
+        L.oref_cheby_spectrum.argtypes = [_pd, ct.c_int, ct.c_double, ct.c_double, ct.c_double,
+                                          ct.c_double, _pd, ct.c_int, _pd]
         L.oref_state_get.argtypes = [ct.c_void_p, _pd, _pd, ct.POINTER(ct.c_int)]
         L.oref_state_put.argtypes = [ct.c_void_p, _pd, _pd, ct.c_int, _pd]
         L.oref_nopr1.argtypes = [ct.c_void_p, ct.c_int]
@@ -165,3 +168,13 @@ class Oracle:
 
 def set_threads(n):
     return lib().oref_set_threads(n)
+
+
+def cheby_spectrum(autoc, emindlt, delta2, hbr, xkgam, e_zot):
+    """Oracle of the cheby-spectra sum (oref_cheby_spectrum)."""
+    a = np.ascontiguousarray(autoc, dtype=float)
+    e = np.ascontiguousarray(e_zot, dtype=float)
+    out = np.zeros(e.size)
+    lib().oref_cheby_spectrum(_dp(a), a.size - 1, emindlt, delta2, hbr, xkgam, _dp(e), e.size,
+                              _dp(out))
+    return out

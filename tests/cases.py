@@ -67,3 +67,18 @@ def box_case(n1=16, n2=24, na=6, Jtot=0, iommax=0, damp=True, vzero=True):
         c.absfr1[:] = 1.0
         c.absfr2[:] = 1.0
     return c
+
+
+def photo_case(name="states", nofe=2):
+    """A named case run as a photodissociation (iphoto = 1): the
+    autocorrelation <psi_0|psi_k> and psi(E) at nofe energies are kept.  The
+    initial packet is the case's collision packet."""
+    import copy
+
+    import numpy as np
+
+    c = copy.copy(case(name))
+    c.iphoto = 1
+    c.nofe = nofe
+    c.efe = np.linspace(float(c.etotS2[len(c.etotS2) // 4]), float(c.etotS2[-1]), nofe)
+    return c

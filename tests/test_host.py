@@ -203,3 +203,18 @@ def test_fortran_dropin_compiles_with_the_reference_main():
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "main_madwave3.f compiled" in r.stdout
+
+
+def test_auto_file_roundtrip(tmp_path):
+    """auto file of check / cheby-spectra: header `xnorm ebndini emindlt delta2`,
+    then `k auto(k)`; auto(0) = 1 and unwritten k read as 0."""
+    from madwave3_amd import spectra
+
+    a = np.array([0.5, -0.25, 0.125])
+    p = str(tmp_path / "auto")
+    spectra.write_auto(p, 2.0, -0.1, 0.7, 1.9, a, k0=2)
+    xn, eb, em, de, auto = spectra.read_auto(p)
+    assert (xn, eb, em, de) == (2.0, -0.1, 0.7, 1.9)
+    assert np.array_equal(auto, [1.0, 0.0, 0.5, -0.25, 0.125])
+    _, _, _, _, cut = spectra.read_auto(p, kmax=3)
+    assert np.array_equal(cut, [1.0, 0.0, 0.5, -0.25])

@@ -276,3 +276,28 @@ def test_zar1flux_is_half_the_fourier_derivative():
         df = 2 * np.pi * m0 / Lbox * np.cos(2 * np.pi * m0 * (r[ir4 - 1] - 0.37) / Lbox)
         s = np.sum(za * f)
         assert abs(s.real - 0.5 * df) < 1e-10 * abs(df) + 1e-12 and abs(s.imag) < 1e-10 * abs(df) + 1e-12
+
+
+def test_cheby_spectrum_closed_form():
+    """cheby-spectra (SRC/photo/cheby-spectra.f:71-90) of auto(k) = cos(k t0)
+    (one eigenstate at Es0 = cos t0): with r = e^-xkgam and kmax large enough
+    for r^kmax to vanish, spec * sqrt(1-Es^2) delta2 / hbr is the sum
+    sum_k (2 - d_k0) r^k cos(k t) cos(k t0) = [P(t - t0) + P(t + t0)] / 2 with the
+    Poisson kernel P(a) = (1 - r^2) / (1 - 2 r cos a + r^2)."""
+    from oracle.oracle import cheby_spectrum
+
+    emindlt, delta2, hbr, xkgam = 0.7, 2.3, 0.9, 0.01
+    kmax = 6000
+    t0 = 1.1
+    autoc = np.cos(np.arange(kmax + 1) * t0)
+    E = emindlt + delta2 * np.linspace(-0.99, 0.99, 397)
+    spec = cheby_spectrum(autoc, emindlt, delta2, hbr, xkgam, E)
+    Es = (E - emindlt) / delta2
+    t = np.arccos(Es)
+    r = np.exp(-xkgam)
+    P = lambda a: (1 - r * r) / (1 - 2 * r * np.cos(a) + r * r)
+    want = 0.5 * (P(t - t0) + P(t + t0)) * hbr / (np.sqrt(1 - Es * Es) * delta2)
+    assert np.allclose(spec, want, rtol=1e-9, atol=1e-12 * np.max(np.abs(want)))
+    # |Es| >= 1 gives NaN in the sum, reported as 0
+    out = cheby_spectrum(autoc[:10], emindlt, delta2, hbr, xkgam, np.array([emindlt + 1.5 * delta2]))
+    assert out[0] == 0.0
