@@ -50,3 +50,32 @@ def test_full_run_matches_oracle(precision, tol_p, tol_n):
     assert dP <= tol_p
     assert dn <= tol_n
     assert dz <= tol_p * max(1.0, float(zr.max()))
+
+
+@pytest.mark.gpu
+def test_full_run_unitarity():
+    """Reaction plus reactant-state probabilities sum to one (the
+    third column of S2prod, e.g. J000-check/S2prod.v00.J000.k00008:1-500):
+    P(E) + sum_{v,j} |zS2/2pi|^2 S2factor over the energies the packet covers."""
+    from madwave3_amd import printprob
+    from madwave3_amd import consts as Cst
+
+    case, nsteps = _case()
+    g = Propagator(case, MW3_FP64)
+    g.begin(case.rpaq0)
+    for i in range(0, nsteps, 4000):
+        g.steps(min(4000, nsteps - i), want_norms=False)
+    P = g.flux_result()[0]
+    z = g.zs2()
+    g.close()
+    _, S2reac = printprob.reactant_probabilities(case, z)
+    E = np.asarray(case.etotS2) / Cst.conve1 / Cst.eV2cm
+    tot = P + S2reac
+    band = (E > 0.5) & (E < 1.2)
+    print("P + S2reac over 0.5-1.2 eV: min %.4f max %.4f" % (tot[band].min(), tot[band].max()),
+          flush=True)
+    for e_, p_, t_ in zip(E[band][::25], P[band][::25], tot[band][::25]):
+        print("  E %.3f eV  P %.4f  P + S2reac %.4f" % (e_, p_, t_), flush=True)
+    # within 10%: the reactant sum stops at v <= 3, j <= 10 (j up to about 14
+    # is open at 1.2 eV) and the run ends with 0.5% of the norm left
+    assert np.all(np.abs(tot[band] - 1.0) < 0.1)

@@ -218,3 +218,39 @@ def test_auto_file_roundtrip(tmp_path):
     assert np.array_equal(auto, [1.0, 0.0, 0.5, -0.25, 0.125])
     _, _, _, _, cut = spectra.read_auto(p, kmax=3)
     assert np.array_equal(cut, [1.0, 0.0, 0.5, -0.25])
+
+
+def test_printprob_number_format_matches_golden_table():
+    """The reference's S2prod table (EXAMPLES/HD+H_collision/J000-check) is in
+    (1x,e15.7) fields: re-formatting its numbers gives its text back."""
+    from madwave3_amd.printprob import fortran_e
+
+    lines = open(os.path.join(ROOT, "tests", "golden", "hd_h_j000_S2prod.txt")).read().splitlines()
+    for ln in lines[:50]:
+        vals = [float(x) for x in ln.split()]
+        assert "".join(" " + fortran_e(v, 15, 7, 2) for v in vals) == ln.rstrip("\n")
+
+
+def test_printprob_tables_roundtrip(tmp_path):
+    """S2prod / S2prodelec columns: E(eV), P, P + S2reac, reacfct; values
+    below 1e-90 written as 0 (SRC/main_madwave3.f:700-731)."""
+    from madwave3_amd import printprob
+
+    c = case("j0")
+    rng = np.random.default_rng(1)
+    P = rng.random(c.netot) * 0.5
+    P[0] = 1e-95
+    Pe = [P.copy()]
+    nj, nv = c.jmax - c.jini + 1, c.nvmax - c.nvini + 1
+    zs2 = rng.standard_normal((c.ncan, nj, nv, c.netot)) + 1j * rng.standard_normal((c.ncan, nj, nv, c.netot))
+    paths = printprob.write_tables(c, str(tmp_path), 3, P, Pe, zs2, iwrt_reac_distri=1)
+    t = printprob.read_table(paths[0])
+    _, S2reac = printprob.reactant_probabilities(c, zs2)
+    assert os.path.basename(paths[0]) == "S2prod.v%02d.J%03d.k00003" % (c.nvref, c.Jtot)
+    assert np.allclose(t[:, 0], np.asarray(c.etotS2) / 1.197e-4 / 8065.5, rtol=1e-6)
+    assert t[0, 1] == 0.0
+    assert np.allclose(t[1:, 1], P[1:], rtol=1e-6)
+    assert np.allclose(t[1:, 2], P[1:] + S2reac[1:], rtol=1e-6)
+    assert np.allclose(t[:, 3], c.reacfct, rtol=1e-6, atol=1e-300)
+    tot = printprob.read_table([p for p in paths if p.endswith("distriS2reac.elec01")][0])
+    assert np.allclose(tot[:, 1], S2reac, rtol=1e-6, atol=1e-12)
