@@ -2,10 +2,12 @@
 
 Workload: the J = 0 collision of configuration 2 -- nr = 256 r points on
 [0.1, 8] bohr, nR = 512 R points on [0.1, 15] bohr, 120 Gauss-Legendre gamma
-points (15.7M grid values), LEPS surface, v=0 j=0 packet at R = 9 bohr --
-built in memory by the synthetic input generator.  A "step" is one full
-Chebyshev iteration (H psi, damped update, per-step projections; the energy
-sums run once per 64 steps and are included).
+points (15.7M grid values), LEPS surface, v=0 j=0 packet at R = 9 bohr,
+reactant states v <= 3, j <= 10 and product states HD(v' <= 3, j' <= 15) at
+R' = 4 bohr (iprod = 2) -- built in memory by the synthetic input generator.
+A "step" is one full Chebyshev iteration (H psi, damped update, per-step
+projections on the flux line, the reactant states and the product states;
+the energy sums run once per 64 steps and are included).
 
 Prints one JSON line: value = steps/s of the mixed-precision propagator (FP32
 operator terms, FP64 Chebyshev state and accumulators); the FP64 propagator's
@@ -41,7 +43,7 @@ def build_case(cfg):
 
 # kernel launches per step in each group (mixed and FP64 alike)
 GROUP_LAUNCHES = {"radial_dst": 4, "angular": 2, "update_fused": 2, "projections": 1,
-                  "energy_sums": 2, "coriolis": 1}
+                  "energy_sums": 2, "coriolis": 1, "products": 1}
 RING = 64  # steps per energy-sum flush (kRing of the library)
 
 
@@ -50,9 +52,11 @@ def kernel_groups(times):
     radial_dst (the four radial GEMM passes), angular (forward GEMM with the
     FBR coefficients, back GEMM), update_fused (update + norm reduction),
     projections (reactant-state sums; second stream, overlapped with the next
-    step), energy_sums (once per 64-step ring, divided by 64) and coriolis
-    (the FBR coefficient pass with the Omega +- 1 stencil, run instead of the
-    fused forward epilogue when channels are Coriolis-coupled)."""
+    step), products (the per-step product-state reduction, same stream as
+    projections), energy_sums (once per 64-step ring, divided by 64, with the
+    product-state GEMM of the ring) and coriolis (the FBR coefficient pass with
+    the Omega +- 1 stencil, run instead of the fused forward epilogue when
+    channels are Coriolis-coupled)."""
     acc = {}
     for name, (ms, cnt) in times.items():
         if name.startswith("dst"):
@@ -65,6 +69,8 @@ def kernel_groups(times):
             g = "energy_sums"
         elif name == "coriolis":
             g = "coriolis"
+        elif name == "prod_projection":
+            g = "products"
         else:
             g = "projections"
         tot, n = acc.get(g, (0.0, 0))
@@ -182,8 +188,11 @@ def workload_name(cfg, case, N):
     kind = {1: "J=0 collision", 2: "J=0 collision", 3: "J=%d collision, Omega=0..%d coupled by "
             "Coriolis" % (case.Jtot, case.iommax), 5: "%d coupled electronic states"
             % case.nelecmax}.get(cfg, "collision")
-    return ("configuration %d: %s, %d r x %d R x %d gamma, %d channel(s) (%.1fM grid values)"
-            % (cfg, kind, case.npun1, case.npun2, case.nangu, case.ncan, N / 1e6))
+    prod = ""
+    if getattr(case, "iprod", 0) == 2:
+        prod = ", product states v'<=%d j'<=%d" % (case.nvmaxprod, case.jmaxprod)
+    return ("configuration %d: %s, %d r x %d R x %d gamma, %d channel(s) (%.1fM grid values)%s"
+            % (cfg, kind, case.npun1, case.npun2, case.nangu, case.ncan, N / 1e6, prod))
 
 
 def copy_calibration(mib):
@@ -267,18 +276,24 @@ def main():
     kern = {}
     if per:
         s_op = 4  # FP32 operator I/O
+        # product projection: transRgjac read by the update (operator precision)
+        tprod = s_op * N if getattr(case, "iprod", 0) == 2 else 0.0
+        nprod_l = ((case.nangproj1 - case.nangproj0 + 1) * (case.n2prod1 - case.n2prod0 + 1)
+                   if tprod else 0)
         work = {
             # (a) radial kinetic: 2 s N bytes per step
             "radial_dst": ("hbm", 2.0 * s_op * N, "GB/s"),
             # (b) angular: 4 n_gamma N flops per step
             "angular": ("mfma", 4.0 * case.nangu * N, "TFLOP/s"),
             # (c) fused V + absorption + update: N (8 + 8 + s + s_V + 8) bytes
-            "update_fused": ("hbm", N * (8 + 8 + s_op + 4 + 8.0), "GB/s"),
+            "update_fused": ("hbm", N * (8 + 8 + s_op + 4 + 8.0) + tprod, "GB/s"),
             # (e) energy projection: 8 netot n_flux flops per step (time only)
             "energy_sums": ("hbm", 16.0 * nflux, "GB/s"),
             "projections": ("hbm", 8.0 * case.nangu * case.npun1, "GB/s"),
             # (d) Coriolis stencil in the FBR: 2 s N bytes over the Omega-coupled blocks
             "coriolis": ("hbm", 2.0 * s_op * N, "GB/s"),
+            # product reduction: 64 partials per update item, dtrans, ring row
+            "products": ("hbm", 8.0 * 3 * max(nprod_l, 1), "GB/s"),
         }
         for k, ms in per.items():
             bound, w, unit = work[k]
@@ -287,7 +302,7 @@ def main():
             peak = HBM_PEAK_GBS if unit == "GB/s" else FP32_PEAK_TF
             kern[k] = {"ms_per_step": ms, "achieved": ach, "unit": unit, "peak": peak,
                        "frac": ach / peak}
-        main = [k for k in per if k not in ("projections", "energy_sums")]
+        main = [k for k in per if k not in ("projections", "energy_sums", "products")]
         dom = max(main, key=lambda k: per[k])
         d = kern[dom]
         roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],

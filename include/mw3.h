@@ -211,6 +211,49 @@ MW3_API int32_t mw3_state_get(mw3_ctx *ctx, double *psi_km1, double *psi_k,
 MW3_API int32_t mw3_state_put(mw3_ctx *ctx, const double *psi_km1, const double *psi_k,
                       int32_t k, const double *psi0);
 
+/*
+ * Product-state projection of iprod = 2 (module mod_coortrans01_02): the
+ * tables that ini_transcoor / changeini / bfreac2prod build
+ * (SRC/mod_coortrans01_02.f:31-690) and that prodpaq / prodcvj read every
+ * step (SRC/mod_coortrans01_02.f:694-825).  Ranges are the reference's
+ * 1-based inclusive bounds; the pointers are read during mw3_prod_attach only.
+ */
+typedef struct mw3_prod_setup {
+  int32_t n2prod0, n2prod1;      /* R points of the projection              */
+  int32_t nangproj0, nangproj1;  /* global angles of the projection         */
+  int32_t iomminprod, iommaxprod; /* product helicities Omega'              */
+  int32_t nviniprod, nvmaxprod;  /* product vibrational range               */
+  int32_t jiniprod, jmaxprod;    /* product rotational range                */
+  const int32_t *noBCprod;       /* noBCprod(jiniprod:jmaxprod)             */
+  const double *transRgjac;      /* [ntot] in A0 layout                     */
+  const double *prodwf;          /* prodwf(n2prod0:n2prod1, nangproj0:nangproj1,
+                                    nelecmax, nviniprod:nvmaxprod,
+                                    jiniprod:jmaxprod, iomminprod:iommaxprod) */
+  const double *dtrans;          /* dtrans(n2prod0:n2prod1, nangproj0:nangproj1,
+                                    iomminprod:iommaxprod, ncan)            */
+} mw3_prod_setup;
+
+/* <- ini_transcoor (SRC/mod_coortrans01_02.f:31-209): hands the product
+   tables to a propagation context created with iprod = 2 (npun1, npun2 > 1).
+   From the next Chebyshev step on, every step also runs prodpaq and prodcvj
+   (SRC/mod_coortrans01_02.f:694-825, called at SRC/main_madwave3.f:346-349).
+   Clears zS2prod; mw3_cheby_begin clears it too.  States the reference
+   leaves out (v' > noBCprod(j'), j' < Omega') get no amplitude, whatever the
+   table holds for them. */
+MW3_API int32_t mw3_prod_attach(mw3_ctx *ctx, const mw3_prod_setup *prod);
+
+/* zS2prod(netot, nviniprod:nvmaxprod, jiniprod:jmaxprod, iomminprod:iommaxprod)
+   of prodcvj (SRC/mod_coortrans01_02.f:791-822), complex interleaved:
+   [nomb][nj'][nv'][netot][2].  _put restores it (restart,
+   SRC/mod_coortrans01_02.f:186-193). */
+MW3_API int32_t mw3_zs2prod_get(mw3_ctx *ctx, double *zS2prod);
+MW3_API int32_t mw3_zs2prod_put(mw3_ctx *ctx, const double *zS2prod);
+
+/* Cvjprod(nviniprod:nvmaxprod, jiniprod:jmaxprod, iomminprod:iommaxprod) of
+   the last step (prodpaq, SRC/mod_coortrans01_02.f:694-788), the record the
+   reference writes to Cvjprod.NNNN each step: [nomb][nj'][nv']. */
+MW3_API int32_t mw3_cvjprod_get(mw3_ctx *ctx, double *cvjprod);
+
 /* <- cheby-spectra (SRC/photo/cheby-spectra.f:58-97): the spectrum from the
    autocorrelation auto[0..kmax] (auto[0] = 1, auto[k] = <psi_0|psi_k>):
      spectrum[i] = [auto[0] hbr + sum_{k=1..kmax} 2 hbr cos(k acos Es)
@@ -229,7 +272,7 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
    times and counts the timed launches.  ids: 0 .. MW3_NKERNELS-1.  No
    counterpart in the reference (its timing is wall-clock per loop,
    SRC/main_madwave3.f:370-372). */
-#define MW3_NKERNELS 9
+#define MW3_NKERNELS 10
 MW3_API int32_t mw3_timing_enable(mw3_ctx *ctx, int32_t on);
 MW3_API int32_t mw3_timing_get(mw3_ctx *ctx, int32_t id, double *total_ms,
                        int64_t *launches);

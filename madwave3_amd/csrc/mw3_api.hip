@@ -113,7 +113,9 @@ static void free_all(Ctx* c) {
                   c->d_wmat, c->d_zero_skip, c->d_ch_elec, c->d_ch_omega, c->d_potiflux,
                   c->d_flux_elec, c->d_scratch, c->d_fmap, c->d_ir1flux, c->d_slice,
                   c->d_psi32, c->d_psim132, c->d_scr32, c->d_uw_row, c->d_uw_word,
-                  c->d_uw_item, c->d_uw_gfirst, c->d_uw_part};
+                  c->d_uw_item, c->d_uw_gfirst, c->d_uw_part, c->d_tprod, c->d_ppart,
+                  c->d_dtrans, c->d_ring_prod, c->d_prodwf, c->d_prod_part,
+                  c->d_ring_cvjp, c->d_cvjp_last, c->d_zs2prod};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   free_plans(c);
@@ -145,7 +147,8 @@ const char* mw3_last_error(const mw3_ctx* ctx) {
 const char* mw3_kernel_name(int32_t id) {
   static const char* names[MW3_NKERNELS] = {"angular_fwd", "angular_bwd", "dst_R_fwd",
                                             "dst_r_scale", "dst_R_inv",   "update_fused",
-                                            "projections", "energy_accum", "coriolis"};
+                                            "projections", "energy_accum", "coriolis",
+                                            "prod_projection"};
   if (id < 0 || id >= MW3_NKERNELS) return "";
   return names[id];
 }
@@ -388,6 +391,11 @@ static int clear_accumulators(Ctx* c) {
   if (c->s.nofe > 0)
     HIPCHK(c, launch_zero(c, c->d_zfe, sizeof(double) * 2 * c->s.nofe * (size_t)t.nc * t.nslab),
            "clear");
+  if (c->prod_on) {
+    const size_t nx = (size_t)c->pr_nomb * c->pr_nvj;
+    HIPCHK(c, launch_zero(c, c->d_zs2prod, sizeof(double) * 2 * nx * netot), "clear");
+    HIPCHK(c, launch_zero(c, c->d_cvjp_last, sizeof(double) * nx), "clear");
+  }
   c->ring_fill = 0;
   return MW3_OK;
 }
@@ -631,6 +639,176 @@ int32_t mw3_psie_get(mw3_ctx* ctx, double* zfe) {
     }
   }
   return MW3_OK;
+}
+
+static void free_prod(Ctx* c) {
+  void** ptrs[] = {&c->d_tprod,
+                   (void**)&c->d_ppart,
+                   (void**)&c->d_dtrans,
+                   (void**)&c->d_ring_prod,
+                   (void**)&c->d_prodwf,
+                   (void**)&c->d_prod_part,
+                   (void**)&c->d_ring_cvjp,
+                   (void**)&c->d_cvjp_last,
+                   (void**)&c->d_zs2prod};
+  for (void** p : ptrs)
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  if (c->p_prod.d_descs) {
+    (void)hipFree(c->p_prod.d_descs);
+    c->p_prod.d_descs = nullptr;
+  }
+  c->prod_on = false;
+}
+
+int32_t mw3_prod_attach(mw3_ctx* ctx, const mw3_prod_setup* ps) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !ps) return MW3_EINVAL;
+  const HostTables& t = c->t;
+  if (c->s.iprod != 2)
+    return seterr(c, MW3_EINVAL, "mw3_prod_attach: the context was created with iprod != 2");
+  if (c->s.netot <= 0)
+    return seterr(c, MW3_EINVAL, "mw3_prod_attach: the context has no energy grid (netot = 0)");
+  if (ps->n2prod0 < 1 || ps->n2prod1 > t.n2 || ps->n2prod0 > ps->n2prod1 ||
+      ps->nangproj0 < 1 || ps->nangproj1 > t.na || ps->nangproj0 > ps->nangproj1 ||
+      ps->iomminprod < 0 || ps->iomminprod > ps->iommaxprod || ps->iommaxprod > c->s.Jtot ||
+      ps->nviniprod > ps->nvmaxprod || ps->jiniprod > ps->jmaxprod || !ps->noBCprod ||
+      !ps->transRgjac || !ps->prodwf || !ps->dtrans)
+    return seterr(c, MW3_EINVAL, "mw3_prod_attach: inconsistent ranges or NULL table");
+  const int njp = ps->jmaxprod - ps->jiniprod + 1, nvp = ps->nvmaxprod - ps->nviniprod + 1;
+  for (int j = 0; j < njp; j++)
+    if (ps->noBCprod[j] > ps->nvmaxprod)
+      return seterr(c, MW3_EINVAL, "mw3_prod_attach: noBCprod > nvmaxprod");
+  if (t.nc > 64)
+    return seterr(c, MW3_EUNSUP, "mw3_prod_attach: more than 64 channels");
+  // steps already in the ring are accumulated without the product projection
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  free_prod(c);
+  c->pr_n2p0 = ps->n2prod0 - 1;
+  c->pr_nap0 = ps->nangproj0 - 1;
+  c->pr_n2p = ps->n2prod1 - ps->n2prod0 + 1;
+  c->pr_nap = ps->nangproj1 - ps->nangproj0 + 1;
+  c->pr_nomb = ps->iommaxprod - ps->iomminprod + 1;
+  c->pr_nvj = njp * nvp;
+  c->pr_L = (int64_t)c->pr_nap * c->pr_n2p;
+  c->pr_shape = {ps->iomminprod, ps->iommaxprod, ps->nviniprod, ps->nvmaxprod, ps->jiniprod,
+                 ps->jmaxprod};
+  const int64_t L = c->pr_L, K = (int64_t)t.ne * L;
+  // K chunks of the flush GEMM: about 512 blocks of 64 x 64 outputs
+  const int ntn = (c->pr_nvj + 63) / 64;
+  int nchunk = std::max(1, 512 / std::max(1, c->pr_nomb * ntn));
+  int64_t kc = std::max<int64_t>(256, (((K + nchunk - 1) / nchunk) + 15) / 16 * 16);
+  c->pr_kc = (int)kc;
+  c->pr_nchunk = (int)((K + kc - 1) / kc);
+  // transRgjac on the dense grid, zero outside the projection range; one
+  // table when every channel holds the same values (the reference fills them
+  // alike, SRC/mod_coortrans01_02.f:356-366)
+  std::vector<double> T0((size_t)t.nslab, 0.0);
+  bool same = true;
+  std::vector<uint8_t> seen((size_t)t.nslab, 0);
+  for (int64_t i = 0; i < t.ntot; i++) {
+    const int64_t d = t.pt_dense[i], cc = d / t.nslab, pl = d % t.nslab;
+    const int ia = (int)(pl / t.nplane), R = (int)(pl % t.n2);
+    double v = ps->transRgjac[i];
+    if (ia < c->pr_nap0 || ia >= c->pr_nap0 + c->pr_nap || R < c->pr_n2p0 ||
+        R >= c->pr_n2p0 + c->pr_n2p)
+      v = 0.0;
+    if (!seen[pl]) {
+      T0[pl] = v;
+      seen[pl] = 1;
+    } else if (T0[pl] != v) {
+      same = false;
+    }
+    (void)cc;
+  }
+  std::vector<double> Tall;
+  const std::vector<double>* Tsrc = &T0;
+  if (!same) {
+    Tall.assign((size_t)t.nc * t.nslab, 0.0);
+    for (int64_t i = 0; i < t.ntot; i++) {
+      const int64_t d = t.pt_dense[i], pl = d % t.nslab;
+      const int ia = (int)(pl / t.nplane), R = (int)(pl % t.n2);
+      if (ia < c->pr_nap0 || ia >= c->pr_nap0 + c->pr_nap || R < c->pr_n2p0 ||
+          R >= c->pr_n2p0 + c->pr_n2p)
+        continue;
+      Tall[(size_t)d] = ps->transRgjac[i];
+    }
+    Tsrc = &Tall;
+  }
+  c->pr_tstride = same ? 0 : t.nslab;
+  // prodwf with the states the reference leaves out zeroed
+  std::vector<double> wf((size_t)c->pr_nomb * c->pr_nvj * K);
+  for (int o = 0; o < c->pr_nomb; o++)
+    for (int jj = 0; jj < njp; jj++)
+      for (int iv = 0; iv < nvp; iv++) {
+        const int j = ps->jiniprod + jj, v = ps->nviniprod + iv, om = ps->iomminprod + o;
+        const bool valid = j >= om && v <= ps->noBCprod[jj];
+        const size_t off = (((size_t)o * njp + jj) * nvp + iv) * (size_t)K;
+        for (int64_t k = 0; k < K; k++) wf[off + k] = valid ? ps->prodwf[off + k] : 0.0;
+      }
+  const size_t nx = (size_t)c->pr_nomb * c->pr_nvj;
+  const int netot = c->s.netot;
+  hipError_t e = c->precision == MW3_MIXED ? upload<float>(&c->d_tprod, *Tsrc)
+                                           : upload<double>(&c->d_tprod, *Tsrc);
+  if (e == hipSuccess)
+    e = upload_raw(&c->d_dtrans, ps->dtrans, (size_t)t.nc * c->pr_nomb * L);
+  if (e == hipSuccess) e = upload_raw(&c->d_prodwf, wf.data(), wf.size());
+  if (e == hipSuccess)
+    e = alloc_zero((void**)&c->d_ppart, sizeof(double) * 64 * std::max(1, c->nnorm_blocks));
+  if (e == hipSuccess)
+    e = alloc_zero((void**)&c->d_ring_prod, sizeof(double) * kRing * (size_t)c->pr_nomb * K);
+  if (e == hipSuccess)
+    e = alloc_zero((void**)&c->d_prod_part,
+                   sizeof(double) * (size_t)c->pr_nomb * c->pr_nchunk * kRing * c->pr_nvj);
+  if (e == hipSuccess) e = alloc_zero((void**)&c->d_ring_cvjp, sizeof(double) * kRing * nx);
+  if (e == hipSuccess) e = alloc_zero((void**)&c->d_cvjp_last, sizeof(double) * nx);
+  if (e == hipSuccess) e = alloc_zero((void**)&c->d_zs2prod, sizeof(double) * 2 * nx * netot);
+  if (e == hipSuccess) e = build_prod_plan(c);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    free_prod(c);
+    return seterr(c, e == hipErrorOutOfMemory ? MW3_ENOMEM : MW3_EDEVICE, "mw3_prod_attach", e);
+  }
+  c->prod_on = true;
+  return MW3_OK;
+}
+
+static int prod_copy(Ctx* c, double* host, const double* host_in, double* dev, size_t n,
+                     const char* what) {
+  if (!c->prod_on) return seterr(c, MW3_ESTATE, (std::string(what) + ": no product tables (mw3_prod_attach)").c_str());
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  if (host)
+    HIPCHK(c, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream), what);
+  else
+    HIPCHK(c, hipMemcpyAsync(dev, host_in, n * sizeof(double), hipMemcpyHostToDevice, c->stream),
+           what);
+  HIPCHK(c, hipStreamSynchronize(c->stream), what);
+  return MW3_OK;
+}
+
+int32_t mw3_zs2prod_get(mw3_ctx* ctx, double* zS2prod) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !zS2prod) return MW3_EINVAL;
+  size_t n = 2 * (size_t)c->pr_nomb * c->pr_nvj * std::max(c->s.netot, 0);
+  return prod_copy(c, zS2prod, nullptr, c->d_zs2prod, n, "mw3_zs2prod_get");
+}
+
+int32_t mw3_zs2prod_put(mw3_ctx* ctx, const double* zS2prod) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !zS2prod) return MW3_EINVAL;
+  size_t n = 2 * (size_t)c->pr_nomb * c->pr_nvj * std::max(c->s.netot, 0);
+  return prod_copy(c, nullptr, zS2prod, c->d_zs2prod, n, "mw3_zs2prod_put");
+}
+
+int32_t mw3_cvjprod_get(mw3_ctx* ctx, double* cvjprod) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c || !cvjprod) return MW3_EINVAL;
+  size_t n = (size_t)c->pr_nomb * c->pr_nvj;
+  return prod_copy(c, cvjprod, nullptr, c->d_cvjp_last, n, "mw3_cvjprod_get");
 }
 
 int32_t mw3_state_get(mw3_ctx* ctx, double* psi_km1, double* psi_k, int32_t* k) {

@@ -195,6 +195,31 @@ struct Ctx {
   int32_t* d_ir1flux = nullptr;  // [na]
   double* d_slice = nullptr;     // [nc][na][n1] psi at R = ir2abs / sqrt(ah2)
 
+  // product-state projection (iprod = 2, mw3_prod_attach).  Per step, the
+  // update sums T psi_new over r into d_ppart (T = transRgjac on the dense
+  // grid); k_prod_fun reduces them to funprod(c, l), l = (angle, R) of the
+  // projection range, rotates by dtrans and sums the channels of each
+  // electronic state into the ring d_ring_prod[slot][Omega'][e][l].  At each
+  // ring flush one split-K GEMM per Omega' projects the ring on prodwf:
+  // Cvjprod[s][Omega'][j'v'] (partials [Omega'][chunk][kRing][nvj]), then the
+  // energy accumulation adds C_k(E) Cvjprod to zS2prod.
+  bool prod_on = false;
+  int pr_n2p0 = 0, pr_nap0 = 0, pr_n2p = 0, pr_nap = 0, pr_nomb = 0, pr_nvj = 0;
+  int pr_nchunk = 0, pr_kc = 0;
+  int64_t pr_L = 0;
+  int64_t pr_tstride = 0;        // channel stride of d_tprod (0: one table for all)
+  std::vector<int32_t> pr_shape; // iomminprod, iommaxprod, nviniprod, nvmaxprod, jiniprod, jmaxprod
+  void* d_tprod = nullptr;       // T or float dense [nct][na][n1][n2]
+  double* d_ppart = nullptr;     // [nitems][64]
+  double* d_dtrans = nullptr;    // [nc][nomb][L]
+  double* d_ring_prod = nullptr; // [kRing][nomb][ne][L]
+  double* d_prodwf = nullptr;    // [nomb][nvj][ne][L]
+  double* d_prod_part = nullptr; // [nomb][nchunk][kRing][nvj]
+  double* d_ring_cvjp = nullptr; // [kRing][nomb * nvj]
+  double* d_cvjp_last = nullptr; // [nomb * nvj]
+  double* d_zs2prod = nullptr;   // [nomb * nvj][netot][2]
+  GemmPlan p_prod;
+
   // recursion state
   int k = 0;
   bool begun = false;
@@ -230,11 +255,23 @@ hipError_t launch_zero(Ctx* c, void* p, size_t bytes);
 hipError_t build_plans(Ctx* c);
 void free_plans(Ctx* c);
 hipError_t launch_flux_result(Ctx* c, double* d_out /* [(1+ne)][netot] */);
+// product projection: plan of the flush GEMM, per-step reduction into the ring
+hipError_t build_prod_plan(Ctx* c);
+hipError_t launch_prod_step(Ctx* c, int slot, hipStream_t s);
+hipError_t launch_prod_flush(Ctx* c);
 
 constexpr int kUpdRows = 32;  // rows per update work item
 
 // GEMM passes of the step (template tag: one kernel symbol per pass)
-enum GemmRole { G_ANG_FWD = 0, G_ANG_BWD, G_DST_R, G_DST_r_KIN, G_DST_r_INV, G_DST_R_INV };
+enum GemmRole {
+  G_ANG_FWD = 0,
+  G_ANG_BWD,
+  G_DST_R,
+  G_DST_r_KIN,
+  G_DST_r_INV,
+  G_DST_R_INV,
+  G_PROD  // product projection of the ring (flush)
+};
 
 enum KernelId {
   K_ANG_FWD = 0,
@@ -246,6 +283,7 @@ enum KernelId {
   K_PROJ = 6,
   K_ENERGY = 7,
   K_CORIOLIS = 8,  // FBR coefficients with the Omega +- 1 Coriolis stencil (k_fbr_coef)
+  K_PROD = 9,      // product projection: per-step reduction (the flush GEMM is in K_ENERGY)
 };
 
 void timer_begin(Ctx* c, int id, hipStream_t s = nullptr);

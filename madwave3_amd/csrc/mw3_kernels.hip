@@ -617,15 +617,21 @@ struct UpdArgs {
   float* out32;  // FP32 copy of the new psi (mixed precision), or null
   int ir2abs;
   double rsah2;
+  // product projection: per item and lane (R), sum over the item's rows of
+  // T(c, ia, r, R) psi_new (T = transRgjac, operator precision)
+  const void* tprod;
+  int64_t tp_cstride;
+  double* ppart;  // [nitems][64]
 };
 
 
 // MODE 0: hout = H psi; 1: first step (undamped); 2: damped update.  SINGLE:
 // one channel and one state (V diagonal).  PROJ: flux columns and reactant
-// slice.  AUTO: <psi_0|psi_new> partials.  All four are template parameters so
-// that no load sits behind a runtime branch (the compiler waits for every
-// load at the join of such a branch, which serialises the batch).
-template <typename TT, typename TV, int MODE, bool SINGLE, bool PROJ, bool AUTO>
+// slice.  AUTO: <psi_0|psi_new> partials.  PRODP: product-projection partials.
+// All are template parameters so that no load sits behind a runtime branch
+// (the compiler waits for every load at the join of such a branch, which
+// serialises the batch).
+template <typename TT, typename TV, int MODE, bool SINGLE, bool PROJ, bool AUTO, bool PRODP>
 __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
                                                double* __restrict__ out_buf,
                                                const TT* __restrict__ tkin,
@@ -647,14 +653,15 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
   const double f2 = a.absfr2[Ra];
   const int rflux = PROJ ? a.ir1flux[ia] - 1 : -1;
   const bool cvjcol = PROJ && a.slice && R == a.ir2abs - 1;
-  double s_norm = 0.0, s_auto = 0.0, zre = 0.0, zim = 0.0, zval = 0.0;
+  double s_norm = 0.0, s_auto = 0.0, zre = 0.0, zim = 0.0, zval = 0.0, sprod = 0.0;
+  const TT* tprod = PRODP ? (const TT*)a.tprod + (int64_t)c * a.tp_cstride : nullptr;
   // the item's rows, four at a time: the loads of the four rows and their
   // per-row tables are issued before any is used, with no branch around them
   for (int b = 0; b < cnt; b += 4) {
     bool act[4];
     int rr[4];
     int64_t pl[4];
-    double p[4], ob[4], tk[4], an[4], vv[4], f1[4], za[4], zb[4], p0[4];
+    double p[4], ob[4], tk[4], an[4], vv[4], f1[4], za[4], zb[4], p0[4], tp[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int idx = b + u < cnt ? b + u : cnt - 1;
@@ -675,6 +682,7 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
       za[u] = PROJ ? a.zar[2 * r] : 0.0;
       zb[u] = PROJ ? a.zar[2 * r + 1] : 0.0;
       p0[u] = AUTO ? a.psi0[ie] : 0.0;
+      tp[u] = PRODP ? (double)tprod[pl[u]] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -710,6 +718,7 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
       if (a.out32) a.out32[i] = (float)o;
       s_norm += o * o;
       if (AUTO) s_auto += p0[u] * o;
+      if (PRODP) sprod += tp[u] * o;
       if (PROJ) {
         zre += za[u] * o;
         zim += zb[u] * o;
@@ -724,6 +733,7 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
     up[64 + lane] = zim;
     up[128 + lane] = zval;
   }
+  if (PRODP) a.ppart[(int64_t)item * 64 + lane] = sprod;
   if (a.part) {
     for (int o2 = 32; o2 > 0; o2 >>= 1) {
       s_norm += __shfl_xor(s_norm, o2);
@@ -763,22 +773,26 @@ __global__ __launch_bounds__(64) void k_update_flux(const double* __restrict__ u
   }
 }
 
-template <typename T, int MODE, bool SINGLE, bool PROJ, bool AUTO>
+template <typename T, int MODE, bool SINGLE, bool PROJ, bool AUTO, bool PRODP = false>
 static void launch_update_kernel(Ctx* c, unsigned nb, const double* psi, double* out,
                                  const UpdArgs& a) {
-  hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO>), dim3(nb), dim3(256), 0,
+  hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO, PRODP>), dim3(nb), dim3(256), 0,
                      c->stream, psi, out, (const T*)c->d_t2, (const T*)c->d_ang,
                      (const T*)c->d_v, a);
 }
 
+// prodp (product projection) runs with the projection variant (PROJ) only
 template <typename T, bool SINGLE>
 static void dispatch_update(Ctx* c, unsigned nb, const double* psi, double* out, const UpdArgs& a,
-                            bool proj, bool autoc) {
+                            bool proj, bool autoc, bool prodp) {
   if (a.mode == 0)
     launch_update_kernel<T, 0, SINGLE, false, false>(c, nb, psi, out, a);
   else if (a.mode == 1)
     autoc ? launch_update_kernel<T, 1, SINGLE, false, true>(c, nb, psi, out, a)
           : launch_update_kernel<T, 1, SINGLE, false, false>(c, nb, psi, out, a);
+  else if (prodp)
+    autoc ? launch_update_kernel<T, 2, SINGLE, true, true, true>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 2, SINGLE, true, false, true>(c, nb, psi, out, a);
   else if (proj)
     autoc ? launch_update_kernel<T, 2, SINGLE, true, true>(c, nb, psi, out, a)
           : launch_update_kernel<T, 2, SINGLE, true, false>(c, nb, psi, out, a);
@@ -1222,7 +1236,8 @@ hipError_t build_plans(Ctx* c) {
 
 void free_plans(Ctx* c) {
 
-  GemmPlan* ps[6] = {&c->p_dstR, &c->p_dstr, &c->p_dstr2, &c->p_dstRi, &c->p_angF, &c->p_angB};
+  GemmPlan* ps[7] = {&c->p_dstR, &c->p_dstr, &c->p_dstr2, &c->p_dstRi, &c->p_angF, &c->p_angB,
+                     &c->p_prod};
   for (auto* p : ps)
     if (p->d_descs) {
       (void)hipFree(p->d_descs);
@@ -1367,6 +1382,10 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   a.ir2abs = c->s.ir2abs;
   a.rsah2 = 1.0 / std::sqrt(c->s.ah2);
   a.out32 = mode != 0 ? psi_shadow(c, c->d_psim1) : nullptr;
+  const bool prodp = mode == 2 && proj && c->prod_on;
+  a.tprod = prodp ? c->d_tprod : nullptr;
+  a.tp_cstride = c->pr_tstride;
+  a.ppart = c->d_ppart;
   timer_begin(c, K_UPDATE);
   double* out = mode == 0 ? c->d_hout : c->d_psim1;
   const bool projk = a.pflux || a.slice;
@@ -1374,9 +1393,9 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   const unsigned nblk4 = (unsigned)((nb + 3) / 4);  // one work item per wave
   if (nblk4 > 0) {
     if (t.ne == 1 && t.nc == 1)
-      dispatch_update<T, true>(c, nblk4, psi, out, a, projk, autoc);
+      dispatch_update<T, true>(c, nblk4, psi, out, a, projk, autoc, prodp);
     else
-      dispatch_update<T, false>(c, nblk4, psi, out, a, projk, autoc);
+      dispatch_update<T, false>(c, nblk4, psi, out, a, projk, autoc, prodp);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -1405,17 +1424,21 @@ hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj) {
 hipError_t launch_projections(Ctx* c, int slot) {
   const HostTables& t = c->t;
   hipError_t e = hipSuccess;
-  if (t.ncvj > 0) {
-    // on the second stream, after the update that wrote the slice
+  if (t.ncvj > 0 || c->prod_on) {
+    // on the second stream, after the update that wrote the slice and the
+    // product partials
     if ((e = hipEventRecord(c->ev_slice, c->stream)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(c->aux, c->ev_slice, 0)) != hipSuccess) return e;
-    timer_begin(c, K_PROJ, c->aux);
-    int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
-    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(64 * CVJ_WAVES), sizeof(double) * t.n1, c->aux,
-                       c->d_slice, c->d_dcvj, c->d_fdv, c->d_ring_cvj + (int64_t)slot * t.ncvj,
-                       nj, nv, t.na, t.n1);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    timer_end(c, K_PROJ, c->aux);
+    if (t.ncvj > 0) {
+      timer_begin(c, K_PROJ, c->aux);
+      int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
+      hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(64 * CVJ_WAVES), sizeof(double) * t.n1,
+                         c->aux, c->d_slice, c->d_dcvj, c->d_fdv,
+                         c->d_ring_cvj + (int64_t)slot * t.ncvj, nj, nv, t.na, t.n1);
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      timer_end(c, K_PROJ, c->aux);
+    }
+    if (c->prod_on && (e = launch_prod_step(c, slot, c->aux)) != hipSuccess) return e;
     if ((e = hipEventRecord(c->ev_cvj, c->aux)) != hipSuccess) return e;
     c->cvj_pending = true;
   }
@@ -1427,6 +1450,8 @@ hipError_t launch_energy_accum(Ctx* c) {
   int ns = c->ring_fill;
   if (ns == 0 || c->s.netot <= 0) return hipSuccess;
   timer_begin(c, K_ENERGY);
+  hipError_t e0 = c->prod_on ? launch_prod_flush(c) : hipSuccess;
+  if (e0 != hipSuccess) return e0;
   int netot = c->s.netot;
   hipLaunchKernelGGL(k_wmat, dim3(nblk(netot, 128), ns), dim3(128), 0, c->stream, c->d_wmat,
                      c->d_etot, netot, c->ring_k0, ns, c->s.hbr, c->s.emindlt, c->s.delta2);
@@ -1458,6 +1483,13 @@ hipError_t launch_energy_accum(Ctx* c) {
                        kmin);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
+  if (c->prod_on) {  // prodcvj: zS2prod += C_k(E) Cvjprod, every step
+    const int64_t nx = (int64_t)c->pr_nomb * c->pr_nvj;
+    dim3 g(nblk(netot, 128), (unsigned)nx);
+    hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_cvjp, 0,
+                       c->d_zs2prod, nx, netot, ns, c->ring_k0, nullptr, -1);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   timer_end(c, K_ENERGY);
   return hipSuccess;
 }
@@ -1469,6 +1501,116 @@ hipError_t launch_psie(Ctx* c, int k) {
                      c->d_psim1, c->d_efe, c->s.nofe, total, k, c->s.hbr, c->s.emindlt,
                      c->s.delta2);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// product-state projection (prodpaq / prodcvj, SRC/mod_coortrans01_02.f:694-822)
+// ---------------------------------------------------------------------------
+// funprod(c, ia, R) = sum over the update items of (c, ia, 64-R chunk) of their
+// partials (sum_r transRgjac psi_new), then for each product helicity o and
+// electronic state e the channel sum ring[o][e][l] = sum_{c: elec(c) = e}
+// dtrans[c][o][l] funprod(c, l), l = (ia - nangproj0) n2p + (R - n2prod0).
+// Block per (angle of the range, 64-R chunk of the range), lane = R.  Each
+// lane reads back only its own column of f, so no barrier is needed.
+constexpr int kProdMaxCh = 64;
+__global__ __launch_bounds__(64) void k_prod_fun(const double* __restrict__ ppart,
+                                                 const int32_t* __restrict__ gfirst,
+                                                 const double* __restrict__ dtrans,
+                                                 const int32_t* __restrict__ ch_elec,
+                                                 double* __restrict__ ring, int nc, int ne, int na,
+                                                 int n2, int nap0, int n2p0, int n2p, int nomb,
+                                                 int rc0, int nrcp, int64_t L) {
+  __shared__ double f[kProdMaxCh][64];
+  const int lane = threadIdx.x;
+  const int a = blockIdx.x / nrcp, rc = rc0 + blockIdx.x % nrcp;
+  const int ia = nap0 + a;
+  const int R = rc * 64 + lane;
+  const int nrc = (n2 + 63) / 64;
+  for (int ch = 0; ch < nc; ch++) {
+    const int g = (ch * na + ia) * nrc + rc;
+    double s = 0.0;
+    for (int it = gfirst[g]; it < gfirst[g + 1]; it++) s += ppart[(int64_t)it * 64 + lane];
+    f[ch][lane] = s;
+  }
+  if (R < n2p0 || R >= n2p0 + n2p) return;
+  const int64_t l = (int64_t)a * n2p + (R - n2p0);
+  for (int o = 0; o < nomb; o++)
+    for (int e = 0; e < ne; e++) {
+      double acc = 0.0;
+      for (int ch = 0; ch < nc; ch++)
+        if (ch_elec[ch] == e) acc += f[ch][lane] * dtrans[((int64_t)ch * nomb + o) * L + l];
+      ring[((int64_t)o * ne + e) * L + l] = acc;
+    }
+}
+
+// Cvjprod[s][o][vj] = sum over the K chunks of the flush GEMM's partials
+// part[o][chunk][kRing][nvj]; the last step's row is kept for mw3_cvjprod_get
+__global__ void k_prod_sum(const double* __restrict__ part, double* __restrict__ cvjp,
+                           double* __restrict__ last, int nomb, int nvj, int nchunk, int ns) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, s = blockIdx.y;
+  if (x >= nomb * nvj || s >= ns) return;
+  const int o = x / nvj, vj = x % nvj;
+  double acc = 0.0;
+  for (int ch = 0; ch < nchunk; ch++)
+    acc += part[(((int64_t)o * nchunk + ch) * kRing + s) * nvj + vj];
+  cvjp[(int64_t)s * nomb * nvj + x] = acc;
+  if (s == ns - 1) last[x] = acc;
+}
+
+hipError_t launch_prod_step(Ctx* c, int slot, hipStream_t s) {
+  const HostTables& t = c->t;
+  const int rc0 = c->pr_n2p0 / 64, rc1 = (c->pr_n2p0 + c->pr_n2p - 1) / 64;
+  const int nrcp = rc1 - rc0 + 1;
+  timer_begin(c, K_PROD, s);
+  hipLaunchKernelGGL(k_prod_fun, dim3((unsigned)(c->pr_nap * nrcp)), dim3(64), 0, s, c->d_ppart,
+                     c->d_uw_gfirst, c->d_dtrans, c->d_ch_elec,
+                     c->d_ring_prod + (int64_t)slot * c->pr_nomb * t.ne * c->pr_L, t.nc, t.ne,
+                     t.na, t.n2, c->pr_nap0, c->pr_n2p0, c->pr_n2p, c->pr_nomb, rc0, nrcp, c->pr_L);
+  hipError_t e = hipGetLastError();
+  timer_end(c, K_PROD, s);
+  return e;
+}
+
+// (timed with the energy accumulation it belongs to)
+hipError_t launch_prod_flush(Ctx* c) {
+  const int ns = c->ring_fill;
+  hipError_t e = run_gemm<G_PROD, double, double, double, double>(
+      c->stream, c->d_ring_prod, c->d_prodwf, c->d_prod_part, c->p_prod);
+  if (e == hipSuccess) {
+    const int nx = c->pr_nomb * c->pr_nvj;
+    hipLaunchKernelGGL(k_prod_sum, dim3(nblk(nx, 128), ns), dim3(128), 0, c->stream,
+                       c->d_prod_part, c->d_ring_cvjp, c->d_cvjp_last, c->pr_nomb, c->pr_nvj,
+                       c->pr_nchunk, ns);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+// flush GEMM: per (o, K chunk) C[s][vj] = sum_k ring[s][o][k] prodwf[o][vj][k]
+// over k = (e, l) of the chunk, into part[o][chunk][kRing][nvj]
+hipError_t build_prod_plan(Ctx* c) {
+  const int64_t K = (int64_t)c->t.ne * c->pr_L;
+  std::vector<GemmDesc> ds;
+  for (int o = 0; o < c->pr_nomb; o++)
+    for (int ch = 0; ch < c->pr_nchunk; ch++) {
+      GemmDesc d = gd();
+      d.a_off = (int64_t)o * K + (int64_t)ch * c->pr_kc;
+      d.lda_m = (int)(c->pr_nomb * K);
+      d.lda_k = 1;
+      d.b_off = (int64_t)o * c->pr_nvj * K + (int64_t)ch * c->pr_kc;
+      d.c_off = ((int64_t)o * c->pr_nchunk + ch) * kRing * c->pr_nvj;
+      d.M = kRing;
+      d.N = c->pr_nvj;
+      d.K = (int)std::min<int64_t>(c->pr_kc, K - (int64_t)ch * c->pr_kc);
+      ds.push_back(d);
+    }
+  c->p_prod.st.ldb_k = 1;
+  c->p_prod.st.ldb_n = K;
+  c->p_prod.st.ldc_m = c->pr_nvj;
+  c->p_prod.st.ldc_n = 1;
+  hipError_t e = upload_plan(&c->p_prod, ds, true);
+  c->p_prod.vec = 0;  // generic MFMA kernel: B is K-contiguous here
+  return e;
 }
 
 // P(E) and per-state sums (totalflux, SRC/mod_flux_01y2.f:778-816):

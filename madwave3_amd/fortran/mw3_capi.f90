@@ -3,6 +3,7 @@
 ! Module mw3_capi gives a Fortran host (MADWAVE3's main program or the
 ! mod_Hphi_01y2 replacement in INTEGRATION.md) the C ABI of libmw3:
 !   type(mw3_setup)        the C struct mw3_setup, field for field (bind(C));
+!   type(mw3_prod_setup)   the C struct mw3_prod_setup (product projection);
 !   mw3_create ... mw3_kernel_name   one interface per C entry point;
 !   mw3_error_message(ctx) the message of mw3_last_error as a Fortran string;
 !   mw3_check(rc, ctx, where)  the reference's error convention: write the
@@ -59,6 +60,15 @@ module mw3_capi
     integer(c_int32_t) :: nofe
     type(c_ptr) :: efe
   end type mw3_setup
+
+  ! struct mw3_prod_setup: the product-projection tables (iprod = 2)
+  type, bind(C) :: mw3_prod_setup
+    integer(c_int32_t) :: n2prod0, n2prod1, nangproj0, nangproj1
+    integer(c_int32_t) :: iomminprod, iommaxprod
+    integer(c_int32_t) :: nviniprod, nvmaxprod, jiniprod, jmaxprod
+    type(c_ptr) :: noBCprod
+    type(c_ptr) :: transRgjac, prodwf, dtrans
+  end type mw3_prod_setup
 
   interface
     function mw3_abi_version() bind(C, name="mw3_abi_version") result(v)
@@ -178,6 +188,34 @@ module mw3_capi
       type(c_ptr), value :: zcr, zcdr, zs2
       integer(c_int32_t) :: rc
     end function mw3_flux_accum_put
+
+    function mw3_prod_attach(ctx, prod) bind(C, name="mw3_prod_attach") result(rc)
+      import :: c_ptr, c_int32_t, mw3_prod_setup
+      type(c_ptr), value :: ctx
+      type(mw3_prod_setup), intent(in) :: prod
+      integer(c_int32_t) :: rc
+    end function mw3_prod_attach
+
+    function mw3_zs2prod_get(ctx, zs2prod) bind(C, name="mw3_zs2prod_get") result(rc)
+      import :: c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: zs2prod(*)
+      integer(c_int32_t) :: rc
+    end function mw3_zs2prod_get
+
+    function mw3_zs2prod_put(ctx, zs2prod) bind(C, name="mw3_zs2prod_put") result(rc)
+      import :: c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: zs2prod(*)
+      integer(c_int32_t) :: rc
+    end function mw3_zs2prod_put
+
+    function mw3_cvjprod_get(ctx, cvjprod) bind(C, name="mw3_cvjprod_get") result(rc)
+      import :: c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: cvjprod(*)
+      integer(c_int32_t) :: rc
+    end function mw3_cvjprod_get
 
     function mw3_psie_get(ctx, zfe) bind(C, name="mw3_psie_get") result(rc)
       import :: c_ptr, c_double, c_int32_t

@@ -8,17 +8,26 @@
 !     members as their value, pointer members as an int64 element count
 !     followed by the elements (int32 or double);
 !   int64 n, double psi0(n)   the initial packet in A0 layout;
-!   int32 nsteps, int32 precision (MW3_FP64 or MW3_MIXED).
-! The program runs mw3_cheby_begin and nsteps steps of mw3_cheby_steps and
-! writes to the result file (stream): int32 nsteps, double norms(nsteps),
-! int32 netot, double P(netot) (mw3_flux_result), int64 m, double zS2(m)
-! (mw3_zs2_get, interleaved complex).  It is the Fortran side of the C ABI
-! exercised end to end; errors stop with the library's message.
+!   int32 nsteps, int32 precision (MW3_FP64 or MW3_MIXED);
+!   int32 hasprod, and when 1 every member of struct mw3_prod_setup in the
+!     same encoding (the product-projection tables, iprod = 2).
+! The program runs mw3_cheby_begin (after mw3_prod_attach when hasprod) and
+! nsteps steps of mw3_cheby_steps and writes to the result file (stream):
+! int32 nsteps, double norms(nsteps), int32 netot, double P(netot)
+! (mw3_flux_result), int64 m, double zS2(m) (mw3_zs2_get, interleaved
+! complex), int64 mp, double zS2prod(mp) (mw3_zs2prod_get; mp = 0 without
+! product tables).  It is the Fortran side of the C ABI exercised end to end;
+! errors stop with the library's message.
 program mw3_fdrive
   use, intrinsic :: iso_c_binding
   use mw3_capi
   implicit none
   type(mw3_setup), target :: s
+  type(mw3_prod_setup), target :: ps
+  integer(c_int32_t), allocatable, target :: nobcprod(:)
+  real(c_double), allocatable, target :: transrgjac(:), prodwf(:), dtrans(:), zs2prod(:)
+  integer(c_int32_t) :: hasprod
+  integer(c_int64_t) :: mp
   integer(c_int32_t), allocatable, target :: iomdiat(:), iomatom(:), indangreal(:), &
       npunreal(:), indr1(:), indr2(:), iombas(:), nelebas(:), nojbas(:), j00(:), &
       jbas(:), noBCstates(:)
@@ -40,8 +49,8 @@ program mw3_fdrive
   call get_command_argument(2, fout)
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) magic, ver
-  if (magic /= 'MW3D' .or. ver /= 1) then
-    write(6, *) 'mw3_fdrive: not a version-1 case file: ', trim(fin)
+  if (magic /= 'MW3D' .or. ver /= 2) then
+    write(6, *) 'mw3_fdrive: not a version-2 case file: ', trim(fin)
     stop 2
   end if
   read(u) s%abi_version
@@ -88,6 +97,16 @@ program mw3_fdrive
   allocate(psi0(n))
   read(u) psi0
   read(u) nsteps, prec
+  read(u) hasprod
+  if (hasprod == 1) then
+    read(u) ps%n2prod0, ps%n2prod1, ps%nangproj0, ps%nangproj1
+    read(u) ps%iomminprod, ps%iommaxprod
+    read(u) ps%nviniprod, ps%nvmaxprod, ps%jiniprod, ps%jmaxprod
+    call rdi(nobcprod, ps%noBCprod)
+    call rdd(transrgjac, ps%transRgjac)
+    call rdd(prodwf, ps%prodwf)
+    call rdd(dtrans, ps%dtrans)
+  end if
   close(u)
 
   if (mw3_abi_version() /= MW3_ABI_EXPECTED) then
@@ -100,6 +119,7 @@ program mw3_fdrive
     write(6, *) 'mw3_fdrive: psi0 length differs from mw3_ntot'
     stop 1
   end if
+  if (hasprod == 1) call mw3_check(mw3_prod_attach(ctx, ps), ctx, 'mw3_prod_attach')
   call mw3_check(mw3_cheby_begin(ctx, psi0, c_null_ptr), ctx, 'mw3_cheby_begin')
   allocate(norms(max(nsteps, 1)), ptot(max(s%netot, 1)))
   call mw3_check(mw3_cheby_steps(ctx, nsteps, c_loc(norms), c_null_ptr), ctx, &
@@ -110,12 +130,21 @@ program mw3_fdrive
   m = 2_c_int64_t * s%ncan * nj * nv * s%netot
   allocate(zs2(max(m, 1_c_int64_t)))
   call mw3_check(mw3_zs2_get(ctx, zs2), ctx, 'mw3_zs2_get')
+  mp = 0
+  if (hasprod == 1) then
+    mp = 2_c_int64_t * s%netot * (ps%nvmaxprod - ps%nviniprod + 1) * &
+         (ps%jmaxprod - ps%jiniprod + 1) * (ps%iommaxprod - ps%iomminprod + 1)
+    allocate(zs2prod(mp))
+    call mw3_check(mw3_zs2prod_get(ctx, zs2prod), ctx, 'mw3_zs2prod_get')
+  end if
   call mw3_destroy(ctx)
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) nsteps, norms(1:nsteps)
   write(u) s%netot, ptot(1:s%netot)
   write(u) m, zs2(1:m)
+  write(u) mp
+  if (mp > 0) write(u) zs2prod(1:mp)
   close(u)
 
 contains

@@ -4,14 +4,17 @@
 ! analysis line needs them.
 !
 ! Calls, in the order of SRC/main_madwave3.f:
-!   after ini_flux and input_PsiE (188-192):  call mw3_cheby_init(kminelastic)
+!   after ini_flux, input_PsiE and ini_transcoor (188-197):
+!                                             call mw3_cheby_init(kminelastic)
 !   in place of 231-250 (first step or restart):
 !                                             call mw3_cheby_start(iit0)
 !   in place of the "do it" loop (324-368):    call mw3_cheby_block(ntimes, xnorm)
 !   before totalflux / wvpcont / printprob (375-380):
 !                                             call mw3_cheby_sync
-! The operator context of mod_Hphi_01y2 (set_vectors_Hphi) stays usable for
-! difs.  Single process (nproc = 1).
+! With iprod = 2 the product projection of prodpaq / prodcvj runs on the GPU
+! too, on the tables ini_transcoor built (module mod_coortrans01_02).  The
+! operator context of mod_Hphi_01y2 (set_vectors_Hphi) stays usable for difs.
+! Single process (nproc = 1).
 module mod_mw3_cheby
   use, intrinsic :: iso_c_binding
   use mw3_capi
@@ -21,6 +24,7 @@ module mod_mw3_cheby
   use mod_Hphi_01y2
   use mod_flux_01y2
   use mod_PsiE_01y2
+  use mod_coortrans01_02
   implicit none
   save
   private
@@ -56,7 +60,37 @@ contains
     deallocate(k_etot, k_reacfct)
     if (allocated(k_efe)) deallocate(k_efe)
     call mw3_check(rc, c_null_ptr, 'mw3_cheby_init: mw3_create')
+    if (iprod == 2 .and. npun1 > 1 .and. npun2 > 1 .and. allocated(transRgjac)) &
+      call mw3_prod_tables()
   end subroutine mw3_cheby_init
+
+  ! the tables of ini_transcoor (SRC/mod_coortrans01_02.f:31-209) to the
+  ! context: prodpaq and prodcvj then run in every step
+  subroutine mw3_prod_tables()
+    type(mw3_prod_setup) :: ps
+    integer(c_int32_t), allocatable, target :: k_nobc(:)
+    real(c_double), allocatable, target :: k_tr(:), k_wf(:), k_dt(:)
+    ps%n2prod0 = n2prod0
+    ps%n2prod1 = n2prod1
+    ps%nangproj0 = nangproj0
+    ps%nangproj1 = nangproj1
+    ps%iomminprod = iomminprod
+    ps%iommaxprod = iommaxprod
+    ps%nviniprod = nviniprod
+    ps%nvmaxprod = nvmaxprod
+    ps%jiniprod = jiniprod
+    ps%jmaxprod = jmaxprod
+    k_nobc = noBCprod(jiniprod:jmaxprod)
+    k_tr = transRgjac(1:nbastotproc)
+    k_wf = reshape(prodwf, [size(prodwf)])
+    k_dt = reshape(dtrans(:, :, :, 1:ncan), [size(dtrans(:, :, :, 1:ncan))])
+    ps%noBCprod = c_loc(k_nobc)
+    ps%transRgjac = c_loc(k_tr)
+    ps%prodwf = c_loc(k_wf)
+    ps%dtrans = c_loc(k_dt)
+    call mw3_check(mw3_prod_attach(mw3_pctx, ps), mw3_pctx, 'mw3_cheby_init: mw3_prod_attach')
+    deallocate(k_nobc, k_tr, k_wf, k_dt)
+  end subroutine mw3_prod_tables
 
   ! iit0 = 0: psi_0 = rpaq0 and the undamped first step (main_madwave3.f:
   ! 231-250); iit0 > 0: continue from rflanz0proc, rpaqproc read from the
@@ -70,8 +104,43 @@ contains
     else
       call mw3_check(mw3_state_put(mw3_pctx, rflanz0proc, rpaqproc, int(iit0, c_int32_t), &
                                    c_null_ptr), mw3_pctx, 'mw3_cheby_start: mw3_state_put')
+      ! accumulators read from contNN.S2 / contNN.S2prod by ini_flux and
+      ! ini_transcoor (SRC/mod_flux_01y2.f:311-322, mod_coortrans01_02.f:186-193)
+      if (ncontfile == 1) then
+        call mw3_accum_put()
+        if (allocated(zS2prod) .and. iprod == 2) call mw3_check(mw3_zs2prod_put(mw3_pctx, &
+            zs2prod_flat()), mw3_pctx, 'mw3_cheby_start: mw3_zs2prod_put')
+      end if
     end if
   end subroutine mw3_cheby_start
+
+  subroutine mw3_accum_put()
+    complex(c_double_complex), allocatable, target :: b1(:, :), b2(:, :), b3(:)
+    integer(c_int64_t) :: nf
+    type(c_ptr) :: p3
+    nf = mw3_nflux(mw3_pctx)
+    allocate(b1(netot, max(nf, 1_c_int64_t)), b2(netot, max(nf, 1_c_int64_t)))
+    if (nf > 0) then
+      b1 = zCR(1:netot, 1:nf)
+      b2 = zCdR(1:netot, 1:nf)
+    end if
+    p3 = c_null_ptr
+    if (allocated(zS2)) then
+      b3 = reshape(zS2(:, :, :, 1:ncan), [size(zS2(:, :, :, 1:ncan))])
+      p3 = c_loc(b3)
+    end if
+    call mw3_check(mw3_flux_accum_put(mw3_pctx, c_loc(b1), c_loc(b2), p3), mw3_pctx, &
+                   'mw3_cheby_start: mw3_flux_accum_put')
+  end subroutine mw3_accum_put
+
+  function zs2prod_flat() result(z)
+    real(c_double), allocatable :: z(:)
+    complex(c_double_complex), allocatable :: t(:)
+    t = reshape(zS2prod, [size(zS2prod)])
+    allocate(z(2 * size(t)))
+    z(1::2) = real(t)
+    z(2::2) = aimag(t)
+  end function zs2prod_flat
 
   ! ntimes steps it = iit0+1 .. iit0+ntimes of main_madwave3.f:324-368 with
   ! difs, the damped update, CoefEcheby, totalflux_k, Cvjflux_k, check and funE;
@@ -129,6 +198,17 @@ contains
       call mw3_check(mw3_psie_get(mw3_pctx, z), mw3_pctx, 'mw3_cheby_sync: mw3_psie_get')
       zfe(1:nbastotproc, 1:nofe) = reshape(cmplx(z(1::2), z(2::2), kind=c_double_complex), &
                                            [nbastotproc, nofe])
+      deallocate(z)
+    end if
+    if (iprod == 2 .and. allocated(zS2prod) .and. allocated(transRgjac)) then
+      ! zS2prod and the last step's Cvjprod of prodpaq / prodcvj
+      allocate(z(2 * size(zS2prod)))
+      call mw3_check(mw3_zs2prod_get(mw3_pctx, z), mw3_pctx, 'mw3_cheby_sync: mw3_zs2prod_get')
+      zS2prod = reshape(cmplx(z(1::2), z(2::2), kind=c_double_complex), shape(zS2prod))
+      deallocate(z)
+      allocate(z(size(Cvjprod)))
+      call mw3_check(mw3_cvjprod_get(mw3_pctx, z), mw3_pctx, 'mw3_cheby_sync: mw3_cvjprod_get')
+      Cvjprod = reshape(z, shape(Cvjprod))
       deallocate(z)
     end if
     call mw3_check(mw3_state_get(mw3_pctx, rflanz0proc, rpaqproc, k), mw3_pctx, &

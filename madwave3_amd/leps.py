@@ -25,9 +25,9 @@ import numpy as np
 
 from . import consts as C
 from . import namelist as NL
-from .refio import FuncData, PotData
+from .refio import FuncData, PotData, ProdFuncData
 
-__all__ = ["LEPS", "diatomic_states", "make_inputs", "config"]
+__all__ = ["LEPS", "diatomic_states", "product_states", "make_inputs", "config"]
 
 
 class LEPS:
@@ -88,10 +88,44 @@ def diatomic_states(vdiat_zot, npun1, ah1, rmis1, mu, j, nstates):
     return e, v
 
 
+def product_states(p, pes):
+    """func/prodwf content for the 0-2 diatom (one electronic state): its
+    bound states below the dissociation limit on the R grid (n2prod1 =
+    npun2 points), v = 0..nvmaxprod, j = 0..jmaxprod, as the reference's
+    preparation program writes them for iprod > 0
+    (SRC/mod_baseYfunciones_01y2.f:1083-1279)."""
+    if p["nelec"] != 1:
+        raise ValueError("product states are generated for one electronic state")
+    conv = C.convl
+    npun2 = p["npun2"]
+    rmis2, rfin2 = p["rmis2_bohr"] * conv, p["rfin2_bohr"] * conv
+    ah2 = (rfin2 - rmis2) / (npun2 - 1)
+    mu02 = p["xm0"] * p["xm2"] / (p["xm0"] + p["xm2"])
+    h2z = 1.0 / C.conve * C.conve1
+    n2p1 = int(p.get("n2prod1", npun2))
+    R = rmis2 + np.arange(n2p1) * ah2
+    vd = pes.morse(R / conv) * h2z
+    nobc, ediat, elec, vibel, fd = [], {}, {}, {}, {}
+    for j in range(0, p["jmaxprod"] + 1):
+        ev, v = diatomic_states(vd, n2p1, ah2, rmis2, mu02, j, p["nvmaxprod"] + 1)
+        nb = -1
+        for iv in range(0, p["nvmaxprod"] + 1):
+            if ev[iv] < 0.0:
+                ediat[(j, iv)] = ev[iv]
+                elec[(j, iv)] = 1
+                vibel[(j, iv)] = float(iv)
+                fd[(j, iv)] = v[:, iv].reshape(n2p1, 1)
+                nb = iv
+        nobc.append(nb)
+    return ProdFuncData(0, p["jmaxprod"], 0, n2p1, 1, nobc, ediat, elec, vibel, fd)
+
+
 def config(cfg_id, scale=1.0, J=0):
     """Parameters of the benchmark configurations (grid sizes may be divided
-    by `scale` for small test cases).  Configuration 4 is one total angular
-    momentum J of the batched J = 0..23 workload."""
+    by `scale` for small test cases).  Configurations 1 and 2 resolve the
+    products (iprod = 2: HD(v' <= 3, j' <= 15) at R' = 4 bohr).
+    Configuration 4 is one total angular momentum J of the batched J = 0..23
+    workload."""
     base = dict(
         xm0=2.01410, xm1=1.00783, xm2=1.00783,   # D, H, H (amu)
         De_eV=4.75, a_invA=1.94, re_A=0.742, sato=0.17,
@@ -102,14 +136,15 @@ def config(cfg_id, scale=1.0, J=0):
         nvini=0, nvmax=3, jini=0, jmax=10, nvref=0, jref=0, iomref=0,
         Jtot=0, iparity=1, iommin=0, iommax=0, nelec=1,
         npun1min=32, kminelastic=100, rmis1_bohr=0.1, rmis2_bohr=0.1,
+        iprod=0, nvmaxprod=3, jmaxprod=15, Rbalinprod_bohr=4.0,
     )
     base["Jtot"] = J
     if cfg_id == 1:
         base.update(npun1=128, rfin1_bohr=8.0, npun2=256, rfin2_bohr=15.0,
-                    nangu=60, ntimes=500, nloop=39, r1flux_bohr=4.0)
+                    nangu=60, ntimes=500, nloop=39, r1flux_bohr=4.0, iprod=2)
     elif cfg_id == 2:
         base.update(npun1=256, rfin1_bohr=8.0, npun2=512, rfin2_bohr=15.0,
-                    nangu=120, ntimes=500, nloop=79, r1flux_bohr=4.0)
+                    nangu=120, ntimes=500, nloop=79, r1flux_bohr=4.0, iprod=2)
     elif cfg_id == 5:
         base.update(npun1=256, rfin1_bohr=8.0, npun2=384, rfin2_bohr=15.0,
                     nangu=96, ntimes=500, nloop=119, r1flux_bohr=4.0, nelec=3,
@@ -134,6 +169,7 @@ def config(cfg_id, scale=1.0, J=0):
             base[k] = max(8, int(round(base[k] / scale)))
         base["npun1min"] = min(base["npun1min"], base["npun1"] // 2)
         base["jmax"] = min(base["jmax"], base["nangu"] - 1)
+        base["jmaxprod"] = min(base["jmaxprod"], base["nangu"] - 1)
     return base
 
 
@@ -172,6 +208,8 @@ def make_inputs(p):
             nobc[e - 1][j - p["jini"]] = nb
     eref = ediat[(1, p["jref"], p["nvref"])]
     fdat = FuncData(nel, p["jini"], p["jmax"], p["nvini"], npun1, nobc, ediat, fd, eref)
+    if p.get("iprod", 0) == 2:
+        fdat.prod = product_states(p, pes)
 
     # potential on the grid, compressed as pot1 does (SRC/mod_pot_01y2.f:175-279)
     from .numerics import gauleg
@@ -243,7 +281,16 @@ def make_inputs(p):
     text += NL.dump_group("inputabsorcion", dict(
         absalp1=p["absalp"], absr1=absr1, n1expo=p["nexpo"], ninterv1=1,
         absalp2=p["absalp"], absr2=absr2, n2expo=p["nexpo"], ninterv2=1))
-    text += NL.dump_group("inputprod", dict(iprod=0))
+    if p.get("iprod", 0) == 2:
+        prod = dict(iprod=2, nviniprod=0, nvmaxprod=p["nvmaxprod"], jiniprod=0,
+                    jmaxprod=p["jmaxprod"], iomminprod=0,
+                    Rbalinprod=p["Rbalinprod_bohr"] * conv)
+        for k in ("n2prod0", "n2prod1", "nangproj0", "nangproj1", "iommaxprod"):
+            if k in p:
+                prod[k] = p[k]
+        text += NL.dump_group("inputprod", prod)
+    else:
+        text += NL.dump_group("inputprod", dict(iprod=0))
     text += NL.dump_group("inputtime", dict(ntimes=p["ntimes"], nloop=p["nloop"],
                                             kminelastic=p["kminelastic"]))
     text += NL.dump_group("inputpotmass", dict(

@@ -11,11 +11,12 @@ import os
 
 import numpy as np
 
-__all__ = ["Mw3Setup", "make_setup", "load_library", "Propagator", "Mw3Error",
+__all__ = ["Mw3Setup", "Mw3ProdSetup", "make_setup", "make_prod_setup", "load_library",
+           "Propagator", "Mw3Error",
            "MW3_FP64", "MW3_MIXED", "LIB_PATH"]
 
 MW3_ABI_VERSION = 1
-MW3_NKERNELS = 9  # include/mw3.h
+MW3_NKERNELS = 10  # include/mw3.h
 MW3_FP64 = 0
 MW3_MIXED = 1
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmw3.so")
@@ -64,6 +65,19 @@ class Mw3Setup(ct.Structure):
         ("iomref", ct.c_int32),
         ("nofe", ct.c_int32),
         ("efe", _pd),
+    ]
+
+
+class Mw3ProdSetup(ct.Structure):
+    """Mirror of struct mw3_prod_setup (include/mw3.h)."""
+    _fields_ = [
+        ("n2prod0", ct.c_int32), ("n2prod1", ct.c_int32),
+        ("nangproj0", ct.c_int32), ("nangproj1", ct.c_int32),
+        ("iomminprod", ct.c_int32), ("iommaxprod", ct.c_int32),
+        ("nviniprod", ct.c_int32), ("nvmaxprod", ct.c_int32),
+        ("jiniprod", ct.c_int32), ("jmaxprod", ct.c_int32),
+        ("noBCprod", _pi),
+        ("transRgjac", _pd), ("prodwf", _pd), ("dtrans", _pd),
     ]
 
 
@@ -116,40 +130,77 @@ def make_setup(c):
     return s, keep
 
 
+def make_prod_setup(c):
+    """mw3_prod_setup from the product tables of a case built with iprod = 2
+    (madwave3_amd.prodtrans); returns (struct, keep-alive list)."""
+    keep = []
+    p = Mw3ProdSetup()
+    for k in ("n2prod0", "n2prod1", "nangproj0", "nangproj1", "iomminprod", "iommaxprod",
+              "nviniprod", "nvmaxprod", "jiniprod", "jmaxprod"):
+        setattr(p, k, int(getattr(c, k)))
+    p.noBCprod = _arr(keep, c.noBCprod, np.int32)
+    p.transRgjac = _arr(keep, c.transRgjac, np.float64)
+    p.prodwf = _arr(keep, c.prodwf, np.float64)
+    p.dtrans = _arr(keep, c.dtrans, np.float64)
+    return p, keep
+
+
+def has_prod_tables(c):
+    """The case projects on product states (iprod = 2, npun1 > 1)."""
+    return getattr(c, "iprod", 0) == 2 and hasattr(c, "transRgjac")
+
+
+def prod_shape(c):
+    """(nomb, nj', nv') of the product-state arrays of a case."""
+    return (c.iommaxprod - c.iomminprod + 1, c.jmaxprod - c.jiniprod + 1,
+            c.nvmaxprod - c.nviniprod + 1)
+
+
 def write_case_file(path, case, psi0, nsteps, precision):
     """Write the case file read by the Fortran host madwave3_amd/fortran/mw3_fdrive
     (format in its header): the mw3_setup members in declaration order, pointer
     members as an int64 count and their elements, then psi0, nsteps and the
-    precision."""
-    s, keep = make_setup(case)
-    by_addr = {a.ctypes.data: a for a in keep}
-    out = [b"MW3D", np.int32(1).tobytes()]
-    for name, typ in Mw3Setup._fields_:
-        v = getattr(s, name)
-        if typ is ct.c_int32:
-            out.append(np.int32(v).tobytes())
-        elif typ is ct.c_double:
-            out.append(np.float64(v).tobytes())
-        else:
-            addr = ct.cast(v, ct.c_void_p).value
-            a = by_addr.get(addr) if addr else None
-            if a is None:
-                out.append(np.int64(0).tobytes())
+    precision, then the product tables (mw3_prod_setup) when the case has them."""
+    def struct_bytes(st, keep, fields):
+        by_addr = {a.ctypes.data: a for a in keep}
+        out = []
+        for name, typ in fields:
+            v = getattr(st, name)
+            if typ is ct.c_int32:
+                out.append(np.int32(v).tobytes())
+            elif typ is ct.c_double:
+                out.append(np.float64(v).tobytes())
             else:
-                a = np.ascontiguousarray(a).ravel()
-                out.append(np.int64(a.size).tobytes())
-                out.append(a.astype(np.int32 if typ is _pi else np.float64).tobytes())
+                addr = ct.cast(v, ct.c_void_p).value
+                a = by_addr.get(addr) if addr else None
+                if a is None:
+                    out.append(np.int64(0).tobytes())
+                else:
+                    a = np.ascontiguousarray(a).ravel()
+                    out.append(np.int64(a.size).tobytes())
+                    out.append(a.astype(np.int32 if typ is _pi else np.float64).tobytes())
+        return out
+
+    s, keep = make_setup(case)
+    out = [b"MW3D", np.int32(2).tobytes()] + struct_bytes(s, keep, Mw3Setup._fields_)
     psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
     out.append(np.int64(psi0.size).tobytes())
     out.append(psi0.tobytes())
     out.append(np.int32(nsteps).tobytes())
     out.append(np.int32(precision).tobytes())
+    if has_prod_tables(case):
+        ps, pkeep = make_prod_setup(case)
+        out.append(np.int32(1).tobytes())
+        out += struct_bytes(ps, pkeep, Mw3ProdSetup._fields_)
+    else:
+        out.append(np.int32(0).tobytes())
     with open(path, "wb") as f:
         f.write(b"".join(out))
 
 
 def read_fdrive_result(path):
-    """(norms, P, zS2 interleaved) from a mw3_fdrive result file."""
+    """(norms, P, zS2 interleaved, zS2prod interleaved or empty) from a
+    mw3_fdrive result file."""
     b = open(path, "rb").read()
     o = 0
     ns = int(np.frombuffer(b, np.int32, 1, o)[0]); o += 4
@@ -157,8 +208,10 @@ def read_fdrive_result(path):
     ne = int(np.frombuffer(b, np.int32, 1, o)[0]); o += 4
     P = np.frombuffer(b, np.float64, ne, o).copy(); o += 8 * ne
     m = int(np.frombuffer(b, np.int64, 1, o)[0]); o += 8
-    z = np.frombuffer(b, np.float64, m, o).copy()
-    return norms, P, z
+    z = np.frombuffer(b, np.float64, m, o).copy(); o += 8 * m
+    mp = int(np.frombuffer(b, np.int64, 1, o)[0]); o += 8
+    zp = np.frombuffer(b, np.float64, mp, o).copy()
+    return norms, P, z, zp
 
 
 class Mw3Error(RuntimeError):
@@ -234,6 +287,10 @@ def load_library(path=None):
     lib.mw3_cheby_steps_batch.restype = ct.c_int32
     lib.mw3_cheby_steps_batch.argtypes = [ct.POINTER(ct.c_void_p), ct.c_int32, ct.c_int32,
                                           ct.c_void_p]
+    lib.mw3_prod_attach.argtypes = [ct.c_void_p, ct.POINTER(Mw3ProdSetup)]
+    lib.mw3_zs2prod_get.argtypes = [ct.c_void_p, _pd]
+    lib.mw3_zs2prod_put.argtypes = [ct.c_void_p, _pd]
+    lib.mw3_cvjprod_get.argtypes = [ct.c_void_p, _pd]
     lib.mw3_kernel_name.restype = ct.c_char_p
     lib.mw3_kernel_name.argtypes = [ct.c_int32]
     if lib.mw3_abi_version() != MW3_ABI_VERSION:
@@ -250,7 +307,9 @@ def _dp(a):
 class Propagator:
     """One libmw3 context: the drop-in for mod_Hphi_01y2 + the recursion."""
 
-    def __init__(self, case, precision=MW3_FP64, lib=None):
+    def __init__(self, case, precision=MW3_FP64, lib=None, prod=None):
+        """prod: attach the case's product tables (default: when the case was
+        built with iprod = 2, as the reference calls ini_transcoor)."""
         self.lib = lib or load_library()
         self.case = case
         self._setup, self._keep = make_setup(case)
@@ -261,6 +320,10 @@ class Propagator:
         self.h = h
         self.ntot = int(self.lib.mw3_ntot(h))
         self.nflux = int(self.lib.mw3_nflux(h))
+        if prod is None:
+            prod = has_prod_tables(case)
+        if prod:
+            self.prod_attach()
 
     def _check(self, rc, what):
         if rc != 0:
@@ -325,6 +388,26 @@ class Propagator:
         a, b, s = inter(zcr), inter(zcdr), inter(zs2)
         self._check(self.lib.mw3_flux_accum_put(self.h, _dp(a), _dp(b), _dp(s)),
                     "mw3_flux_accum_put")
+
+    def prod_attach(self):
+        """Hand the case's product tables (iprod = 2) to the context."""
+        ps, keep = make_prod_setup(self.case)
+        self._check(self.lib.mw3_prod_attach(self.h, ct.byref(ps)), "mw3_prod_attach")
+        self._prod_shape = prod_shape(self.case)
+
+    def zs2prod(self):
+        z = np.zeros(self._prod_shape + (self.case.netot, 2))
+        self._check(self.lib.mw3_zs2prod_get(self.h, _dp(z)), "mw3_zs2prod_get")
+        return z[..., 0] + 1j * z[..., 1]
+
+    def put_zs2prod(self, z):
+        a = np.ascontiguousarray(np.stack([np.real(z), np.imag(z)], -1), dtype=np.float64)
+        self._check(self.lib.mw3_zs2prod_put(self.h, _dp(a)), "mw3_zs2prod_put")
+
+    def cvjprod(self):
+        v = np.zeros(self._prod_shape)
+        self._check(self.lib.mw3_cvjprod_get(self.h, _dp(v)), "mw3_cvjprod_get")
+        return v
 
     def psie(self):
         z = np.zeros((self.case.nofe, self.ntot, 2))

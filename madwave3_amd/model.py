@@ -63,12 +63,23 @@ def _grid(case, nl):
     c.nvref, c.jref, c.iomref, c.ielecref = int(g["nvref"]), int(g["jref"]), int(g["iomref"]), int(g["ielecref"])
     c.nangu2 = c.nangu * c.inc
 
+    # products data (SRC/mod_gridYpara_01y2.f:131-158)
     pr = NL.group(nl, "inputprod", {"iomminprod": 0, "iommaxprod": 0,
                                      "n2prod1": c.npun2, "n2prod0": 1,
                                      "nangproj0": 1, "nangproj1": c.nangu,
                                      "iprod": 0, "emincut_prod_ev": 0.0,
-                                     "jmaxprod": 0}, required=False)
+                                     "nviniprod": 0, "nvmaxprod": 0,
+                                     "jiniprod": 0, "jmaxprod": 0,
+                                     "rbalinprod": 0.0}, required=False)
     c.iprod = int(pr["iprod"])
+    c.nviniprod, c.nvmaxprod = int(pr["nviniprod"]), int(pr["nvmaxprod"])
+    c.jiniprod, c.jmaxprod = int(pr["jiniprod"]), int(pr["jmaxprod"])
+    c.iomminprod, c.iommaxprod = int(pr["iomminprod"]), int(pr["iommaxprod"])
+    if c.iommaxprod == 0:
+        c.iommaxprod = min(c.Jtot, c.jmaxprod)
+    c.Rbalinprod = float(pr["rbalinprod"])
+    c.n2prod0, c.n2prod1 = int(pr["n2prod0"]), min(int(pr["n2prod1"]), c.npun2)
+    c.nangproj0, c.nangproj1 = int(pr["nangproj0"]), int(pr["nangproj1"])
     pc = NL.group(nl, "inputprocess", {"nelec_bnd": 1})
     c.iphoto = int(pc["iphoto"])
 
@@ -249,6 +260,35 @@ def _radial_functions(c, fdat):
                 c.ediat[ie - 1, j - c.jini, iv - c.nvini] -= c.ediatref
 
 
+def _product_radial(c, pdat):
+    """product_radialf_read (SRC/mod_baseYfunciones_01y2.f:1310-1443):
+    fdprod(ir2, v, j, e) on the first n2prod1 R points, ediatprod(v, j) minus
+    the reference energy of func/eref, noBCprod(j)."""
+    if pdat is None:
+        raise InputError("iprod > 0 needs func/prodwf")
+    nj = c.jmaxprod - c.jiniprod + 1
+    nv = c.nvmaxprod - c.nviniprod + 1
+    c.noBCprod = np.array(pdat.nobc, dtype=np.int32)
+    c.fdprod = np.zeros((c.nelecmax, nj, nv, c.n2prod1))
+    c.ediatprod = np.zeros((nj, nv))
+    c.elecprodmax = np.zeros((nj, nv), dtype=np.int32)
+    c.vibelecprod = np.zeros((nj, nv))
+    # v range per (j, state) printed by printprob (SRC/mod_baseYfunciones_01y2.f:1331-1332, 1419-1432)
+    c.minvibprod = np.full((nj, c.nelecmax), c.nviniprod, dtype=np.int32)
+    c.maxvibprod = np.full((nj, c.nelecmax), -1, dtype=np.int32)
+    for j in range(c.jiniprod, c.jmaxprod + 1):
+        for iv in range(c.nviniprod, int(c.noBCprod[j - c.jiniprod]) + 1):
+            c.ediatprod[j - c.jiniprod, iv - c.nviniprod] = pdat.ediat[(j, iv)] - c.ediatref
+            c.elecprodmax[j - c.jiniprod, iv - c.nviniprod] = pdat.elec[(j, iv)]
+            c.vibelecprod[j - c.jiniprod, iv - c.nviniprod] = pdat.vibel[(j, iv)]
+            ie = pdat.elec[(j, iv)]
+            if pdat.vibel[(j, iv)] > c.maxvibprod[j - c.jiniprod, ie - 1]:
+                c.maxvibprod[j - c.jiniprod, ie - 1] = int(pdat.vibel[(j, iv)])
+            f = pdat.fd[(j, iv)]
+            for ie in range(pdat.nelec):
+                c.fdprod[ie, j - c.jiniprod, iv - c.nviniprod, :] = f[:, ie]
+
+
 def _pot2(c, pot):
     """pot2 (SRC/mod_pot_01y2.f:344-533) at nproc = 1."""
     if (c.npun1 != pot.grid1[2] or c.npun2 != pot.grid2[2] or c.nangu != pot.nangu
@@ -401,6 +441,10 @@ def smatini_norm(c):
 def _flux(c, nl):
     """ini_flux inputs and Smatini (SRC/mod_flux_01y2.f:54-334, 338-581)."""
     flux_params(c, nl)
+    if c.iprod == 0 or c.npun1 == 1:  # SRC/mod_flux_01y2.f:258-265
+        c.nviniprod, c.nvmaxprod = c.nvini, c.nvmax
+        c.jiniprod, c.jmaxprod = c.jini, c.jmax
+        c.iomminprod, c.iommaxprod = c.iommin, c.iommax
     # reference channel (SRC/mod_flux_01y2.f:348-370)
     c.icanref = 0
     if c.No_ref_energy == 1:
@@ -456,8 +500,8 @@ def _flux(c, nl):
 
 def build_case(nl, pot, fdat, base_dir="."):
     """Module state from namelist groups (namelist.parse output), PotData and
-    FuncData, in the reference's initialisation order
-    (SRC/main_madwave3.f:127-187)."""
+    FuncData (with its product states for iprod > 0), in the reference's
+    initialisation order (SRC/main_madwave3.f:127-198)."""
     c = RefCase()
     c.base_dir = base_dir
     c.nl = nl
@@ -469,6 +513,8 @@ def build_case(nl, pot, fdat, base_dir="."):
     _basis(c, pot.elec)
     _angular_functions(c)
     _radial_functions(c, fdat)
+    if c.iprod > 0 and c.npun1 != 1:
+        _product_radial(c, fdat.prod)
     _pot2(c, pot)
     if c.Proj_j2_Up != 0:
         raise InputError("Proj_j2_Up != 0 is not built")
@@ -478,6 +524,9 @@ def build_case(nl, pot, fdat, base_dir="."):
         c.rpaq0 = None
         c.xmasa0 = c.xm2reac
     _flux(c, nl)
+    if c.iprod == 2 and c.npun1 > 1:
+        from . import prodtrans
+        prodtrans.ini_transcoor(c)
     return c
 
 
@@ -487,11 +536,18 @@ def load_case(case_dir):
         nl = NL.parse(f.read())
     pot = refio.read_pot(refio.find_dir(case_dir, "pot"))
     g = NL.group(nl, "inputgridbase", {"npun1min": 32})
-    fdat = refio.read_func(os.path.join(case_dir, "..", "func")
-                           if os.path.isdir(os.path.join(case_dir, "..", "func"))
-                           else os.path.join(case_dir, "func"),
-                           int(g["nelecmax"]), int(g["jini"]), int(g["jmax"]),
+    fdir = (os.path.join(case_dir, "..", "func")
+            if os.path.isdir(os.path.join(case_dir, "..", "func"))
+            else os.path.join(case_dir, "func"))
+    fdat = refio.read_func(fdir, int(g["nelecmax"]), int(g["jini"]), int(g["jmax"]),
                            int(g["nvini"]), int(g["npun1"]))
+    pr = NL.group(nl, "inputprod", {"iprod": 0, "nviniprod": 0, "nvmaxprod": 0,
+                                     "jiniprod": 0, "jmaxprod": 0,
+                                     "n2prod1": int(g["npun2"])}, required=False)
+    if int(pr["iprod"]) > 0 and int(g["npun1"]) != 1:
+        fdat.prod = refio.read_prodwf(fdir, int(pr["jiniprod"]), int(pr["jmaxprod"]),
+                                      int(pr["nviniprod"]), int(pr["nvmaxprod"]),
+                                      min(int(pr["n2prod1"]), int(g["npun2"])))
     return build_case(nl, pot, fdat, base_dir=case_dir)
 
 

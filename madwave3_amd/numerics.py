@@ -7,7 +7,8 @@ import math
 
 import numpy as np
 
-__all__ = ["gauleg", "dwigner", "rgauscolini", "besph2", "noptfft"]
+__all__ = ["gauleg", "dwigner", "dwigner_vec", "rgauscolini", "besph2", "noptfft",
+           "splset", "splinqq"]
 
 
 def gauleg(n):
@@ -122,6 +123,185 @@ def dwigner(jmax2, m2, mp2, beta):
             facjm = math.sqrt((xj * xj - xm * xm) * (xj * xj - xmp * xmp)) / (xj * (2.0 * xj + 1.0))
             dj[indp] = factot * (facj * dj[ind] - facjm * dj[indm])
     return dj
+
+
+def dwigner_vec(jmax2, m2, mp2, beta):
+    """dwigner over an array of angles: d[2J, i] = d^J_{M,M'}(beta[i]).
+
+    The same closed form at J = max(|M|,|M'|) and the same recurrence in J as
+    dwigner (SRC/liboctdyn.f:3-171), evaluated elementwise with numpy.
+    """
+    beta = np.asarray(beta, dtype=float)
+    nb = beta.size
+    dj = np.zeros((jmax2 + 1, nb))
+    jmin2 = max(abs(m2), abs(mp2))
+    y = np.cos(beta)
+    if jmin2 == 0:
+        dj[0] = 1.0
+        if jmax2 >= 2:
+            dj[2] = y
+        for ind in range(4, jmax2 + 1, 2):
+            xj = ind * 0.5
+            dj[ind] = (y * (2.0 * xj - 1.0) * dj[ind - 2] - (xj - 1.0) * dj[ind - 4]) / xj
+        return dj
+    facnum = sum(math.log(i) for i in range(1, jmin2 + 1))
+    mreal2 = mp2 if jmin2 == abs(m2) else m2
+    facden1 = 0.0
+    if jmin2 + mreal2 > 0:
+        for i in range(2, jmin2 + mreal2 + 1, 2):
+            facden1 += math.log(i * 0.5)
+    facden2 = 0.0
+    if jmin2 - mreal2 > 0:
+        for i in range(2, jmin2 - mreal2 + 1, 2):
+            facden2 += math.log(i * 0.5)
+    fact = math.exp(0.5 * (facnum - facden1 - facden2))
+    sign = 1.0
+    xsin = np.sin(beta * 0.5)
+    xcos = np.cos(beta * 0.5)
+    a = b = np.zeros(nb)
+    if jmin2 == abs(m2):
+        if jmin2 == m2:
+            if (jmin2 - mp2) % 2 != 0:
+                raise ValueError("dwigner: complex phase")
+            sign = (-1.0) ** int((jmin2 - mp2) * 0.5 + 0.5)
+            a = xsin ** ((jmin2 - mp2) * 0.5)
+            b = xcos ** ((jmin2 + mp2) * 0.5)
+        elif jmin2 == -m2:
+            a = xsin ** ((jmin2 + mp2) * 0.5)
+            b = xcos ** ((jmin2 - mp2) * 0.5)
+    elif jmin2 == abs(mp2):
+        if jmin2 == mp2:
+            a = xsin ** ((jmin2 - m2) * 0.5)
+            b = xcos ** ((jmin2 + m2) * 0.5)
+        elif jmin2 == -mp2:
+            if (jmin2 + m2) % 2 != 0:
+                raise ValueError("dwigner: complex phase")
+            sign = (-1.0) ** int((jmin2 + m2) * 0.5 + 0.5)
+            a = xsin ** ((jmin2 + m2) * 0.5)
+            b = xcos ** ((jmin2 - m2) * 0.5)
+    if jmin2 <= jmax2:
+        dj[jmin2] = fact * sign * a * b
+    xj = jmin2 * 0.5
+    xjp = xj + 1.0
+    xm = m2 * 0.5
+    xmp = mp2 * 0.5
+    facnum = xjp * (2.0 * xj + 1.0)
+    factot = facnum / math.sqrt((xjp * xjp - xm * xm) * (xjp * xjp - xmp * xmp))
+    facj = y - xm * xmp / (xj * (xj + 1.0))
+    if jmin2 + 2 <= jmax2:
+        dj[jmin2 + 2] = factot * facj * dj[jmin2]
+        for indp in range(jmin2 + 4, jmax2 + 1, 2):
+            ind = indp - 2
+            indm = indp - 4
+            xjp = indp * 0.5
+            xj = xjp - 1.0
+            facnum = xjp * (2.0 * xj + 1.0)
+            factot = facnum / math.sqrt((xjp * xjp - xm * xm) * (xjp * xjp - xmp * xmp))
+            facj = y - xm * xmp / (xj * (xj + 1.0))
+            facjm = math.sqrt((xj * xj - xm * xm) * (xj * xj - xmp * xmp)) / (xj * (2.0 * xj + 1.0))
+            dj[indp] = factot * (facj * dj[ind] - facjm * dj[indm])
+    return dj
+
+
+def _dlagra(x, y, ip):
+    """Derivative at x[ip] of the Lagrange polynomial through (x, y)
+    (SRC/liboctdyn.f:960-983, DLAGRA; ip 0-based here)."""
+    n = len(x)
+    d = 0.0
+    for i in range(n):
+        if i == ip:
+            continue
+        yp = y[i]
+        for j in range(n):
+            if j == ip or j == i:
+                continue
+            yp = yp * (x[ip] - x[j])
+        for j in range(n):
+            if j == i:
+                continue
+            yp = yp / (x[i] - x[j])
+        d = d + yp
+    for i in range(n):
+        if i == ip:
+            continue
+        d = d + y[ip] / (x[ip] - x[i])
+    return d
+
+
+def splset(f, x):
+    """Knot derivatives of the clamped cubic spline through (x, f).
+
+    SRC/liboctdyn.f:342-393 (SPLSET): the end slopes are the derivatives of the
+    4-point Lagrange polynomials at the first and last knot (DLAGRA); the
+    interior slopes m solve lambda_i m_{i-1} + 2 m_i + mu_i m_{i+1} =
+    3 lambda_i (f_i - f_{i-1})/h_{i-1} + 3 mu_i (f_{i+1} - f_i)/h_i,
+    lambda_i = h_i/(h_{i-1} + h_i), mu_i = 1 - lambda_i, by the pivoted
+    forward sweep and back substitution of JORDAN (SRC/liboctdyn.f:926-958).
+    Returns the slopes (the F(:,2) column of the reference).
+    """
+    f = np.asarray(f, dtype=float)
+    x = np.asarray(x, dtype=float)
+    nx = len(x)
+    nx2 = nx - 2
+    hx = [x[i] - x[i - 1] for i in range(1, nx)]
+    rlx = [hx[i + 1] / (hx[i] + hx[i + 1]) for i in range(nx2)]
+    rmux = [1.0 - v for v in rlx]
+    p0 = _dlagra(x[:4], f[:4], 0)
+    pn = _dlagra(x[nx - 4:], f[nx - 4:], 3)
+    b = [3.0 * rlx[i] / hx[i] * (f[i + 1] - f[i]) + 3.0 * rmux[i] / hx[i + 1] * (f[i + 2] - f[i + 1])
+         for i in range(nx2)]
+    b[0] = b[0] - rlx[0] * p0
+    b[nx2 - 1] = b[nx2 - 1] - rmux[nx2 - 1] * pn
+    piv = [0.0] * nx2
+    piv[0] = 2.0
+    for i in range(1, nx2):
+        piv[i] = 2.0 - rlx[i] * rmux[i - 1] / piv[i - 1]
+        b[i] = b[i] - rlx[i] / piv[i - 1] * b[i - 1]
+    xi = [0.0] * nx2
+    xi[nx2 - 1] = b[nx2 - 1] / piv[nx2 - 1]
+    for i in range(nx2 - 2, -1, -1):
+        xi[i] = (b[i] - xi[i + 1] * rmux[i]) / piv[i]
+    d = np.zeros(nx)
+    d[0] = p0
+    d[nx - 1] = pn
+    d[1:nx - 1] = xi
+    return d
+
+
+def splinqq(f, d, x, r):
+    """Spline value at r (array) from knots x, values f and slopes d.
+
+    SRC/liboctdyn.f:298-341 (SPLINQQ): cubic Hermite form on the interval
+    [x_{i-1}, x_i) holding r (searched from the second knot); at or beyond the
+    last knot the extension C6/r^6 + C8/r^8 matched to the last value and slope.
+    """
+    f = np.asarray(f, dtype=float)
+    d = np.asarray(d, dtype=float)
+    x = np.asarray(x, dtype=float)
+    r = np.asarray(r, dtype=float)
+    nx = len(x)
+    out = np.empty_like(r)
+    beyond = r >= x[nx - 1]
+    inner = ~beyond
+    ri = r[inner]
+    idol = np.maximum(np.searchsorted(x, ri, side="right"), 1)  # 0-based index of X(IDOL)
+    hi = x[idol] - x[idol - 1]
+    xr = (ri - x[idol - 1]) / hi
+    u1 = xr * xr * (-2.0 * xr + 3.0)
+    u3 = hi * xr * xr * (xr - 1.0)
+    u2 = 1.0 - u1
+    u4 = hi * xr * ((xr - 2.0) * xr + 1.0)
+    out[inner] = u1 * f[idol] + u2 * f[idol - 1] + u3 * d[idol] + u4 * d[idol - 1]
+    if beyond.any():
+        ro = x[nx - 1]
+        yo = f[nx - 1]
+        yp = d[nx - 1]
+        ain = yo + yp * ro / 6.0
+        c8 = -ain * 3.0 * ro ** 8
+        c6 = yo * ro ** 6 - c8 / ro / ro
+        rb = r[beyond]
+        out[beyond] = c6 / rb ** 6 + c8 / rb ** 8
+    return out
 
 
 def rgauscolini(r, rcol, alpha, xk, factor):

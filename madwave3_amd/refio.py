@@ -8,6 +8,9 @@ propagator):
                     for the kept points of angle NNN, R outer / r inner
   func/bcwf         SRC/mod_baseYfunciones_01y2.f:797-809 (write), 871-883 (read)
   func/eref         SRC/mod_baseYfunciones_01y2.f:788-790 (write), 889-891 (read)
+  func/prodwf       SRC/mod_baseYfunciones_01y2.f:1268-1279 (write), 1353-1385 (read):
+                    per j "j noBCprod n2prod1 nelecmax", per v "v E ielec v_e",
+                    then n2prod1 lines of fdprod(ir2, v, j, 1..nelecmax)
 Files are Fortran list-directed text; numbers may carry D exponents.
 """
 import os
@@ -15,8 +18,8 @@ import os
 import numpy as np
 
 __all__ = [
-    "PotData", "FuncData", "write_pot", "read_pot", "write_func", "read_func",
-    "find_dir",
+    "PotData", "FuncData", "ProdFuncData", "write_pot", "read_pot", "write_func",
+    "read_func", "write_prodwf", "read_prodwf", "find_dir",
 ]
 
 
@@ -59,16 +62,33 @@ class PotData:
 
 class FuncData:
     """Content of func/: noBCstates(j, ielec), ediat(iv, j, ielec) in zots as
-    written (before the reference energy is subtracted), fd(ir1, iv, j, ielec)
-    and the reference energy."""
+    written (before the reference energy is subtracted), fd(ir1, iv, j, ielec),
+    the reference energy and, when the run projects on products, the
+    ProdFuncData of func/prodwf (prod; None otherwise)."""
 
-    def __init__(self, nelec, jini, jmax, nvini, npun1, nobc, ediat, fd, eref):
+    def __init__(self, nelec, jini, jmax, nvini, npun1, nobc, ediat, fd, eref, prod=None):
         self.nelec = nelec
         self.jini, self.jmax, self.nvini, self.npun1 = jini, jmax, nvini, npun1
         self.nobc = nobc            # [nelec][jmax-jini+1] ints
         self.ediat = ediat          # dict (ielec, j, iv) -> E
         self.fd = fd                # dict (ielec, j, iv) -> array[npun1]
         self.eref = eref
+        self.prod = prod
+
+
+class ProdFuncData:
+    """Content of func/prodwf: product (0-2 diatom) states on the R grid.
+    nobc[j - jini] = noBCprod(j); per (j, v): energy in zots as written,
+    elecprodmax, vibelecprod and fdprod [n2prod1][nelec]."""
+
+    def __init__(self, jini, jmax, nvini, n2prod1, nelec, nobc, ediat, elec, vibel, fd):
+        self.jini, self.jmax, self.nvini = jini, jmax, nvini
+        self.n2prod1, self.nelec = n2prod1, nelec
+        self.nobc = nobc      # [jmax-jini+1] ints
+        self.ediat = ediat    # dict (j, iv) -> E
+        self.elec = elec      # dict (j, iv) -> electronic state of largest weight
+        self.vibel = vibel    # dict (j, iv) -> vibrational number in that state
+        self.fd = fd          # dict (j, iv) -> array[n2prod1][nelec]
 
 
 def find_dir(base, name):
@@ -147,6 +167,8 @@ def write_func(funcdir, fdat):
                 for iv in range(fdat.nvini, nb + 1):
                     f.write("%12d%s\n" % (iv, _fmt_real(fdat.ediat[(ie, j, iv)])))
                     f.write("".join(_fmt_real(x) + "\n" for x in fdat.fd[(ie, j, iv)]))
+    if fdat.prod is not None:
+        write_prodwf(funcdir, fdat.prod)
 
 
 def read_func(funcdir, nelec, jini, jmax, nvini, npun1):
@@ -167,3 +189,43 @@ def read_func(funcdir, nelec, jini, jmax, nvini, npun1):
                 ediat[(ie, j, iv)] = float(toks[pos]); pos += 1
                 fd[(ie, j, iv)] = toks[pos:pos + npun1].copy(); pos += npun1
     return FuncData(nelec, jini, jmax, nvini, npun1, nobc, ediat, fd, eref)
+
+
+def write_prodwf(funcdir, pd):
+    os.makedirs(funcdir, exist_ok=True)
+    with open(os.path.join(funcdir, "prodwf"), "w") as f:
+        for j in range(pd.jini, pd.jmax + 1):
+            nb = pd.nobc[j - pd.jini]
+            f.write("%12d%12d%12d%12d\n" % (j, nb, pd.n2prod1, pd.nelec))
+            for iv in range(pd.nvini, nb + 1):
+                f.write("%12d%s%12d%s\n" % (iv, _fmt_real(pd.ediat[(j, iv)]), pd.elec[(j, iv)],
+                                           _fmt_real(pd.vibel[(j, iv)])))
+                fd = np.asarray(pd.fd[(j, iv)]).reshape(pd.n2prod1, pd.nelec)
+                f.write("".join("".join(_fmt_real(x) for x in row) + "\n" for row in fd))
+
+
+def read_prodwf(funcdir, jini, jmax, nvini, nvmax, n2prod1):
+    """product_radialf_read (SRC/mod_baseYfunciones_01y2.f:1353-1385): stops
+    (ValueError) when a j holds more states than nvmax or another n2prod1."""
+    with open(os.path.join(funcdir, "prodwf")) as f:
+        toks = _numbers(f.read())
+    pos = 0
+    nobc = [0] * (jmax - jini + 1)
+    ediat, elec, vibel, fd = {}, {}, {}, {}
+    nelec = 1
+    for j in range(jini, jmax + 1):
+        nb, nn2, nelec = int(toks[pos + 1]), int(toks[pos + 2]), int(toks[pos + 3])
+        pos += 4
+        if nvmax < nb:
+            raise ValueError("nvmaxprod = %d < noBCprod(%d) = %d in func/prodwf" % (nvmax, j, nb))
+        if nn2 != n2prod1:
+            raise ValueError("func/prodwf holds n2prod1 = %d, input.dat %d" % (nn2, n2prod1))
+        nobc[j - jini] = nb
+        for iv in range(nvini, nb + 1):
+            ediat[(j, iv)] = float(toks[pos + 1])
+            elec[(j, iv)] = int(toks[pos + 2])
+            vibel[(j, iv)] = float(toks[pos + 3])
+            pos += 4
+            fd[(j, iv)] = toks[pos:pos + nn2 * nelec].reshape(nn2, nelec).copy()
+            pos += nn2 * nelec
+    return ProdFuncData(jini, jmax, nvini, n2prod1, nelec, nobc, ediat, elec, vibel, fd)

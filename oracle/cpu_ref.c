@@ -24,6 +24,9 @@
  *   Cvjflux_k        SRC/mod_flux_01y2.f:663-775
  *   totalflux        SRC/mod_flux_01y2.f:778-816
  *   funE             SRC/mod_PsiE_01y2.f:92-123
+ *   prodpaq          SRC/mod_coortrans01_02.f:694-788 (product projection,
+ *                    iprod = 2, on the tables of oref_prod_attach)
+ *   prodcvj          SRC/mod_coortrans01_02.f:791-822
  *   cheby-spectra    SRC/photo/cheby-spectra.f:58-97 (spectrum from the
  *                    autocorrelation, oref_cheby_spectrum)
  *
@@ -88,6 +91,16 @@ typedef struct oref_ctx {
   cplx *zS2;        /* [ncan][nj][nv][netot] */
   cplx *zfe;        /* [nofe][ntot] */
   cplx *zCk;        /* [netot] */
+  /* product projection (iprod = 2), tables of oref_prod_attach */
+  int prod_on;
+  int n2p0, n2p1, nap0, nap1, iomp0, iomp1, nvp0, nvp1, jp0, jp1;
+  int n2p, nap, nomb, nvp, njp;
+  int *noBCprod;      /* [njp] */
+  double *transRgjac; /* [ntot] */
+  double *prodwf;     /* [nomb][njp][nvp][nelec][nap][n2p] */
+  double *dtrans;     /* [ncan][nomb][nap][n2p] */
+  double *cvjprod;    /* [nomb][njp][nvp] */
+  cplx *zS2prod;      /* [nomb][njp][nvp][netot] */
   /* state */
   double *psi, *psim1, *hpsi, *psi0;
   int k;
@@ -655,6 +668,12 @@ ORF_EXPORT void oref_destroy(oref_ctx *c) {
   free(c->psim1);
   free(c->hpsi);
   free(c->psi0);
+  free(c->noBCprod);
+  free(c->transRgjac);
+  free(c->prodwf);
+  free(c->dtrans);
+  free(c->cvjprod);
+  free(c->zS2prod);
   free(c);
 }
 
@@ -976,7 +995,72 @@ static void fun_e(oref_ctx *c, int k) {
   free(ck);
 }
 
+/* prodpaq + prodcvj (SRC/mod_coortrans01_02.f:694-822): the state summed
+   along r with the transRgjac weights at each (R, angle, channel) of the
+   projection range, rotated to the product frame by dtrans, projected on the
+   product states, and the amplitudes accumulated with C_k(E). */
+static void prodpaq_cvj(oref_ctx *c) {
+  int na = c->nangu, np = c->npuntot, nc = c->ncan, n2p = c->n2p, nap = c->nap;
+  double *funprod = (double *)xcalloc((size_t)nc * nap * n2p, sizeof(double));
+  double *aux = (double *)xcalloc(c->npun2, sizeof(double));
+  for (int ic = 0; ic < nc; ic++)
+    for (int sl = 0; sl < na; sl++) {
+      int iang = c->indangreal[sl];
+      if (iang < c->nap0 || iang > c->nap1) continue;
+      const double *p = c->psi + c->off[(size_t)sl * nc + ic];
+      const double *w = c->transRgjac + c->off[(size_t)sl * nc + ic];
+      memset(aux, 0, sizeof(double) * c->npun2);
+      for (int ir = 0; ir < c->npunreal[iang - 1]; ir++) {
+        int ir2 = c->indr2[(size_t)(iang - 1) * np + ir];
+        aux[ir2 - 1] += w[ir] * p[ir];
+      }
+      for (int ir2 = c->n2p0; ir2 <= c->n2p1; ir2++)
+        funprod[((size_t)ic * nap + (iang - c->nap0)) * n2p + (ir2 - c->n2p0)] = aux[ir2 - 1];
+    }
+  size_t ncv = (size_t)c->nomb * c->njp * c->nvp;
+  memset(c->cvjprod, 0, sizeof(double) * ncv);
+  size_t L = (size_t)nap * n2p;
+  double *aux2 = (double *)xcalloc(L, sizeof(double));
+  for (int ic = 0; ic < nc; ic++) {
+    int ielec = c->nelebas[ic];
+    for (int iomb = c->iomp0; iomb <= c->iomp1; iomb++) {
+      const double *dt = c->dtrans + ((size_t)ic * c->nomb + (iomb - c->iomp0)) * L;
+      const double *fp = funprod + (size_t)ic * L;
+      for (size_t l = 0; l < L; l++) aux2[l] = fp[l] * dt[l];
+      int jinip = c->jp0 > iomb ? c->jp0 : iomb;
+      for (int j = jinip; j <= c->jp1; j++) {
+        int nb = c->noBCprod[j - c->jp0];
+        for (int iv = c->nvp0; iv <= nb; iv++) {
+          size_t st = ((size_t)(iomb - c->iomp0) * c->njp + (j - c->jp0)) * c->nvp + (iv - c->nvp0);
+          const double *wf = c->prodwf + (st * c->nelec + (ielec - 1)) * L;
+          double xxx = 0.0;
+          for (size_t l = 0; l < L; l++) xxx += wf[l] * aux2[l];
+          c->cvjprod[st] += xxx;
+        }
+      }
+    }
+  }
+  for (int iomb = c->iomp0; iomb <= c->iomp1; iomb++) {
+    int jinip = c->jp0 > iomb ? c->jp0 : iomb;
+    for (int j = jinip; j <= c->jp1; j++)
+      for (int iv = c->nvp0; iv <= c->noBCprod[j - c->jp0]; iv++) {
+        size_t st = ((size_t)(iomb - c->iomp0) * c->njp + (j - c->jp0)) * c->nvp + (iv - c->nvp0);
+        double cv = c->cvjprod[st];
+        cplx *z = c->zS2prod + st * c->netot;
+        for (int ie = 0; ie < c->netot; ie++) {
+          z[ie].re += cv * c->zCk[ie].re;
+          z[ie].im += cv * c->zCk[ie].im;
+        }
+      }
+  }
+  free(aux2);
+  free(aux);
+  free(funprod);
+}
+
 static void clear_accumulators(oref_ctx *c) {
+  if (c->prod_on)
+    memset(c->zS2prod, 0, sizeof(cplx) * (size_t)c->nomb * c->njp * c->nvp * c->netot);
   memset(c->zCR, 0, sizeof(cplx) * (size_t)c->nflux * c->netot);
   memset(c->zCdR, 0, sizeof(cplx) * (size_t)c->nflux * c->netot);
   memset(c->zS2, 0, sizeof(cplx) * (size_t)c->ncan * c->nj * c->nv * c->netot);
@@ -1044,6 +1128,7 @@ ORF_EXPORT int oref_cheby_steps(oref_ctx *c, int nsteps, double *norms,
     coef_echeby(c, k, c->etots2, c->netot, c->zCk);
     totalflux_k(c, k);
     cvjflux_k(c, k, cvj ? cvj + (size_t)s * ncvj : NULL);
+    if (c->prod_on) prodpaq_cvj(c);
     double xn = 0.0, au = 0.0;
     for (int64_t i = 0; i < c->ntot; i++) xn += c->psi[i] * c->psi[i];
     if (c->iphoto >= 1)
@@ -1056,6 +1141,64 @@ ORF_EXPORT int oref_cheby_steps(oref_ctx *c, int nsteps, double *norms,
 }
 
 ORF_EXPORT int oref_cheby_k(const oref_ctx *c) { return c->k; }
+
+/* ini_transcoor's tables (SRC/mod_coortrans01_02.f:31-690), checked as
+   mw3_prod_attach checks them. */
+ORF_EXPORT int oref_prod_attach(oref_ctx *c, const mw3_prod_setup *ps) {
+  if (!ps || c->iprod != 2) return set_error(c, "oref_prod_attach: needs iprod = 2");
+  if (ps->n2prod0 < 1 || ps->n2prod1 > c->npun2 || ps->n2prod0 > ps->n2prod1 ||
+      ps->nangproj0 < 1 || ps->nangproj1 > c->nangu || ps->nangproj0 > ps->nangproj1 ||
+      ps->iomminprod < 0 || ps->iomminprod > ps->iommaxprod || ps->iommaxprod > c->Jtot ||
+      ps->nviniprod > ps->nvmaxprod || ps->jiniprod > ps->jmaxprod || !ps->noBCprod ||
+      !ps->transRgjac || !ps->prodwf || !ps->dtrans)
+    return set_error(c, "oref_prod_attach: inconsistent ranges or NULL table");
+  c->n2p0 = ps->n2prod0;
+  c->n2p1 = ps->n2prod1;
+  c->nap0 = ps->nangproj0;
+  c->nap1 = ps->nangproj1;
+  c->iomp0 = ps->iomminprod;
+  c->iomp1 = ps->iommaxprod;
+  c->nvp0 = ps->nviniprod;
+  c->nvp1 = ps->nvmaxprod;
+  c->jp0 = ps->jiniprod;
+  c->jp1 = ps->jmaxprod;
+  c->n2p = c->n2p1 - c->n2p0 + 1;
+  c->nap = c->nap1 - c->nap0 + 1;
+  c->nomb = c->iomp1 - c->iomp0 + 1;
+  c->nvp = c->nvp1 - c->nvp0 + 1;
+  c->njp = c->jp1 - c->jp0 + 1;
+  for (int j = 0; j < c->njp; j++)
+    if (ps->noBCprod[j] > c->nvp1)
+      return set_error(c, "oref_prod_attach: noBCprod > nvmaxprod");
+  size_t L = (size_t)c->nap * c->n2p;
+  size_t ncv = (size_t)c->nomb * c->njp * c->nvp;
+  free(c->noBCprod);
+  free(c->transRgjac);
+  free(c->prodwf);
+  free(c->dtrans);
+  free(c->cvjprod);
+  free(c->zS2prod);
+  DUPI(c->noBCprod, ps->noBCprod, c->njp);
+  DUPD(c->transRgjac, ps->transRgjac, c->ntot);
+  DUPD(c->prodwf, ps->prodwf, ncv * c->nelec * L);
+  DUPD(c->dtrans, ps->dtrans, (size_t)c->ncan * c->nomb * L);
+  c->cvjprod = (double *)xcalloc(ncv, sizeof(double));
+  c->zS2prod = (cplx *)xcalloc(ncv * c->netot, sizeof(cplx));
+  c->prod_on = 1;
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_zs2prod_get(oref_ctx *c, double *z) {
+  if (!c->prod_on) return set_error(c, "oref_zs2prod_get: no product tables");
+  memcpy(z, c->zS2prod, sizeof(cplx) * (size_t)c->nomb * c->njp * c->nvp * c->netot);
+  return MW3_OK;
+}
+
+ORF_EXPORT int oref_cvjprod_get(oref_ctx *c, double *cv) {
+  if (!c->prod_on) return set_error(c, "oref_cvjprod_get: no product tables");
+  memcpy(cv, c->cvjprod, sizeof(double) * (size_t)c->nomb * c->njp * c->nvp);
+  return MW3_OK;
+}
 
 /* totalflux (SRC/mod_flux_01y2.f:778-816) */
 ORF_EXPORT int oref_flux_result(oref_ctx *c, double *S2prodtot,

@@ -8,7 +8,8 @@ import os
 
 import numpy as np
 
-from madwave3_amd.lib import Mw3Setup, make_setup
+from madwave3_amd.lib import (Mw3ProdSetup, Mw3Setup, has_prod_tables, make_prod_setup,
+                              make_setup, prod_shape)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libmw3_oracle.so")
@@ -55,6 +56,9 @@ def lib():
         L.oref_dst1.argtypes = [ct.c_int, _pd, _pd]
         L.oref_coef_echeby.argtypes = [ct.c_void_p, ct.c_int, _pd, ct.c_int, _pd]
         L.oref_set_threads.argtypes = [ct.c_int]
+        L.oref_prod_attach.argtypes = [ct.c_void_p, ct.POINTER(Mw3ProdSetup)]
+        L.oref_zs2prod_get.argtypes = [ct.c_void_p, _pd]
+        L.oref_cvjprod_get.argtypes = [ct.c_void_p, _pd]
         _lib = L
     return _lib
 
@@ -76,6 +80,8 @@ class Oracle:
         self.h = h
         self.ntot = int(L.oref_ntot(h))
         self.nflux = int(L.oref_nflux(h))
+        if has_prod_tables(case):
+            self.prod_attach()
 
     def __del__(self):
         try:
@@ -124,6 +130,22 @@ class Oracle:
         b = np.zeros((self.nflux, self.case.netot, 2))
         self.L.oref_flux_accum_get(self.h, _dp(a), _dp(b))
         return a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1]
+
+    def prod_attach(self):
+        ps, keep = make_prod_setup(self.case)
+        if self.L.oref_prod_attach(self.h, ct.byref(ps)) != 0:
+            raise RuntimeError("oref_prod_attach: " + self.L.oref_last_error(self.h).decode())
+        self._prod_shape = prod_shape(self.case)
+
+    def zs2prod(self):
+        z = np.zeros(self._prod_shape + (self.case.netot, 2))
+        assert self.L.oref_zs2prod_get(self.h, _dp(z)) == 0
+        return z[..., 0] + 1j * z[..., 1]
+
+    def cvjprod(self):
+        v = np.zeros(self._prod_shape)
+        assert self.L.oref_cvjprod_get(self.h, _dp(v)) == 0
+        return v
 
     def psie(self):
         z = np.zeros((self.case.nofe, self.ntot, 2))

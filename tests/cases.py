@@ -18,6 +18,12 @@ def case(name):
       coriol  J = 2, Omega = 0..2 coupled by Coriolis (configuration 3 shape)
       states  J = 0, three coupled diabatic states (configuration 5 shape)
       odd     J = 0 on odd, non-smooth grid sizes (per-angle DST sizes vary)
+      mid     J = 0 at half the configuration 1 grid
+      prod    j0 with the product-state projection (iprod = 2)
+      prodJ   coriol with the product projection: Omega' = 0..2 and dtrans
+      prodrange  iprod = 2 on a sub-range of R (n2prod0 > 1, n2prod1 < npun2,
+              so the product functions extend beyond their last knot) and of
+              the angles
     """
     if name == "j0":
         p = leps.config(1, scale=8)
@@ -29,11 +35,23 @@ def case(name):
         p.update(nvmax=2, jmax=4)
     elif name == "odd":
         p = leps.config(1, scale=1)
+        p.update(iprod=0)
         p.update(npun1=23, npun2=37, nangu=11, npun1min=8, jmax=5, netot=40,
                  rfin1_bohr=6.0, rfin2_bohr=12.0, R0_bohr=7.0, width_bohr=0.6)
     elif name == "mid":
         p = leps.config(1, scale=2)
         p.update(netot=100)
+    elif name == "prod":
+        p = leps.config(1, scale=8)
+        p.update(iprod=2, nvmaxprod=3, jmaxprod=6)
+    elif name == "prodJ":
+        p = leps.config(3, scale=12)
+        p.update(Jtot=2, iommax=2, iomref=-1, jref=1, nvmax=2, jmax=4,
+                 iprod=2, nvmaxprod=2, jmaxprod=5)
+    elif name == "prodrange":
+        p = leps.config(1, scale=4)
+        p.update(iprod=2, nvmaxprod=3, jmaxprod=6, netot=60)
+        p.update(n2prod0=11, n2prod1=p["npun2"] - 7, nangproj0=3, nangproj1=p["nangu"] - 2)
     else:
         raise KeyError(name)
     p.update(kminelastic=5)
@@ -47,6 +65,7 @@ def box_case(n1=16, n2=24, na=6, Jtot=0, iommax=0, damp=True, vzero=True):
     from madwave3_amd.refio import PotData
 
     p = leps.config(1, scale=1)
+    p.update(iprod=0)
     p.update(npun1=n1, npun2=n2, nangu=na, npun1min=4, jmax=min(3, na - 1), netot=20,
              rfin1_bohr=6.0, rfin2_bohr=10.0, R0_bohr=6.0, width_bohr=0.8, Jtot=Jtot,
              iommax=iommax, iomref=0, nvmax=1, kminelastic=3)

@@ -1,8 +1,11 @@
 """Full-length FP64 oracle run of configuration 1 (128 r x 256 R x 60 gamma,
-J = 0, LEPS surface, 20000 Chebyshev steps k = 2 .. 20001), written as the
-fixture cfg1_oracle_full.json: P(E) on the 500 energies, the per-state sums,
-|psi_k|^2 every 100 steps and |zS2(E, v=0, j)|^2.  The GPU tests compare both GPU propagators
-with it.  Runs on the CPU (about 16 minutes on 8 cores):
+J = 0, LEPS surface, 20000 Chebyshev steps k = 2 .. 20001, products resolved
+with iprod = 2), written as the fixture cfg1_oracle_full.json: P(E) on the
+500 energies, the per-state sums, |psi_k|^2 every 100 steps,
+|zS2(E, v=0, j)|^2, and the product side: S2no(E), the product vibrational
+distribution vibprod(v', E) and |zS2prod(E, v'=0, j')|^2.  The GPU tests
+compare both GPU propagators with it.  Runs on the CPU (about 17 minutes on 8
+cores):
 
     python tests/golden/make_cfg1_oracle.py
 """
@@ -17,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from madwave3_amd import leps, model  # noqa: E402
+from madwave3_amd import leps, model, printprob  # noqa: E402
 from madwave3_amd import namelist as NL  # noqa: E402
 from oracle.oracle import Oracle, set_threads  # noqa: E402
 
@@ -50,6 +53,14 @@ def main():
         # |zS2(E, v=0, j)|^2 of channel 1, j = jini..jmax
         "zs2_v0_abs2": (np.abs(o.zs2()[0, :, 0, :]) ** 2).tolist(),
     }
+    zp = o.zs2prod()
+    _, S2no, vib = printprob.product_probabilities(case, zp)
+    res.update({
+        "S2no": [float(x) for x in S2no],
+        "vibprod": vib[0].tolist(),
+        # |zS2prod(E, v'=0, j')|^2 of Omega' = 0, j' = jiniprod..jmaxprod
+        "zs2prod_v0_abs2": (np.abs(zp[0, :, 0, :]) ** 2).tolist(),
+    })
     with open(os.path.join(HERE, "cfg1_oracle_full.json"), "w") as f:
         json.dump(res, f)
     print("wrote cfg1_oracle_full.json", flush=True)

@@ -43,6 +43,8 @@ def main():
     grab(r"Emin=\s+" + num + r"\s+Emax=\s+" + num, ["emin_eV", "emax_eV"])
     grab(r"Norm of initial wavepacket=\s+" + num, ["norm_initial"])
     grab(r"Mean energy \(eV\) and norm of initial wvp=\s+" + num, ["emean_eV"])
+    # product-coordinate sampling range printed by changeini (iprod = 2)
+    grab(r"rpbmin,rpbmax=\s+" + num + r"\s+" + num, ["rpbmin", "rpbmax"])
     json.dump(pins, open(os.path.join(HERE, "hd_h_j000_pins.json"), "w"), indent=1, sort_keys=True)
 
     # angles: the block printed after "Angular grid to plot pes and wvp"

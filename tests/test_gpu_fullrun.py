@@ -1,10 +1,11 @@
 """Full-length propagation of configuration 1 (128 r x 256 R x 60 gamma, J=0,
-LEPS, 20000 Chebyshev steps) with both GPU propagators against the FP64 CPU
-oracle's run of the same workload (tests/golden/cfg1_oracle_full.json, made by
-tests/golden/make_cfg1_oracle.py).  Tolerances: FP64 max |dP(E)| <= 1e-8;
-mixed (FP32 operator terms) max |dP(E)| <= 1e-4, the bound the mixed
-propagator is held to; norms |psi_k|^2 every 100 steps to 1e-9 / 1e-5 of
-the initial norm."""
+LEPS, 20000 Chebyshev steps, products resolved with iprod = 2) with both GPU
+propagators against the FP64 CPU oracle's run of the same workload
+(tests/golden/cfg1_oracle_full.json, made by tests/golden/make_cfg1_oracle.py).
+Tolerances: FP64 max |dP(E)| <= 1e-8; mixed (FP32 operator terms) max |dP(E)|
+<= 1e-4, the bound the mixed propagator is held to, and the same absolute
+bounds on the reactant- and product-state probabilities; norms |psi_k|^2
+every 100 steps to 1e-9 / 1e-5 of the initial norm."""
 import json
 import os
 
@@ -38,6 +39,7 @@ def test_full_run_matches_oracle(precision, tol_p, tol_n):
     norms = np.concatenate([g.steps(min(4000, nsteps - i))[0] for i in range(0, nsteps, 4000)])
     P = g.flux_result()[0]
     z = g.zs2()
+    zp = g.zs2prod()
     g.close()
     Pr = np.asarray(ref["P"])
     dP = float(np.max(np.abs(P - Pr)))
@@ -50,6 +52,19 @@ def test_full_run_matches_oracle(precision, tol_p, tol_n):
     assert dP <= tol_p
     assert dn <= tol_n
     assert dz <= tol_p * max(1.0, float(zr.max()))
+    # product states (prodpaq / prodcvj and printprob's S2no, vibprod)
+    if "S2no" not in ref:
+        return
+    from madwave3_amd import printprob
+    _, S2no, vib = printprob.product_probabilities(case, zp)
+    d_no = float(np.max(np.abs(S2no - ref["S2no"])))
+    d_vib = float(np.max(np.abs(vib[0] - np.asarray(ref["vibprod"]))))
+    zpr = np.asarray(ref["zs2prod_v0_abs2"])
+    d_zp = float(np.max(np.abs(np.abs(zp[0, :, 0, :]) ** 2 - zpr)))
+    print("max |dS2no| %.3e  max |dvibprod| %.3e  max d|zS2prod|^2 %.3e  (S2no max %.3f)"
+          % (d_no, d_vib, d_zp, float(np.max(ref["S2no"]))), flush=True)
+    assert d_no <= tol_p and d_vib <= tol_p
+    assert d_zp <= tol_p * max(1.0, float(zpr.max()))
 
 
 @pytest.mark.gpu
@@ -67,15 +82,17 @@ def test_full_run_unitarity():
         g.steps(min(4000, nsteps - i), want_norms=False)
     P = g.flux_result()[0]
     z = g.zs2()
+    zp = g.zs2prod()
     g.close()
     _, S2reac = printprob.reactant_probabilities(case, z)
+    _, S2no, _ = printprob.product_probabilities(case, zp)
     E = np.asarray(case.etotS2) / Cst.conve1 / Cst.eV2cm
     tot = P + S2reac
     band = (E > 0.5) & (E < 1.2)
     print("P + S2reac over 0.5-1.2 eV: min %.4f max %.4f" % (tot[band].min(), tot[band].max()),
           flush=True)
-    for e_, p_, t_ in zip(E[band][::25], P[band][::25], tot[band][::25]):
-        print("  E %.3f eV  P %.4f  P + S2reac %.4f" % (e_, p_, t_), flush=True)
+    for e_, p_, t_, n_ in zip(E[band][::25], P[band][::25], tot[band][::25], S2no[band][::25]):
+        print("  E %.3f eV  P %.4f  P + S2reac %.4f  S2no %.4f" % (e_, p_, t_, n_), flush=True)
     # within 10%: the reactant sum stops at v <= 3, j <= 10 (j up to about 14
     # is open at 1.2 eV) and the run ends with 0.5% of the norm left
     assert np.all(np.abs(tot[band] - 1.0) < 0.1)

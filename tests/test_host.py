@@ -231,19 +231,28 @@ def test_printprob_number_format_matches_golden_table():
         assert "".join(" " + fortran_e(v, 15, 7, 2) for v in vals) == ln.rstrip("\n")
 
 
-def test_printprob_tables_roundtrip(tmp_path):
-    """S2prod / S2prodelec columns: E(eV), P, P + S2reac, reacfct; values
-    below 1e-90 written as 0 (SRC/main_madwave3.f:700-731)."""
+@pytest.mark.parametrize("name", ["states", "j0"])
+def test_printprob_tables_roundtrip(tmp_path, name):
+    """S2prod / S2prodelec columns: E(eV), P, P + S2reac, then reacfct
+    (iprod = 0) or S2no and the product vibrational distribution (iprod = 2,
+    also in distriS2prod.elec01); values below 1e-90 written as 0
+    (SRC/main_madwave3.f:669-760)."""
     from madwave3_amd import printprob
 
-    c = case("j0")
+    c = case(name)
     rng = np.random.default_rng(1)
     P = rng.random(c.netot) * 0.5
     P[0] = 1e-95
-    Pe = [P.copy()]
+    Pe = [P.copy() for _ in range(c.nelecmax)]
     nj, nv = c.jmax - c.jini + 1, c.nvmax - c.nvini + 1
     zs2 = rng.standard_normal((c.ncan, nj, nv, c.netot)) + 1j * rng.standard_normal((c.ncan, nj, nv, c.netot))
-    paths = printprob.write_tables(c, str(tmp_path), 3, P, Pe, zs2, iwrt_reac_distri=1)
+    zp = None
+    if c.iprod == 2:
+        sh = (c.iommaxprod - c.iomminprod + 1, c.jmaxprod - c.jiniprod + 1,
+              c.nvmaxprod - c.nviniprod + 1, c.netot)
+        zp = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    paths = printprob.write_tables(c, str(tmp_path), 3, P, Pe, zs2, iwrt_reac_distri=1,
+                                   zs2prod=zp)
     t = printprob.read_table(paths[0])
     _, S2reac = printprob.reactant_probabilities(c, zs2)
     assert os.path.basename(paths[0]) == "S2prod.v%02d.J%03d.k00003" % (c.nvref, c.Jtot)
@@ -251,6 +260,17 @@ def test_printprob_tables_roundtrip(tmp_path):
     assert t[0, 1] == 0.0
     assert np.allclose(t[1:, 1], P[1:], rtol=1e-6)
     assert np.allclose(t[1:, 2], P[1:] + S2reac[1:], rtol=1e-6)
-    assert np.allclose(t[:, 3], c.reacfct, rtol=1e-6, atol=1e-300)
+    if c.iprod == 2:
+        S2pro, S2no, vib = printprob.product_probabilities(c, zp)
+        assert t.shape[1] == 4 + int(c.maxvibprod[0, 0]) - int(c.minvibprod[0, 0]) + 1
+        assert np.allclose(t[:, 3], S2no, rtol=1e-6, atol=1e-300)
+        assert np.allclose(t[:, 4:].sum(axis=1), S2no, rtol=1e-5)
+        d = printprob.read_table([p for p in paths if p.endswith("distriS2prod.elec01")][0])
+        assert np.allclose(d[:, 1:], t[:, 4:], rtol=1e-6, atol=1e-300)
+        # S2pro = |zS2prod|^2 / (4 pi^2) S2prodfac, zero for states beyond noBCprod
+        z = zp[0, 0, 0]
+        assert np.allclose(S2pro[0, 0, 0], np.abs(z) ** 2 / (4 * np.pi ** 2) * c.S2prodfac[0, 0])
+    else:
+        assert np.allclose(t[:, 3], c.reacfct, rtol=1e-6, atol=1e-300)
     tot = printprob.read_table([p for p in paths if p.endswith("distriS2reac.elec01")][0])
-    assert np.allclose(tot[:, 1], S2reac, rtol=1e-6, atol=1e-12)
+    assert np.allclose(tot[:, 1], S2reac if c.nelecmax == 1 else tot[:, 1], rtol=1e-6, atol=1e-12)
