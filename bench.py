@@ -7,7 +7,7 @@ reactant states v <= 3, j <= 10 and product states HD(v' <= 3, j' <= 15) at
 R' = 4 bohr (iprod = 2) -- built in memory by the synthetic input generator.
 A "step" is one full Chebyshev iteration (H psi, damped update, per-step
 projections on the flux line, the reactant states and the product states;
-the energy sums run once per 64 steps and are included).
+the energy sums run once per 256 steps and are included).
 
 Prints one JSON line: value = steps/s of the mixed-precision propagator (FP32
 operator terms, FP64 Chebyshev state and accumulators); the FP64 propagator's
@@ -44,7 +44,7 @@ def build_case(cfg):
 # kernel launches per step in each group (mixed and FP64 alike)
 GROUP_LAUNCHES = {"radial_dst": 4, "angular": 2, "update_fused": 2, "projections": 1,
                   "energy_sums": 2, "coriolis": 1, "products": 1}
-RING = 64  # steps per energy-sum flush (kRing of the library)
+RING = 256  # steps per energy-sum flush (kRing of the library)
 
 
 def kernel_groups(times):
@@ -53,7 +53,7 @@ def kernel_groups(times):
     FBR coefficients, back GEMM), update_fused (update + norm reduction),
     projections (reactant-state sums; second stream, overlapped with the next
     step), products (the per-step product-state reduction, same stream as
-    projections), energy_sums (once per 64-step ring, divided by 64, with the
+    projections), energy_sums (once per 256-step ring, divided by 256, with the
     product-state GEMM of the ring) and coriolis (the FBR coefficient pass with
     the Omega +- 1 stencil, run instead of the fused forward epilogue when
     channels are Coriolis-coupled)."""

@@ -18,7 +18,10 @@
 
 namespace mw3 {
 
-constexpr int kRing = 64;  // Chebyshev steps buffered between energy GEMMs
+// Chebyshev steps buffered between energy sums: the FP64 accumulators (about
+// 1 GB at 500 energies and 61,440 flux points) are read and written once per
+// ring, so a longer ring moves fewer accumulator bytes per step
+constexpr int kRing = 256;
 
 // Host-side operator tables built from mw3_setup (mw3_setup.cpp).
 struct HostTables {

@@ -890,30 +890,71 @@ __global__ void k_wmat(double* __restrict__ w, const double* __restrict__ etot, 
 }
 
 // real samples z[s][x] (zr, cvj) or complex (zdr); skip: per-x flag of the
-// reference state whose steps k < kmin are left out (Cvjflux_k)
-__global__ void k_eacc(const double* __restrict__ w, const double* __restrict__ z, int cplx,
-                       double* __restrict__ acc, int64_t nx, int netot, int ns, int k0,
-                       const uint8_t* __restrict__ skip, int kmin) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t x = blockIdx.y;
-  if (e >= netot || x >= nx) return;
-  double ar = 0, ai = 0;
-  bool sk = skip && skip[x];
+// reference state whose steps k < kmin are left out (Cvjflux_k).  A thread
+// owns one energy and EACC_X consecutive x, so each C_k(E) it loads serves
+// EACC_X samples; the samples of a step are the same for the whole block.
+constexpr int EACC_X = 16;
+template <bool CPLX>
+__global__ __launch_bounds__(128) void k_eacc(const double* __restrict__ w,
+                                              const double* __restrict__ z,
+                                              double* __restrict__ acc, int64_t nx, int netot,
+                                              int ns, int k0, const uint8_t* __restrict__ skip,
+                                              int kmin, int64_t xbegin) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t x0 = xbegin + (int64_t)blockIdx.y * EACC_X;
+  if (e >= netot) return;
+  double ar[EACC_X], ai[EACC_X];
+  bool sk[EACC_X];
+#pragma unroll
+  for (int j = 0; j < EACC_X; j++) {
+    ar[j] = ai[j] = 0.0;
+    sk[j] = skip && x0 + j < nx && skip[x0 + j];
+  }
   for (int s = 0; s < ns; s++) {
-    if (sk && k0 + s < kmin) continue;
-    double wr = w[2 * ((int64_t)s * netot + e)], wi = w[2 * ((int64_t)s * netot + e) + 1];
-    if (cplx) {
-      double zr = z[2 * ((int64_t)s * nx + x)], zi = z[2 * ((int64_t)s * nx + x) + 1];
-      ar += wr * zr - wi * zi;
-      ai += wr * zi + wi * zr;
-    } else {
-      double zr = z[(int64_t)s * nx + x];
-      ar += wr * zr;
-      ai += wi * zr;
+    const double wr = w[2 * ((int64_t)s * netot + e)], wi = w[2 * ((int64_t)s * netot + e) + 1];
+    const bool early = k0 + s < kmin;
+#pragma unroll
+    for (int j = 0; j < EACC_X; j++) {
+      const int64_t x = x0 + j < nx ? x0 + j : nx - 1;
+      const bool off = (sk[j] && early) || x0 + j >= nx;
+      if (CPLX) {
+        const double zr = off ? 0.0 : z[2 * ((int64_t)s * nx + x)];
+        const double zi = off ? 0.0 : z[2 * ((int64_t)s * nx + x) + 1];
+        ar[j] += wr * zr - wi * zi;
+        ai[j] += wr * zi + wi * zr;
+      } else {
+        const double zr = off ? 0.0 : z[(int64_t)s * nx + x];
+        ar[j] += wr * zr;
+        ai[j] += wi * zr;
+      }
     }
   }
-  acc[2 * (x * netot + e)] += ar;
-  acc[2 * (x * netot + e) + 1] += ai;
+#pragma unroll
+  for (int j = 0; j < EACC_X; j++)
+    if (x0 + j < nx) {
+      acc[2 * ((x0 + j) * netot + e)] += ar[j];
+      acc[2 * ((x0 + j) * netot + e) + 1] += ai[j];
+    }
+}
+
+// acc[x][E] += sum_s C_{k0+s}(E) z[s][x] over the ring, x in chunks of the grid limit
+static hipError_t launch_eacc(hipStream_t st, const double* w, const double* z, bool cplx,
+                              double* acc, int64_t nx, int netot, int ns, int k0,
+                              const uint8_t* skip, int kmin) {
+  const int64_t maxy = 65535LL * EACC_X;  // x per launch (grid.y limit)
+  for (int64_t x0 = 0; x0 < nx; x0 += maxy) {
+    const int64_t n = std::min<int64_t>(maxy, nx - x0);
+    dim3 g((unsigned)((netot + 127) / 128), (unsigned)((n + EACC_X - 1) / EACC_X));
+    if (cplx)
+      hipLaunchKernelGGL(k_eacc<true>, g, dim3(128), 0, st, w, z, acc, nx, netot, ns, k0, skip,
+                         kmin, x0);
+    else
+      hipLaunchKernelGGL(k_eacc<false>, g, dim3(128), 0, st, w, z, acc, nx, netot, ns, k0, skip,
+                         kmin, x0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // psi(E): zfe[e][i] += C_k(E_e) psi[i] on the dense slab (mask implied by psi)
@@ -1459,36 +1500,27 @@ hipError_t launch_energy_accum(Ctx* c) {
   if (e != hipSuccess) return e;
   // totalflux_k zeroes C_k for k < kminelastic when iphoto = 0 and iprod = 1
   int kmin_flux = (c->s.iphoto == 0 && c->s.iprod == 1) ? c->s.kminelastic : -1;
-  const int64_t maxy = 65535;
   if (t.nflux > 0) {
-    // x runs over grid.y in chunks of 65535
-    for (int64_t x0 = 0; x0 < t.nflux; x0 += maxy) {
-      int64_t nx = std::min<int64_t>(maxy, t.nflux - x0);
-      dim3 g(nblk(netot, 128), (unsigned)nx);
-      hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_zr + x0, 0,
-                         c->d_zcr + 2 * x0 * netot, t.nflux, netot, ns, c->ring_k0,
-                         kmin_flux >= 0 ? c->d_zero_skip : nullptr, kmin_flux);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-      hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_zdr + 2 * x0, 1,
-                         c->d_zcdr + 2 * x0 * netot, t.nflux, netot, ns, c->ring_k0,
-                         kmin_flux >= 0 ? c->d_zero_skip : nullptr, kmin_flux);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
+    const uint8_t* sk = kmin_flux >= 0 ? c->d_zero_skip : nullptr;
+    if ((e = launch_eacc(c->stream, c->d_wmat, c->d_ring_zr, false, c->d_zcr, t.nflux, netot, ns,
+                         c->ring_k0, sk, kmin_flux)) != hipSuccess)
+      return e;
+    if ((e = launch_eacc(c->stream, c->d_wmat, c->d_ring_zdr, true, c->d_zcdr, t.nflux, netot, ns,
+                         c->ring_k0, sk, kmin_flux)) != hipSuccess)
+      return e;
   }
   if (t.ncvj > 0) {
     int kmin = (c->s.iprod != 1 && c->s.iphoto == 0) ? c->s.kminelastic : -1;
-    dim3 g(nblk(netot, 128), (unsigned)t.ncvj);
-    hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_cvj, 0, c->d_zs2,
-                       (int64_t)t.ncvj, netot, ns, c->ring_k0, kmin >= 0 ? c->d_cvj_skip : nullptr,
-                       kmin);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_eacc(c->stream, c->d_wmat, c->d_ring_cvj, false, c->d_zs2, (int64_t)t.ncvj,
+                         netot, ns, c->ring_k0, kmin >= 0 ? c->d_cvj_skip : nullptr, kmin)) !=
+        hipSuccess)
+      return e;
   }
   if (c->prod_on) {  // prodcvj: zS2prod += C_k(E) Cvjprod, every step
-    const int64_t nx = (int64_t)c->pr_nomb * c->pr_nvj;
-    dim3 g(nblk(netot, 128), (unsigned)nx);
-    hipLaunchKernelGGL(k_eacc, g, dim3(128), 0, c->stream, c->d_wmat, c->d_ring_cvjp, 0,
-                       c->d_zs2prod, nx, netot, ns, c->ring_k0, nullptr, -1);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_eacc(c->stream, c->d_wmat, c->d_ring_cvjp, false, c->d_zs2prod,
+                         (int64_t)c->pr_nomb * c->pr_nvj, netot, ns, c->ring_k0, nullptr, -1)) !=
+        hipSuccess)
+      return e;
   }
   timer_end(c, K_ENERGY);
   return hipSuccess;

@@ -13,7 +13,7 @@ NAMES = ["j0", "coriol", "states"]
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [MW3_FP64, MW3_MIXED])
 def test_batch_equals_separate_runs(precision):
-    nsteps = 130  # crosses a 64-step ring boundary
+    nsteps = 300  # crosses a 256-step ring boundary
     print("batch of %s, %d steps, precision %d" % (NAMES, nsteps, precision), flush=True)
     alone = []
     for n in NAMES:

@@ -58,7 +58,7 @@ def test_recursion_and_projections_fp64(name):
     st_o = o.begin(c.rpaq0)
     st_g = g.begin(c.rpaq0)
     assert rel(st_g, st_o) < 1e-11
-    nsteps = 150                     # crosses the 64-step ring twice
+    nsteps = 600                     # crosses the 256-step ring twice
     n_o, _, _ = o.steps(nsteps)
     n_g, _ = g.steps(nsteps)
     assert rel(n_g, n_o) < 1e-10

@@ -3,7 +3,7 @@ SRC/mod_coortrans01_02.f:694-822) on the GPU against the CPU oracle.
 
 Bar: FP64 Cvjprod and zS2prod agree with the oracle within 1e-9 relative to
 their largest value.  The mixed path's psi carries the FP32 operator error
-(about 1e-8 of |psi| per element after 150 steps); the product amplitudes are
+(about 1e-8 of |psi| per element after 600 steps); the product amplitudes are
 projections of order 1e-4 |psi|, so they agree within 1e-3 relative.
 """
 import ctypes
@@ -34,7 +34,7 @@ def test_product_projection_matches_oracle(name, prec, tol):
     g = Propagator(c, prec)
     o.begin(c.rpaq0)
     g.begin(c.rpaq0)
-    nsteps = 150                      # two full 64-step rings and a partial one
+    nsteps = 600                      # two full 256-step rings and a partial one
     o.steps(nsteps)
     g.steps(nsteps)
     e_cv, e_z = rel(g.cvjprod(), o.cvjprod()), rel(g.zs2prod(), o.zs2prod())
