@@ -89,14 +89,13 @@ struct GemmStrides {
   int64_t ldc_m, ldc_n;
 };
 
-// one pass of the step as a batch of GEMMs; tile = 10 TM + TN selects the
-// MFMA tile layout (wave tile TM x TN blocks of 16 x 16); vec: the operands
-// meet the alignment rules of the vectorised kernel; group / share_b: block
-// order (consecutive descriptors sharing the A panel / blocks sharing the B
-// panel kept on one XCD)
+// one pass of the step as a batch of GEMMs; vec: the operands meet the
+// alignment rules of the vectorised kernel; group / share_b: block order
+// (consecutive descriptors sharing the A panel / blocks sharing the B panel
+// kept on one XCD)
 struct GemmPlan {
   GemmDesc* d_descs = nullptr;
-  int nb = 0, maxM = 0, maxN = 0, tile = 22, vec = 0, group = 1, share_b = 0;
+  int nb = 0, maxM = 0, maxN = 0, vec = 0, group = 1, share_b = 0;
   GemmStrides st{};
 };
 

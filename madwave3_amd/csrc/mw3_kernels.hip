@@ -497,13 +497,7 @@ static hipError_t run_gemm(hipStream_t s, const TA* A, const TB* B, TC* C, const
                        p.d_descs, p.st, kp);
     return hipGetLastError();
   }
-  switch (p.tile) {
-    case 22: launch_gemm_vec<ROLE, 2, 2, TACC>(s, A, B, C, p, kp); break;
-    case 42: launch_gemm_vec<ROLE, 4, 2, TACC>(s, A, B, C, p, kp); break;
-    case 24: launch_gemm_vec<ROLE, 2, 4, TACC>(s, A, B, C, p, kp); break;
-    case 44: launch_gemm_vec<ROLE, 4, 4, TACC>(s, A, B, C, p, kp); break;
-    default: return hipErrorInvalidValue;
-  }
+  launch_gemm_vec<ROLE, 2, 2, TACC>(s, A, B, C, p, kp);
   return hipGetLastError();
 }
 
@@ -1085,27 +1079,15 @@ static bool plan_vec_ok(const std::vector<GemmDesc>& ds, const GemmStrides& st, 
   return true;
 }
 
-// MFMA tile per plan (angular fwd, angular back, R passes, r pass with the
-// kinetic factor, r pass back); MW3_TILES="t0,t1,t2,t3,t4" overrides
+// block order of each plan; every vectorised plan runs the 64 x 64 block tile
+// (2 x 2 waves of 2 x 2 MFMA tiles), which beat 128 x 64, 64 x 128 and
+// 128 x 128 on every pass
 static void plan_tiles(Ctx* c) {
-  int t[5] = {22, 22, 22, 22, 22};
-  if (const char* env = getenv("MW3_TILES")) {
-    int v[5];
-    if (sscanf(env, "%d,%d,%d,%d,%d", &v[0], &v[1], &v[2], &v[3], &v[4]) == 5)
-      for (int i = 0; i < 5; i++)
-        if (v[i] == 22 || v[i] == 24 || v[i] == 42 || v[i] == 44) t[i] = v[i];
-  }
   // R pass (1): the two parities of one (channel, angle) fold the same rows
   c->p_dstR.group = 2;
   c->p_dstR.share_b = 0;
   c->p_dstRi.share_b = 0;
   c->p_dstr.share_b = c->p_dstr2.share_b = c->p_angF.share_b = c->p_angB.share_b = 1;
-  c->p_dstRi.tile = t[2];
-  c->p_angF.tile = t[0];
-  c->p_angB.tile = t[1];
-  c->p_dstR.tile = t[2];
-  c->p_dstr.tile = t[3];
-  c->p_dstr2.tile = t[4];
 }
 
 static GemmDesc gd() {
