@@ -185,6 +185,8 @@ struct Ctx {
   // GEMM passes: R rows, r columns, angular forward, angular back
   GemmPlan p_dstR, p_dstr, p_dstr2, p_dstRi, p_angF, p_angB;
   bool ang_fused = false;  // FBR coefficients applied in the forward GEMM (no Coriolis)
+  int ang_nt = 0;          // 16-row tiles of the one-pass angular kernel (0: two GEMMs)
+  int ang_grid = 0;        // its persistent blocks
   void* d_phi = nullptr;        // FBR after the Omega stencil
   double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors
   uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)

@@ -1079,6 +1079,268 @@ static bool plan_vec_ok(const std::vector<GemmDesc>& ds, const GemmStrides& st, 
   return true;
 }
 
+// ---------------------------------------------------------------------------
+// Angular term in one pass, for contexts with no Coriolis neighbour (the FBR
+// coefficients are diagonal in j):
+//   F[j][p]       = sum_ia D_c(ia, j) psi[c][ia][p]
+//   F[j][p]      *= a_j(r1) + b_j(R)                     (p = r1 n2 + R)
+//   ang[c][ia][p] = sum_j D_c(ia, j) F[j][p]
+// the forward product, the coefficients and the back product of the two-GEMM
+// form, with F kept out of HBM.  Persistent blocks of ANG_WAVES waves walk
+// tiles of 16 ANG_WAVES columns p; each wave owns 16 columns and every row of
+// both products (NT row tiles of 16, zero padded).  D_c stays in LDS while a
+// block works on channel c.  F never leaves the registers: the MFMA result
+// layout (column = lane % 16, row MfmaAcc::row(lane, r) in register r) is the
+// B-operand layout of four k steps in which lane group g takes
+// k = row(lane, r) at step r, so the forward accumulators are the back
+// product's B operands as they stand.  The forward product runs over ia in
+// chunks of 16, lane group g taking ia = 16 c + 4 g + s at step s; the LDS
+// row pitch (4 mod 64 words for FP32, 2 mod 32 doubles for FP64) spreads the
+// four groups' rows over distinct banks.
+// ---------------------------------------------------------------------------
+constexpr int ANG_WAVES = 8;
+
+// an opaque copy of a lane value: address arithmetic built on it stays where
+// it is written instead of being hoisted to the top of the tile, where its
+// results would hold registers through both products
+__device__ __forceinline__ int lane_opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+template <typename T>
+struct AngPitch;
+template <>
+struct AngPitch<float> {
+  static constexpr int extra = 4;
+};
+template <>
+struct AngPitch<double> {
+  static constexpr int extra = 2;
+};
+
+template <typename T, int NT>
+__global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
+    const T* __restrict__ psi, const T* __restrict__ Dmat, const double* __restrict__ acoef,
+    const double* __restrict__ bcoef, T* __restrict__ ang, int nc, int na, int nJ, int nJp,
+    int n1, int n2) {
+  typedef MfmaAcc<T> M;
+  typedef typename M::V V;
+  constexpr int NR = 16 * NT;                 // padded rows of D (ia) and of F (j)
+  constexpr int P = NR + AngPitch<T>::extra;  // LDS row pitch
+  constexpr int XA = sizeof(T) == 8 ? 2 : 3;  // psi chunks loaded ahead of the MFMAs
+  __shared__ __align__(16) T Ds[NR * P];
+  const int pl = n1 * n2;  // the host checks that a channel's slab is below 2 GiB
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, i = lane & 15;
+  const int tpc = (pl + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES);  // tiles per channel
+  const int ntile = tpc * nc;
+  // loads take a scalar channel base and a 32-bit per-lane byte offset
+  // x[4 c + s] = psi[ch][16 c + 4 g + s][p]: the B operand of forward step
+  // 4 c + s; rows past na read row na - 1 (finite), which meets zero rows of D
+  T x[4 * NT];
+  auto ldx = [&](int c, int tl) {
+    const int ch = tl / tpc;
+    const char* pb = (const char*)(psi + (int64_t)ch * na * pl);
+    int pp = (tl - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + lane_opaque(i);
+    pp = pp < pl ? pp : pl - 1;
+    const int gg = lane_opaque(g);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      int ia = 16 * c + 4 * gg + s;
+      ia = ia < na ? ia : na - 1;
+      x[4 * c + s] = *(const T*)(pb + (uint32_t)(ia * pl + pp) * (uint32_t)sizeof(T));
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile >= ntile) return;
+#pragma unroll
+  for (int c = 0; c < XA && c < NT; c++) ldx(c, tile);
+  int cur = -1;
+  for (; tile < ntile; tile += gridDim.x) {
+    const int ch = tile / tpc;
+    if (ch != cur) {  // block-uniform
+      __syncthreads();
+      // D_c rows in 4-element vectors (nJp is a multiple of 4), every load
+      // issued before the first LDS store
+      const T* Dc = Dmat + (int64_t)ch * na * nJp;
+      constexpr int NV = (NR * NR / 4 + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
+      constexpr int NG = NV < 4 ? NV : 4;  // vectors in flight per thread
+#pragma unroll
+      for (int l0 = 0; l0 < NV; l0 += NG) {
+        T dv[NG][4];
+#pragma unroll
+        for (int l = 0; l < NG; l++) {
+          const int v = threadIdx.x + 64 * ANG_WAVES * (l0 + l);
+          const int ia = v / (NR / 4), j0 = 4 * (v - ia * (NR / 4));
+#pragma unroll
+          for (int e = 0; e < 4; e++) dv[l][e] = 0;
+          if (l0 + l < NV && ia < na && j0 < nJp) {
+            ld4(Dc + (int64_t)ia * nJp + j0, dv[l]);
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+              if (j0 + e >= nJ) dv[l][e] = 0;
+          }
+        }
+#pragma unroll
+        for (int l = 0; l < NG; l++) {
+          const int v = threadIdx.x + 64 * ANG_WAVES * (l0 + l);
+          const int ia = v / (NR / 4), j0 = 4 * (v - ia * (NR / 4));
+          if (l0 + l < NV && ia < NR) st4(&Ds[ia * P + j0], dv[l]);
+        }
+      }
+      __syncthreads();
+      cur = ch;
+    }
+    // the next tile of this block (the last one repeats itself), whose first
+    // psi chunks load during this tile's back product
+    const int tnext = tile + (int)gridDim.x < ntile ? tile + (int)gridDim.x : tile;
+    const int p = (tile - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + i;
+    const int pc = p < pl ? p : pl - 1;
+    const int r1 = pc / n2, R = pc - r1 * n2;
+    const char* ac = (const char*)(acoef + (int64_t)ch * nJ * n1);
+    const char* bc = (const char*)(bcoef + ((int64_t)ch * 3 + 1) * nJ * n2);
+    // coefficient pairs of row tile t (rows j >= nJ are zero in F)
+    double ca[2][4], cb[2][4];
+    auto ldcoef = [&](int t, double (&a)[4], double (&b)[4]) {
+      const int ll = lane_opaque(lane), rr = lane_opaque(r1), RR = lane_opaque(R);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        int j = 16 * t + M::row(ll, r);
+        j = j < nJ ? j : nJ - 1;
+        a[r] = *(const double*)(ac + (uint32_t)(j * n1 + rr) * 8u);
+        b[r] = *(const double*)(bc + (uint32_t)(j * n2 + RR) * 8u);
+      }
+    };
+    // forward product; each stage reads the next stage's D values from LDS
+    // before its own MFMAs (the scheduling barriers keep that order and stop
+    // the compiler from hoisting every read of the unrolled loop)
+    V f[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) f[t] = (V){0, 0, 0, 0};
+    T a1[2][NT];
+    auto rd1 = [&](int k, T (&a)[NT]) {
+      const T* drow = &Ds[(16 * (k >> 2) + 4 * g + (k & 3)) * P + i];
+#pragma unroll
+      for (int t = 0; t < NT; t++) a[t] = drow[16 * t];
+    };
+    rd1(0, a1[0]);
+#pragma unroll
+    for (int k = 0; k < 4 * NT; k++) {
+      if (k + 1 < 4 * NT) rd1(k + 1, a1[(k + 1) & 1]);
+      if ((k & 3) == 0 && (k >> 2) + XA < NT) ldx((k >> 2) + XA, tile);
+      if (k == 4 * NT - 4) ldcoef(0, ca[0], cb[0]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < NT; t++) f[t] = M::mma(a1[k & 1][t], x[k], f[t]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // back product: row tile t of F is scaled by its coefficients, then
+    // feeds NT output tiles; stage (t, u) reads a[r] = D(16 u + i, 16 t +
+    // row(lane, r)) for the next stage
+    V o[NT];
+#pragma unroll
+    for (int u = 0; u < NT; u++) o[u] = (V){0, 0, 0, 0};
+    T a2[2][4];
+    auto rd2 = [&](int q, T (&a)[4]) {
+      const int t = q / NT, u = q - t * NT;
+      const T* drow = &Ds[(16 * u + i) * P + 16 * t];
+      if (sizeof(T) == 4) {
+        ld4(drow + 4 * g, a);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) a[r] = drow[M::row(lane, r)];
+      }
+    };
+    rd2(0, a2[0]);
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      if (t + 1 < NT) ldcoef(t + 1, ca[(t + 1) & 1], cb[(t + 1) & 1]);
+      if (t == NT / 2) {
+#pragma unroll
+        for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
+#pragma unroll
+      for (int u = 0; u < NT; u++) {
+        const int q = t * NT + u;
+        if (q + 1 < NT * NT) rd2(q + 1, a2[(q + 1) & 1]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[u] = M::mma(a2[q & 1][r], f[t][r], o[u]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    char* ob = (char*)(ang + (int64_t)ch * na * pl);
+    const int ll = lane_opaque(lane), pp = lane_opaque(p);
+#pragma unroll
+    for (int u = 0; u < NT; u++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ia = 16 * u + M::row(ll, r);
+        if (ia < na && pp < pl)
+          *(T*)(ob + (uint32_t)(ia * pl + pp) * (uint32_t)sizeof(T)) = o[u][r];
+      }
+  }
+}
+
+template <typename T, int NT>
+static hipError_t ang_fused_grid(int* blocks) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ang_fused<T, NT>,
+                                                              64 * ANG_WAVES, 0);
+  if (e != hipSuccess) return e;
+  int dev = 0, ncu = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  *blocks = (nb > 0 ? nb : 1) * ncu;
+  return hipSuccess;
+}
+
+template <typename T, int NT>
+static void launch_ang_fused_nt(Ctx* c, const T* psi) {
+  const HostTables& t = c->t;
+  const int64_t tiles = (t.nplane + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES) * t.nc;
+  const unsigned nblk = (unsigned)(tiles < c->ang_grid ? tiles : c->ang_grid);
+  hipLaunchKernelGGL((k_ang_fused<T, NT>), dim3(nblk), dim3(64 * ANG_WAVES), 0, c->stream, psi,
+                     (const T*)c->d_Dmat, (const double*)c->d_acoef, (const double*)c->d_bcoef,
+                     (T*)c->d_ang, t.nc, t.na, t.nJ, t.nJp, t.n1, t.n2);
+}
+
+template <typename T>
+static hipError_t launch_ang_fused(Ctx* c, const T* psi) {
+  switch (c->ang_nt) {
+    case 1: launch_ang_fused_nt<T, 1>(c, psi); break;
+    case 2: launch_ang_fused_nt<T, 2>(c, psi); break;
+    case 3: launch_ang_fused_nt<T, 3>(c, psi); break;
+    case 4: launch_ang_fused_nt<T, 4>(c, psi); break;
+    case 5: launch_ang_fused_nt<T, 5>(c, psi); break;
+    case 6: launch_ang_fused_nt<T, 6>(c, psi); break;
+    case 7: launch_ang_fused_nt<T, 7>(c, psi); break;
+    case 8: launch_ang_fused_nt<T, 8>(c, psi); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t ang_fused_setup(Ctx* c, int nt) {
+  switch (nt) {
+    case 1: return ang_fused_grid<T, 1>(&c->ang_grid);
+    case 2: return ang_fused_grid<T, 2>(&c->ang_grid);
+    case 3: return ang_fused_grid<T, 3>(&c->ang_grid);
+    case 4: return ang_fused_grid<T, 4>(&c->ang_grid);
+    case 5: return ang_fused_grid<T, 5>(&c->ang_grid);
+    case 6: return ang_fused_grid<T, 6>(&c->ang_grid);
+    case 7: return ang_fused_grid<T, 7>(&c->ang_grid);
+    case 8: return ang_fused_grid<T, 8>(&c->ang_grid);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 // block order of each plan; every vectorised plan runs the 64 x 64 block tile
 // (2 x 2 waves of 2 x 2 MFMA tiles), which beat 128 x 64, 64 x 128 and
 // 128 x 128 on every pass
@@ -1254,6 +1516,17 @@ hipError_t build_plans(Ctx* c) {
     if ((e = upload_plan(&c->p_angB, d, true)) != hipSuccess) return e;
   }
   plan_tiles(c);
+  // one-pass angular kernel when both products fit its register tiles
+  c->ang_nt = 0;
+  const int nt = ((t.na > t.nJ ? t.na : t.nJ) + 15) / 16;
+  const size_t es = c->precision == MW3_MIXED ? sizeof(float) : sizeof(double);
+  const bool offs32 = (double)t.na * (double)plane * (double)es < 2147483648.0 &&
+                      (double)t.nJ * (double)(t.n1 > t.n2 ? t.n1 : t.n2) * 8.0 < 2147483648.0;
+  if (c->ang_fused && nt <= 8 && offs32) {
+    e = c->precision == MW3_MIXED ? ang_fused_setup<float>(c, nt) : ang_fused_setup<double>(c, nt);
+    if (e != hipSuccess) return e;
+    c->ang_nt = nt;
+  }
   return hipSuccess;
 }
 
@@ -1293,7 +1566,10 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   hipError_t e;
   // angular term
   timer_begin(c, K_ANG_FWD);
-  if (c->ang_fused) {
+  if (c->ang_nt) {
+    if ((e = launch_ang_fused<T>(c, psi)) != hipSuccess) return e;
+    timer_end(c, K_ANG_FWD);
+  } else if (c->ang_fused) {
     KinParams ka{};
     ka.n1 = t.n1;
     ka.acoef = (const double*)c->d_acoef;
@@ -1316,10 +1592,13 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     if ((e = hipGetLastError()) != hipSuccess) return e;
     timer_end(c, K_CORIOLIS);
   }
-  timer_begin(c, K_ANG_BWD);
-  e = run_gemm<G_ANG_BWD, T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang, c->p_angB);
-  if (e != hipSuccess) return e;
-  timer_end(c, K_ANG_BWD);
+  if (!c->ang_nt) {
+    timer_begin(c, K_ANG_BWD);
+    e = run_gemm<G_ANG_BWD, T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang,
+                                        c->p_angB);
+    if (e != hipSuccess) return e;
+    timer_end(c, K_ANG_BWD);
+  }
   // radial kinetic correction on the low modes: folded DST over R, folded DST
   // over r times (c - E)_+ divi, back over r, back over R into d_t2; the
   // update forms T psi = radcut psi - d_t2 on rows r < nn1
