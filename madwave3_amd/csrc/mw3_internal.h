@@ -48,6 +48,12 @@ struct HostTables {
   std::vector<int64_t> low_off;          // offset of each size's low-mode table L[k][j] = S(j,k)
   std::vector<int> low_ld;               // its row stride
   std::vector<int32_t> k1lo, k2lo;       // [na] r / R modes with p2r1 + p2r2 < radcut possible
+  // tables of the one-wave-per-row-tile R passes (n2 a multiple of 64): with
+  // h = n2 / 2 and k2p = 32 rad_nkb columns k2 (zero for k2 >= n2),
+  // Ti[R][k2] = S(R, k2) (R < h) at rad_ti_off and Tt[k2][j] = S(j, k2)
+  // (j < h) at rad_tt_off; rad_nkb = 0 when they are not built
+  int rad_nkb = 0;
+  int64_t rad_ti_off = 0, rad_tt_off = 0;
   std::vector<double> dst_tables;
   // angular FBR
   std::vector<int32_t> jlist;   // [nJ] j values of the FBR axis
@@ -186,6 +192,9 @@ struct Ctx {
   GemmPlan p_dstR, p_dstr, p_dstr2, p_dstRi, p_angF, p_angB;
   bool ang_fused = false;  // FBR coefficients applied in the forward GEMM (no Coriolis)
   int ang_nt = 0;          // 16-row tiles of the one-pass angular kernel (0: two GEMMs)
+  int32_t* d_rad_items = nullptr;  // R-pass work items {channel-angle, 16-row tile, nn1, k2lo}
+  int rad_nitems = 0;              // 0: the R passes run as GEMMs
+  int rad_grid[2] = {0, 0};        // persistent blocks of the forward / inverse R pass
   int ang_grid = 0;        // its persistent blocks
   void* d_phi = nullptr;        // FBR after the Omega stencil
   double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors

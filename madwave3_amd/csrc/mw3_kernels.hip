@@ -1229,21 +1229,32 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
     for (int k = 0; k < 4 * NT; k++) {
       if (k + 1 < 4 * NT) rd1(k + 1, a1[(k + 1) & 1]);
       if ((k & 3) == 0 && (k >> 2) + XA < NT) ldx((k >> 2) + XA, tile);
-      if (k == 4 * NT - 4) ldcoef(0, ca[0], cb[0]);
+      if (k == (NT > 1 ? 4 * NT - 8 : 0)) ldcoef(0, ca[0], cb[0]);
+      if (NT > 1 && k == 4 * NT - 4) ldcoef(1, ca[1], cb[1]);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int t = 0; t < NT; t++) f[t] = M::mma(a1[k & 1][t], x[k], f[t]);
       __builtin_amdgcn_sched_barrier(0);
     }
-    // back product: row tile t of F is scaled by its coefficients, then
-    // feeds NT output tiles; stage (t, u) reads a[r] = D(16 u + i, 16 t +
-    // row(lane, r)) for the next stage
-    V o[NT];
+    // back product in NH passes over the output tiles (NT / NH accumulators
+    // live at a time); row tile t of F, scaled by its coefficients during
+    // the first pass, feeds the pass's output tiles.  Stage (h, t, uu) reads
+    // a[r] = D(16 u + i, 16 t + row(lane, r)), u = h UN + uu, two stages
+    // ahead; halfway through row tile t of the first pass the VALU scales row
+    // tile t + 1 (its coefficients loaded a row tile earlier) while the MFMAs
+    // run
+    constexpr int NH = NT % 2 == 0 ? 2 : 1;
+    constexpr int UN = NT / NH;
+    constexpr int NQ = NH * NT * UN;  // stages
+    auto scale = [&](int t) {
 #pragma unroll
-    for (int u = 0; u < NT; u++) o[u] = (V){0, 0, 0, 0};
-    T a2[2][4];
+      for (int r = 0; r < 4; r++)
+        f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
+    };
+    T a2[3][4];
     auto rd2 = [&](int q, T (&a)[4]) {
-      const int t = q / NT, u = q - t * NT;
+      const int h = q / (NT * UN), t = (q - h * NT * UN) / UN;
+      const int u = h * UN + (q - h * NT * UN - t * UN);
       const T* drow = &Ds[(16 * u + i) * P + 16 * t];
       if (sizeof(T) == 4) {
         ld4(drow + 4 * g, a);
@@ -1253,36 +1264,43 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
       }
     };
     rd2(0, a2[0]);
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      if (t + 1 < NT) ldcoef(t + 1, ca[(t + 1) & 1], cb[(t + 1) & 1]);
-      if (t == NT / 2) {
-#pragma unroll
-        for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
-#pragma unroll
-      for (int u = 0; u < NT; u++) {
-        const int q = t * NT + u;
-        if (q + 1 < NT * NT) rd2(q + 1, a2[(q + 1) & 1]);
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int r = 0; r < 4; r++) o[u] = M::mma(a2[q & 1][r], f[t][r], o[u]);
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
+    if (NQ > 1) rd2(1, a2[1]);
+    scale(0);
+    constexpr int UH = UN / 2 > 0 ? UN / 2 - 1 : 0;
     char* ob = (char*)(ang + (int64_t)ch * na * pl);
-    const int ll = lane_opaque(lane), pp = lane_opaque(p);
 #pragma unroll
-    for (int u = 0; u < NT; u++)
+    for (int h = 0; h < NH; h++) {
+      V o[UN];
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int ia = 16 * u + M::row(ll, r);
-        if (ia < na && pp < pl)
-          *(T*)(ob + (uint32_t)(ia * pl + pp) * (uint32_t)sizeof(T)) = o[u][r];
+      for (int uu = 0; uu < UN; uu++) o[uu] = (V){0, 0, 0, 0};
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+#pragma unroll
+        for (int uu = 0; uu < UN; uu++) {
+          const int q = (h * NT + t) * UN + uu;
+          if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
+          if (h == 0 && uu == 0 && t + 2 < NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
+          if (uu == 0 && t == 0 && h == NH - 1) {
+#pragma unroll
+            for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int r = 0; r < 4; r++) o[uu] = M::mma(a2[q % 3][r], f[t][r], o[uu]);
+          __builtin_amdgcn_sched_barrier(0);
+          if (h == 0 && uu == UH && t + 1 < NT) scale(t + 1);
+        }
       }
+      const int ll = lane_opaque(lane), pp = lane_opaque(p);
+#pragma unroll
+      for (int uu = 0; uu < UN; uu++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int ia = 16 * (h * UN + uu) + M::row(ll, r);
+          if (ia < na && pp < pl)
+            *(T*)(ob + (uint32_t)(ia * pl + pp) * (uint32_t)sizeof(T)) = o[uu][r];
+        }
+    }
   }
 }
 
@@ -1340,6 +1358,264 @@ static hipError_t ang_fused_setup(Ctx* c, int nt) {
     default: return hipErrorInvalidValue;
   }
 }
+
+// ---------------------------------------------------------------------------
+// First and last passes of the radial correction over R, for n2 = 32 HB with
+// HB in {8, 10, 12, 16}, on work items of 16 rows r < nn1(ia) of one
+// (channel, angle): int4 {channel * na + angle, row tile, nn1, k2lo}.
+// With h = n2 / 2 and S(n2-1-R, k) = (-1)^k S(R, k):
+//   k_rad_fwd: t1[r][k2] = sum_R S(R, k2) psi[r][R] for k2 < k2lo(ia), as
+//     even k2: sum_{j<h} (psi[j] + psi[n2-1-j]) S(j, k2)
+//     odd  k2: sum_{j<h} (psi[j] - psi[n2-1-j]) S(j, k2);
+//   k_rad_inv: corr[r][R] = E + O and corr[r][n2-1-R] = E - O for R < h,
+//     E / O the sums of t1[r][k2] S(R, k2) over even / odd k2 < k2lo(ia)
+//     (half the MFMAs of the unfolded product, every output written once).
+// Both are persistent: each wave keeps its slice of the sine table in
+// registers for the whole launch (the forward pass: one 16-wide k2 tile of
+// one parity, all j; the inverse pass: one pair of 16-wide R tiles, all
+// k2), and the block walks the work items with the item's 16-row tile staged
+// in LDS; the next item's tile loads into registers during the MFMAs.
+// MFMA k index of lane group g at step s: j = 16 kb + 4 g + s (forward) and
+// k2 = 32 kb + 8 g + 2 s + parity (inverse), so a lane's operands for four
+// steps are one 16-byte read.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RadPad;  // LDS row padding (elements): rows start 4 banks apart
+template <>
+struct RadPad<float> {
+  static constexpr int v = 4;
+};
+template <>
+struct RadPad<double> {
+  static constexpr int v = 2;
+};
+
+template <typename T, int NKB, int HB>
+__global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi,
+                                                       const T* __restrict__ Tt,
+                                                       T* __restrict__ t1,
+                                                       const int4* __restrict__ items, int nitems,
+                                                       int n1) {
+  typedef MfmaAcc<T> M;
+  typedef typename M::V V;
+  constexpr int N2 = 32 * HB, H = N2 / 2, NT = 128 * NKB;  // threads
+  constexpr int LP = N2 + RadPad<T>::v;
+  constexpr int NV4 = 16 * N2 / 4, PER = (NV4 + NT - 1) / NT;
+  __shared__ __align__(16) T tile[16 * LP];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, wave = threadIdx.x >> 6;
+  const int p = wave / NKB, t = wave - p * NKB;  // parity and k' tile of this wave
+  const int64_t plane = (int64_t)n1 * N2;
+  // b[kb][s] = S(16 kb + 4 g + s, 2 (16 t + i) + p) for every j block
+  T b[HB][4];
+  {
+    const T* trow = Tt + (int64_t)(2 * (16 * t + i) + p) * H + 4 * g;
+#pragma unroll
+    for (int kb = 0; kb < HB; kb++) ld4(trow + 16 * kb, b[kb]);
+  }
+  T pf[PER][4];
+  auto fetch = [&](const int4& it) {
+#pragma unroll
+    for (int l = 0; l < PER; l++) {
+      int v = threadIdx.x + NT * l;
+      v = v < NV4 ? v : NV4 - 1;  // spare threads repeat the last vector
+      const int row = v / (N2 / 4), c4 = v - row * (N2 / 4);
+      int r = 16 * it.y + row;
+      r = r < it.z ? r : it.z - 1;
+      ld4(psi + it.x * plane + (int64_t)r * N2 + 4 * c4, pf[l]);
+    }
+  };
+  auto put = [&]() {
+#pragma unroll
+    for (int l = 0; l < PER; l++) {
+      int v = threadIdx.x + NT * l;
+      v = v < NV4 ? v : NV4 - 1;
+      const int row = v / (N2 / 4), c4 = v - row * (N2 / 4);
+      st4(&tile[row * LP + 4 * c4], pf[l]);
+    }
+  };
+  int n = blockIdx.x;
+  if (n >= nitems) return;  // block-uniform
+  int4 cur = items[n];
+  fetch(cur);
+  for (; n < nitems; n += gridDim.x) {
+    const int4 nxt = items[n + (int)gridDim.x < nitems ? n + (int)gridDim.x : n];
+    __syncthreads();  // the previous tile is no longer read
+    put();
+    __syncthreads();
+    fetch(nxt);  // the last item fetches itself again, unused
+    V acc = (V){0, 0, 0, 0};
+    const T* trow = &tile[i * LP + 4 * g];
+#pragma unroll
+    for (int kb = 0; kb < HB; kb++) {
+      T lo[4], hi[4];
+      ld4(trow + 16 * kb, lo);
+      ld4(trow + (N2 - 4 - 8 * g) - 16 * kb, hi);  // psi[n2-4-j .. n2-1-j], j = 16 kb + 4 g
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        const T x = p ? lo[s] - hi[3 - s] : lo[s] + hi[3 - s];
+        acc = M::mma(x, b[kb][s], acc);
+      }
+    }
+    T* orow = t1 + cur.x * plane;
+    const int k2 = 2 * (16 * t + i) + p;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const int row = 16 * cur.y + M::row(lane, rr);
+      if (row < cur.z && k2 < cur.w) orow[(int64_t)row * N2 + k2] = acc[rr];
+    }
+    cur = nxt;
+  }
+}
+
+template <typename T, int NKB, int HB>
+__global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
+                                                     const T* __restrict__ Ti,
+                                                     T* __restrict__ corr,
+                                                     const int4* __restrict__ items, int nitems,
+                                                     int n1) {
+  typedef MfmaAcc<T> M;
+  typedef typename M::V V;
+  constexpr int N2 = 32 * HB, NT = 32 * HB;  // one wave per pair of R tiles of the first half
+  constexpr int K2P = 32 * NKB, LP = K2P + RadPad<T>::v;
+  constexpr int NV4 = 16 * K2P / 4, PER = (NV4 + NT - 1) / NT;
+  __shared__ __align__(16) T tile[16 * LP];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, pair = threadIdx.x >> 6;
+  const int64_t plane = (int64_t)n1 * N2;
+  // b[q][kb][e] = S(R, 32 kb + 8 g + e), R = 16 (2 pair + q) + i
+  T b[2][NKB][2][4];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const T* trow = Ti + (int64_t)(16 * (2 * pair + q) + i) * K2P + 8 * g;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+      ld4(trow + 32 * kb, b[q][kb][0]);
+      ld4(trow + 32 * kb + 4, b[q][kb][1]);
+    }
+  }
+  T pf[PER][4];
+  auto fetch = [&](const int4& it) {
+#pragma unroll
+    for (int l = 0; l < PER; l++) {
+      int v = threadIdx.x + NT * l;
+      v = v < NV4 ? v : NV4 - 1;
+      const int row = v / (K2P / 4), c4 = v - row * (K2P / 4);
+      int r = 16 * it.y + row;
+      r = r < it.z ? r : it.z - 1;
+      ld4(t1 + it.x * plane + (int64_t)r * N2 + 4 * c4, pf[l]);
+    }
+  };
+  auto put = [&](const int4& it) {  // columns k2 >= k2lo enter as zero
+#pragma unroll
+    for (int l = 0; l < PER; l++) {
+      int v = threadIdx.x + NT * l;
+      v = v < NV4 ? v : NV4 - 1;
+      const int row = v / (K2P / 4), c4 = v - row * (K2P / 4);
+#pragma unroll
+      for (int e = 0; e < 4; e++)
+        if (4 * c4 + e >= it.w) pf[l][e] = 0;
+      st4(&tile[row * LP + 4 * c4], pf[l]);
+    }
+  };
+  int n = blockIdx.x;
+  if (n >= nitems) return;  // block-uniform
+  int4 cur = items[n];
+  fetch(cur);
+  for (; n < nitems; n += gridDim.x) {
+    const int4 nxt = items[n + (int)gridDim.x < nitems ? n + (int)gridDim.x : n];
+    __syncthreads();
+    put(cur);
+    __syncthreads();
+    fetch(nxt);
+    T a[NKB][2][4];
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+      ld4(&tile[i * LP + 32 * kb + 8 * g], a[kb][0]);
+      ld4(&tile[i * LP + 32 * kb + 8 * g + 4], a[kb][1]);
+    }
+    T* ob = corr + cur.x * plane;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      V E = (V){0, 0, 0, 0}, O = (V){0, 0, 0, 0};
+#pragma unroll
+      for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+          const int e0 = 2 * s, e1 = 2 * s + 1;
+          E = M::mma(a[kb][e0 >> 2][e0 & 3], b[q][kb][e0 >> 2][e0 & 3], E);
+          O = M::mma(a[kb][e1 >> 2][e1 & 3], b[q][kb][e1 >> 2][e1 & 3], O);
+        }
+      const int R = 16 * (2 * pair + q) + i;
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int row = 16 * cur.y + M::row(lane, rr);
+        if (row < cur.z) {
+          ob[(int64_t)row * N2 + R] = E[rr] + O[rr];
+          ob[(int64_t)row * N2 + (N2 - 1 - R)] = E[rr] - O[rr];
+        }
+      }
+    }
+    cur = nxt;
+  }
+}
+
+template <typename K>
+static hipError_t persistent_grid(K kernel, int threads, int* blocks) {
+  int nb = 0, dev = 0, ncu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0);
+  if (e != hipSuccess) return e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  *blocks = (nb > 0 ? nb : 1) * ncu;
+  return hipSuccess;
+}
+
+// launch (or, with setup, size the grids of) the R passes for NKB, HB
+template <typename T, int NKB, int HB>
+static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
+  const HostTables& t = c->t;
+  const T* tab = (const T*)c->d_dst_tables;
+  const int4* items = (const int4*)c->d_rad_items;
+  if (mode == 2) {
+    hipError_t e = persistent_grid(k_rad_fwd<T, NKB, HB>, 128 * NKB, &c->rad_grid[0]);
+    if (e != hipSuccess) return e;
+    return persistent_grid(k_rad_inv<T, NKB, HB>, 32 * HB, &c->rad_grid[1]);
+  }
+  const int n = c->rad_nitems;
+  if (mode == 0) {
+    hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB>), dim3(n < c->rad_grid[0] ? n : c->rad_grid[0]),
+                       dim3(128 * NKB), 0, c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+  } else {
+    hipLaunchKernelGGL((k_rad_inv<T, NKB, HB>), dim3(n < c->rad_grid[1] ? n : c->rad_grid[1]),
+                       dim3(32 * HB), 0, c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int NKB>
+static hipError_t rad_nkb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
+  switch (c->t.n2) {
+    case 256: return rad_nkb_hb<T, NKB, 8>(c, psi, t1, corr, mode);
+    case 320: return rad_nkb_hb<T, NKB, 10>(c, psi, t1, corr, mode);
+    case 384: return rad_nkb_hb<T, NKB, 12>(c, psi, t1, corr, mode);
+    case 512: return rad_nkb_hb<T, NKB, 16>(c, psi, t1, corr, mode);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// mode 0: forward R pass (psi -> t1); 1: inverse (t1 -> corr); 2: size the
+// persistent grids
+template <typename T>
+static hipError_t run_rad(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
+  switch (c->t.rad_nkb) {
+    case 1: return rad_nkb<T, 1>(c, psi, t1, corr, mode);
+    case 2: return rad_nkb<T, 2>(c, psi, t1, corr, mode);
+    case 3: return rad_nkb<T, 3>(c, psi, t1, corr, mode);
+    case 4: return rad_nkb<T, 4>(c, psi, t1, corr, mode);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 
 // block order of each plan; every vectorised plan runs the 64 x 64 block tile
 // (2 x 2 waves of 2 x 2 MFMA tiles), which beat 128 x 64, 64 x 128 and
@@ -1516,6 +1792,28 @@ hipError_t build_plans(Ctx* c) {
     if ((e = upload_plan(&c->p_angB, d, true)) != hipSuccess) return e;
   }
   plan_tiles(c);
+  // R passes one wave per 16-row tile when their tables were built
+  c->rad_nitems = 0;
+  if (t.rad_nkb > 0 && (t.n2 == 256 || t.n2 == 320 || t.n2 == 384 || t.n2 == 512)) {
+    // items {channel * na + angle, row tile, nn1, k2lo}
+    std::vector<int32_t> items;
+    for (int ch = 0; ch < t.nc; ch++)
+      for (int ia = 0; ia < t.na; ia++)
+        for (int rt = 0; 16 * rt < t.nn1[ia]; rt++) {
+          items.push_back(ch * t.na + ia);
+          items.push_back(rt);
+          items.push_back(t.nn1[ia]);
+          items.push_back(t.k2lo[ia]);
+        }
+    if ((e = hipMalloc(&c->d_rad_items, items.size() * sizeof(int32_t))) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_rad_items, items.data(), items.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice)) != hipSuccess)
+      return e;
+    c->rad_nitems = (int)(items.size() / 4);
+    e = c->precision == MW3_MIXED ? run_rad<float>(c, nullptr, nullptr, nullptr, 2)
+                                  : run_rad<double>(c, nullptr, nullptr, nullptr, 2);
+    if (e != hipSuccess) return e;
+  }
   // one-pass angular kernel when both products fit its register tiles
   c->ang_nt = 0;
   const int nt = ((t.na > t.nJ ? t.na : t.nJ) + 15) / 16;
@@ -1604,7 +1902,8 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // update forms T psi = radcut psi - d_t2 on rows r < nn1
   KinParams kp{c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.n1};
   timer_begin(c, K_DST_R);
-  e = run_gemm<G_DST_R, T, T, T, T>(s, psi, tab, t1, c->p_dstR);
+  e = c->rad_nitems ? run_rad<T>(c, psi, t1, t2, 0)
+                    : run_gemm<G_DST_R, T, T, T, T>(s, psi, tab, t1, c->p_dstR);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_R);
   timer_begin(c, K_DST_r);
@@ -1614,7 +1913,8 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_r);
   timer_begin(c, K_DST_RINV);
-  e = run_gemm<G_DST_R_INV, T, T, T, T>(s, t1, tab, t2, c->p_dstRi);
+  e = c->rad_nitems ? run_rad<T>(c, psi, t1, t2, 1)
+                    : run_gemm<G_DST_R_INV, T, T, T, T>(s, t1, tab, t2, c->p_dstRi);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_RINV);
   return hipSuccess;

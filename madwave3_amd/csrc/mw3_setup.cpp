@@ -291,6 +291,29 @@ int build_tables(const mw3_setup* s, HostTables* t, std::string* err) {
         }
       off += (int64_t)kcap * ld;
     }
+    // transposed n2 tables for the one-wave-per-row-tile R passes
+    int kmax = 0;
+    for (int ia = 0; ia < na; ia++) kmax = std::max(kmax, t->k2lo[ia]);
+    const int nkb = ((kmax + 1) / 2 + 15) / 16;
+    if (n2 % 64 == 0 && nkb >= 1 && nkb <= 4) {
+      const int h = n2 / 2, k2p = 32 * nkb;
+      const long m2 = 2L * (n2 + 1);
+      auto S = [&](int j, int k) {
+        if (k >= n2) return 0.0;
+        long m = ((long)(j + 1) * (k + 1)) % m2;
+        if (m == 0 || m == n2 + 1) return 0.0;
+        return 2.0 * std::sin(pi * (double)m / (double)(n2 + 1));
+      };
+      t->rad_nkb = nkb;
+      t->rad_ti_off = off;
+      for (int R = 0; R < h; R++)
+        for (int k = 0; k < k2p; k++) t->dst_tables.push_back(S(R, k));
+      off += (int64_t)h * k2p;
+      t->rad_tt_off = off;
+      for (int k = 0; k < k2p; k++)
+        for (int j = 0; j < h; j++) t->dst_tables.push_back(S(j, k));
+      off += (int64_t)k2p * h;
+    }
   }
 
   // angular FBR tables (angkinj2 / bfgridl2mat restated in factor form)

@@ -24,6 +24,9 @@ def case(name):
       prodrange  iprod = 2 on a sub-range of R (n2prod0 > 1, n2prod1 < npun2,
               so the product functions extend beyond their last knot) and of
               the angles
+      r256, r320, r384, r512  few r points and angles on the R grids of the
+              configurations (the R passes' one-wave-per-row-tile kernels);
+              r384 with three coupled states
     """
     if name == "j0":
         p = leps.config(1, scale=8)
@@ -52,6 +55,10 @@ def case(name):
         p = leps.config(1, scale=4)
         p.update(iprod=2, nvmaxprod=3, jmaxprod=6, netot=60)
         p.update(n2prod0=11, n2prod1=p["npun2"] - 7, nangproj0=3, nangproj1=p["nangu"] - 2)
+    elif name in ("r256", "r320", "r384", "r512"):
+        n2 = int(name[1:])
+        p = leps.config(5 if n2 == 384 else 1)
+        p.update(iprod=0, npun1=40, npun2=n2, nangu=10, npun1min=16, jmax=5, nvmax=2, netot=40)
     else:
         raise KeyError(name)
     p.update(kminelastic=5)

@@ -22,7 +22,10 @@ def rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("name", CASES + ["mid"])
+RGRIDS = ["r256", "r320", "r384", "r512"]
+
+
+@pytest.mark.parametrize("name", CASES + ["mid"] + RGRIDS)
 def test_hphi_fp64(name):
     c = case(name)
     print("case", name, c, flush=True)
@@ -38,7 +41,7 @@ def test_hphi_fp64(name):
     assert rel(h, o.hphi(c.rpaq0)) < 1e-11
 
 
-@pytest.mark.parametrize("name", CASES + ["mid"])
+@pytest.mark.parametrize("name", CASES + ["mid"] + RGRIDS)
 def test_hphi_mixed(name):
     c = case(name)
     print("case", name, c, flush=True)
