@@ -196,6 +196,10 @@ struct Ctx {
   int rad_nitems = 0;              // 0: the R passes run as GEMMs
   int rad_grid[2] = {0, 0};        // persistent blocks of the forward / inverse R pass
   int ang_grid = 0;        // its persistent blocks
+  int ang_fold = 0;        // row tiles of the parity-folded one-pass kernel (0: not used)
+  void* d_fold_D = nullptr;    // [nc][16 ang_fold][32 ang_fold] D(ia < na/2, j by parity class)
+  double* d_fold_a = nullptr;  // [nc][32 ang_fold][n1] a_j(r1) in the same j order
+  double* d_fold_b = nullptr;  // [nc][32 ang_fold][n2] b_j(R)
   void* d_phi = nullptr;        // FBR after the Omega stencil
   double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors
   uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)

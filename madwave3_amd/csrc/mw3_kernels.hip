@@ -1304,6 +1304,306 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
   }
 }
 
+// ---------------------------------------------------------------------------
+// The one-pass angular term folded on the parity of the angular functions.
+// For an even number of angles whose functions satisfy D(na-1-ia, j) =
+// s_j D(ia, j) with s_j = +-1 (the host checks it to 1e-12 of each
+// function's largest value), with h = na / 2, u = psi[ia] + psi[na-1-ia]
+// and v = psi[ia] - psi[na-1-ia] (ia < h):
+//   F[j] = sum_{ia<h} D(ia, j) u[ia] (s_j = +1) or D(ia, j) v[ia] (s_j = -1)
+//   G[j] = (a_j(r1) + b_j(R)) F[j]
+//   E[ia] = sum_{s_j=+1} D(ia, j) G[j],   O[ia] = sum_{s_j=-1} D(ia, j) G[j]
+//   ang[ia] = E + O,   ang[na-1-ia] = E - O
+// which halves both products.  The j axis runs class by class: columns
+// 0 .. NR-1 of the LDS image of D hold the s_j = +1 functions and NR ..
+// 2 NR-1 the s_j = -1 ones (zero padded), and the coefficient tables follow
+// the same order, so row tile T of F (T < 2 NT) is class T / NT.  Otherwise
+// as k_ang_fused.
+// ---------------------------------------------------------------------------
+template <typename T, int NT>
+__global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
+    const T* __restrict__ psi, const T* __restrict__ Dfold, const double* __restrict__ afold,
+    const double* __restrict__ bfold, T* __restrict__ ang, int nc, int na, int n1, int n2) {
+  typedef MfmaAcc<T> M;
+  typedef typename M::V V;
+  constexpr int NR = 16 * NT, NC2 = 2 * NR;    // rows (ia < h) and columns (both classes)
+  constexpr int P = NC2 + AngPitch<T>::extra;  // LDS row pitch
+  constexpr int XA = sizeof(T) == 8 ? 2 : 3;   // psi chunks loaded ahead of the MFMAs
+  __shared__ __align__(16) T Ds[NR * P];
+  const int pl = n1 * n2, h = na >> 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, i = lane & 15;
+  const int tpc = (pl + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES);  // tiles per channel
+  const int ntile = tpc * nc;
+  // xl[4 c + s] = psi[16 c + 4 g + s][p], xh[..] = psi[na-1-(16 c + 4 g + s)][p];
+  // angles past h read angle h-1 and its mirror (finite), against zero rows of D
+  T xl[4 * NT], xh[4 * NT];
+  auto ldx = [&](int c, int tl) {
+    const int ch = tl / tpc;
+    const char* pb = (const char*)(psi + (int64_t)ch * na * pl);
+    int pp = (tl - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + lane_opaque(i);
+    pp = pp < pl ? pp : pl - 1;
+    const int gg = lane_opaque(g);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      int ia = 16 * c + 4 * gg + s;
+      ia = ia < h ? ia : h - 1;
+      xl[4 * c + s] = *(const T*)(pb + (uint32_t)(ia * pl + pp) * (uint32_t)sizeof(T));
+      xh[4 * c + s] = *(const T*)(pb + (uint32_t)((na - 1 - ia) * pl + pp) * (uint32_t)sizeof(T));
+    }
+  };
+  int tile = blockIdx.x;
+  if (tile >= ntile) return;
+#pragma unroll
+  for (int c = 0; c < XA && c < NT; c++) ldx(c, tile);
+  int cur = -1;
+  for (; tile < ntile; tile += gridDim.x) {
+    const int ch = tile / tpc;
+    if (ch != cur) {  // block-uniform
+      __syncthreads();
+      const T* Dc = Dfold + (int64_t)ch * NR * NC2;
+      constexpr int NV = (NR * NC2 / 4 + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
+      constexpr int NG = NV < 4 ? NV : 4;  // vectors in flight per thread
+#pragma unroll
+      for (int l0 = 0; l0 < NV; l0 += NG) {
+        T dv[NG][4];
+#pragma unroll
+        for (int l = 0; l < NG; l++) {
+          int v = threadIdx.x + 64 * ANG_WAVES * (l0 + l);
+          v = v < NR * NC2 / 4 ? v : NR * NC2 / 4 - 1;
+          ld4(Dc + 4 * (int64_t)v, dv[l]);
+        }
+#pragma unroll
+        for (int l = 0; l < NG; l++) {
+          int v = threadIdx.x + 64 * ANG_WAVES * (l0 + l);
+          v = v < NR * NC2 / 4 ? v : NR * NC2 / 4 - 1;
+          const int row = v / (NC2 / 4), c4 = v - row * (NC2 / 4);
+          st4(&Ds[row * P + 4 * c4], dv[l]);
+        }
+      }
+      __syncthreads();
+      cur = ch;
+    }
+    const int tnext = tile + (int)gridDim.x < ntile ? tile + (int)gridDim.x : tile;
+    const int p = (tile - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + i;
+    const int pc = p < pl ? p : pl - 1;
+    const int r1 = pc / n2, R = pc - r1 * n2;
+    const char* ac = (const char*)(afold + (int64_t)ch * NC2 * n1);
+    const char* bc = (const char*)(bfold + (int64_t)ch * NC2 * n2);
+    double ca[2][4], cb[2][4];
+    auto ldcoef = [&](int t, double (&a)[4], double (&b)[4]) {
+      const int ll = lane_opaque(lane), rr = lane_opaque(r1), RR = lane_opaque(R);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = 16 * t + M::row(ll, r);
+        a[r] = *(const double*)(ac + (uint32_t)(j * n1 + rr) * 8u);
+        b[r] = *(const double*)(bc + (uint32_t)(j * n2 + RR) * 8u);
+      }
+    };
+    // forward products: f[t] (t < NT) the s = +1 rows from u, f[NT + t] the
+    // s = -1 rows from v
+    V f[2 * NT];
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++) f[t] = (V){0, 0, 0, 0};
+    T a1[2][2 * NT];
+    auto rd1 = [&](int k, T (&a)[2 * NT]) {
+      const T* drow = &Ds[(16 * (k >> 2) + 4 * g + (k & 3)) * P + i];
+#pragma unroll
+      for (int t = 0; t < 2 * NT; t++) a[t] = drow[16 * t];
+    };
+    rd1(0, a1[0]);
+#pragma unroll
+    for (int k = 0; k < 4 * NT; k++) {
+      if (k + 1 < 4 * NT) rd1(k + 1, a1[(k + 1) & 1]);
+      if ((k & 3) == 0 && (k >> 2) + XA < NT) ldx((k >> 2) + XA, tile);
+      if (k == (NT > 1 ? 4 * NT - 8 : 0)) ldcoef(0, ca[0], cb[0]);
+      if (k == 4 * NT - 4) ldcoef(1, ca[1], cb[1]);
+      __builtin_amdgcn_sched_barrier(0);
+      const T u = xl[k] + xh[k], v = xl[k] - xh[k];
+#pragma unroll
+      for (int t = 0; t < NT; t++) f[t] = M::mma(a1[k & 1][t], u, f[t]);
+#pragma unroll
+      for (int t = NT; t < 2 * NT; t++) f[t] = M::mma(a1[k & 1][t], v, f[t]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // back products: row tile t of F (class t / NT), scaled by its
+    // coefficients, feeds the class's NT output tiles o[(t / NT) NT + u];
+    // stage (t, u) reads a[r] = D(16 u + i, 16 t + row(lane, r)) two stages
+    // ahead, and halfway through row tile t the VALU scales row tile t + 1
+    constexpr int NQ = 2 * NT * NT;
+    auto scale = [&](int t) {
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
+    };
+    V o[2 * NT];
+#pragma unroll
+    for (int u = 0; u < 2 * NT; u++) o[u] = (V){0, 0, 0, 0};
+    T a2[3][4];
+    auto rd2 = [&](int q, T (&a)[4]) {
+      const int t = q / NT, u = q - t * NT;
+      const T* drow = &Ds[(16 * u + i) * P + 16 * t];
+      if (sizeof(T) == 4) {
+        ld4(drow + 4 * g, a);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) a[r] = drow[M::row(lane, r)];
+      }
+    };
+    rd2(0, a2[0]);
+    if (NQ > 1) rd2(1, a2[1]);
+    scale(0);
+    constexpr int UH = NT / 2 > 0 ? NT / 2 - 1 : 0;
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++) {
+#pragma unroll
+      for (int u = 0; u < NT; u++) {
+        const int q = t * NT + u;
+        if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
+        if (u == 0 && t + 2 < 2 * NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
+        if (u == 0 && t == NT) {
+#pragma unroll
+          for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        const int ou = (t / NT) * NT + u;
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[ou] = M::mma(a2[q % 3][r], f[t][r], o[ou]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (u == UH && t + 1 < 2 * NT) scale(t + 1);
+      }
+    }
+    char* ob = (char*)(ang + (int64_t)ch * na * pl);
+    const int ll = lane_opaque(lane), pp = lane_opaque(p);
+#pragma unroll
+    for (int u = 0; u < NT; u++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ia = 16 * u + M::row(ll, r);
+        if (ia < h && pp < pl) {
+          const T e = o[u][r], od = o[NT + u][r];
+          *(T*)(ob + (uint32_t)(ia * pl + pp) * (uint32_t)sizeof(T)) = e + od;
+          *(T*)(ob + (uint32_t)((na - 1 - ia) * pl + pp) * (uint32_t)sizeof(T)) = e - od;
+        }
+      }
+  }
+}
+
+template <typename T, int NT>
+static hipError_t ang_fold_grid(int* blocks) {
+  int nb = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ang_fold<T, NT>,
+                                                              64 * ANG_WAVES, 0);
+  if (e != hipSuccess) return e;
+  int dev = 0, ncu = 0;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  *blocks = (nb > 0 ? nb : 1) * ncu;
+  return hipSuccess;
+}
+
+template <typename T, int NT>
+static void launch_ang_fold_nt(Ctx* c, const T* psi) {
+  const HostTables& t = c->t;
+  const int64_t tiles = (t.nplane + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES) * t.nc;
+  const unsigned nblk = (unsigned)(tiles < c->ang_grid ? tiles : c->ang_grid);
+  hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3(nblk), dim3(64 * ANG_WAVES), 0, c->stream, psi,
+                     (const T*)c->d_fold_D, c->d_fold_a, c->d_fold_b, (T*)c->d_ang, t.nc, t.na,
+                     t.n1, t.n2);
+}
+
+template <typename T>
+static hipError_t launch_ang_fold(Ctx* c, const T* psi) {
+  switch (c->ang_fold) {
+    case 1: launch_ang_fold_nt<T, 1>(c, psi); break;
+    case 2: launch_ang_fold_nt<T, 2>(c, psi); break;
+    case 3: launch_ang_fold_nt<T, 3>(c, psi); break;
+    case 4: launch_ang_fold_nt<T, 4>(c, psi); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t ang_fold_setup(Ctx* c, int nt) {
+  switch (nt) {
+    case 1: return ang_fold_grid<T, 1>(&c->ang_grid);
+    case 2: return ang_fold_grid<T, 2>(&c->ang_grid);
+    case 3: return ang_fold_grid<T, 3>(&c->ang_grid);
+    case 4: return ang_fold_grid<T, 4>(&c->ang_grid);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// the folded angular tables when every channel's functions have the parity
+// symmetry; returns the kernel's tile count (0: not folded)
+static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
+  const HostTables& t = c->t;
+  *nt_out = 0;
+  if (t.na % 2 != 0) return hipSuccess;
+  const int na = t.na, h = na / 2, nJ = t.nJ, nc = t.nc;
+  std::vector<int> cls((size_t)nc * nJ, 1);
+  int nemax = 0, nomax = 0;
+  for (int ch = 0; ch < nc; ch++) {
+    int ne = 0, no = 0;
+    for (int j = 0; j < nJ; j++) {
+      auto D = [&](int ia) { return t.Dmat[((size_t)ch * na + ia) * t.nJp + j]; };
+      double dmax = 0, dot = 0;
+      for (int ia = 0; ia < na; ia++) dmax = std::fmax(dmax, std::fabs(D(ia)));
+      for (int ia = 0; ia < h; ia++) dot += D(na - 1 - ia) * D(ia);
+      const int sj = dot >= 0 ? 1 : -1;
+      for (int ia = 0; ia < h; ia++)
+        if (std::fabs(D(na - 1 - ia) - sj * D(ia)) > 1e-12 * dmax) return hipSuccess;
+      cls[(size_t)ch * nJ + j] = sj;
+      (sj > 0 ? ne : no)++;
+    }
+    nemax = std::max(nemax, ne);
+    nomax = std::max(nomax, no);
+  }
+  const int nt = (std::max(h, std::max(nemax, nomax)) + 15) / 16;
+  if (nt > 4) return hipSuccess;
+  const int NR = 16 * nt, NC2 = 2 * NR;
+  std::vector<double> Df((size_t)nc * NR * NC2, 0.0), af((size_t)nc * NC2 * t.n1, 0.0),
+      bf((size_t)nc * NC2 * t.n2, 0.0);
+  for (int ch = 0; ch < nc; ch++) {
+    int slot[2] = {0, NR};
+    for (int j = 0; j < nJ; j++) {
+      const int col = slot[cls[(size_t)ch * nJ + j] > 0 ? 0 : 1]++;
+      for (int ia = 0; ia < h; ia++)
+        Df[((size_t)ch * NR + ia) * NC2 + col] = t.Dmat[((size_t)ch * na + ia) * t.nJp + j];
+      for (int r = 0; r < t.n1; r++)
+        af[((size_t)ch * NC2 + col) * t.n1 + r] = t.acoef[((size_t)ch * nJ + j) * t.n1 + r];
+      for (int R = 0; R < t.n2; R++)
+        bf[((size_t)ch * NC2 + col) * t.n2 + R] =
+            t.bcoef[(((size_t)ch * 3 + 1) * nJ + j) * t.n2 + R];
+    }
+  }
+  hipError_t e;
+  if (c->precision == MW3_MIXED) {
+    std::vector<float> d32(Df.begin(), Df.end());
+    if ((e = hipMalloc(&c->d_fold_D, d32.size() * sizeof(float))) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_fold_D, d32.data(), d32.size() * sizeof(float), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return e;
+  } else {
+    if ((e = hipMalloc(&c->d_fold_D, Df.size() * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_fold_D, Df.data(), Df.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+      return e;
+  }
+  if ((e = hipMalloc(&c->d_fold_a, af.size() * sizeof(double))) != hipSuccess) return e;
+  if ((e = hipMemcpy(c->d_fold_a, af.data(), af.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return e;
+  if ((e = hipMalloc(&c->d_fold_b, bf.size() * sizeof(double))) != hipSuccess) return e;
+  if ((e = hipMemcpy(c->d_fold_b, bf.data(), bf.size() * sizeof(double), hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return e;
+  *nt_out = nt;
+  return hipSuccess;
+}
+
 template <typename T, int NT>
 static hipError_t ang_fused_grid(int* blocks) {
   int nb = 0;
@@ -1825,6 +2125,17 @@ hipError_t build_plans(Ctx* c) {
     if (e != hipSuccess) return e;
     c->ang_nt = nt;
   }
+  // the parity-folded form of the one-pass kernel where the functions allow it
+  c->ang_fold = 0;
+  if (c->ang_fused && offs32) {
+    int ntf = 0;
+    if ((e = build_ang_fold(c, &ntf)) != hipSuccess) return e;
+    if (ntf > 0) {
+      e = c->precision == MW3_MIXED ? ang_fold_setup<float>(c, ntf) : ang_fold_setup<double>(c, ntf);
+      if (e != hipSuccess) return e;
+      c->ang_fold = ntf;
+    }
+  }
   return hipSuccess;
 }
 
@@ -1864,7 +2175,10 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   hipError_t e;
   // angular term
   timer_begin(c, K_ANG_FWD);
-  if (c->ang_nt) {
+  if (c->ang_fold) {
+    if ((e = launch_ang_fold<T>(c, psi)) != hipSuccess) return e;
+    timer_end(c, K_ANG_FWD);
+  } else if (c->ang_nt) {
     if ((e = launch_ang_fused<T>(c, psi)) != hipSuccess) return e;
     timer_end(c, K_ANG_FWD);
   } else if (c->ang_fused) {
@@ -1890,7 +2204,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     if ((e = hipGetLastError()) != hipSuccess) return e;
     timer_end(c, K_CORIOLIS);
   }
-  if (!c->ang_nt) {
+  if (!c->ang_nt && !c->ang_fold) {
     timer_begin(c, K_ANG_BWD);
     e = run_gemm<G_ANG_BWD, T, T, T, T>(s, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang,
                                         c->p_angB);
