@@ -195,6 +195,11 @@ struct Ctx {
   int32_t* d_rad_items = nullptr;  // R-pass work items {channel-angle, 16-row tile, nn1, k2lo}
   int rad_nitems = 0;              // 0: the R passes run as GEMMs
   int rad_grid[2] = {0, 0};        // persistent blocks of the forward / inverse R pass
+  void* d_lowT = nullptr;          // transposed low-mode r tables of k_rad_mid
+  int32_t* d_mid_items = nullptr;  // its work items (int4)
+  int32_t* d_mid_sizes = nullptr;  // its per-size table records (int4)
+  int mid_nitems = 0;              // 0: the r passes run as two GEMMs
+  int mid_nk1 = 0;                 // 16-row tiles of the low r modes
   int ang_grid = 0;        // its persistent blocks
   int ang_fold = 0;        // row tiles of the parity-folded one-pass kernel (0: not used)
   void* d_fold_D = nullptr;    // [nc][16 ang_fold][32 ang_fold] D(ia < na/2, j by parity class)

@@ -1858,6 +1858,230 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
   }
 }
 
+// ---------------------------------------------------------------------------
+// The two r passes of the radial correction in one kernel, one block of four
+// waves per work item = (channel-angle, 16 columns k2 of the low R modes):
+//   t2[k1][k2] = (radcut - p2r1(k1) - p2r2(k2))_+ divi sum_{r<nn1} S(r, k1) t1[r][k2]
+//   t1[r][k2]  = sum_{k1<k1lo} S(r, k1) t2[k1][k2]            (in place)
+// S(r, k1) from the low-mode table L[k1][r] of size nn1 (forward) and its
+// transpose LT[r][k1] (back; within each 16-column block of FP64 tables the
+// columns run k1 = 16 m + 4 rr + g at position 16 m + 4 g + rr, so that a
+// lane's four k1 of one step group are one read).  t2 stays in the MFMA
+// accumulators: their layout is the back product's B operand (as in
+// k_ang_fused).  Items: int4 {channel * na + angle, column block | k2lo << 16,
+// nn1 | k1lo << 16, size index}; sizes: int4 {L offset, L row stride,
+// LT offset, 0}.
+// ---------------------------------------------------------------------------
+template <typename T, int NK1>
+__global__ __launch_bounds__(256) void k_rad_mid(T* __restrict__ t1, const T* __restrict__ tab,
+                                                 const T* __restrict__ tabT,
+                                                 const int4* __restrict__ items, int nitems,
+                                                 const int4* __restrict__ sizes,
+                                                 const double* __restrict__ p2r1,
+                                                 const double* __restrict__ p2r2,
+                                                 const double* __restrict__ divi, double radcut,
+                                                 int na, int n1, int n2) {
+  typedef MfmaAcc<T> M;
+  typedef typename M::V V;
+  constexpr int KTP = 16 * NK1;  // row stride of LT
+  __shared__ V red[4][NK1][64];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  const int4 it = items[blockIdx.x];
+  const int cg = it.x, cc = it.y & 0xffff, k2l = it.y >> 16, nn = it.z & 0xffff,
+            k1l = it.z >> 16;
+  const int4 sz = sizes[it.w];
+  const int ia = cg % na;
+  T* tb = t1 + (int64_t)cg * n1 * n2;
+  const T* L = tab + sz.x;
+  const int ld = sz.y;
+  const T* LT = tabT + sz.z;
+  const int col = 16 * cc + i;
+  const int colc = col < n2 ? col : n2 - 1;
+  const int nrb = (nn + 15) >> 4;
+  // forward over r, r blocks rb = w, w + 4, ..: acc[m] rows k1 = 16 m + (A
+  // row), columns k2 = col; the four waves' partial tiles are summed in LDS
+  V acc[NK1];
+#pragma unroll
+  for (int m = 0; m < NK1; m++) acc[m] = (V){0, 0, 0, 0};
+  T a[2][NK1][4], b[2][4];
+  auto load = [&](int rb, int q) {
+    const int r0 = 16 * rb + 4 * g;
+    const int c0 = r0 + 4 <= ld ? r0 : ld - 4;  // groups past the row are masked below
+#pragma unroll
+    for (int m = 0; m < NK1; m++) {
+      const int k1 = 16 * m + i;
+      ld4(L + (int64_t)(k1 < k1l ? k1 : 0) * ld + c0, a[q][m]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int r = r0 + s;
+      b[q][s] = tb[(int64_t)(r < nn ? r : nn - 1) * n2 + colc];
+    }
+  };
+  auto compute = [&](int rb, int q) {
+    const int r0 = 16 * rb + 4 * g;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const bool rin = r0 + s < nn;
+      const T bv = rin ? b[q][s] : (T)0;
+#pragma unroll
+      for (int m = 0; m < NK1; m++) {
+        const T av = (rin && 16 * m + i < k1l) ? a[q][m][s] : (T)0;
+        acc[m] = M::mma(av, bv, acc[m]);
+      }
+    }
+  };
+  // this wave's blocks w, w + 4, ..; a missing second block repeats the
+  // first with its rows masked (rin false past nn)
+  const int nmine = nrb > w ? (nrb - w + 3) / 4 : 0;
+  if (nmine > 0) {
+    load(w, 0);
+    for (int m2 = 0; m2 < nmine; m2 += 2) {
+      const int rb0 = w + 4 * m2, rb1 = rb0 + 4;
+      load(m2 + 1 < nmine ? rb1 : rb0, 1);
+      compute(rb0, 0);
+      load(m2 + 2 < nmine ? rb1 + 4 : rb0, 0);
+      if (m2 + 1 < nmine) compute(rb1, 1);
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < NK1; m++) red[w][m][lane] = acc[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NK1; m++) {
+    acc[m] = red[0][m][lane];
+#pragma unroll
+    for (int q = 1; q < 4; q++) acc[m] += red[q][m][lane];
+  }
+  // (radcut - E)_+ divi, rounded to the operator precision like the stored t2
+  {
+    const double e2 = p2r2[colc], dv = divi[ia];
+#pragma unroll
+    for (int m = 0; m < NK1; m++)
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        int k1 = 16 * m + M::row(lane, rr);
+        k1 = k1 < n1 ? k1 : n1 - 1;
+        double wk = radcut - (p2r1[(int64_t)ia * n1 + k1] + e2);
+        wk = wk > 0.0 ? wk * dv : 0.0;
+        acc[m][rr] = (T)((double)acc[m][rr] * wk);
+      }
+  }
+  // back over r: output tiles mt = w, w + 4, ..; B operand = acc
+  for (int mt = w; mt < nrb; mt += 4) {
+    T at[NK1][4];
+#pragma unroll
+    for (int m = 0; m < NK1; m++) ld4(LT + (int64_t)(16 * mt + i) * KTP + 16 * m + 4 * g, at[m]);
+    V o = (V){0, 0, 0, 0};
+#pragma unroll
+    for (int m = 0; m < NK1; m++)
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) o = M::mma(at[m][rr], acc[m][rr], o);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const int r = 16 * mt + M::row(lane, rr);
+      if (r < nn && col < k2l) tb[(int64_t)r * n2 + col] = o[rr];
+    }
+  }
+}
+
+template <typename T>
+static hipError_t launch_rad_mid(Ctx* c) {
+  const HostTables& t = c->t;
+  const unsigned nblk = (unsigned)c->mid_nitems;  // one block of 4 waves per item
+  const T* tab = (const T*)c->d_dst_tables;
+  const T* tabT = (const T*)c->d_lowT;
+  const int4* items = (const int4*)c->d_mid_items;
+  const int4* sizes = (const int4*)c->d_mid_sizes;
+  switch (c->mid_nk1) {
+#define MW3_MID(NK)                                                                               \
+  case NK:                                                                                        \
+    hipLaunchKernelGGL((k_rad_mid<T, NK>), dim3(nblk), dim3(256), 0, c->stream, (T*)c->d_t1, tab, \
+                       tabT, items, c->mid_nitems, sizes, c->d_p2r1, c->d_p2r2, c->d_divi,        \
+                       t.radcut, t.na, t.n1, t.n2);                                              \
+    break;
+    MW3_MID(1)
+    MW3_MID(2)
+    MW3_MID(3)
+    MW3_MID(4)
+#undef MW3_MID
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// items, size records and transposed low tables of k_rad_mid (r sizes only)
+static hipError_t build_rad_mid(Ctx* c) {
+  const HostTables& t = c->t;
+  c->mid_nitems = 0;
+  int k1max = 0;
+  for (int ia = 0; ia < t.na; ia++) k1max = std::max(k1max, t.k1lo[ia]);
+  const int nk1 = (k1max + 15) / 16;
+  if (nk1 < 1 || nk1 > 4 || t.n1 >= 65536 || t.n2 >= 65536 * 16) return hipSuccess;
+  const int ktp = 16 * nk1;
+  const bool f64 = c->precision != MW3_MIXED;
+  std::vector<double> lt;
+  std::vector<int32_t> sizes;
+  std::vector<int> qrec(t.dst_sizes.size(), -1);
+  for (int ia = 0; ia < t.na; ia++) {
+    int q = 0;
+    while (t.dst_sizes[q] != t.nn1[ia]) q++;
+    if (qrec[q] >= 0) continue;
+    const int n = t.nn1[ia], ld = t.low_ld[q], rows = (n + 15) / 16 * 16;
+    int kcap = 0;
+    for (int ja = 0; ja < t.na; ja++)
+      if (t.nn1[ja] == n) kcap = std::max(kcap, t.k1lo[ja]);
+    qrec[q] = (int)sizes.size() / 4;
+    // a size with no low r modes has no table rows: its (fully masked) reads
+    // go to the start of the tables instead
+    sizes.push_back(kcap > 0 ? (int32_t)t.low_off[q] : 0);
+    sizes.push_back(ld);
+    sizes.push_back((int32_t)lt.size());
+    sizes.push_back(0);
+    for (int r = 0; r < rows; r++)
+      for (int p = 0; p < ktp; p++) {
+        // position p holds k1 = p (FP32) or the 4 x 4 transpose within its 16-block (FP64)
+        const int k1 = f64 ? (p & ~15) + 4 * (p & 3) + ((p >> 2) & 3) : p;
+        double v = 0.0;
+        if (r < n && k1 < kcap) v = t.dst_tables[(size_t)t.low_off[q] + (size_t)k1 * ld + r];
+        lt.push_back(v);
+      }
+  }
+  if (lt.size() >= (size_t)1 << 31 || t.dst_tables.size() >= (size_t)1 << 31) return hipSuccess;
+  std::vector<int32_t> items;
+  for (int ch = 0; ch < t.nc; ch++)
+    for (int ia = 0; ia < t.na; ia++) {
+      int q = 0;
+      while (t.dst_sizes[q] != t.nn1[ia]) q++;
+      for (int cc = 0; 16 * cc < t.k2lo[ia]; cc++) {
+        items.push_back(ch * t.na + ia);
+        items.push_back(cc | (t.k2lo[ia] << 16));
+        items.push_back(t.nn1[ia] | (t.k1lo[ia] << 16));
+        items.push_back(qrec[q]);
+      }
+    }
+  if (items.empty()) return hipSuccess;
+  hipError_t e;
+  auto up = [&](void** d, const void* h, size_t bytes) {
+    hipError_t r = hipMalloc(d, bytes);
+    if (r != hipSuccess) return r;
+    return hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice);
+  };
+  if (f64) {
+    e = up(&c->d_lowT, lt.data(), lt.size() * sizeof(double));
+  } else {
+    std::vector<float> l32(lt.begin(), lt.end());
+    e = up(&c->d_lowT, l32.data(), l32.size() * sizeof(float));
+  }
+  if (e != hipSuccess) return e;
+  if ((e = up((void**)&c->d_mid_items, items.data(), items.size() * 4)) != hipSuccess) return e;
+  if ((e = up((void**)&c->d_mid_sizes, sizes.data(), sizes.size() * 4)) != hipSuccess) return e;
+  c->mid_nitems = (int)(items.size() / 4);
+  c->mid_nk1 = nk1;
+  return hipSuccess;
+}
+
 template <typename K>
 static hipError_t persistent_grid(K kernel, int threads, int* blocks) {
   int nb = 0, dev = 0, ncu = 0;
@@ -2092,6 +2316,8 @@ hipError_t build_plans(Ctx* c) {
     if ((e = upload_plan(&c->p_angB, d, true)) != hipSuccess) return e;
   }
   plan_tiles(c);
+  // the two r passes in one kernel where the low r modes fit its tiles
+  if ((e = build_rad_mid(c)) != hipSuccess) return e;
   // R passes one wave per 16-row tile when their tables were built
   c->rad_nitems = 0;
   if (t.rad_nkb > 0 && (t.n2 == 256 || t.n2 == 320 || t.n2 == 384 || t.n2 == 512)) {
@@ -2221,10 +2447,14 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_R);
   timer_begin(c, K_DST_r);
-  e = run_gemm<G_DST_r_KIN, T, T, T, T>(s, tab, t1, t2, c->p_dstr, kp);
-  if (e != hipSuccess) return e;
-  e = run_gemm<G_DST_r_INV, T, T, T, T>(s, tab, t2, t1, c->p_dstr2);
-  if (e != hipSuccess) return e;
+  if (c->mid_nitems) {
+    if ((e = launch_rad_mid<T>(c)) != hipSuccess) return e;
+  } else {
+    e = run_gemm<G_DST_r_KIN, T, T, T, T>(s, tab, t1, t2, c->p_dstr, kp);
+    if (e != hipSuccess) return e;
+    e = run_gemm<G_DST_r_INV, T, T, T, T>(s, tab, t2, t1, c->p_dstr2);
+    if (e != hipSuccess) return e;
+  }
   timer_end(c, K_DST_r);
   timer_begin(c, K_DST_RINV);
   e = c->rad_nitems ? run_rad<T>(c, psi, t1, t2, 1)
