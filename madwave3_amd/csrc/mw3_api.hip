@@ -44,6 +44,7 @@ static void timer_collect(Ctx* c) {
   if (c->pending.empty()) return;
   (void)hipStreamSynchronize(c->stream);
   if (c->aux) (void)hipStreamSynchronize(c->aux);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (auto& p : c->pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, p.second.a, p.second.b) == hipSuccess) {
@@ -131,8 +132,14 @@ static void free_all(Ctx* c) {
     (void)hipStreamSynchronize(c->aux);
     (void)hipStreamDestroy(c->aux);
   }
+  if (c->side) {
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamDestroy(c->side);
+  }
   if (c->ev_slice) (void)hipEventDestroy(c->ev_slice);
   if (c->ev_cvj) (void)hipEventDestroy(c->ev_cvj);
+  if (c->ev_psi) (void)hipEventDestroy(c->ev_psi);
+  if (c->ev_ang) (void)hipEventDestroy(c->ev_ang);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -193,6 +200,9 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "mw3_create: stream");
   CK(hipEventCreateWithFlags(&c->ev_slice, hipEventDisableTiming), "mw3_create: event");
   CK(hipEventCreateWithFlags(&c->ev_cvj, hipEventDisableTiming), "mw3_create: event");
+  CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "mw3_create: stream");
+  CK(hipEventCreateWithFlags(&c->ev_psi, hipEventDisableTiming), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_ang, hipEventDisableTiming), "mw3_create: event");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
 

@@ -120,6 +120,10 @@ struct Ctx {
   // wrote the slice, ev_cvj orders the next update and the ring flush after them
   hipStream_t aux = nullptr;
   hipEvent_t ev_slice = nullptr, ev_cvj = nullptr;
+  // third stream: the angular term runs beside the radial passes; ev_psi
+  // orders it after the update that wrote psi, ev_ang the next update after it
+  hipStream_t side = nullptr;
+  hipEvent_t ev_psi = nullptr, ev_ang = nullptr;
   bool cvj_pending = false;
   int device = 0;
 
