@@ -42,15 +42,17 @@ def build_case(cfg):
 
 
 # kernel launches per step in each group (mixed and FP64 alike)
-GROUP_LAUNCHES = {"radial_dst": 4, "angular": 2, "update_fused": 2, "projections": 1,
+GROUP_LAUNCHES = {"radial_dst": 3, "angular": 1, "update_fused": 2, "projections": 1,
                   "energy_sums": 2, "coriolis": 1, "products": 1}
 RING = 256  # steps per energy-sum flush (kRing of the library)
 
 
 def kernel_groups(times):
     """Per-step milliseconds of each kernel group from mw3_timing_get totals:
-    radial_dst (the four radial GEMM passes), angular (forward GEMM with the
-    FBR coefficients, back GEMM), update_fused (update + norm reduction),
+    radial_dst (the radial correction passes: k_rad_fwd, k_rad_mid, k_rad_inv
+    at configuration 2), angular (the one-pass angular kernel; in mixed
+    precision on a third stream beside the radial passes), update_fused
+    (update + norm reduction),
     projections (reactant-state sums; second stream, overlapped with the next
     step), products (the per-step product-state reduction, same stream as
     projections), energy_sums (once per 256-step ring, divided by 256, with the

@@ -272,7 +272,7 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
    times and counts the timed launches.  ids: 0 .. MW3_NKERNELS-1.  No
    counterpart in the reference (its timing is wall-clock per loop,
    SRC/main_madwave3.f:370-372). */
-#define MW3_NKERNELS 10
+#define MW3_NKERNELS 11
 MW3_API int32_t mw3_timing_enable(mw3_ctx *ctx, int32_t on);
 MW3_API int32_t mw3_timing_get(mw3_ctx *ctx, int32_t id, double *total_ms,
                        int64_t *launches);

@@ -156,7 +156,7 @@ const char* mw3_kernel_name(int32_t id) {
   static const char* names[MW3_NKERNELS] = {"angular_fwd", "angular_bwd", "dst_R_fwd",
                                             "dst_r_scale", "dst_R_inv",   "update_fused",
                                             "projections", "energy_accum", "coriolis",
-                                            "prod_projection"};
+                                            "prod_projection", "update_tail"};
   if (id < 0 || id >= MW3_NKERNELS) return "";
   return names[id];
 }

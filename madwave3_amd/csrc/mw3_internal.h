@@ -310,8 +310,10 @@ enum KernelId {
   K_ENERGY = 7,
   K_CORIOLIS = 8,  // FBR coefficients with the Omega +- 1 Coriolis stencil (k_fbr_coef)
   K_PROD = 9,      // product projection: per-step reduction (the flush GEMM is in K_ENERGY)
+  K_UPDATE_TAIL = 10,  // flux columns and norm from the update partials (second stream)
 };
 
+hipError_t launch_update_tail(Ctx* c, int mode, int slot, bool proj, hipStream_t st);
 void timer_begin(Ctx* c, int id, hipStream_t s = nullptr);
 void timer_end(Ctx* c, int id, hipStream_t s = nullptr);
 

@@ -2556,19 +2556,28 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (mode == 2 && a.pflux) {  // flux columns from the work-item partials
-    const int ng = t.nc * t.na * ((t.n2 + 63) / 64);
-    hipLaunchKernelGGL(k_update_flux, dim3(ng), dim3(64), 0, c->stream, c->d_uw_part,
-                       c->d_uw_gfirst, c->d_fmap, a.zr, a.zdr, t.n2);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  if (mode != 0) {
-    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, c->stream, c->d_norm_part, nb,
-                       c->d_ring_norm + 2 * slot);
-    e = hipGetLastError();
-  }
+  // the step loop (mode 2) reduces the partials on the second stream, beside
+  // the next step's H psi (launch_projections)
+  if (mode == 1) e = launch_update_tail(c, mode, slot, proj, c->stream);
   timer_end(c, K_UPDATE);
   return e;
+}
+
+// flux columns and norm of the update's work-item partials, into ring slot
+hipError_t launch_update_tail(Ctx* c, int mode, int slot, bool proj, hipStream_t st) {
+  const HostTables& t = c->t;
+  if (mode == 2 && proj && t.nflux > 0 && c->d_fmap) {
+    const int ng = t.nc * t.na * ((t.n2 + 63) / 64);
+    hipLaunchKernelGGL(k_update_flux, dim3(ng), dim3(64), 0, st, c->d_uw_part, c->d_uw_gfirst,
+                       c->d_fmap, c->d_ring_zr + (int64_t)slot * t.nflux,
+                       c->d_ring_zdr + (int64_t)2 * slot * t.nflux, t.n2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (mode != 0)
+    hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, st, c->d_norm_part,
+                       c->nnorm_blocks, c->d_ring_norm + 2 * slot);
+  return hipGetLastError();
 }
 
 hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj) {
@@ -2581,11 +2590,15 @@ hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj) {
 hipError_t launch_projections(Ctx* c, int slot) {
   const HostTables& t = c->t;
   hipError_t e = hipSuccess;
-  if (t.ncvj > 0 || c->prod_on) {
+  {
     // on the second stream, after the update that wrote the slice and the
-    // product partials
+    // partials: the flux columns and the norm of the step, then the
+    // reactant- and product-state sums
     if ((e = hipEventRecord(c->ev_slice, c->stream)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(c->aux, c->ev_slice, 0)) != hipSuccess) return e;
+    timer_begin(c, K_UPDATE_TAIL, c->aux);
+    if ((e = launch_update_tail(c, 2, slot, true, c->aux)) != hipSuccess) return e;
+    timer_end(c, K_UPDATE_TAIL, c->aux);
     if (t.ncvj > 0) {
       timer_begin(c, K_PROJ, c->aux);
       int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;

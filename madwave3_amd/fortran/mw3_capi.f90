@@ -21,7 +21,7 @@ module mw3_capi
   integer(c_int32_t), parameter :: MW3_OK = 0, MW3_EINVAL = -1, MW3_ENOMEM = -2, &
                                    MW3_EDEVICE = -3, MW3_ESTATE = -4, MW3_EUNSUP = -5
   integer(c_int32_t), parameter :: MW3_FP64 = 0, MW3_MIXED = 1
-  integer(c_int32_t), parameter :: MW3_NKERNELS = 8
+  integer(c_int32_t), parameter :: MW3_NKERNELS = 11
 
   ! struct mw3_setup: same members in the same order (C layout by bind(C))
   type, bind(C) :: mw3_setup

@@ -16,7 +16,7 @@ __all__ = ["Mw3Setup", "Mw3ProdSetup", "make_setup", "make_prod_setup", "load_li
            "MW3_FP64", "MW3_MIXED", "LIB_PATH"]
 
 MW3_ABI_VERSION = 1
-MW3_NKERNELS = 10  # include/mw3.h
+MW3_NKERNELS = 11  # include/mw3.h
 MW3_FP64 = 0
 MW3_MIXED = 1
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmw3.so")
