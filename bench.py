@@ -42,8 +42,8 @@ def build_case(cfg):
 
 
 # kernel launches per step in each group (mixed and FP64 alike)
-GROUP_LAUNCHES = {"radial_dst": 3, "angular": 1, "update_fused": 2, "projections": 1,
-                  "energy_sums": 2, "coriolis": 1, "products": 1}
+GROUP_LAUNCHES = {"radial_dst": 3, "angular": 1, "update_fused": 1, "update_tail": 2,
+                  "projections": 1, "energy_sums": 2, "coriolis": 1, "products": 1}
 RING = 256  # steps per energy-sum flush (kRing of the library)
 
 
@@ -52,7 +52,8 @@ def kernel_groups(times):
     radial_dst (the radial correction passes: k_rad_fwd, k_rad_mid, k_rad_inv
     at configuration 2), angular (the one-pass angular kernel; in mixed
     precision on a third stream beside the radial passes), update_fused
-    (update + norm reduction),
+    (the update kernel), update_tail (flux columns and norm of the step,
+    second stream),
     projections (reactant-state sums; second stream, overlapped with the next
     step), products (the per-step product-state reduction, same stream as
     projections), energy_sums (once per 256-step ring, divided by 256, with the
@@ -67,6 +68,8 @@ def kernel_groups(times):
             g = "angular"
         elif name == "update_fused":
             g = "update_fused"
+        elif name == "update_tail":
+            g = "update_tail"
         elif name == "energy_accum":
             g = "energy_sums"
         elif name == "coriolis":
@@ -292,6 +295,9 @@ def main():
             # (e) energy projection: 8 netot n_flux flops per step (time only)
             "energy_sums": ("hbm", 16.0 * nflux, "GB/s"),
             "projections": ("hbm", 8.0 * case.nangu * case.npun1, "GB/s"),
+            # flux columns and norm of the step (second stream): the flux
+            # line values written to the ring (time only, like energy_sums)
+            "update_tail": ("hbm", 16.0 * nflux, "GB/s"),
             # (d) Coriolis stencil in the FBR: 2 s N bytes over the Omega-coupled blocks
             "coriolis": ("hbm", 2.0 * s_op * N, "GB/s"),
             # product reduction: 64 partials per update item, dtrans, ring row
@@ -304,7 +310,8 @@ def main():
             peak = HBM_PEAK_GBS if unit == "GB/s" else FP32_PEAK_TF
             kern[k] = {"ms_per_step": ms, "achieved": ach, "unit": unit, "peak": peak,
                        "frac": ach / peak}
-        main = [k for k in per if k not in ("projections", "energy_sums", "products")]
+        main = [k for k in per if k not in ("projections", "energy_sums", "products",
+                                            "update_tail")]
         dom = max(main, key=lambda k: per[k])
         d = kern[dom]
         roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],
