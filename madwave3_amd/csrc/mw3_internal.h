@@ -206,6 +206,7 @@ struct Ctx {
   int mid_nk1 = 0;                 // 16-row tiles of the low r modes
   int ang_grid = 0;        // its persistent blocks
   int ang_fold = 0;        // row tiles of the parity-folded one-pass kernel (0: not used)
+  bool fbr_all = false;    // Coriolis coefficients by k_fbr_coef_all (2..8 channels)
   void* d_fold_D = nullptr;    // [nc][16 ang_fold][32 ang_fold] D(ia < na/2, j by parity class)
   double* d_fold_a = nullptr;  // [nc][32 ang_fold][n1] a_j(r1) in the same j order
   double* d_fold_b = nullptr;  // [nc][32 ang_fold][n2] b_j(R)

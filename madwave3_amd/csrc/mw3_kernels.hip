@@ -574,6 +574,54 @@ __global__ void k_fbr_coef(const T* __restrict__ fbr, T* __restrict__ phi,
   }
 }
 
+// The same coefficients with every channel of one (j, 16-row chunk of r) in
+// one block: each thread takes an R column, loads the b_j(R) triples of all
+// NC channels once for the chunk, and per row reads the NC FBR values once
+// (each FBR value is read once instead of three times, and the coefficient
+// rows once per 16 rows instead of once per row).
+constexpr int FBR_RB = 16;
+template <typename T, int NC>
+__global__ __launch_bounds__(256) void k_fbr_coef_all(const T* __restrict__ fbr,
+                                                      T* __restrict__ phi,
+                                                      const double* __restrict__ acoef,
+                                                      const double* __restrict__ bcoef,
+                                                      const int32_t* __restrict__ nbr, int nJ,
+                                                      int n1, int n2) {
+  const int nrc = (n1 + FBR_RB - 1) / FBR_RB;
+  const int jj = blockIdx.x / nrc, rc = blockIdx.x - jj * nrc;
+  const int64_t plane = (int64_t)n1 * n2, cst = (int64_t)nJ * plane;
+  const int r0 = rc * FBR_RB, r1 = r0 + FBR_RB < n1 ? r0 + FBR_RB : n1;
+  for (int R = threadIdx.x; R < n2; R += blockDim.x) {
+    double b0[NC], bm[NC], bp[NC];
+#pragma unroll
+    for (int t = 0; t < NC; t++) {
+      b0[t] = bcoef[(((int64_t)t * 3 + 1) * nJ + jj) * n2 + R];
+      bm[t] = bcoef[(((int64_t)t * 3 + 0) * nJ + jj) * n2 + R];
+      bp[t] = bcoef[(((int64_t)t * 3 + 2) * nJ + jj) * n2 + R];
+    }
+    const T* fb = fbr + (int64_t)jj * plane + R;
+    T* ob = phi + (int64_t)jj * plane + R;
+    for (int r = r0; r < r1; r++) {
+      double f[NC];
+#pragma unroll
+      for (int t = 0; t < NC; t++) f[t] = (double)fb[t * cst + (int64_t)r * n2];
+#pragma unroll
+      for (int t = 0; t < NC; t++) {
+        double v = (acoef[((int64_t)t * nJ + jj) * n1 + r] + b0[t]) * f[t];
+        const int sm = nbr[t * 3 + 0], sp = nbr[t * 3 + 2];  // uniform
+        double fm = 0.0, fp = 0.0;
+#pragma unroll
+        for (int s2 = 0; s2 < NC; s2++) {
+          fm = s2 == sm ? f[s2] : fm;
+          fp = s2 == sp ? f[s2] : fp;
+        }
+        v += bm[t] * fm + bp[t] * fp;  // bm / bp are zero where the neighbour is absent
+        ob[t * cst + (int64_t)r * n2] = (T)v;
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused V psi + compressed-grid projection + damped update
 // (SRC/mod_Hphi_01y2.f:838-853, SRC/main_madwave3.f:245-248, 332-339) with the
@@ -2351,6 +2399,20 @@ hipError_t build_plans(Ctx* c) {
     if (e != hipSuccess) return e;
     c->ang_nt = nt;
   }
+  // all-channel Coriolis coefficient kernel: up to 8 channels, and the
+  // q = 0 / 2 coefficient rows zero wherever a channel has no such neighbour
+  // (the kernel adds them against a zero stand-in)
+  c->fbr_all = t.nc >= 2 && t.nc <= 8;
+  for (int ch = 0; ch < t.nc && c->fbr_all; ch++)
+    for (int q = 0; q < 3; q += 2) {
+      if (t.nbr[ch * 3 + q] >= 0) continue;
+      const double* row = &t.bcoef[(((size_t)ch * 3 + q) * t.nJ) * t.n2];
+      for (size_t k = 0; k < (size_t)t.nJ * t.n2; k++)
+        if (row[k] != 0.0) {
+          c->fbr_all = false;
+          break;
+        }
+    }
   // the parity-folded form of the one-pass kernel where the functions allow it
   c->ang_fold = 0;
   if (c->ang_fused && offs32) {
@@ -2434,9 +2496,28 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     if (e != hipSuccess) return e;
     timer_end(c, K_ANG_FWD, sa);
     timer_begin(c, K_CORIOLIS, sa);
-    hipLaunchKernelGGL(k_fbr_coef<T>, dim3((unsigned)((int64_t)t.nc * t.nJ * t.n1)), dim3(256),
-                       0, sa, (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,
-                       (const double*)c->d_bcoef, c->d_nbr, t.nc, t.nJ, t.n1, t.n2);
+    const unsigned nall = (unsigned)((int64_t)t.nJ * ((t.n1 + FBR_RB - 1) / FBR_RB));
+    switch (c->fbr_all ? t.nc : 0) {
+#define MW3_FBR(NC)                                                                              \
+  case NC:                                                                                       \
+    hipLaunchKernelGGL((k_fbr_coef_all<T, NC>), dim3(nall), dim3(256), 0, sa,                   \
+                       (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,             \
+                       (const double*)c->d_bcoef, c->d_nbr, t.nJ, t.n1, t.n2);                  \
+    break;
+      MW3_FBR(2)
+      MW3_FBR(3)
+      MW3_FBR(4)
+      MW3_FBR(5)
+      MW3_FBR(6)
+      MW3_FBR(7)
+      MW3_FBR(8)
+#undef MW3_FBR
+      default:
+        hipLaunchKernelGGL(k_fbr_coef<T>, dim3((unsigned)((int64_t)t.nc * t.nJ * t.n1)),
+                           dim3(256), 0, sa, (const T*)c->d_fbr, (T*)c->d_phi,
+                           (const double*)c->d_acoef, (const double*)c->d_bcoef, c->d_nbr, t.nc,
+                           t.nJ, t.n1, t.n2);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     timer_end(c, K_CORIOLIS, sa);
   }
