@@ -205,6 +205,8 @@ struct Ctx {
   int mid_nitems = 0;              // 0: the r passes run as two GEMMs
   int mid_nk1 = 0;                 // 16-row tiles of the low r modes
   int ang_grid = 0;        // its persistent blocks
+  int32_t* d_ang_tiles = nullptr;  // tiles of the one-pass kernels with compressed-grid points
+  int ang_ntiles = 0;
   int ang_fold = 0;        // row tiles of the parity-folded one-pass kernel (0: not used)
   bool fbr_all = false;    // Coriolis coefficients by k_fbr_coef_all (2..8 channels)
   void* d_fold_D = nullptr;    // [nc][16 ang_fold][32 ang_fold] D(ia < na/2, j by parity class)

@@ -1170,8 +1170,8 @@ struct AngPitch<double> {
 template <typename T, int NT>
 __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
     const T* __restrict__ psi, const T* __restrict__ Dmat, const double* __restrict__ acoef,
-    const double* __restrict__ bcoef, T* __restrict__ ang, int nc, int na, int nJ, int nJp,
-    int n1, int n2) {
+    const double* __restrict__ bcoef, T* __restrict__ ang, const int32_t* __restrict__ tiles,
+    int ntiles, int na, int nJ, int nJp, int n1, int n2) {
   typedef MfmaAcc<T> M;
   typedef typename M::V V;
   constexpr int NR = 16 * NT;                 // padded rows of D (ia) and of F (j)
@@ -1182,7 +1182,6 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, i = lane & 15;
   const int tpc = (pl + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES);  // tiles per channel
-  const int ntile = tpc * nc;
   // loads take a scalar channel base and a 32-bit per-lane byte offset
   // x[4 c + s] = psi[ch][16 c + 4 g + s][p]: the B operand of forward step
   // 4 c + s; rows past na read row na - 1 (finite), which meets zero rows of D
@@ -1200,12 +1199,17 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
       x[4 * c + s] = *(const T*)(pb + (uint32_t)(ia * pl + pp) * (uint32_t)sizeof(T));
     }
   };
-  int tile = blockIdx.x;
-  if (tile >= ntile) return;
+  // the block walks the listed tiles (channel * tiles per channel + tile);
+  // tiles with no point of the compressed grid are not listed: psi is zero
+  // there and the update reads no angular value there
+  int kt = blockIdx.x;
+  if (kt >= ntiles) return;
+  int tile = tiles[kt];
 #pragma unroll
   for (int c = 0; c < XA && c < NT; c++) ldx(c, tile);
   int cur = -1;
-  for (; tile < ntile; tile += gridDim.x) {
+  for (; kt < ntiles; kt += gridDim.x) {
+    tile = tiles[kt];
     const int ch = tile / tpc;
     if (ch != cur) {  // block-uniform
       __syncthreads();
@@ -1242,7 +1246,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
     }
     // the next tile of this block (the last one repeats itself), whose first
     // psi chunks load during this tile's back product
-    const int tnext = tile + (int)gridDim.x < ntile ? tile + (int)gridDim.x : tile;
+    const int tnext = kt + (int)gridDim.x < ntiles ? tiles[kt + (int)gridDim.x] : tile;
     const int p = (tile - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + i;
     const int pc = p < pl ? p : pl - 1;
     const int r1 = pc / n2, R = pc - r1 * n2;
@@ -1371,7 +1375,8 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
 template <typename T, int NT>
 __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
     const T* __restrict__ psi, const T* __restrict__ Dfold, const double* __restrict__ afold,
-    const double* __restrict__ bfold, T* __restrict__ ang, int nc, int na, int n1, int n2) {
+    const double* __restrict__ bfold, T* __restrict__ ang, const int32_t* __restrict__ tiles,
+    int ntiles, int na, int n1, int n2) {
   typedef MfmaAcc<T> M;
   typedef typename M::V V;
   constexpr int NR = 16 * NT, NC2 = 2 * NR;    // rows (ia < h) and columns (both classes)
@@ -1382,7 +1387,6 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, i = lane & 15;
   const int tpc = (pl + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES);  // tiles per channel
-  const int ntile = tpc * nc;
   // xl[4 c + s] = psi[16 c + 4 g + s][p], xh[..] = psi[na-1-(16 c + 4 g + s)][p];
   // angles past h read angle h-1 and its mirror (finite), against zero rows of D
   T xl[4 * NT], xh[4 * NT];
@@ -1400,12 +1404,17 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
       xh[4 * c + s] = *(const T*)(pb + (uint32_t)((na - 1 - ia) * pl + pp) * (uint32_t)sizeof(T));
     }
   };
-  int tile = blockIdx.x;
-  if (tile >= ntile) return;
+  // the block walks the listed tiles (channel * tiles per channel + tile);
+  // tiles with no point of the compressed grid are not listed: psi is zero
+  // there and the update reads no angular value there
+  int kt = blockIdx.x;
+  if (kt >= ntiles) return;
+  int tile = tiles[kt];
 #pragma unroll
   for (int c = 0; c < XA && c < NT; c++) ldx(c, tile);
   int cur = -1;
-  for (; tile < ntile; tile += gridDim.x) {
+  for (; kt < ntiles; kt += gridDim.x) {
+    tile = tiles[kt];
     const int ch = tile / tpc;
     if (ch != cur) {  // block-uniform
       __syncthreads();
@@ -1432,7 +1441,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
       __syncthreads();
       cur = ch;
     }
-    const int tnext = tile + (int)gridDim.x < ntile ? tile + (int)gridDim.x : tile;
+    const int tnext = kt + (int)gridDim.x < ntiles ? tiles[kt + (int)gridDim.x] : tile;
     const int p = (tile - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + i;
     const int pc = p < pl ? p : pl - 1;
     const int r1 = pc / n2, R = pc - r1 * n2;
@@ -1554,11 +1563,12 @@ static hipError_t ang_fold_grid(int* blocks) {
 template <typename T, int NT>
 static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   const HostTables& t = c->t;
-  const int64_t tiles = (t.nplane + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES) * t.nc;
-  const unsigned nblk = (unsigned)(tiles < c->ang_grid ? tiles : c->ang_grid);
+  const int nt = c->ang_ntiles;
+  const unsigned nblk = (unsigned)(nt < c->ang_grid ? nt : c->ang_grid);
+  if (nblk == 0) return;
   hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3(nblk), dim3(64 * ANG_WAVES), 0, st, psi,
-                     (const T*)c->d_fold_D, c->d_fold_a, c->d_fold_b, (T*)c->d_ang, t.nc, t.na,
-                     t.n1, t.n2);
+                     (const T*)c->d_fold_D, c->d_fold_a, c->d_fold_b, (T*)c->d_ang,
+                     c->d_ang_tiles, nt, t.na, t.n1, t.n2);
 }
 
 template <typename T>
@@ -1669,11 +1679,12 @@ static hipError_t ang_fused_grid(int* blocks) {
 template <typename T, int NT>
 static void launch_ang_fused_nt(Ctx* c, const T* psi, hipStream_t st) {
   const HostTables& t = c->t;
-  const int64_t tiles = (t.nplane + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES) * t.nc;
-  const unsigned nblk = (unsigned)(tiles < c->ang_grid ? tiles : c->ang_grid);
+  const int nt = c->ang_ntiles;
+  const unsigned nblk = (unsigned)(nt < c->ang_grid ? nt : c->ang_grid);
+  if (nblk == 0) return;
   hipLaunchKernelGGL((k_ang_fused<T, NT>), dim3(nblk), dim3(64 * ANG_WAVES), 0, st, psi,
                      (const T*)c->d_Dmat, (const double*)c->d_acoef, (const double*)c->d_bcoef,
-                     (T*)c->d_ang, t.nc, t.na, t.nJ, t.nJp, t.n1, t.n2);
+                     (T*)c->d_ang, c->d_ang_tiles, nt, t.na, t.nJ, t.nJp, t.n1, t.n2);
 }
 
 template <typename T>
@@ -2413,6 +2424,26 @@ hipError_t build_plans(Ctx* c) {
           break;
         }
     }
+  // tiles of 16 ANG_WAVES plane columns with a point of the compressed grid at
+  // some angle (the mask is the same for every channel)
+  if (c->ang_nt || (c->ang_fused && offs32)) {
+    const int64_t tw = 16 * ANG_WAVES, tpc = (plane + tw - 1) / tw;
+    std::vector<uint8_t> act((size_t)tpc, 0);
+    for (int ia = 0; ia < t.na; ia++)
+      for (int64_t q = 0; q < plane; q++)
+        if (t.mask[(size_t)ia * plane + q]) act[(size_t)(q / tw)] = 1;
+    std::vector<int32_t> tl;
+    for (int ch = 0; ch < t.nc; ch++)
+      for (int64_t k = 0; k < tpc; k++)
+        if (act[(size_t)k]) tl.push_back((int32_t)(ch * tpc + k));
+    c->ang_ntiles = (int)tl.size();
+    if (!tl.empty()) {
+      if ((e = hipMalloc(&c->d_ang_tiles, tl.size() * sizeof(int32_t))) != hipSuccess) return e;
+      if ((e = hipMemcpy(c->d_ang_tiles, tl.data(), tl.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+    }
+  }
   // the parity-folded form of the one-pass kernel where the functions allow it
   c->ang_fold = 0;
   if (c->ang_fused && offs32) {
