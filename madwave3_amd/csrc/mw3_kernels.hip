@@ -1820,7 +1820,7 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
       const int row = 16 * cur.y + M::row(lane, rr);
-      if (row < cur.z && k2 < cur.w) orow[(int64_t)row * N2 + k2] = acc[rr];
+      if (row < cur.z && k2 < (cur.w & 0xffff)) orow[(int64_t)row * N2 + k2] = acc[rr];
     }
     cur = nxt;
   }
@@ -1871,7 +1871,7 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
       const int row = v / (K2P / 4), c4 = v - row * (K2P / 4);
 #pragma unroll
       for (int e = 0; e < 4; e++)
-        if (4 * c4 + e >= it.w) pf[l][e] = 0;
+        if (4 * c4 + e >= (it.w & 0xffff)) pf[l][e] = 0;
       st4(&tile[row * LP + 4 * c4], pf[l]);
     }
   };
@@ -1885,6 +1885,12 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
     put(cur);
     __syncthreads();
     fetch(nxt);
+    // pairs whose 16 rows hold no compressed-grid point in their columns or
+    // the mirrored ones: the update reads none of their output
+    if (!((cur.w >> (16 + pair)) & 1)) {
+      cur = nxt;
+      continue;
+    }
     T a[NKB][2][4];
 #pragma unroll
     for (int kb = 0; kb < NKB; kb++) {
@@ -2380,15 +2386,28 @@ hipError_t build_plans(Ctx* c) {
   // R passes one wave per 16-row tile when their tables were built
   c->rad_nitems = 0;
   if (t.rad_nkb > 0 && (t.n2 == 256 || t.n2 == 320 || t.n2 == 384 || t.n2 == 512)) {
-    // items {channel * na + angle, row tile, nn1, k2lo}
+    // items {channel * na + angle, row tile, nn1, k2lo | active R-tile pairs << 16}
     std::vector<int32_t> items;
     for (int ch = 0; ch < t.nc; ch++)
       for (int ia = 0; ia < t.na; ia++)
         for (int rt = 0; 16 * rt < t.nn1[ia]; rt++) {
+          // bit 16 + q: R-tile pair q (columns 32 q .. 32 q + 31 and their
+          // mirrors) holds a compressed-grid point in the item's rows
+          int pm = 0;
+          const int rhi = std::min(16 * rt + 16, t.nn1[ia]);
+          for (int q = 0; q < t.n2 / 64; q++)
+            for (int r = 16 * rt; r < rhi && !((pm >> q) & 1); r++)
+              for (int k = 0; k < 32; k++) {
+                const uint8_t* mrow = &t.mask[((size_t)ia * t.n1 + r) * t.n2];
+                if (mrow[32 * q + k] || mrow[t.n2 - 1 - 32 * q - k]) {
+                  pm |= 1 << q;
+                  break;
+                }
+              }
           items.push_back(ch * t.na + ia);
           items.push_back(rt);
           items.push_back(t.nn1[ia]);
-          items.push_back(t.k2lo[ia]);
+          items.push_back(t.k2lo[ia] | (pm << 16));
         }
     if ((e = hipMalloc(&c->d_rad_items, items.size() * sizeof(int32_t))) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_rad_items, items.data(), items.size() * sizeof(int32_t),
