@@ -1204,12 +1204,14 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
   // there and the update reads no angular value there
   int kt = blockIdx.x;
   if (kt >= ntiles) return;
-  int tile = tiles[kt];
+  // entries: tile index | (waves whose 16 columns hold a grid point) << 24
+  int tile = tiles[kt] & 0xffffff;
 #pragma unroll
   for (int c = 0; c < XA && c < NT; c++) ldx(c, tile);
   int cur = -1;
   for (; kt < ntiles; kt += gridDim.x) {
-    tile = tiles[kt];
+    const int tent = tiles[kt];
+    tile = tent & 0xffffff;
     const int ch = tile / tpc;
     if (ch != cur) {  // block-uniform
       __syncthreads();
@@ -1246,7 +1248,13 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
     }
     // the next tile of this block (the last one repeats itself), whose first
     // psi chunks load during this tile's back product
-    const int tnext = kt + (int)gridDim.x < ntiles ? tiles[kt + (int)gridDim.x] : tile;
+    const int tnext =
+        kt + (int)gridDim.x < ntiles ? tiles[kt + (int)gridDim.x] & 0xffffff : tile;
+    if (!((tent >> (24 + wave)) & 1)) {  // no grid point in this wave's columns
+#pragma unroll
+      for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+      continue;
+    }
     const int p = (tile - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + i;
     const int pc = p < pl ? p : pl - 1;
     const int r1 = pc / n2, R = pc - r1 * n2;
@@ -1409,12 +1417,14 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
   // there and the update reads no angular value there
   int kt = blockIdx.x;
   if (kt >= ntiles) return;
-  int tile = tiles[kt];
+  // entries: tile index | (waves whose 16 columns hold a grid point) << 24
+  int tile = tiles[kt] & 0xffffff;
 #pragma unroll
   for (int c = 0; c < XA && c < NT; c++) ldx(c, tile);
   int cur = -1;
   for (; kt < ntiles; kt += gridDim.x) {
-    tile = tiles[kt];
+    const int tent = tiles[kt];
+    tile = tent & 0xffffff;
     const int ch = tile / tpc;
     if (ch != cur) {  // block-uniform
       __syncthreads();
@@ -1441,7 +1451,13 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
       __syncthreads();
       cur = ch;
     }
-    const int tnext = kt + (int)gridDim.x < ntiles ? tiles[kt + (int)gridDim.x] : tile;
+    const int tnext =
+        kt + (int)gridDim.x < ntiles ? tiles[kt + (int)gridDim.x] & 0xffffff : tile;
+    if (!((tent >> (24 + wave)) & 1)) {  // no grid point in this wave's columns
+#pragma unroll
+      for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+      continue;
+    }
     const int p = (tile - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + i;
     const int pc = p < pl ? p : pl - 1;
     const int r1 = pc / n2, R = pc - r1 * n2;
@@ -2445,16 +2461,20 @@ hipError_t build_plans(Ctx* c) {
     }
   // tiles of 16 ANG_WAVES plane columns with a point of the compressed grid at
   // some angle (the mask is the same for every channel)
-  if (c->ang_nt || (c->ang_fused && offs32)) {
+  const bool tiles_fit = (plane + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES) * t.nc < (1 << 24);
+  if (!tiles_fit) c->ang_nt = 0;  // tile entries keep the index in 24 bits
+  if (tiles_fit && (c->ang_nt || (c->ang_fused && offs32))) {
     const int64_t tw = 16 * ANG_WAVES, tpc = (plane + tw - 1) / tw;
-    std::vector<uint8_t> act((size_t)tpc, 0);
+    std::vector<uint8_t> act((size_t)tpc, 0);  // bit w: wave w's 16 columns
     for (int ia = 0; ia < t.na; ia++)
       for (int64_t q = 0; q < plane; q++)
-        if (t.mask[(size_t)ia * plane + q]) act[(size_t)(q / tw)] = 1;
+        if (t.mask[(size_t)ia * plane + q])
+          act[(size_t)(q / tw)] |= (uint8_t)(1u << ((q % tw) / 16));
     std::vector<int32_t> tl;
     for (int ch = 0; ch < t.nc; ch++)
       for (int64_t k = 0; k < tpc; k++)
-        if (act[(size_t)k]) tl.push_back((int32_t)(ch * tpc + k));
+        if (act[(size_t)k])
+          tl.push_back((int32_t)((uint32_t)(ch * tpc + k) | ((uint32_t)act[(size_t)k] << 24)));
     c->ang_ntiles = (int)tl.size();
     if (!tl.empty()) {
       if ((e = hipMalloc(&c->d_ang_tiles, tl.size() * sizeof(int32_t))) != hipSuccess) return e;
@@ -2465,7 +2485,7 @@ hipError_t build_plans(Ctx* c) {
   }
   // the parity-folded form of the one-pass kernel where the functions allow it
   c->ang_fold = 0;
-  if (c->ang_fused && offs32) {
+  if (c->ang_fused && offs32 && tiles_fit) {
     int ntf = 0;
     if ((e = build_ang_fold(c, &ntf)) != hipSuccess) return e;
     if (ntf > 0) {
