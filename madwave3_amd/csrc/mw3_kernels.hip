@@ -801,7 +801,9 @@ __global__ __launch_bounds__(64) void k_update_flux(const double* __restrict__ u
   const int R = rc * 64 + lane;
   if (R >= n2) return;
   double re = 0.0, im = 0.0, zv = 0.0;
-  for (int it = gfirst[g]; it < gfirst[g + 1]; it++) {
+  const int i0 = gfirst[g], i1 = gfirst[g + 1];
+#pragma unroll 4
+  for (int it = i0; it < i1; it++) {
     const double* up = upart + (int64_t)it * 3 * 64;
     re += up[lane];
     im += up[64 + lane];
@@ -845,11 +847,21 @@ static void dispatch_update(Ctx* c, unsigned nb, const double* psi, double* out,
 
 __global__ void k_reduce_parts(const double* __restrict__ part, int nb, double* __restrict__ out) {
   __shared__ double sh[2][256];
-  double a = 0, b = 0;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    a += part[2 * i];
-    b += part[2 * i + 1];
+  // four independent partial sums per thread, so that four loads are in
+  // flight at a time instead of one
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  const d2* p2 = (const d2*)part;
+  d2 s0 = {0, 0}, s1 = {0, 0}, s2 = {0, 0}, s3 = {0, 0};
+  int i = threadIdx.x;
+  for (; i + 3 * (int)blockDim.x < nb; i += 4 * blockDim.x) {
+    s0 += p2[i];
+    s1 += p2[i + blockDim.x];
+    s2 += p2[i + 2 * blockDim.x];
+    s3 += p2[i + 3 * blockDim.x];
   }
+  for (; i < nb; i += blockDim.x) s0 += p2[i];
+  const d2 st = (s0 + s1) + (s2 + s3);
+  double a = st.x, b = st.y;
   sh[0][threadIdx.x] = a;
   sh[1][threadIdx.x] = b;
   __syncthreads();
@@ -891,9 +903,16 @@ __global__ __launch_bounds__(64 * CVJ_WAVES) void k_cvj(const double* __restrict
   const double* sl = slice + (int64_t)c * na * n1;
   for (int r0 = 0; r0 < n1; r0 += 64) {
     const int r = r0 + lane;
-    double s = 0;
-    if (r < n1)
-      for (int ia = wv; ia < na; ia += CVJ_WAVES) s += sl[(int64_t)ia * n1 + r] * dj[ia];
+    double s = 0, s1 = 0;
+    if (r < n1) {
+      int ia = wv;
+      for (; ia + CVJ_WAVES < na; ia += 2 * CVJ_WAVES) {  // two loads in flight
+        s += sl[(int64_t)ia * n1 + r] * dj[ia];
+        s1 += sl[(int64_t)(ia + CVJ_WAVES) * n1 + r] * dj[ia + CVJ_WAVES];
+      }
+      if (ia < na) s += sl[(int64_t)ia * n1 + r] * dj[ia];
+      s += s1;
+    }
     part[wv][lane] = s;
     __syncthreads();
     if (wv == 0 && r < n1) {

@@ -65,9 +65,10 @@ def case(name):
     return _build(p)
 
 
-def box_case(n1=16, n2=24, na=6, Jtot=0, iommax=0, damp=True, vzero=True):
+def box_case(n1=16, n2=24, na=6, Jtot=0, iommax=0, damp=True, vzero=True, r1range=None):
     """Every grid point kept (Vcut above the whole surface) and, with vzero,
-    V = 0: the operator is the bare kinetic + angular part."""
+    V = 0: the operator is the bare kinetic + angular part.  r1range = (lo, hi)
+    keeps only the rows lo <= ir1 <= hi at every angle."""
     import numpy as np
     from madwave3_amd.refio import PotData
 
@@ -80,6 +81,9 @@ def box_case(n1=16, n2=24, na=6, Jtot=0, iommax=0, damp=True, vzero=True):
     i2, i1 = np.meshgrid(np.arange(1, n2 + 1), np.arange(1, n1 + 1), indexing="ij")
     ii1 = i1.ravel().astype(np.int32)
     ii2 = i2.ravel().astype(np.int32)
+    if r1range is not None:
+        keep = (ii1 >= r1range[0]) & (ii1 <= r1range[1])
+        ii1, ii2 = ii1[keep], ii2[keep]
     if vzero:
         vm = np.zeros((len(ii1), 1, 1))
     else:   # smooth well in r and R, inside [0, Vcut]

@@ -128,6 +128,69 @@ def test_undamped_recursion_is_chebyshev_polynomial():
         assert np.max(np.abs(pm - refm)) < 1e-10 * np.max(np.abs(refm))
 
 
+def test_radial_kinetic_past_the_sine_box_is_the_identity():
+    """Rows ir1 > nopr1 of an angle lie outside its 2-D sine transform: raux
+    keeps psi there and rHpaqrec adds it unchanged (SRC/mod_Hphi_01y2.f:768-831).
+    With the kept rows 5..14 of 16, nopr1 = noptFFT(10) = 10.  At J = 0 and
+    V = 0 the column of a point past row 10 has no entry off its (r1, R) line,
+    and the line block is I + A with A the angular block, whose j = 0
+    eigenvalue is 0, so its smallest eigenvalue is exactly 1.  Returning zero
+    there, or the transform's value, or taking row nopr1 itself out of the
+    transform fails it."""
+    c = box_case(n1=16, n2=12, na=2, damp=False, vzero=True, r1range=(5, 14))
+    o = Oracle(c)
+    n = c.ntot
+    H = _hmatrix(o, n)
+    _, _, _, i1, i2 = model.point_index(c)
+    nn1 = int(o.tables()["nopr1"][0])
+    assert nn1 == 10 and i1.min() == 5 and i1.max() == 14
+    for r1 in (nn1, nn1 + 1, 14):
+        for r2 in (1, 6, 12):
+            line = np.where((i1 == r1) & (i2 == r2))[0]
+            off = np.setdiff1d(np.arange(n), line)
+            coupling = np.max(np.abs(H[np.ix_(off, line)]))
+            lo = np.linalg.eigvalsh(H[np.ix_(line, line)]).min()
+            if r1 <= nn1:        # inside the transform: the kinetic term couples rows
+                assert coupling > 1e-3 and abs(lo - 1.0) > 1e-2
+            else:
+                assert coupling == 0.0
+                assert abs(lo - 1.0) < 1e-12
+
+
+def test_autocorrelation_and_psie_are_chebyshev_sums():
+    """Photodissociation outputs with no absorption: the autocorrelation
+    <psi_0|psi_k> (SRC/main_madwave3.f:444-455) is sum_i c_i^2 cos(k theta_i)
+    in the eigenbasis of H_norm (c = V^T psi_0, theta = arccos lambda), and
+    psi(E) (funE, SRC/mod_PsiE_01y2.f:92-123, called for it = 2, 3, ... at
+    SRC/main_madwave3.f:350-353) is V diag(sum_k C_k(E) cos(k theta)) c with the
+    coefficients C_k(E) of CoefEcheby (pinned by its closed form).  Pairing a
+    coefficient with the wrong step, a missing step or a wrong real/imaginary
+    part fails it."""
+    c = box_case(n1=8, n2=10, na=4, damp=False, vzero=False)
+    c.iphoto = 1
+    c.nofe = 3
+    c.efe = c.emindlt + c.delta2 * np.array([-0.55, 0.1, 0.7])  # |E_s| < 1
+    o = Oracle(c)
+    n = c.ntot
+    H = _hmatrix(o, n)
+    Hn = (H - c.emindlt * np.eye(n)) / c.delta2
+    lam, V = np.linalg.eigh(Hn)
+    th = np.arccos(lam)
+    psi0 = np.random.default_rng(5).standard_normal(n)
+    cf = V.T @ psi0
+    o.begin(psi0)
+    nst = 40
+    _, au, _ = o.steps(nst)
+    ks = np.arange(2, 2 + nst)          # the steps' Chebyshev orders
+    auto_ref = np.cos(np.outer(ks, th)) @ (cf * cf)
+    assert np.max(np.abs(au - auto_ref)) < 1e-11 * np.max(np.abs(auto_ref))
+    ze = o.psie()
+    for ie, E in enumerate(c.efe):
+        ck = np.array([o.coef_echeby(int(k), np.array([E]))[0] for k in ks])
+        ref = V @ ((np.cos(np.outer(th, ks)) @ ck) * cf)
+        assert np.max(np.abs(ze[ie] - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
 def test_damped_recursion_matches_matrix_iteration():
     """Damped update psi_{k+1} = f (2 H_norm psi_k - f psi_{k-1}) with
     f = absfr1(r) absfr2(R) decoded per point (SRC/main_madwave3.f:332-339)."""
