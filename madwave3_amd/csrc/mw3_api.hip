@@ -117,7 +117,8 @@ static void free_all(Ctx* c) {
                   c->d_uw_item, c->d_uw_gfirst, c->d_uw_part, c->d_tprod, c->d_ppart,
                   c->d_dtrans, c->d_ring_prod, c->d_prodwf, c->d_prod_part,
                   c->d_ring_cvjp, c->d_cvjp_last, c->d_zs2prod, c->d_rad_items, c->d_fold_D, c->d_fold_a, c->d_fold_b,
-                  c->d_lowT, c->d_mid_items, c->d_mid_sizes, c->d_ang_tiles};
+                  c->d_lowT, c->d_mid_items, c->d_mid_sizes, c->d_ang_tiles,
+                  c->d_ang_groups};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   free_plans(c);
@@ -198,11 +199,17 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   } while (0)
   CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "mw3_create: stream");
   CK(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking), "mw3_create: stream");
-  CK(hipEventCreateWithFlags(&c->ev_slice, hipEventDisableTiming), "mw3_create: event");
-  CK(hipEventCreateWithFlags(&c->ev_cvj, hipEventDisableTiming), "mw3_create: event");
+  // the stream-ordering events only order device work between the library's
+  // streams (the host never queries them; host reads follow a stream
+  // synchronize and a copy on the main stream), so they carry no system-scope
+  // fence: with one, every record and wait wrote back and invalidated the
+  // caches, about 10 us of idle GPU at each of the step's two crossings
+  const unsigned evf = hipEventDisableTiming | hipEventDisableSystemFence;
+  CK(hipEventCreateWithFlags(&c->ev_slice, evf), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_cvj, evf), "mw3_create: event");
   CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "mw3_create: stream");
-  CK(hipEventCreateWithFlags(&c->ev_psi, hipEventDisableTiming), "mw3_create: event");
-  CK(hipEventCreateWithFlags(&c->ev_ang, hipEventDisableTiming), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_psi, evf), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_ang, evf), "mw3_create: event");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
 
@@ -485,10 +492,13 @@ static int step_enqueue(Ctx* c, int32_t* done, int32_t nsteps, double* norms, do
   int base = *done - c->ring_fill;  // output index of ring slot 0
   c->timing_step = (c->k % c->timing_every) == 0;
   HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
-  if (c->cvj_pending)  // the update rewrites the slice the last k_cvj reads
+  // the update rewrites the slice the last k_cvj reads (already waited for
+  // through ev_ang when the angular term ran on the third stream)
+  if (c->cvj_pending && !c->cvj_joined)
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
   HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
   HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
+  c->slice_fresh = true;  // nothing below writes psi before the next H psi
   c->k += 1;
   HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
   std::swap(c->d_psi, c->d_psim1);
@@ -510,11 +520,10 @@ int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* aut
   if (!c->begun) return seterr(c, MW3_ESTATE, "mw3_cheby_steps: call mw3_cheby_begin first");
   if (nsteps < 0) return seterr(c, MW3_EINVAL, "mw3_cheby_steps: nsteps < 0");
   int32_t done = 0;
-  while (done < nsteps) {
-    int rc = step_enqueue(c, &done, nsteps, norms, autocorr);
-    if (rc) return rc;
-  }
-  return MW3_OK;
+  int rc = MW3_OK;
+  while (done < nsteps && rc == MW3_OK) rc = step_enqueue(c, &done, nsteps, norms, autocorr);
+  c->slice_fresh = false;  // the caller may write psi before the next H psi
+  return rc;
 }
 
 int32_t mw3_cheby_steps_batch(mw3_ctx** ctxs, int32_t n, int32_t nsteps, double* norms) {
@@ -533,14 +542,16 @@ int32_t mw3_cheby_steps_batch(mw3_ctx** ctxs, int32_t n, int32_t nsteps, double*
   std::vector<int32_t> done(n, 0);
   // step-major order: every context's step s is enqueued before any step s+1,
   // so the streams of all contexts hold work at the same time
-  for (int32_t st = 0; st < nsteps; st++)
-    for (int i = 0; i < n; i++) {
+  int rc = MW3_OK;
+  for (int32_t st = 0; st < nsteps && rc == MW3_OK; st++)
+    for (int i = 0; i < n && rc == MW3_OK; i++) {
       Ctx* c = reinterpret_cast<Ctx*>(ctxs[i]);
-      int rc = step_enqueue(c, &done[i], nsteps, norms ? norms + (size_t)i * nsteps : nullptr,
-                            nullptr);
-      if (rc) return rc;
+      rc = step_enqueue(c, &done[i], nsteps, norms ? norms + (size_t)i * nsteps : nullptr,
+                        nullptr);
     }
-  return MW3_OK;
+  for (int i = 0; i < n; i++)  // the caller may write psi before the next H psi
+    reinterpret_cast<Ctx*>(ctxs[i])->slice_fresh = false;
+  return rc;
 }
 
 int32_t mw3_flux_result(mw3_ctx* ctx, double* S2prodtot, double* S2prodelec) {

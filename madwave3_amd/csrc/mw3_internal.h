@@ -125,6 +125,10 @@ struct Ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_psi = nullptr, ev_ang = nullptr;
   bool cvj_pending = false;
+  // step loop only: ev_slice follows main's last write of psi (the update),
+  // so the next H psi's third stream waits on it instead of a new record
+  bool slice_fresh = false;
+  bool cvj_joined = false;  // the third stream waited ev_cvj before ev_ang
   int device = 0;
 
   // device buffers
@@ -207,6 +211,9 @@ struct Ctx {
   int ang_grid = 0;        // its persistent blocks
   int32_t* d_ang_tiles = nullptr;  // tiles of the one-pass kernels with compressed-grid points
   int ang_ntiles = 0;
+  int32_t* d_ang_groups = nullptr;  // 16-column groups with compressed-grid points (folded kernel)
+  int ang_ngroups = 0;
+  int ang_fold_nbc = 0;             // folded kernel: blocks per channel
   int ang_fold = 0;        // row tiles of the parity-folded one-pass kernel (0: not used)
   bool fbr_all = false;    // Coriolis coefficients by k_fbr_coef_all (2..8 channels)
   void* d_fold_D = nullptr;    // [nc][16 ang_fold][32 ang_fold] D(ia < na/2, j by parity class)

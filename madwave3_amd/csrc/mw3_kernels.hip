@@ -1396,14 +1396,20 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
 // which halves both products.  The j axis runs class by class: columns
 // 0 .. NR-1 of the LDS image of D hold the s_j = +1 functions and NR ..
 // 2 NR-1 the s_j = -1 ones (zero padded), and the coefficient tables follow
-// the same order, so row tile T of F (T < 2 NT) is class T / NT.  Otherwise
-// as k_ang_fused.
+// the same order, so row tile T of F (T < 2 NT) is class T / NT.
+//
+// Work: the 16-column groups of the plane that hold a compressed-grid point
+// (the same list for every channel), one group per wave at a time.  The
+// grid is nbc blocks per channel; block b of a channel takes D once and its
+// wave w walks the groups b + nbc w, b + nbc w + 8 nbc, ...  so the groups
+// of a last partial round go one per block before any block takes a second,
+// and the waves of one SIMD differ by at most one group.
 // ---------------------------------------------------------------------------
 template <typename T, int NT>
 __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
     const T* __restrict__ psi, const T* __restrict__ Dfold, const double* __restrict__ afold,
-    const double* __restrict__ bfold, T* __restrict__ ang, const int32_t* __restrict__ tiles,
-    int ntiles, int na, int n1, int n2) {
+    const double* __restrict__ bfold, T* __restrict__ ang, const int32_t* __restrict__ groups,
+    int ngroups, int nbc, int na, int n1, int n2) {
   typedef MfmaAcc<T> M;
   typedef typename M::V V;
   constexpr int NR = 16 * NT, NC2 = 2 * NR;    // rows (ia < h) and columns (both classes)
@@ -1413,14 +1419,14 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
   const int pl = n1 * n2, h = na >> 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, i = lane & 15;
-  const int tpc = (pl + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES);  // tiles per channel
+  const int ch = blockIdx.x / nbc, bl = blockIdx.x - ch * nbc;
+  const int stride = nbc * ANG_WAVES;
+  const char* pb = (const char*)(psi + (int64_t)ch * na * pl);
   // xl[4 c + s] = psi[16 c + 4 g + s][p], xh[..] = psi[na-1-(16 c + 4 g + s)][p];
   // angles past h read angle h-1 and its mirror (finite), against zero rows of D
   T xl[4 * NT], xh[4 * NT];
-  auto ldx = [&](int c, int tl) {
-    const int ch = tl / tpc;
-    const char* pb = (const char*)(psi + (int64_t)ch * na * pl);
-    int pp = (tl - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + lane_opaque(i);
+  auto ldx = [&](int c, int col0) {  // col0: first column of the wave's group
+    int pp = col0 + lane_opaque(i);
     pp = pp < pl ? pp : pl - 1;
     const int gg = lane_opaque(g);
 #pragma unroll
@@ -1431,22 +1437,16 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
       xh[4 * c + s] = *(const T*)(pb + (uint32_t)((na - 1 - ia) * pl + pp) * (uint32_t)sizeof(T));
     }
   };
-  // the block walks the listed tiles (channel * tiles per channel + tile);
-  // tiles with no point of the compressed grid are not listed: psi is zero
-  // there and the update reads no angular value there
-  int kt = blockIdx.x;
-  if (kt >= ntiles) return;
-  // entries: tile index | (waves whose 16 columns hold a grid point) << 24
-  int tile = tiles[kt] & 0xffffff;
+  // groups with no point of the compressed grid are not listed: psi is zero
+  // there and the update reads no angular value there.  The first group's
+  // psi loads go out before the D image's, so that the two latencies overlap
+  int gi = bl + nbc * wave;
+  const bool busy = gi < ngroups;
+  if (busy) {
 #pragma unroll
-  for (int c = 0; c < XA && c < NT; c++) ldx(c, tile);
-  int cur = -1;
-  for (; kt < ntiles; kt += gridDim.x) {
-    const int tent = tiles[kt];
-    tile = tent & 0xffffff;
-    const int ch = tile / tpc;
-    if (ch != cur) {  // block-uniform
-      __syncthreads();
+    for (int c = 0; c < XA && c < NT; c++) ldx(c, 16 * groups[gi]);
+  }
+  {
       const T* Dc = Dfold + (int64_t)ch * NR * NC2;
       constexpr int NV = (NR * NC2 / 4 + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
       constexpr int NG = NV < 4 ? NV : 4;  // vectors in flight per thread
@@ -1468,21 +1468,21 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
         }
       }
       __syncthreads();
-      cur = ch;
-    }
-    const int tnext =
-        kt + (int)gridDim.x < ntiles ? tiles[kt + (int)gridDim.x] & 0xffffff : tile;
-    if (!((tent >> (24 + wave)) & 1)) {  // no grid point in this wave's columns
-#pragma unroll
-      for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
-      continue;
-    }
-    const int p = (tile - ch * tpc) * (16 * ANG_WAVES) + 16 * wave + i;
+  }
+  if (!busy) return;  // no block-wide barrier follows
+  const char* ac = (const char*)(afold + (int64_t)ch * NC2 * n1);
+  const char* bc = (const char*)(bfold + (int64_t)ch * NC2 * n2);
+  char* ob = (char*)(ang + (int64_t)ch * na * pl);
+  for (; gi < ngroups; gi += stride) {
+    const int col = 16 * groups[gi];
+    const int cnext = 16 * groups[gi + stride < ngroups ? gi + stride : gi];
+    const int p = col + i;
     const int pc = p < pl ? p : pl - 1;
     const int r1 = pc / n2, R = pc - r1 * n2;
-    const char* ac = (const char*)(afold + (int64_t)ch * NC2 * n1);
-    const char* bc = (const char*)(bfold + (int64_t)ch * NC2 * n2);
-    double ca[2][4], cb[2][4];
+    // coefficient ring of three stages: the loads of the last stages go out
+    // before the next group's psi loads, so that waiting for them never
+    // waits for those
+    double ca[3][4], cb[3][4];
     auto ldcoef = [&](int t, double (&a)[4], double (&b)[4]) {
       const int ll = lane_opaque(lane), rr = lane_opaque(r1), RR = lane_opaque(R);
 #pragma unroll
@@ -1507,7 +1507,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
 #pragma unroll
     for (int k = 0; k < 4 * NT; k++) {
       if (k + 1 < 4 * NT) rd1(k + 1, a1[(k + 1) & 1]);
-      if ((k & 3) == 0 && (k >> 2) + XA < NT) ldx((k >> 2) + XA, tile);
+      if ((k & 3) == 0 && (k >> 2) + XA < NT) ldx((k >> 2) + XA, col);
       if (k == (NT > 1 ? 4 * NT - 8 : 0)) ldcoef(0, ca[0], cb[0]);
       if (k == 4 * NT - 4) ldcoef(1, ca[1], cb[1]);
       __builtin_amdgcn_sched_barrier(0);
@@ -1526,7 +1526,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
     auto scale = [&](int t) {
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
+        f[t][r] = (T)((double)f[t][r] * (ca[t % 3][r] + cb[t % 3][r]));
     };
     V o[2 * NT];
 #pragma unroll
@@ -1552,10 +1552,12 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
       for (int u = 0; u < NT; u++) {
         const int q = t * NT + u;
         if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
-        if (u == 0 && t + 2 < 2 * NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
+        if (u == 0 && t + 2 < 2 * NT && t != NT + 1)
+          ldcoef(t + 2, ca[(t + 2) % 3], cb[(t + 2) % 3]);
         if (u == 0 && t == NT) {
+          if (t + 3 < 2 * NT) ldcoef(t + 3, ca[(t + 3) % 3], cb[(t + 3) % 3]);
 #pragma unroll
-          for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+          for (int c = 0; c < XA && c < NT; c++) ldx(c, cnext);
         }
         __builtin_amdgcn_sched_barrier(0);
         const int ou = (t / NT) * NT + u;
@@ -1565,7 +1567,6 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
         if (u == UH && t + 1 < 2 * NT) scale(t + 1);
       }
     }
-    char* ob = (char*)(ang + (int64_t)ch * na * pl);
     const int ll = lane_opaque(lane), pp = lane_opaque(p);
 #pragma unroll
     for (int u = 0; u < NT; u++)
@@ -1581,8 +1582,10 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
   }
 }
 
+// blocks per channel: the resident blocks shared evenly by the channels, and
+// no more than the group list can feed
 template <typename T, int NT>
-static hipError_t ang_fold_grid(int* blocks) {
+static hipError_t ang_fold_grid(int nc, int ngroups, int* nbc) {
   int nb = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ang_fold<T, NT>,
                                                               64 * ANG_WAVES, 0);
@@ -1591,19 +1594,20 @@ static hipError_t ang_fold_grid(int* blocks) {
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
     return e;
-  *blocks = (nb > 0 ? nb : 1) * ncu;
+  const int per = (nb > 0 ? nb : 1) * ncu / (nc > 0 ? nc : 1);
+  const int need = (ngroups + ANG_WAVES - 1) / ANG_WAVES;
+  *nbc = std::max(1, std::min(per, need));
   return hipSuccess;
 }
 
 template <typename T, int NT>
 static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   const HostTables& t = c->t;
-  const int nt = c->ang_ntiles;
-  const unsigned nblk = (unsigned)(nt < c->ang_grid ? nt : c->ang_grid);
-  if (nblk == 0) return;
-  hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3(nblk), dim3(64 * ANG_WAVES), 0, st, psi,
-                     (const T*)c->d_fold_D, c->d_fold_a, c->d_fold_b, (T*)c->d_ang,
-                     c->d_ang_tiles, nt, t.na, t.n1, t.n2);
+  if (c->ang_ngroups == 0) return;
+  hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3((unsigned)(c->ang_fold_nbc * t.nc)),
+                     dim3(64 * ANG_WAVES), 0, st, psi, (const T*)c->d_fold_D, c->d_fold_a,
+                     c->d_fold_b, (T*)c->d_ang, c->d_ang_groups, c->ang_ngroups,
+                     c->ang_fold_nbc, t.na, t.n1, t.n2);
 }
 
 template <typename T>
@@ -1620,11 +1624,12 @@ static hipError_t launch_ang_fold(Ctx* c, const T* psi, hipStream_t st) {
 
 template <typename T>
 static hipError_t ang_fold_setup(Ctx* c, int nt) {
+  const int nc = c->t.nc, ng = c->ang_ngroups;
   switch (nt) {
-    case 1: return ang_fold_grid<T, 1>(&c->ang_grid);
-    case 2: return ang_fold_grid<T, 2>(&c->ang_grid);
-    case 3: return ang_fold_grid<T, 3>(&c->ang_grid);
-    case 4: return ang_fold_grid<T, 4>(&c->ang_grid);
+    case 1: return ang_fold_grid<T, 1>(nc, ng, &c->ang_fold_nbc);
+    case 2: return ang_fold_grid<T, 2>(nc, ng, &c->ang_fold_nbc);
+    case 3: return ang_fold_grid<T, 3>(nc, ng, &c->ang_fold_nbc);
+    case 4: return ang_fold_grid<T, 4>(nc, ng, &c->ang_fold_nbc);
     default: return hipErrorInvalidValue;
   }
 }
@@ -2489,15 +2494,24 @@ hipError_t build_plans(Ctx* c) {
       for (int64_t q = 0; q < plane; q++)
         if (t.mask[(size_t)ia * plane + q])
           act[(size_t)(q / tw)] |= (uint8_t)(1u << ((q % tw) / 16));
-    std::vector<int32_t> tl;
+    std::vector<int32_t> tl, gl;
     for (int ch = 0; ch < t.nc; ch++)
       for (int64_t k = 0; k < tpc; k++)
         if (act[(size_t)k])
           tl.push_back((int32_t)((uint32_t)(ch * tpc + k) | ((uint32_t)act[(size_t)k] << 24)));
+    // the same columns as 16-column groups (group q: columns 16 q .. 16 q + 15)
+    for (int64_t k = 0; k < tpc; k++)
+      for (int w = 0; w < ANG_WAVES; w++)
+        if ((act[(size_t)k] >> w) & 1) gl.push_back((int32_t)(k * ANG_WAVES + w));
     c->ang_ntiles = (int)tl.size();
+    c->ang_ngroups = (int)gl.size();
     if (!tl.empty()) {
       if ((e = hipMalloc(&c->d_ang_tiles, tl.size() * sizeof(int32_t))) != hipSuccess) return e;
       if ((e = hipMemcpy(c->d_ang_tiles, tl.data(), tl.size() * sizeof(int32_t),
+                         hipMemcpyHostToDevice)) != hipSuccess)
+        return e;
+      if ((e = hipMalloc(&c->d_ang_groups, gl.size() * sizeof(int32_t))) != hipSuccess) return e;
+      if ((e = hipMemcpy(c->d_ang_groups, gl.data(), gl.size() * sizeof(int32_t),
                          hipMemcpyHostToDevice)) != hipSuccess)
         return e;
     }
@@ -2558,9 +2572,15 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // whole CU's LDS)
   hipStream_t sa = c->precision == MW3_MIXED ? c->side : s;
   if (sa != s) {
-    if ((e = hipEventRecord(c->ev_psi, s)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(sa, c->ev_psi, 0)) != hipSuccess) return e;
+    if (c->slice_fresh) {  // recorded right after the update that wrote psi
+      if ((e = hipStreamWaitEvent(sa, c->ev_slice, 0)) != hipSuccess) return e;
+    } else {
+      if ((e = hipEventRecord(c->ev_psi, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(sa, c->ev_psi, 0)) != hipSuccess) return e;
+    }
   }
+  c->slice_fresh = false;
+  c->cvj_joined = false;
   timer_begin(c, K_ANG_FWD, sa);
   if (c->ang_fold) {
     if ((e = launch_ang_fold<T>(c, psi, sa)) != hipSuccess) return e;
@@ -2617,7 +2637,15 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     if (e != hipSuccess) return e;
     timer_end(c, K_ANG_BWD, sa);
   }
-  if (sa != s && (e = hipEventRecord(c->ev_ang, sa)) != hipSuccess) return e;
+  if (sa != s) {
+    // the second stream's sums of the last step end before ev_ang too, so
+    // that the next update, which rewrites what they read, waits once
+    if (c->cvj_pending) {
+      if ((e = hipStreamWaitEvent(sa, c->ev_cvj, 0)) != hipSuccess) return e;
+      c->cvj_joined = true;
+    }
+    if ((e = hipEventRecord(c->ev_ang, sa)) != hipSuccess) return e;
+  }
   // radial kinetic correction on the low modes: folded DST over R, folded DST
   // over r times (c - E)_+ divi, back over r, back over R into d_t2; the
   // update forms T psi = radcut psi - d_t2 on rows r < nn1
