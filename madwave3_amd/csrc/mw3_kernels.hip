@@ -1407,8 +1407,9 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
 // partial round go one per block before any block takes a second, and the
 // waves of one SIMD differ by at most one group.
 // ---------------------------------------------------------------------------
+// FP32: at most 128 VGPRs (4 waves per SIMD), so that two blocks share a CU
 template <typename T, int NT>
-__global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
+__global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_fold(
     const T* __restrict__ psi, const T* __restrict__ Dfold, const double* __restrict__ afold,
     const double* __restrict__ bfold, T* __restrict__ ang, const int32_t* __restrict__ groups,
     int ngroups, int nbc, int na, int n1, int n2) {
@@ -1416,7 +1417,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
   typedef typename M::V V;
   constexpr int NR = 16 * NT, NC2 = 2 * NR;    // rows (ia < h) and columns (both classes)
   constexpr int P = NC2 + AngPitch<T>::extra;  // LDS row pitch
-  constexpr int XA = sizeof(T) == 8 ? 2 : 3;   // psi chunks loaded ahead of the MFMAs
+  constexpr int XA = 2;                        // psi chunks loaded ahead of the MFMAs
   __shared__ __align__(16) T Ds[NR * P];
   const int pl = n1 * n2, h = na >> 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1481,10 +1482,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
     const int p = col + i;
     const int pc = p < pl ? p : pl - 1;
     const int r1 = pc / n2, R = pc - r1 * n2;
-    // coefficient ring of three stages: the loads of the last stages go out
-    // before the next group's psi loads, so that waiting for them never
-    // waits for those
-    double ca[3][4], cb[3][4];
+    double ca[2][4], cb[2][4];
     auto ldcoef = [&](int t, double (&a)[4], double (&b)[4]) {
       const int ll = lane_opaque(lane), rr = lane_opaque(r1), RR = lane_opaque(R);
 #pragma unroll
@@ -1528,7 +1526,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
     auto scale = [&](int t) {
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        f[t][r] = (T)((double)f[t][r] * (ca[t % 3][r] + cb[t % 3][r]));
+        f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
     };
     V o[2 * NT];
 #pragma unroll
@@ -1554,10 +1552,8 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fold(
       for (int u = 0; u < NT; u++) {
         const int q = t * NT + u;
         if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
-        if (u == 0 && t + 2 < 2 * NT && t != NT + 1)
-          ldcoef(t + 2, ca[(t + 2) % 3], cb[(t + 2) % 3]);
+        if (u == 0 && t + 2 < 2 * NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
         if (u == 0 && t == NT) {
-          if (t + 3 < 2 * NT) ldcoef(t + 3, ca[(t + 3) % 3], cb[(t + 3) % 3]);
 #pragma unroll
           for (int c = 0; c < XA && c < NT; c++) ldx(c, cnext);
         }
