@@ -78,6 +78,29 @@ def test_recursion_and_projections_fp64(name):
     assert rel(a_g, a_o) < 1e-9 and rel(b_g, b_o) < 1e-9
 
 
+def test_mixed_state_put_between_step_calls():
+    """Mixed precision runs the angular term on its own stream, ordered after
+    the last write of psi on the main stream.  A state put between two step
+    calls of one context must reach the next step: going back to a saved
+    state and stepping on gives what a fresh context gives from that state."""
+    c = case("j0")
+    ga = Propagator(c, MW3_MIXED)
+    ga.begin(c.rpaq0)
+    ga.steps(25)
+    pm, p, k = ga.state()
+    ga.steps(17)
+    ga.put_state(pm, p, k)
+    na, _ = ga.steps(30)
+    gb = Propagator(c, MW3_MIXED)
+    gb.put_state(pm, p, k)
+    nb, _ = gb.steps(30)
+    assert rel(na, nb) < 1e-12
+    assert rel(ga.state()[1], gb.state()[1]) < 1e-12
+    # and H psi between step calls sees the state that the steps left
+    h = ga.hphi(ga.state()[1])
+    assert rel(h, Oracle(c).hphi(ga.state()[1])) < 2e-5
+
+
 def test_restart_continues_identically():
     """wvpcont/restart: state_get -> new context -> state_put continues the
     same recursion (SRC/main_madwave3.f:223-261, 494-552)."""
