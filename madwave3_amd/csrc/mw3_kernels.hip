@@ -657,6 +657,10 @@ struct UpdArgs {
   double* zdr;
   double* slice;
   float* out32;  // FP32 copy of the new psi (mixed precision), or null
+  // mixed precision, damped update: psi_k enters only through the operator
+  // (its diagonal V, radcut and emindlt terms), so it is read from its FP32
+  // copy like every other operator input
+  const float* psi32;
   int ir2abs;
   double rsah2;
   // product projection: per item and lane (R), sum over the item's rows of
@@ -697,6 +701,7 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
   const bool cvjcol = PROJ && a.slice && R == a.ir2abs - 1;
   double s_norm = 0.0, s_auto = 0.0, zre = 0.0, zim = 0.0, zval = 0.0, sprod = 0.0;
   const TT* tprod = PRODP ? (const TT*)a.tprod + (int64_t)c * a.tp_cstride : nullptr;
+  constexpr bool P32 = MODE == 2 && sizeof(TT) == 4;  // psi_k from its FP32 copy
   // the item's rows, four at a time: the loads of the four rows and their
   // per-row tables are issued before any is used, with no branch around them
   for (int b = 0; b < cnt; b += 4) {
@@ -715,7 +720,7 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
     for (int u = 0; u < 4; u++) {
       const int r = rr[u];
       const int64_t ie = (int64_t)c * slab + pl[u];
-      p[u] = psi[ie];
+      p[u] = P32 ? (double)a.psi32[ie] : psi[ie];
       tk[u] = (double)tkin[ie];
       an[u] = (double)ang[ie];
       vv[u] = SINGLE ? (double)vd[pl[u]] : 0.0;
@@ -740,7 +745,8 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
           if (a.ch_omega[s2] != om) continue;
           const int es = a.ch_elec[s2];
           const double v = (double)vd[((int64_t)et * a.ne + es) * slab + pl[u]];
-          const double ps = (s2 == c) ? p[u] : psi[(int64_t)s2 * slab + pl[u]];
+          const int64_t is2 = (int64_t)s2 * slab + pl[u];
+          const double ps = (s2 == c) ? p[u] : (P32 ? (double)a.psi32[is2] : psi[is2]);
           h += ps * v;
         }
       }
@@ -2743,6 +2749,11 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   a.ir2abs = c->s.ir2abs;
   a.rsah2 = 1.0 / std::sqrt(c->s.ah2);
   a.out32 = mode != 0 ? psi_shadow(c, c->d_psim1) : nullptr;
+  a.psi32 = nullptr;
+  if (sizeof(T) == 4 && mode == 2) {
+    a.psi32 = psi_shadow(c, psi);  // current: H psi has just read it
+    if (!a.psi32) return hipErrorInvalidValue;
+  }
   const bool prodp = mode == 2 && proj && c->prod_on;
   a.tprod = prodp ? c->d_tprod : nullptr;
   a.tp_cstride = c->pr_tstride;
