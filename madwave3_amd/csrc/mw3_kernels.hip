@@ -2583,8 +2583,15 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // the FP64 pair ran 2% slower side by side (its angular kernel holds a
   // whole CU's LDS)
   hipStream_t sa = c->precision == MW3_MIXED ? c->side : s;
+  // the forward R pass first (measured 1.3% faster per FP64 step than after
+  // the angular term)
+  timer_begin(c, K_DST_R);
+  e = c->rad_nitems ? run_rad<T>(c, psi, t1, t2, 0)
+                    : run_gemm<G_DST_R, T, T, T, T>(s, psi, tab, t1, c->p_dstR);
+  if (e != hipSuccess) return e;
+  timer_end(c, K_DST_R);
   if (sa != s) {
-    if (c->slice_fresh) {  // recorded right after the update that wrote psi
+    if (c->slice_fresh) {  // recorded after the update that wrote psi, nothing writing it since
       if ((e = hipStreamWaitEvent(sa, c->ev_slice, 0)) != hipSuccess) return e;
     } else {
       if ((e = hipEventRecord(c->ev_psi, s)) != hipSuccess) return e;
@@ -2662,11 +2669,6 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // over r times (c - E)_+ divi, back over r, back over R into d_t2; the
   // update forms T psi = radcut psi - d_t2 on rows r < nn1
   KinParams kp{c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.n1};
-  timer_begin(c, K_DST_R);
-  e = c->rad_nitems ? run_rad<T>(c, psi, t1, t2, 0)
-                    : run_gemm<G_DST_R, T, T, T, T>(s, psi, tab, t1, c->p_dstR);
-  if (e != hipSuccess) return e;
-  timer_end(c, K_DST_R);
   timer_begin(c, K_DST_r);
   if (c->mid_nitems) {
     if ((e = launch_rad_mid<T>(c)) != hipSuccess) return e;
