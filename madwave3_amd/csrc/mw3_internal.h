@@ -217,8 +217,9 @@ struct Ctx {
   int ang_fold = 0;        // row tiles of the parity-folded one-pass kernel (0: not used)
   bool fbr_all = false;    // Coriolis coefficients by k_fbr_coef_all (2..8 channels)
   void* d_fold_D = nullptr;    // [nc][16 ang_fold][32 ang_fold] D(ia < na/2, j by parity class)
-  double* d_fold_a = nullptr;  // [nc][32 ang_fold][n1] a_j(r1) in the same j order
-  double* d_fold_b = nullptr;  // [nc][32 ang_fold][n2] b_j(R)
+  // operator precision (FP32 in mixed mode, like the D tables)
+  void* d_fold_a = nullptr;  // [nc][32 ang_fold][n1] a_j(r1) in the same j order
+  void* d_fold_b = nullptr;  // [nc][32 ang_fold][n2] b_j(R)
   void* d_phi = nullptr;        // FBR after the Omega stencil
   double* d_wmat = nullptr;     // [kRing][netot][2] Chebyshev energy factors
   uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)

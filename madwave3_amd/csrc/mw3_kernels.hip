@@ -1416,8 +1416,8 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
 // FP32: at most 128 VGPRs (4 waves per SIMD), so that two blocks share a CU
 template <typename T, int NT>
 __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_fold(
-    const T* __restrict__ psi, const T* __restrict__ Dfold, const double* __restrict__ afold,
-    const double* __restrict__ bfold, T* __restrict__ ang, const int32_t* __restrict__ groups,
+    const T* __restrict__ psi, const T* __restrict__ Dfold, const T* __restrict__ afold,
+    const T* __restrict__ bfold, T* __restrict__ ang, const int32_t* __restrict__ groups,
     int ngroups, int nbc, int na, int n1, int n2) {
   typedef MfmaAcc<T> M;
   typedef typename M::V V;
@@ -1488,14 +1488,14 @@ __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_
     const int p = col + i;
     const int pc = p < pl ? p : pl - 1;
     const int r1 = pc / n2, R = pc - r1 * n2;
-    double ca[2][4], cb[2][4];
-    auto ldcoef = [&](int t, double (&a)[4], double (&b)[4]) {
+    T ca[2][4], cb[2][4];
+    auto ldcoef = [&](int t, T (&a)[4], T (&b)[4]) {
       const int ll = lane_opaque(lane), rr = lane_opaque(r1), RR = lane_opaque(R);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
         const int j = 16 * t + M::row(ll, r);
-        a[r] = *(const double*)(ac + (uint32_t)(j * n1 + rr) * 8u);
-        b[r] = *(const double*)(bc + (uint32_t)(j * n2 + RR) * 8u);
+        a[r] = *(const T*)(ac + (uint32_t)(j * n1 + rr) * (uint32_t)sizeof(T));
+        b[r] = *(const T*)(bc + (uint32_t)(j * n2 + RR) * (uint32_t)sizeof(T));
       }
     };
     // forward products: f[t] (t < NT) the s = +1 rows from u, f[NT + t] the
@@ -1532,7 +1532,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_
     auto scale = [&](int t) {
 #pragma unroll
       for (int r = 0; r < 4; r++)
-        f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
+        f[t][r] = f[t][r] * (ca[t & 1][r] + cb[t & 1][r]);
     };
     V o[2 * NT];
 #pragma unroll
@@ -1609,8 +1609,9 @@ static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   const HostTables& t = c->t;
   if (c->ang_ngroups == 0) return;
   hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3((unsigned)(c->ang_fold_nbc * t.nc)),
-                     dim3(64 * ANG_WAVES), 0, st, psi, (const T*)c->d_fold_D, c->d_fold_a,
-                     c->d_fold_b, (T*)c->d_ang, c->d_ang_groups, c->ang_ngroups,
+                     dim3(64 * ANG_WAVES), 0, st, psi, (const T*)c->d_fold_D,
+                     (const T*)c->d_fold_a, (const T*)c->d_fold_b, (T*)c->d_ang,
+                     c->d_ang_groups, c->ang_ngroups,
                      c->ang_fold_nbc, t.na, t.n1, t.n2);
 }
 
@@ -1694,14 +1695,20 @@ static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
         hipSuccess)
       return e;
   }
-  if ((e = hipMalloc(&c->d_fold_a, af.size() * sizeof(double))) != hipSuccess) return e;
-  if ((e = hipMemcpy(c->d_fold_a, af.data(), af.size() * sizeof(double), hipMemcpyHostToDevice)) !=
-      hipSuccess)
-    return e;
-  if ((e = hipMalloc(&c->d_fold_b, bf.size() * sizeof(double))) != hipSuccess) return e;
-  if ((e = hipMemcpy(c->d_fold_b, bf.data(), bf.size() * sizeof(double), hipMemcpyHostToDevice)) !=
-      hipSuccess)
-    return e;
+  // the coefficient tables in the operator precision
+  auto put_table = [&](void** dst, const std::vector<double>& v) -> hipError_t {
+    if (c->precision == MW3_MIXED) {
+      std::vector<float> v32(v.begin(), v.end());
+      hipError_t err = hipMalloc(dst, v32.size() * sizeof(float));
+      if (err != hipSuccess) return err;
+      return hipMemcpy(*dst, v32.data(), v32.size() * sizeof(float), hipMemcpyHostToDevice);
+    }
+    hipError_t err = hipMalloc(dst, v.size() * sizeof(double));
+    if (err != hipSuccess) return err;
+    return hipMemcpy(*dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice);
+  };
+  if ((e = put_table(&c->d_fold_a, af)) != hipSuccess) return e;
+  if ((e = put_table(&c->d_fold_b, bf)) != hipSuccess) return e;
   *nt_out = nt;
   return hipSuccess;
 }
