@@ -2896,7 +2896,9 @@ hipError_t launch_psie(Ctx* c, int k) {
 // electronic state e the channel sum ring[o][e][l] = sum_{c: elec(c) = e}
 // dtrans[c][o][l] funprod(c, l), l = (ia - nangproj0) n2p + (R - n2prod0).
 // Block per (angle of the range, 64-R chunk of the range), lane = R.  Each
-// lane reads back only its own column of f, so no barrier is needed.
+// lane reads back only its own column of f, so no barrier is needed; f is
+// sized by the launch to the channel count (nc 64-lane rows), so that the
+// block holds only the LDS it uses beside the kernels of the other streams.
 constexpr int kProdMaxCh = 64;
 __global__ __launch_bounds__(64) void k_prod_fun(const double* __restrict__ ppart,
                                                  const int32_t* __restrict__ gfirst,
@@ -2905,7 +2907,7 @@ __global__ __launch_bounds__(64) void k_prod_fun(const double* __restrict__ ppar
                                                  double* __restrict__ ring, int nc, int ne, int na,
                                                  int n2, int nap0, int n2p0, int n2p, int nomb,
                                                  int rc0, int nrcp, int64_t L) {
-  __shared__ double f[kProdMaxCh][64];
+  extern __shared__ double f[];  // [nc][64]
   const int lane = threadIdx.x;
   const int a = blockIdx.x / nrcp, rc = rc0 + blockIdx.x % nrcp;
   const int ia = nap0 + a;
@@ -2915,7 +2917,7 @@ __global__ __launch_bounds__(64) void k_prod_fun(const double* __restrict__ ppar
     const int g = (ch * na + ia) * nrc + rc;
     double s = 0.0;
     for (int it = gfirst[g]; it < gfirst[g + 1]; it++) s += ppart[(int64_t)it * 64 + lane];
-    f[ch][lane] = s;
+    f[ch * 64 + lane] = s;
   }
   if (R < n2p0 || R >= n2p0 + n2p) return;
   const int64_t l = (int64_t)a * n2p + (R - n2p0);
@@ -2923,7 +2925,7 @@ __global__ __launch_bounds__(64) void k_prod_fun(const double* __restrict__ ppar
     for (int e = 0; e < ne; e++) {
       double acc = 0.0;
       for (int ch = 0; ch < nc; ch++)
-        if (ch_elec[ch] == e) acc += f[ch][lane] * dtrans[((int64_t)ch * nomb + o) * L + l];
+        if (ch_elec[ch] == e) acc += f[ch * 64 + lane] * dtrans[((int64_t)ch * nomb + o) * L + l];
       ring[((int64_t)o * ne + e) * L + l] = acc;
     }
 }
@@ -2947,7 +2949,9 @@ hipError_t launch_prod_step(Ctx* c, int slot, hipStream_t s) {
   const int rc0 = c->pr_n2p0 / 64, rc1 = (c->pr_n2p0 + c->pr_n2p - 1) / 64;
   const int nrcp = rc1 - rc0 + 1;
   timer_begin(c, K_PROD, s);
-  hipLaunchKernelGGL(k_prod_fun, dim3((unsigned)(c->pr_nap * nrcp)), dim3(64), 0, s, c->d_ppart,
+  if (t.nc > kProdMaxCh) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_prod_fun, dim3((unsigned)(c->pr_nap * nrcp)), dim3(64),
+                     sizeof(double) * 64 * t.nc, s, c->d_ppart,
                      c->d_uw_gfirst, c->d_dtrans, c->d_ch_elec,
                      c->d_ring_prod + (int64_t)slot * c->pr_nomb * t.ne * c->pr_L, t.nc, t.ne,
                      t.na, t.n2, c->pr_nap0, c->pr_n2p0, c->pr_n2p, c->pr_nomb, rc0, nrcp, c->pr_L);
