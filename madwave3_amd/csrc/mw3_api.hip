@@ -23,8 +23,11 @@ void timer_begin(Ctx* c, int id, hipStream_t s) {
     t = c->pool.back();
     c->pool.pop_back();
   } else {
-    (void)hipEventCreate(&t.a);
-    (void)hipEventCreate(&t.b);
+    // timing only: elapsed times are read after the streams synchronize, so
+    // the records need no system-scope fence (which would write back and
+    // invalidate the caches at every record, inside the timed work)
+    (void)hipEventCreateWithFlags(&t.a, hipEventDisableSystemFence);
+    (void)hipEventCreateWithFlags(&t.b, hipEventDisableSystemFence);
   }
   (void)hipEventRecord(t.a, s);
   c->pending.push_back({id, t});
