@@ -959,43 +959,64 @@ __global__ void k_wmat(double* __restrict__ w, const double* __restrict__ etot, 
 // real samples z[s][x] (zr, cvj) or complex (zdr); skip: per-x flag of the
 // reference state whose steps k < kmin are left out (Cvjflux_k).  A thread
 // owns one energy and EACC_X consecutive x, so each C_k(E) it loads serves
-// EACC_X samples; the samples of a step are the same for the whole block.
-constexpr int EACC_X = 16;
+// EACC_X samples.  The samples of EACC_S steps are the same for the whole
+// block: they are staged in LDS (zero where left out or past nx), so the
+// inner loop has no branch and reads them as broadcasts.  Each (x, E) still
+// sums its steps in order.
+constexpr int EACC_X = 16, EACC_S = 32;
 template <bool CPLX>
 __global__ __launch_bounds__(128) void k_eacc(const double* __restrict__ w,
                                               const double* __restrict__ z,
                                               double* __restrict__ acc, int64_t nx, int netot,
                                               int ns, int k0, const uint8_t* __restrict__ skip,
                                               int kmin, int64_t xbegin) {
+  __shared__ double zs[EACC_S][EACC_X][2];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool eok = e < netot;
+  const int ec = eok ? e : netot - 1;
   const int64_t x0 = xbegin + (int64_t)blockIdx.y * EACC_X;
-  if (e >= netot) return;
   double ar[EACC_X], ai[EACC_X];
-  bool sk[EACC_X];
 #pragma unroll
-  for (int j = 0; j < EACC_X; j++) {
-    ar[j] = ai[j] = 0.0;
-    sk[j] = skip && x0 + j < nx && skip[x0 + j];
-  }
-  for (int s = 0; s < ns; s++) {
-    const double wr = w[2 * ((int64_t)s * netot + e)], wi = w[2 * ((int64_t)s * netot + e) + 1];
-    const bool early = k0 + s < kmin;
+  for (int j = 0; j < EACC_X; j++) ar[j] = ai[j] = 0.0;
+  for (int s0 = 0; s0 < ns; s0 += EACC_S) {
+    const int nsl = ns - s0 < EACC_S ? ns - s0 : EACC_S;
+    __syncthreads();  // the previous pass no longer reads zs
+    for (int q = threadIdx.x; q < EACC_S * EACC_X; q += blockDim.x) {
+      const int sl = q / EACC_X, j = q - sl * EACC_X;
+      const int64_t x = x0 + j;
+      const int s = s0 + sl;
+      const bool off = sl >= nsl || x >= nx || (skip && skip[x] && k0 + s < kmin);
+      double zr = 0.0, zi = 0.0;
+      if (!off) {
+        if (CPLX) {
+          zr = z[2 * ((int64_t)s * nx + x)];
+          zi = z[2 * ((int64_t)s * nx + x) + 1];
+        } else {
+          zr = z[(int64_t)s * nx + x];
+        }
+      }
+      zs[sl][j][0] = zr;
+      zs[sl][j][1] = zi;
+    }
+    __syncthreads();
+    for (int sl = 0; sl < nsl; sl++) {
+      const int s = s0 + sl;
+      const double wr = w[2 * ((int64_t)s * netot + ec)], wi = w[2 * ((int64_t)s * netot + ec) + 1];
 #pragma unroll
-    for (int j = 0; j < EACC_X; j++) {
-      const int64_t x = x0 + j < nx ? x0 + j : nx - 1;
-      const bool off = (sk[j] && early) || x0 + j >= nx;
-      if (CPLX) {
-        const double zr = off ? 0.0 : z[2 * ((int64_t)s * nx + x)];
-        const double zi = off ? 0.0 : z[2 * ((int64_t)s * nx + x) + 1];
-        ar[j] += wr * zr - wi * zi;
-        ai[j] += wr * zi + wi * zr;
-      } else {
-        const double zr = off ? 0.0 : z[(int64_t)s * nx + x];
-        ar[j] += wr * zr;
-        ai[j] += wi * zr;
+      for (int j = 0; j < EACC_X; j++) {
+        const double zr = zs[sl][j][0];
+        if (CPLX) {
+          const double zi = zs[sl][j][1];
+          ar[j] += wr * zr - wi * zi;
+          ai[j] += wr * zi + wi * zr;
+        } else {
+          ar[j] += wr * zr;
+          ai[j] += wi * zr;
+        }
       }
     }
   }
+  if (!eok) return;
 #pragma unroll
   for (int j = 0; j < EACC_X; j++)
     if (x0 + j < nx) {
