@@ -578,10 +578,14 @@ __global__ void k_fbr_coef(const T* __restrict__ fbr, T* __restrict__ phi,
 // one block: each thread takes an R column, loads the b_j(R) triples of all
 // NC channels once for the chunk, and per row reads the NC FBR values once
 // (each FBR value is read once instead of three times, and the coefficient
-// rows once per 16 rows instead of once per row).
-constexpr int FBR_RB = 16;
+// rows once per 16 rows instead of once per row).  The row's NC values also
+// go to the thread's own LDS column, from which the Omega +- 1 neighbours are
+// read by index (selects over every channel pair kept NC^2 compare masks in
+// scalar registers, which spilled from NC = 3 on); a thread reads back only
+// its own column, so no barrier is needed.
+constexpr int FBR_RB = 16, FBR_THREADS = 256;
 template <typename T, int NC>
-__global__ __launch_bounds__(256) void k_fbr_coef_all(const T* __restrict__ fbr,
+__global__ __launch_bounds__(FBR_THREADS) void k_fbr_coef_all(const T* __restrict__ fbr,
                                                       T* __restrict__ phi,
                                                       const double* __restrict__ acoef,
                                                       const double* __restrict__ bcoef,
@@ -591,6 +595,7 @@ __global__ __launch_bounds__(256) void k_fbr_coef_all(const T* __restrict__ fbr,
   const int jj = blockIdx.x / nrc, rc = blockIdx.x - jj * nrc;
   const int64_t plane = (int64_t)n1 * n2, cst = (int64_t)nJ * plane;
   const int r0 = rc * FBR_RB, r1 = r0 + FBR_RB < n1 ? r0 + FBR_RB : n1;
+  __shared__ double fl[NC][FBR_THREADS];
   for (int R = threadIdx.x; R < n2; R += blockDim.x) {
     double b0[NC], bm[NC], bp[NC];
 #pragma unroll
@@ -604,18 +609,18 @@ __global__ __launch_bounds__(256) void k_fbr_coef_all(const T* __restrict__ fbr,
     for (int r = r0; r < r1; r++) {
       double f[NC];
 #pragma unroll
-      for (int t = 0; t < NC; t++) f[t] = (double)fb[t * cst + (int64_t)r * n2];
+      for (int t = 0; t < NC; t++) {
+        f[t] = (double)fb[t * cst + (int64_t)r * n2];
+        fl[t][threadIdx.x] = f[t];
+      }
 #pragma unroll
       for (int t = 0; t < NC; t++) {
         double v = (acoef[((int64_t)t * nJ + jj) * n1 + r] + b0[t]) * f[t];
         const int sm = nbr[t * 3 + 0], sp = nbr[t * 3 + 2];  // uniform
-        double fm = 0.0, fp = 0.0;
-#pragma unroll
-        for (int s2 = 0; s2 < NC; s2++) {
-          fm = s2 == sm ? f[s2] : fm;
-          fp = s2 == sp ? f[s2] : fp;
-        }
-        v += bm[t] * fm + bp[t] * fp;  // bm / bp are zero where the neighbour is absent
+        // an absent neighbour reads the channel's own value: bm / bp are zero there
+        const double fm = fl[sm >= 0 ? sm : t][threadIdx.x];
+        const double fp = fl[sp >= 0 ? sp : t][threadIdx.x];
+        v += bm[t] * fm + bp[t] * fp;
         ob[t * cst + (int64_t)r * n2] = (T)v;
       }
     }
@@ -2656,7 +2661,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     switch (c->fbr_all ? t.nc : 0) {
 #define MW3_FBR(NC)                                                                              \
   case NC:                                                                                       \
-    hipLaunchKernelGGL((k_fbr_coef_all<T, NC>), dim3(nall), dim3(256), 0, sa,                   \
+    hipLaunchKernelGGL((k_fbr_coef_all<T, NC>), dim3(nall), dim3(FBR_THREADS), 0, sa,           \
                        (const T*)c->d_fbr, (T*)c->d_phi, (const double*)c->d_acoef,             \
                        (const double*)c->d_bcoef, c->d_nbr, t.nJ, t.n1, t.n2);                  \
     break;
