@@ -677,7 +677,7 @@ struct UpdArgs {
 
 
 // MODE 0: hout = H psi; 1: first step (undamped); 2: damped update.  SINGLE:
-// one channel and one state (V diagonal).  PROJ: flux columns and reactant
+// one electronic state (V diagonal; any number of Omega channels).  PROJ: flux columns and reactant
 // slice.  AUTO: <psi_0|psi_new> partials.  PRODP: product-projection partials.
 // All are template parameters so that no load sits behind a runtime branch
 // (the compiler waits for every load at the join of such a branch, which
@@ -2799,7 +2799,9 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   const bool autoc = a.psi0 != nullptr;
   const unsigned nblk4 = (unsigned)((nb + 3) / 4);  // one work item per wave
   if (nblk4 > 0) {
-    if (t.ne == 1 && t.nc == 1)
+    // one electronic state: V is diagonal (each Omega channel couples only to
+    // itself, through the one V slab), whatever the channel count
+    if (t.ne == 1)
       dispatch_update<T, true>(c, nblk4, psi, out, a, projk, autoc, prodp);
     else
       dispatch_update<T, false>(c, nblk4, psi, out, a, projk, autoc, prodp);
