@@ -114,7 +114,7 @@ static void free_all(Ctx* c) {
                   c->d_flux_dense, c->d_flux_col, c->d_zar1flux, c->d_ring_zr, c->d_ring_zdr,
                   c->d_ring_cvj, c->d_zcr, c->d_zcdr, c->d_zs2, c->d_etot, c->d_fdv, c->d_dcvj,
                   c->d_cvj_skip, c->d_norm_part, c->d_ring_norm, c->d_efe, c->d_zfe, c->d_phi,
-                  c->d_wmat, c->d_zero_skip, c->d_ch_elec, c->d_ch_omega, c->d_potiflux,
+                  c->d_wmat, c->d_zero_skip, c->d_ch_elec, c->d_ch_omega, c->d_ch_cpl, c->d_potiflux,
                   c->d_flux_elec, c->d_scratch, c->d_fmap, c->d_ir1flux, c->d_slice,
                   c->d_psi32, c->d_psim132, c->d_scr32, c->d_uw_row, c->d_uw_word,
                   c->d_uw_item, c->d_uw_gfirst, c->d_uw_part, c->d_tprod, c->d_ppart,
@@ -259,6 +259,22 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(upload_raw(&c->d_nbr, t.nbr.data(), t.nbr.size()), "mw3_create: tables");
   CK(upload_raw(&c->d_ch_elec, t.ch_elec.data(), t.ch_elec.size()), "mw3_create: tables");
   CK(upload_raw(&c->d_ch_omega, t.ch_omega.data(), t.ch_omega.size()), "mw3_create: tables");
+  {
+    std::vector<int32_t> cpl((size_t)4 * t.nc, -1);
+    bool fits = true;
+    for (int ch = 0; ch < t.nc && fits; ch++) {
+      int k = 0;
+      for (int s2 = 0; s2 < t.nc; s2++)
+        if (t.ch_omega[s2] == t.ch_omega[ch]) {
+          if (k == 4) {
+            fits = false;
+            break;
+          }
+          cpl[(size_t)4 * ch + k++] = s2;
+        }
+    }
+    if (fits) CK(upload_raw(&c->d_ch_cpl, cpl.data(), cpl.size()), "mw3_create: tables");
+  }
   // projections
   int netot = std::max(setup->netot, 0);
   CK(upload_raw(&c->d_flux_dense, t.flux_dense.data(), t.flux_dense.size()), "mw3_create: flux");

@@ -225,6 +225,9 @@ struct Ctx {
   uint8_t* d_zero_skip = nullptr;  // [nflux] all ones (kminelastic zeroing)
   int32_t* d_ch_elec = nullptr;
   int32_t* d_ch_omega = nullptr;
+  // [nc][4] the channels of each channel's Omega (itself included) in
+  // ascending order, -1 past their count; null when some Omega has more than 4
+  int32_t* d_ch_cpl = nullptr;
   double* d_potiflux = nullptr;
   int32_t* d_flux_elec = nullptr;
 

@@ -649,6 +649,7 @@ struct UpdArgs {
   const double* absfr2;
   const int32_t* ch_elec;
   const int32_t* ch_omega;
+  const int32_t* ch_cpl;  // [nc][4] coupled channels (CPL4 kernels)
   const double* psi0;  // autocorrelation reference (null: none)
   double* part;        // per-block (norm, autocorrelation) partials
   int nc, ne, na, n1, n2;
@@ -682,7 +683,10 @@ struct UpdArgs {
 // All are template parameters so that no load sits behind a runtime branch
 // (the compiler waits for every load at the join of such a branch, which
 // serialises the batch).
-template <typename TT, typename TV, int MODE, bool SINGLE, bool PROJ, bool AUTO, bool PRODP>
+// CPL4 (several states): at most 4 channels share an Omega; their V and psi
+// values are loaded with the row's other loads, from the host-built list.
+template <typename TT, typename TV, int MODE, bool SINGLE, bool PROJ, bool AUTO, bool PRODP,
+          bool CPL4 = false>
 __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
                                                double* __restrict__ out_buf,
                                                const TT* __restrict__ tkin,
@@ -709,11 +713,25 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
   constexpr bool P32 = MODE == 2 && sizeof(TT) == 4;  // psi_k from its FP32 copy
   // the item's rows, four at a time: the loads of the four rows and their
   // per-row tables are issued before any is used, with no branch around them
+  constexpr int NK = CPL4 ? 4 : 1;
+  // coupled channels: V and psi offsets and the 0 / 1 weight (entries past
+  // the count repeat the channel itself with weight 0), once per wave
+  int64_t cvo[NK], cpo[NK];
+  double cw[NK];
+#pragma unroll
+  for (int k = 0; k < NK; k++) {
+    const int s2r = CPL4 ? a.ch_cpl[4 * c + k] : c;
+    const int s2 = s2r >= 0 ? s2r : c;
+    cvo[k] = CPL4 ? ((int64_t)et * a.ne + a.ch_elec[s2]) * slab : 0;
+    cpo[k] = (int64_t)s2 * slab;
+    cw[k] = s2r >= 0 ? 1.0 : 0.0;
+  }
   for (int b = 0; b < cnt; b += 4) {
     bool act[4];
     int rr[4];
     int64_t pl[4];
     double p[4], ob[4], tk[4], an[4], vv[4], f1[4], za[4], zb[4], p0[4], tp[4];
+    double cv[4][NK], cp[4][NK];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const int idx = b + u < cnt ? b + u : cnt - 1;
@@ -735,6 +753,13 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
       zb[u] = PROJ ? a.zar[2 * r + 1] : 0.0;
       p0[u] = AUTO ? a.psi0[ie] : 0.0;
       tp[u] = PRODP ? (double)tprod[pl[u]] : 0.0;
+      if (CPL4) {
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+          cv[u][k] = (double)vd[cvo[k] + pl[u]];
+          cp[u][k] = P32 ? (double)a.psi32[cpo[k] + pl[u]] : psi[cpo[k] + pl[u]];
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
@@ -745,6 +770,9 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
       double h = (r < nn ? a.radcut * p[u] - tk[u] : p[u]) + an[u];
       if (SINGLE) {
         h += p[u] * vv[u];
+      } else if (CPL4) {
+#pragma unroll
+        for (int k = 0; k < NK; k++) h += cp[u][k] * (cv[u][k] * cw[k]);
       } else {
         for (int s2 = 0; s2 < a.nc; s2++) {
           if (a.ch_omega[s2] != om) continue;
@@ -828,32 +856,33 @@ __global__ __launch_bounds__(64) void k_update_flux(const double* __restrict__ u
   }
 }
 
-template <typename T, int MODE, bool SINGLE, bool PROJ, bool AUTO, bool PRODP = false>
+template <typename T, int MODE, bool SINGLE, bool PROJ, bool AUTO, bool PRODP = false,
+          bool CPL4 = false>
 static void launch_update_kernel(Ctx* c, unsigned nb, const double* psi, double* out,
                                  const UpdArgs& a) {
-  hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO, PRODP>), dim3(nb), dim3(256), 0,
+  hipLaunchKernelGGL((k_update<T, T, MODE, SINGLE, PROJ, AUTO, PRODP, CPL4>), dim3(nb), dim3(256), 0,
                      c->stream, psi, out, (const T*)c->d_t2, (const T*)c->d_ang,
                      (const T*)c->d_v, a);
 }
 
 // prodp (product projection) runs with the projection variant (PROJ) only
-template <typename T, bool SINGLE>
+template <typename T, bool SINGLE, bool CPL4>
 static void dispatch_update(Ctx* c, unsigned nb, const double* psi, double* out, const UpdArgs& a,
                             bool proj, bool autoc, bool prodp) {
   if (a.mode == 0)
-    launch_update_kernel<T, 0, SINGLE, false, false>(c, nb, psi, out, a);
+    launch_update_kernel<T, 0, SINGLE, false, false, false, CPL4>(c, nb, psi, out, a);
   else if (a.mode == 1)
-    autoc ? launch_update_kernel<T, 1, SINGLE, false, true>(c, nb, psi, out, a)
-          : launch_update_kernel<T, 1, SINGLE, false, false>(c, nb, psi, out, a);
+    autoc ? launch_update_kernel<T, 1, SINGLE, false, true, false, CPL4>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 1, SINGLE, false, false, false, CPL4>(c, nb, psi, out, a);
   else if (prodp)
-    autoc ? launch_update_kernel<T, 2, SINGLE, true, true, true>(c, nb, psi, out, a)
-          : launch_update_kernel<T, 2, SINGLE, true, false, true>(c, nb, psi, out, a);
+    autoc ? launch_update_kernel<T, 2, SINGLE, true, true, true, CPL4>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 2, SINGLE, true, false, true, CPL4>(c, nb, psi, out, a);
   else if (proj)
-    autoc ? launch_update_kernel<T, 2, SINGLE, true, true>(c, nb, psi, out, a)
-          : launch_update_kernel<T, 2, SINGLE, true, false>(c, nb, psi, out, a);
+    autoc ? launch_update_kernel<T, 2, SINGLE, true, true, false, CPL4>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 2, SINGLE, true, false, false, CPL4>(c, nb, psi, out, a);
   else
-    autoc ? launch_update_kernel<T, 2, SINGLE, false, true>(c, nb, psi, out, a)
-          : launch_update_kernel<T, 2, SINGLE, false, false>(c, nb, psi, out, a);
+    autoc ? launch_update_kernel<T, 2, SINGLE, false, true, false, CPL4>(c, nb, psi, out, a)
+          : launch_update_kernel<T, 2, SINGLE, false, false, false, CPL4>(c, nb, psi, out, a);
 }
 
 __global__ void k_reduce_parts(const double* __restrict__ part, int nb, double* __restrict__ out) {
@@ -2763,6 +2792,7 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
   a.absfr2 = c->d_absfr2;
   a.ch_elec = c->d_ch_elec;
   a.ch_omega = c->d_ch_omega;
+  a.ch_cpl = c->d_ch_cpl;
   a.psi0 = (c->s.iphoto >= 1) ? c->d_psi0 : nullptr;
   a.part = mode != 0 ? c->d_norm_part : nullptr;
   a.nc = t.nc;
@@ -2802,9 +2832,11 @@ static hipError_t update_impl(Ctx* c, int mode, int slot, const double* psi, boo
     // one electronic state: V is diagonal (each Omega channel couples only to
     // itself, through the one V slab), whatever the channel count
     if (t.ne == 1)
-      dispatch_update<T, true>(c, nblk4, psi, out, a, projk, autoc, prodp);
+      dispatch_update<T, true, false>(c, nblk4, psi, out, a, projk, autoc, prodp);
+    else if (c->d_ch_cpl)  // several states, at most 4 channels per Omega
+      dispatch_update<T, false, true>(c, nblk4, psi, out, a, projk, autoc, prodp);
     else
-      dispatch_update<T, false>(c, nblk4, psi, out, a, projk, autoc, prodp);
+      dispatch_update<T, false, false>(c, nblk4, psi, out, a, projk, autoc, prodp);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
