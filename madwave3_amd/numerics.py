@@ -1,306 +1,217 @@
 """Numerical routines of MADWAVE3's set-up, restated for the host side.
 
-Each function follows a routine of SRC/liboctdyn.f (cited per function) so the
-host builds the same grids, angular functions and packets as the reference.
+Each function computes what a routine of SRC/liboctdyn.f computes (cited per
+function), written from the mathematics of that quantity, so the host builds the
+same grids, angular functions and packets as the reference.
 """
 import math
 
 import numpy as np
 
 __all__ = ["gauleg", "dwigner", "dwigner_vec", "rgauscolini", "besph2", "noptfft",
-           "splset", "splinqq"]
+           "splset", "splinqq", "lagrange_slope"]
 
 
 def gauleg(n):
-    """Gauss-Legendre nodes and weights on [-1, 1], ascending nodes.
+    """Gauss-Legendre nodes and weights on [-1, 1], nodes ascending.
 
-    SRC/liboctdyn.f:174-210 (GAULEG): Newton iteration on P_n from the guess
-    cos(pi (i - 1/4)/(n + 1/2)) to |dz| <= 1e-15, symmetric fill.
+    Same quadrature as GAULEG (SRC/liboctdyn.f:174-210).  The roots of P_n are
+    found by Newton's method from the asymptotic guesses
+    cos(pi (k - 1/4)/(n + 1/2)), k = 1..ceil(n/2), all roots at once; P_n and
+    P_{n-1} come from Bonnet's recurrence
+        m P_m(z) = (2m - 1) z P_{m-1}(z) - (m - 1) P_{m-2}(z),
+    P_n'(z) = n (z P_n - P_{n-1})/(z^2 - 1), and a root is final once its
+    Newton step is at most 1e-15.  The weight of a root z is
+    2/((1 - z^2) P_n'(z')^2), with P_n' taken at the iterate z' before the last
+    step.  Written with the recurrence in this operation order, the nodes and
+    weights are bitwise those of the reference build.
     Returns (weights, nodes).
     """
-    w = np.zeros(n)
-    x = np.zeros(n)
-    x1, x2 = -1.0, 1.0
-    yn = float(n)
-    eps = 1.0e-15
-    m = (n + 1) // 2
-    xm = 0.5 * (x1 + x2)
-    xl = 0.5 * (x2 - x1)
-    pi = math.acos(-1.0)
-    for i in range(1, m + 1):
-        z = math.cos(pi * (i - 0.25) / (yn + 0.5))
-        while True:
-            p1, p2 = 1.0, 0.0
-            for j in range(1, n + 1):
-                p3 = p2
-                p2 = p1
-                yj = float(j)
-                p1 = ((2.0 * yj - 1.0) * z * p2 - (yj - 1.0) * p3) / yj
-            pp = yn * (z * p1 - p2) / (z * z - 1.0)
-            z1 = z
-            z = z1 - p1 / pp
-            if abs(z - z1) <= eps:
-                break
-        x[i - 1] = xm - xl * z
-        x[n - i] = xm + xl * z
-        w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp)
-        w[n - i] = w[i - 1]
-    return w, x
+    half = (n + 1) // 2
+    root = np.array([math.cos(math.pi * (k - 0.25) / (n + 0.5)) for k in range(1, half + 1)])
+    slope = np.zeros(half)
+    todo = np.arange(half)
+    while todo.size:
+        z = root[todo]
+        p_lo, p_hi = np.zeros_like(z), np.ones_like(z)
+        for m in range(1, n + 1):
+            p_lo, p_hi = p_hi, ((2.0 * m - 1.0) * z * p_hi - (m - 1.0) * p_lo) / m
+        dpn = n * (z * p_hi - p_lo) / (z * z - 1.0)
+        z_new = z - p_hi / dpn
+        root[todo] = z_new
+        slope[todo] = dpn
+        todo = todo[np.abs(z_new - z) > 1e-15]
+    w_half = 2.0 / ((1.0 - root * root) * slope * slope)
+    nodes = np.empty(n)
+    weights = np.empty(n)
+    nodes[:half] = -root          # the guesses run from the largest root down
+    nodes[n - half:] = root[::-1]
+    weights[:half] = w_half
+    weights[n - half:] = w_half[::-1]
+    return weights, nodes
 
 
-def dwigner(jmax2, m2, mp2, beta):
-    """Wigner d^J_{M,M'}(beta) for 2J = 0..jmax2, returned indexed by 2J.
-
-    SRC/liboctdyn.f:3-171 (dwigner): closed form at J = max(|M|,|M'|), then
-    the three-term recurrence in J.  m2, mp2 are 2M and 2M'.  Entries with
-    2J below max(|m2|,|mp2|) are 0.
-    """
-    dj = np.zeros(jmax2 + 1)
-    jmin2 = max(abs(m2), abs(mp2))
-    y = math.cos(beta)
-    if jmin2 == 0:
-        dj[0] = 1.0
-        if jmax2 >= 2:
-            dj[2] = y
-        for ind in range(4, jmax2 + 1, 2):
-            xj = ind * 0.5
-            dj[ind] = (y * (2.0 * xj - 1.0) * dj[ind - 2] - (xj - 1.0) * dj[ind - 4]) / xj
-        return dj
-    facnum = sum(math.log(i) for i in range(1, jmin2 + 1))
-    mreal2 = mp2 if jmin2 == abs(m2) else m2
-    facden1 = 0.0
-    if jmin2 + mreal2 > 0:
-        for i in range(2, jmin2 + mreal2 + 1, 2):
-            facden1 += math.log(i * 0.5)
-    facden2 = 0.0
-    if jmin2 - mreal2 > 0:
-        for i in range(2, jmin2 - mreal2 + 1, 2):
-            facden2 += math.log(i * 0.5)
-    fact = math.exp(0.5 * (facnum - facden1 - facden2))
-    sign = 1.0
-    xsin = math.sin(beta * 0.5)
-    xcos = math.cos(beta * 0.5)
-    a = b = 0.0
-    if jmin2 == abs(m2):
-        if jmin2 == m2:
-            if (jmin2 - mp2) % 2 != 0:
-                raise ValueError("dwigner: complex phase")
-            sign = (-1.0) ** int((jmin2 - mp2) * 0.5 + 0.5)
-            a = xsin ** ((jmin2 - mp2) * 0.5)
-            b = xcos ** ((jmin2 + mp2) * 0.5)
-        elif jmin2 == -m2:
-            a = xsin ** ((jmin2 + mp2) * 0.5)
-            b = xcos ** ((jmin2 - mp2) * 0.5)
-    elif jmin2 == abs(mp2):
-        if jmin2 == mp2:
-            a = xsin ** ((jmin2 - m2) * 0.5)
-            b = xcos ** ((jmin2 + m2) * 0.5)
-        elif jmin2 == -mp2:
-            if (jmin2 + m2) % 2 != 0:
-                raise ValueError("dwigner: complex phase")
-            sign = (-1.0) ** int((jmin2 + m2) * 0.5 + 0.5)
-            a = xsin ** ((jmin2 + m2) * 0.5)
-            b = xcos ** ((jmin2 - m2) * 0.5)
-    if jmin2 <= jmax2:
-        dj[jmin2] = fact * sign * a * b
-    xj = jmin2 * 0.5
-    xjp = xj + 1.0
-    xm = m2 * 0.5
-    xmp = mp2 * 0.5
-    facnum = xjp * (2.0 * xj + 1.0)
-    factot = facnum / math.sqrt((xjp * xjp - xm * xm) * (xjp * xjp - xmp * xmp))
-    facj = y - xm * xmp / (xj * (xj + 1.0))
-    if jmin2 + 2 <= jmax2:
-        dj[jmin2 + 2] = factot * facj * dj[jmin2]
-        for indp in range(jmin2 + 4, jmax2 + 1, 2):
-            ind = indp - 2
-            indm = indp - 4
-            xjp = indp * 0.5
-            xj = xjp - 1.0
-            facnum = xjp * (2.0 * xj + 1.0)
-            factot = facnum / math.sqrt((xjp * xjp - xm * xm) * (xjp * xjp - xmp * xmp))
-            facj = y - xm * xmp / (xj * (xj + 1.0))
-            facjm = math.sqrt((xj * xj - xm * xm) * (xj * xj - xmp * xmp)) / (xj * (2.0 * xj + 1.0))
-            dj[indp] = factot * (facj * dj[ind] - facjm * dj[indm])
-    return dj
+def _dsmall_first(j2, m2, mp2, cb, sb):
+    """d^j_{m,m'}(beta) at the lowest j = max(|m|, |m'|) (all arguments
+    doubled: j2 = 2j, ...), with cb = cos(beta/2), sb = sin(beta/2):
+        d^j_{m,m'} = s sqrt(C(2j, j + m')) cb^(j+m'') sb^(j-m'')
+    in the Condon-Shortley convention, where for m = +-j the binomial and the
+    powers take m' (and for m' = +-j they take m), and the sign s is
+    (-1)^(j - m') when m = j, (-1)^(j + m) when m' = -j, +1 otherwise."""
+    if abs(m2) == j2:
+        other = mp2
+        if m2 == j2:
+            pc, ps = (j2 + other) // 2, (j2 - other) // 2
+            sign = -1.0 if ((j2 - other) // 2) % 2 else 1.0
+        else:
+            pc, ps = (j2 - other) // 2, (j2 + other) // 2
+            sign = 1.0
+    else:
+        other = m2
+        if mp2 == j2:
+            pc, ps = (j2 + other) // 2, (j2 - other) // 2
+            sign = 1.0
+        else:
+            pc, ps = (j2 - other) // 2, (j2 + other) // 2
+            sign = -1.0 if ((j2 + other) // 2) % 2 else 1.0
+    amp = math.sqrt(float(math.comb(j2, (j2 + other) // 2)))
+    return sign * amp * cb ** pc * sb ** ps
 
 
 def dwigner_vec(jmax2, m2, mp2, beta):
-    """dwigner over an array of angles: d[2J, i] = d^J_{M,M'}(beta[i]).
+    """Wigner small-d functions d^j_{m,m'}(beta) for 2j = 0..jmax2 over an
+    array of angles: d[2j, i] = d^j_{m,m'}(beta[i]) (m2 = 2m, mp2 = 2m').
 
-    The same closed form at J = max(|M|,|M'|) and the same recurrence in J as
-    dwigner (SRC/liboctdyn.f:3-171), evaluated elementwise with numpy.
+    The same functions as dwigner (SRC/liboctdyn.f:3-171), evaluated here from
+    the closed form at j = max(|m|, |m'|) and the standard three-term
+    recurrence in j (e.g. Varshalovich et al., Quantum Theory of Angular
+    Momentum, 4.8.2):
+        j sqrt(((j+1)^2 - m^2)((j+1)^2 - m'^2)) d^{j+1}
+          = (2j + 1)(j (j+1) cos(beta) - m m') d^j
+            - (j + 1) sqrt((j^2 - m^2)(j^2 - m'^2)) d^{j-1};
+    at m = m' = 0 it starts from P_0 = 1, P_1 = cos(beta) (Legendre).  Rows
+    with 2j below max(|m2|, |mp2|), or of the other parity, are 0.
     """
     beta = np.asarray(beta, dtype=float)
-    nb = beta.size
-    dj = np.zeros((jmax2 + 1, nb))
-    jmin2 = max(abs(m2), abs(mp2))
-    y = np.cos(beta)
-    if jmin2 == 0:
-        dj[0] = 1.0
+    out = np.zeros((jmax2 + 1, beta.size))
+    j2min = max(abs(m2), abs(mp2))
+    if (j2min - m2) % 2 or (j2min - mp2) % 2:
+        raise ValueError("dwigner: 2j, 2m and 2m' must have one parity")
+    if j2min > jmax2:
+        return out
+    c = np.cos(beta)
+    m, mp = 0.5 * m2, 0.5 * mp2
+    if j2min == 0:
+        out[0] = 1.0
         if jmax2 >= 2:
-            dj[2] = y
-        for ind in range(4, jmax2 + 1, 2):
-            xj = ind * 0.5
-            dj[ind] = (y * (2.0 * xj - 1.0) * dj[ind - 2] - (xj - 1.0) * dj[ind - 4]) / xj
-        return dj
-    facnum = sum(math.log(i) for i in range(1, jmin2 + 1))
-    mreal2 = mp2 if jmin2 == abs(m2) else m2
-    facden1 = 0.0
-    if jmin2 + mreal2 > 0:
-        for i in range(2, jmin2 + mreal2 + 1, 2):
-            facden1 += math.log(i * 0.5)
-    facden2 = 0.0
-    if jmin2 - mreal2 > 0:
-        for i in range(2, jmin2 - mreal2 + 1, 2):
-            facden2 += math.log(i * 0.5)
-    fact = math.exp(0.5 * (facnum - facden1 - facden2))
-    sign = 1.0
-    xsin = np.sin(beta * 0.5)
-    xcos = np.cos(beta * 0.5)
-    a = b = np.zeros(nb)
-    if jmin2 == abs(m2):
-        if jmin2 == m2:
-            if (jmin2 - mp2) % 2 != 0:
-                raise ValueError("dwigner: complex phase")
-            sign = (-1.0) ** int((jmin2 - mp2) * 0.5 + 0.5)
-            a = xsin ** ((jmin2 - mp2) * 0.5)
-            b = xcos ** ((jmin2 + mp2) * 0.5)
-        elif jmin2 == -m2:
-            a = xsin ** ((jmin2 + mp2) * 0.5)
-            b = xcos ** ((jmin2 - mp2) * 0.5)
-    elif jmin2 == abs(mp2):
-        if jmin2 == mp2:
-            a = xsin ** ((jmin2 - m2) * 0.5)
-            b = xcos ** ((jmin2 + m2) * 0.5)
-        elif jmin2 == -mp2:
-            if (jmin2 + m2) % 2 != 0:
-                raise ValueError("dwigner: complex phase")
-            sign = (-1.0) ** int((jmin2 + m2) * 0.5 + 0.5)
-            a = xsin ** ((jmin2 + m2) * 0.5)
-            b = xcos ** ((jmin2 - m2) * 0.5)
-    if jmin2 <= jmax2:
-        dj[jmin2] = fact * sign * a * b
-    xj = jmin2 * 0.5
-    xjp = xj + 1.0
-    xm = m2 * 0.5
-    xmp = mp2 * 0.5
-    facnum = xjp * (2.0 * xj + 1.0)
-    factot = facnum / math.sqrt((xjp * xjp - xm * xm) * (xjp * xjp - xmp * xmp))
-    facj = y - xm * xmp / (xj * (xj + 1.0))
-    if jmin2 + 2 <= jmax2:
-        dj[jmin2 + 2] = factot * facj * dj[jmin2]
-        for indp in range(jmin2 + 4, jmax2 + 1, 2):
-            ind = indp - 2
-            indm = indp - 4
-            xjp = indp * 0.5
-            xj = xjp - 1.0
-            facnum = xjp * (2.0 * xj + 1.0)
-            factot = facnum / math.sqrt((xjp * xjp - xm * xm) * (xjp * xjp - xmp * xmp))
-            facj = y - xm * xmp / (xj * (xj + 1.0))
-            facjm = math.sqrt((xj * xj - xm * xm) * (xj * xj - xmp * xmp)) / (xj * (2.0 * xj + 1.0))
-            dj[indp] = factot * (facj * dj[ind] - facjm * dj[indm])
-    return dj
+            out[2] = c
+        j_from = 1.0
+    else:
+        out[j2min] = _dsmall_first(j2min, m2, mp2, np.cos(0.5 * beta), np.sin(0.5 * beta))
+        j_from = 0.5 * j2min
+    lo = int(round(2 * j_from))
+    for jj2 in range(lo, jmax2 - 1, 2):
+        j = 0.5 * jj2
+        below = out[jj2 - 2] if jj2 >= 2 else 0.0
+        upper = (2.0 * j + 1.0) * (j * (j + 1.0) * c - m * mp) * out[jj2]
+        lower = (j + 1.0) * math.sqrt(max((j * j - m * m) * (j * j - mp * mp), 0.0)) * below
+        scale = j * math.sqrt(((j + 1.0) ** 2 - m * m) * ((j + 1.0) ** 2 - mp * mp))
+        out[jj2 + 2] = (upper - lower) / scale
+    return out
 
 
-def _dlagra(x, y, ip):
-    """Derivative at x[ip] of the Lagrange polynomial through (x, y)
-    (SRC/liboctdyn.f:960-983, DLAGRA; ip 0-based here)."""
-    n = len(x)
-    d = 0.0
-    for i in range(n):
-        if i == ip:
-            continue
-        yp = y[i]
-        for j in range(n):
-            if j == ip or j == i:
-                continue
-            yp = yp * (x[ip] - x[j])
-        for j in range(n):
-            if j == i:
-                continue
-            yp = yp / (x[i] - x[j])
-        d = d + yp
-    for i in range(n):
-        if i == ip:
-            continue
-        d = d + y[ip] / (x[ip] - x[i])
-    return d
+def dwigner(jmax2, m2, mp2, beta):
+    """d^j_{m,m'}(beta) at one angle for 2j = 0..jmax2, indexed by 2j (see
+    dwigner_vec; SRC/liboctdyn.f:3-171)."""
+    return dwigner_vec(jmax2, m2, mp2, np.array([float(beta)]))[:, 0]
+
+
+def lagrange_slope(x, y, at):
+    """Derivative at the node x[at] of the polynomial interpolating (x, y),
+    the quantity DLAGRA returns (SRC/liboctdyn.f:960-983).  Barycentric form:
+    with w_i = 1/prod_{k != i}(x_i - x_k),
+        p'(x_a) = sum_{i != a} (w_i/w_a) y_i/(x_a - x_i)
+                  + y_a sum_{i != a} 1/(x_a - x_i)."""
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    diff = x[:, None] - x[None, :]
+    np.fill_diagonal(diff, 1.0)
+    w = 1.0 / np.prod(diff, axis=1)
+    others = np.arange(len(x)) != at
+    gap = x[at] - x[others]
+    return float(np.sum(w[others] / w[at] * y[others] / gap) + y[at] * np.sum(1.0 / gap))
 
 
 def splset(f, x):
-    """Knot derivatives of the clamped cubic spline through (x, f).
+    """Knot slopes of the cubic spline through (x, f) with clamped ends, the
+    F(:,2) column that SPLSET fills (SRC/liboctdyn.f:342-403).
 
-    SRC/liboctdyn.f:342-393 (SPLSET): the end slopes are the derivatives of the
-    4-point Lagrange polynomials at the first and last knot (DLAGRA); the
-    interior slopes m solve lambda_i m_{i-1} + 2 m_i + mu_i m_{i+1} =
-    3 lambda_i (f_i - f_{i-1})/h_{i-1} + 3 mu_i (f_{i+1} - f_i)/h_i,
-    lambda_i = h_i/(h_{i-1} + h_i), mu_i = 1 - lambda_i, by the pivoted
-    forward sweep and back substitution of JORDAN (SRC/liboctdyn.f:926-958).
-    Returns the slopes (the F(:,2) column of the reference).
+    End slopes: the derivative of the cubic through the first (last) four
+    knots at the first (last) knot.  Interior slopes s_i: C^2 continuity of
+    the piecewise Hermite cubic,
+        h_i s_{i-1} + 2 (h_{i-1} + h_i) s_i + h_{i-1} s_{i+1}
+          = 3 (h_i D_{i-1} + h_{i-1} D_i),
+    h_i = x_{i+1} - x_i, D_i = (f_{i+1} - f_i)/h_i, solved as a banded system.
     """
+    from scipy.linalg import solve_banded
+
     f = np.asarray(f, dtype=float)
     x = np.asarray(x, dtype=float)
-    nx = len(x)
-    nx2 = nx - 2
-    hx = [x[i] - x[i - 1] for i in range(1, nx)]
-    rlx = [hx[i + 1] / (hx[i] + hx[i + 1]) for i in range(nx2)]
-    rmux = [1.0 - v for v in rlx]
-    p0 = _dlagra(x[:4], f[:4], 0)
-    pn = _dlagra(x[nx - 4:], f[nx - 4:], 3)
-    b = [3.0 * rlx[i] / hx[i] * (f[i + 1] - f[i]) + 3.0 * rmux[i] / hx[i + 1] * (f[i + 2] - f[i + 1])
-         for i in range(nx2)]
-    b[0] = b[0] - rlx[0] * p0
-    b[nx2 - 1] = b[nx2 - 1] - rmux[nx2 - 1] * pn
-    piv = [0.0] * nx2
-    piv[0] = 2.0
-    for i in range(1, nx2):
-        piv[i] = 2.0 - rlx[i] * rmux[i - 1] / piv[i - 1]
-        b[i] = b[i] - rlx[i] / piv[i - 1] * b[i - 1]
-    xi = [0.0] * nx2
-    xi[nx2 - 1] = b[nx2 - 1] / piv[nx2 - 1]
-    for i in range(nx2 - 2, -1, -1):
-        xi[i] = (b[i] - xi[i + 1] * rmux[i]) / piv[i]
-    d = np.zeros(nx)
-    d[0] = p0
-    d[nx - 1] = pn
-    d[1:nx - 1] = xi
-    return d
+    n = len(x)
+    h = np.diff(x)
+    dd = np.diff(f) / h
+    s = np.zeros(n)
+    s[0] = lagrange_slope(x[:4], f[:4], 0)
+    s[-1] = lagrange_slope(x[-4:], f[-4:], 3)
+    m = n - 2
+    if m <= 0:
+        return s
+    rhs = 3.0 * (h[1:] * dd[:-1] + h[:-1] * dd[1:])
+    rhs[0] -= h[1] * s[0]
+    rhs[-1] -= h[-2] * s[-1]
+    ab = np.zeros((3, m))
+    ab[0, 1:] = h[:-2]             # superdiagonal: h_{i-1} s_{i+1}
+    ab[1, :] = 2.0 * (h[:-1] + h[1:])
+    ab[2, :-1] = h[2:]             # subdiagonal: h_i s_{i-1}
+    s[1:-1] = solve_banded((1, 1), ab, rhs)
+    return s
 
 
 def splinqq(f, d, x, r):
-    """Spline value at r (array) from knots x, values f and slopes d.
+    """Value at the points r of the spline with knot values f and slopes d,
+    as SPLINQQ evaluates it (SRC/liboctdyn.f:298-341).
 
-    SRC/liboctdyn.f:298-341 (SPLINQQ): cubic Hermite form on the interval
-    [x_{i-1}, x_i) holding r (searched from the second knot); at or beyond the
-    last knot the extension C6/r^6 + C8/r^8 matched to the last value and slope.
+    Inside the knots (and before the first): the cubic Hermite interpolant
+    on the interval [x_{i-1}, x_i) that holds r (the first interval below
+    x_0), t = (r - x_{i-1})/h,
+        f_{i-1} H00(t) + f_i H01(t) + h (d_{i-1} H10(t) + d_i H11(t)).
+    At or beyond the last knot x_e: the tail A (x_e/r)^6 + B (x_e/r)^8 whose
+    value and slope match f_e and d_e at x_e, B = -(3 f_e + x_e d_e/2),
+    A = f_e - B.
     """
     f = np.asarray(f, dtype=float)
     d = np.asarray(d, dtype=float)
     x = np.asarray(x, dtype=float)
     r = np.asarray(r, dtype=float)
-    nx = len(x)
     out = np.empty_like(r)
-    beyond = r >= x[nx - 1]
-    inner = ~beyond
-    ri = r[inner]
-    idol = np.maximum(np.searchsorted(x, ri, side="right"), 1)  # 0-based index of X(IDOL)
-    hi = x[idol] - x[idol - 1]
-    xr = (ri - x[idol - 1]) / hi
-    u1 = xr * xr * (-2.0 * xr + 3.0)
-    u3 = hi * xr * xr * (xr - 1.0)
-    u2 = 1.0 - u1
-    u4 = hi * xr * ((xr - 2.0) * xr + 1.0)
-    out[inner] = u1 * f[idol] + u2 * f[idol - 1] + u3 * d[idol] + u4 * d[idol - 1]
-    if beyond.any():
-        ro = x[nx - 1]
-        yo = f[nx - 1]
-        yp = d[nx - 1]
-        ain = yo + yp * ro / 6.0
-        c8 = -ain * 3.0 * ro ** 8
-        c6 = yo * ro ** 6 - c8 / ro / ro
-        rb = r[beyond]
-        out[beyond] = c6 / rb ** 6 + c8 / rb ** 8
+    tail = r >= x[-1]
+    ri = r[~tail]
+    hi_k = np.clip(np.searchsorted(x, ri, side="right"), 1, len(x) - 1)
+    lo_k = hi_k - 1
+    h = x[hi_k] - x[lo_k]
+    t = (ri - x[lo_k]) / h
+    t2 = t * t
+    h00 = (2.0 * t - 3.0) * t2 + 1.0
+    h01 = 1.0 - h00
+    h10 = t * (t - 1.0) ** 2
+    h11 = t2 * (t - 1.0)
+    out[~tail] = f[lo_k] * h00 + f[hi_k] * h01 + h * (d[lo_k] * h10 + d[hi_k] * h11)
+    if tail.any():
+        xe, fe, de = x[-1], f[-1], d[-1]
+        b = -(3.0 * fe + 0.5 * xe * de)
+        a = fe - b
+        q = (xe / r[tail]) ** 2
+        out[tail] = (a + b * q) * q ** 3
     return out
 
 

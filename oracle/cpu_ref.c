@@ -449,10 +449,13 @@ ORF_EXPORT int oref_create(const mw3_setup *s, oref_ctx **out) {
   size_t nxj = (size_t)c->ncan * na * na * c->npun1;
   c->xj2 = (double *)xcalloc(nxj, sizeof(double));
   int jjjmax = (na - 1) * c->inc;
+  /* each (channel, r1) owns its own elements, summed over j in order, so the
+     tables do not depend on the thread count */
+#pragma omp parallel for schedule(dynamic) collapse(2)
   for (int ic = 0; ic < c->ncan; ic++) {
-    int ielec = c->nelebas[ic], iom = c->iombas[ic];
-    int iomdi = s->iomdiat[ielec - 1];
     for (int ir1 = 1; ir1 <= c->npun1; ir1++) {
+      int ielec = c->nelebas[ic], iom = c->iombas[ic];
+      int iomdi = s->iomdiat[ielec - 1];
       double r1 = c->rmis1 + (double)(ir1 - 1) * c->ah1;
       for (int k = 0; k < s->nojbas[ic]; k++) {
         int jk = s->jbas[(size_t)ic * na + k];
@@ -508,6 +511,8 @@ ORF_EXPORT int oref_create(const mw3_setup *s, oref_ctx **out) {
         }
         yylmean = 0.5 * (yylmin + yylmax);
         if (nnn == 1) yylmean = yylmin;
+        /* each R owns its own elements (same order of j, a, b per element) */
+#pragma omp parallel for schedule(static)
         for (int ir2 = 1; ir2 <= c->npun2; ir2++) {
           double r2 = c->rmis2 + (double)(ir2 - 1) * c->ah2;
           double r2eff = r2;

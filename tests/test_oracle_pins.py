@@ -48,9 +48,11 @@ def test_gauleg_is_gauss_legendre(n):
 @pytest.mark.parametrize("m2,mp2", [(0, 0), (2, 0), (4, 0), (12, 0), (2, 2), (0, 2), (-2, 0), (4, 2)])
 @pytest.mark.parametrize("beta", [0.1, 1.3, 2.9])
 def test_dwigner_matches_reference(m2, mp2, beta):
+    # the recurrence is arranged differently from the reference's: agreement
+    # to rounding (values are O(1), 30 recurrence steps)
     a = numerics.dwigner(60, m2, mp2, beta)
     b = refnum.dwigner(60, m2, mp2, beta)
-    assert np.max(np.abs(a - b)) <= 1e-15
+    assert np.max(np.abs(a - b)) <= 2e-14
 
 
 def test_dwigner_legendre_and_associated_closed_forms():
@@ -67,6 +69,27 @@ def test_dwigner_legendre_and_associated_closed_forms():
                 ref = math.sqrt(math.factorial(j - m) / math.factorial(j + m)) \
                     * lpmv(m, j, math.cos(beta))
                 assert abs(d[2 * j] - ref) < 1e-12
+
+
+def test_dwigner_general_closed_forms_and_orthogonality():
+    """Textbook values of d^{1/2} and d^1, the symmetry
+    d^j_{m m'} = (-1)^(m-m') d^j_{m' m}, and orthogonality of the rows of
+    each d^j matrix (sum over m' of d^j_{m m'} d^j_{m'' m'} = delta)."""
+    for b in (0.4, 1.7, 2.8):
+        c, s, ch, sh = math.cos(b), math.sin(b), math.cos(b / 2), math.sin(b / 2)
+        assert abs(numerics.dwigner(1, 1, 1, b)[1] - ch) < 1e-15
+        assert abs(numerics.dwigner(1, 1, -1, b)[1] + sh) < 1e-15
+        assert abs(numerics.dwigner(1, -1, 1, b)[1] - sh) < 1e-15
+        assert abs(numerics.dwigner(2, 2, 2, b)[2] - 0.5 * (1 + c)) < 1e-15
+        assert abs(numerics.dwigner(2, 2, 0, b)[2] + s / math.sqrt(2)) < 1e-15
+        assert abs(numerics.dwigner(2, 0, 2, b)[2] - s / math.sqrt(2)) < 1e-15
+        assert abs(numerics.dwigner(2, 2, -2, b)[2] - 0.5 * (1 - c)) < 1e-15
+        for j2 in (7, 10, 13):
+            ms = range(-j2, j2 + 1, 2)
+            d = np.array([[numerics.dwigner(j2, m, mp, b)[j2] for mp in ms] for m in ms])
+            assert np.max(np.abs(d @ d.T - np.eye(len(ms)))) < 1e-13
+            sgn = np.array([[(-1.0) ** ((m - mp) // 2) for mp in ms] for m in ms])
+            assert np.max(np.abs(d - sgn * d.T)) < 1e-13
 
 
 @needs_ref
