@@ -33,21 +33,17 @@ FP32_PEAK_TF = 157.3
 
 
 def build_case(cfg):
-    from madwave3_amd import leps, model
-    from madwave3_amd import namelist as NL
+    from madwave3_amd import leps
 
-    p = leps.config(cfg)
-    text, pot, fdat = leps.make_inputs(p)
-    return model.build_case(NL.parse(text), pot, fdat)
+    return leps.build(leps.config(cfg))
 
 
 # kernel launches per step in each group (mixed and FP64 alike)
 GROUP_LAUNCHES = {"radial_dst": 3, "angular": 1, "update_fused": 1, "update_tail": 2,
                   "projections": 1, "energy_sums": 2, "coriolis": 1, "products": 1}
-RING = 256  # steps per energy-sum flush (kRing of the library)
 
 
-def kernel_groups(times):
+def kernel_groups(times, steps):
     """Per-step milliseconds of each kernel group from mw3_timing_get totals:
     radial_dst (the radial correction passes: k_rad_fwd, k_rad_mid, k_rad_inv
     at configuration 2), angular (the one-pass angular kernel; in mixed
@@ -56,10 +52,11 @@ def kernel_groups(times):
     second stream),
     projections (reactant-state sums; second stream, overlapped with the next
     step), products (the per-step product-state reduction, same stream as
-    projections), energy_sums (once per 256-step ring, divided by 256, with the
-    product-state GEMM of the ring) and coriolis (the FBR coefficient pass with
-    the Omega +- 1 stencil, run instead of the fused forward epilogue when
-    channels are Coriolis-coupled)."""
+    projections), energy_sums (once per ring flush, with the product-state
+    GEMM of the ring: its total time over the timed region divided by the
+    `steps` timed steps, since every flush is timed) and coriolis (the FBR
+    coefficient pass with the Omega +- 1 stencil, run instead of the fused
+    forward epilogue when channels are Coriolis-coupled)."""
     acc = {}
     for name, (ms, cnt) in times.items():
         if name.startswith("dst"):
@@ -83,8 +80,16 @@ def kernel_groups(times):
     per = {}
     for g, (ms, n) in acc.items():
         if n:
-            per[g] = ms / n / (RING if g == "energy_sums" else 1)
+            per[g] = ms / steps if g == "energy_sums" else ms / n
     return per
+
+
+def per_kernel(times, steps):
+    """Milliseconds per step of each timed kernel id: per timed launch,
+    except the energy accumulation (once per ring flush, every flush timed),
+    whose total is divided by the timed steps."""
+    return {k: (ms / steps if k == "energy_accum" else ms / max(cnt, 1))
+            for k, (ms, cnt) in times.items()}
 
 
 def run_gpu(case, precision, steps, warmup, timing):
@@ -109,12 +114,9 @@ def run_gpu(case, precision, steps, warmup, timing):
 
 
 def build_case_j(J):
-    from madwave3_amd import leps, model
-    from madwave3_amd import namelist as NL
+    from madwave3_amd import leps
 
-    p = leps.config(4, J=J)
-    text, pot, fdat = leps.make_inputs(p)
-    return model.build_case(NL.parse(text), pot, fdat)
+    return leps.build(leps.config(4, J=J))
 
 
 def run_batch(cases, precision, steps, warmup):
@@ -189,6 +191,90 @@ def main_batched(a):
     print(json.dumps(line))
 
 
+def run_full(case, precision, nsteps, label, chunk=5120):
+    """One propagation of nsteps Chebyshev steps (the configuration's full
+    length), in chunks with a progress line each; returns the results the
+    reference prints at the analysis line."""
+    from madwave3_amd.lib import Propagator
+
+    g = Propagator(case, precision)
+    g.begin(case.rpaq0)
+    norms, autoc = [], []
+    t0 = time.perf_counter()
+    done = 0
+    while done < nsteps:
+        n = min(chunk, nsteps - done)
+        nn, au = g.steps(n, want_auto=case.iphoto > 0)
+        norms.append(nn)
+        if au is not None:
+            autoc.append(au)
+        done += n
+        print("# %s: %d / %d steps, %.1fs" % (label, done, nsteps, time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
+    out = {"seconds": time.perf_counter() - t0, "norms": np.concatenate(norms),
+           "P": g.flux_result()[0], "zs2": g.zs2()}
+    if autoc:
+        out["auto"] = np.concatenate(autoc)
+    if getattr(case, "iprod", 0) == 2:
+        out["zs2prod"] = g.zs2prod()
+    g.close()
+    return out
+
+
+def spectrum_of(case, autoc, ne=400):
+    """cheby-spectra's sum over the autocorrelation (auto[0] = 1) on ne
+    energies spanning the run's energy window, in the reference's units
+    before the cross-section factor (SRC/photo/cheby-spectra.f:58-97)."""
+    from madwave3_amd import consts as C
+    from madwave3_amd.lib import cheby_spectrum
+
+    a = np.concatenate([[1.0], autoc])
+    e = np.linspace(float(case.etotS2[0]), float(case.etotS2[-1]), ne)
+    return cheby_spectrum(a, case.emindlt, case.delta2, C.hbr, 0.0, e)
+
+
+def main_accuracy(a):
+    """Full-length mixed against FP64 on the GPU for configuration a.config:
+    the FP64 propagator is the reference of the long runs (it matches the CPU
+    oracle over the first steps at full size, tests/test_gpu_configs.py).
+    Prints one JSON line with the largest differences of what the run
+    reports: P(E), the reactant state-resolved probabilities |S|^2, the
+    product v'j'-resolved probabilities (iprod = 2), the norm series and,
+    for photodissociation, the autocorrelation and the spectrum."""
+    from madwave3_amd import printprob
+    from madwave3_amd.lib import MW3_FP64, MW3_MIXED
+
+    case = build_case(a.config)
+    nsteps = a.full_steps or int(case.ntimes) * (int(case.nloop) + 1)
+    print("# accuracy: configuration %d %s, %d steps" % (a.config, case, nsteps), file=sys.stderr,
+          flush=True)
+    r64 = run_full(case, MW3_FP64, nsteps, "fp64")
+    rmx = run_full(case, MW3_MIXED, nsteps, "mixed")
+    res = {"config": a.config, "workload": workload_name(a.config, case,
+                                                         case.ncan * case.nangu * case.npun1 * case.npun2),
+           "steps": nsteps, "fp64_seconds": r64["seconds"], "mixed_seconds": rmx["seconds"],
+           "max_abs_dP": float(np.max(np.abs(rmx["P"] - r64["P"]))),
+           "max_P": float(np.max(np.abs(r64["P"]))),
+           "max_rel_dnorm": float(np.max(np.abs(rmx["norms"] - r64["norms"]) / r64["norms"])),
+           "final_norm": float(r64["norms"][-1])}
+    s64, t64 = printprob.reactant_probabilities(case, r64["zs2"])
+    smx, tmx = printprob.reactant_probabilities(case, rmx["zs2"])
+    res["max_abs_dS2_reactant"] = float(np.max(np.abs(smx - s64)))
+    res["max_S2_reactant"] = float(np.max(np.abs(s64)))
+    if "zs2prod" in r64:
+        p64, q64, _ = printprob.product_probabilities(case, r64["zs2prod"])
+        pmx, qmx, _ = printprob.product_probabilities(case, rmx["zs2prod"])
+        res["max_abs_dS2_product"] = float(np.max(np.abs(pmx - p64)))
+        res["max_S2_product"] = float(np.max(np.abs(p64)))
+        res["max_abs_dS2no"] = float(np.max(np.abs(qmx - q64)))
+    if "auto" in r64:
+        res["max_abs_dauto"] = float(np.max(np.abs(rmx["auto"] - r64["auto"])))
+        sp64 = spectrum_of(case, r64["auto"])
+        spmx = spectrum_of(case, rmx["auto"])
+        res["max_rel_dspectrum"] = float(np.max(np.abs(spmx - sp64)) / np.max(np.abs(sp64)))
+    print(json.dumps(res))
+
+
 def workload_name(cfg, case, N):
     kind = {1: "J=0 collision", 2: "J=0 collision", 3: "J=%d collision, Omega=0..%d coupled by "
             "Coriolis" % (case.Jtot, case.iommax), 5: "%d coupled electronic states"
@@ -252,6 +338,10 @@ def main():
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run a copy of this many MiB first (byte-counter calibration)")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="full-length mixed vs FP64 run of --config instead of the timing")
+    ap.add_argument("--full-steps", type=int, default=0,
+                    help="with --accuracy: steps instead of the configuration's ntimes (nloop+1)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (--gpus %d)" % a.gpus,
@@ -259,6 +349,9 @@ def main():
         sys.exit(2)
     from madwave3_amd.lib import MW3_FP64, MW3_MIXED
 
+    if a.accuracy:
+        main_accuracy(a)
+        return
     if a.config == 4:
         main_batched(a)
         return
@@ -276,13 +369,17 @@ def main():
 
     # roofline of the dominant kernel group of the headline (mixed) run: the
     # largest per-step time among the groups on the library's main stream
-    per = kernel_groups(timmx)
+    per = kernel_groups(timmx, a.steps)
     roof = None
     kern = {}
     if per:
         s_op = 4  # FP32 operator I/O
+        # the update visits only the compressed grid (the reference's loop over
+        # ntotproc): N_c = case.ntot points over all channels
+        n_c = float(case.ntot) / case.nelecmax
+        ne = case.nelecmax
         # product projection: transRgjac read by the update (operator precision)
-        tprod = s_op * N if getattr(case, "iprod", 0) == 2 else 0.0
+        tprod = s_op * n_c * ne if getattr(case, "iprod", 0) == 2 else 0.0
         nprod_l = ((case.nangproj1 - case.nangproj0 + 1) * (case.n2prod1 - case.n2prod0 + 1)
                    if tprod else 0)
         work = {
@@ -290,8 +387,12 @@ def main():
             "radial_dst": ("hbm", 2.0 * s_op * N, "GB/s"),
             # (b) angular: 4 n_gamma N flops per step
             "angular": ("mfma", 4.0 * case.nangu * N, "TFLOP/s"),
-            # (c) fused V + absorption + update: N (8 + 8 + s + s_V + 8) bytes
-            "update_fused": ("hbm", N * (8 + 8 + s_op + 4 + 8.0) + tprod, "GB/s"),
+            # (c) fused V + absorption + update: 8 + 8 + s + (V) + 8 bytes per
+            # state-point on the N_c compressed points, V being the unique
+            # entries of the symmetric n_e x n_e matrix per spatial point
+            # (s_V (n_e + 1)/2 per state-point: 4 B at one state)
+            "update_fused": ("hbm", n_c * ne * (8 + 8 + s_op + 4.0 * (ne + 1) / 2 + 8) + tprod,
+                             "GB/s"),
             # (e) energy projection: 8 netot n_flux flops per step (time only)
             "energy_sums": ("hbm", 16.0 * nflux, "GB/s"),
             "projections": ("hbm", 8.0 * case.nangu * case.npun1, "GB/s"),
@@ -310,12 +411,17 @@ def main():
             peak = HBM_PEAK_GBS if unit == "GB/s" else FP32_PEAK_TF
             kern[k] = {"ms_per_step": ms, "achieved": ach, "unit": unit, "peak": peak,
                        "frac": ach / peak}
+            tr = load_traffic(k) if a.config == 2 else None
+            if tr and unit == "GB/s":
+                # HBM bytes the counters saw per step over the algorithmic bytes
+                kern[k]["traffic"] = tr
+                kern[k]["traffic_vs_algorithmic"] = tr / w
         main = [k for k in per if k not in ("projections", "energy_sums", "products",
                                             "update_tail")]
         dom = max(main, key=lambda k: per[k])
         d = kern[dom]
         roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],
-                "unit": d["unit"], "frac": d["frac"], "traffic": load_traffic(dom),
+                "unit": d["unit"], "frac": d["frac"], "traffic": load_traffic(dom) if a.config == 2 else None,
                 "kernel": dom, "launches_per_unit": GROUP_LAUNCHES[dom],
                 "unit_of_work": "one Chebyshev step of the group (the scope table's per-step "
                                 "formula); traffic = HBM bytes per step from the committed "
@@ -346,9 +452,12 @@ def main():
         "dP_steps": a.warmup + a.steps,
         "roofline": roof,
         "kernels": kern,
-        "kernels_fp64": {k: {"ms_per_step": v} for k, v in kernel_groups(tim64).items()},
-        "kernel_ms_per_step": {k: ms / max(cnt, 1) for k, (ms, cnt) in timmx.items()},
-        "fp64_kernel_ms_per_step": {k: ms / max(cnt, 1) for k, (ms, cnt) in tim64.items()},
+        "kernels_fp64": {k: {"ms_per_step": v} for k, v in kernel_groups(tim64, a.steps).items()},
+        "kernel_ms_per_step": per_kernel(timmx, a.steps),
+        "fp64_kernel_ms_per_step": per_kernel(tim64, a.steps),
+        "energy_accum_ms_per_flush": {k: ms / max(cnt, 1) for k, (ms, cnt) in
+                                      (("mixed", timmx.get("energy_accum", (0.0, 0))),
+                                       ("fp64", tim64.get("energy_accum", (0.0, 0))))},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

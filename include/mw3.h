@@ -10,7 +10,9 @@
  *
  * Conventions
  *   - Plain C types only.  Every array argument is a host pointer; the library
- *     owns its device copies.  All floating-point data are double precision.
+ *     owns its device copies (one entry point, mw3_cheby_spectrum_noalloc,
+ *     takes device buffers instead).  All floating-point data are double
+ *     precision.
  *   - Multi-dimensional arrays are passed in the memory order of the reference's
  *     Fortran arrays: a Fortran array A(n1,n2,n3) is the C array A[n3][n2][n1].
  *     A Fortran caller therefore passes its module arrays unchanged.
@@ -264,6 +266,21 @@ MW3_API int32_t mw3_cvjprod_get(mw3_ctx *ctx, double *cvjprod);
 MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emindlt,
                                    double delta2, double hbr, double xkgam, const double *e,
                                    int32_t ne, double *spectrum);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory -- code under HIP graph capture, serving loops, harnesses that
+   forbid hipMalloc: the sum of mw3_cheby_spectrum on device buffers the
+   caller owns, d_autoc[kmax+1], d_e[ne] and d_spectrum[ne], enqueued on
+   `stream` (a hipStream_t; NULL is the null stream) of the current device.
+   It returns once the work is enqueued; the caller synchronizes.  It calls
+   no allocating routine: the coefficients go to a device array of the
+   library (2^17 doubles, 1 MiB per device, held until the process ends), so
+   kmax is at most 131071 (MW3_EINVAL beyond), and calls that overlap on one
+   device race on that array.  Errors: mw3_last_error(NULL). */
+MW3_API int32_t mw3_cheby_spectrum_noalloc(const double *d_autoc, int32_t kmax, double emindlt,
+                                           double delta2, double hbr, double xkgam,
+                                           const double *d_e, int32_t ne, double *d_spectrum,
+                                           void *stream);
 
 /* Per-kernel timing with HIP events on the stream each kernel runs on.  on = 0
    disables it; on = 1 brackets every launch of a timed kernel with an event

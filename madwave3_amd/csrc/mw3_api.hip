@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "mw3_internal.h"
@@ -12,6 +13,17 @@
 using namespace mw3;
 
 static thread_local std::string g_create_err;
+
+// Device array of mw3_cheby_spectrum_noalloc: the Chebyshev coefficients of
+// one call (kmax + 1 doubles).  Every device the library runs on holds its
+// own copy (1 MiB) until the process ends; calls that overlap on one device
+// share it and race.  In host code `scratch` names a host-side stand-in, so
+// host code uses only the per-device address from hipGetSymbolAddress.
+constexpr int kSpecScratch = 1 << 17;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kSpecScratch];
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
 
 namespace mw3 {
 
@@ -722,7 +734,7 @@ int32_t mw3_prod_attach(mw3_ctx* ctx, const mw3_prod_setup* ps) {
   for (int j = 0; j < njp; j++)
     if (ps->noBCprod[j] > ps->nvmaxprod)
       return seterr(c, MW3_EINVAL, "mw3_prod_attach: noBCprod > nvmaxprod");
-  if (t.nc > 64)
+  if (t.nc > kProdMaxCh)
     return seterr(c, MW3_EUNSUP, "mw3_prod_attach: more than 64 channels");
   // steps already in the ring are accumulated without the product projection
   int rc = flush_ring(c, nullptr, nullptr);
@@ -923,6 +935,54 @@ int32_t mw3_timing_get(mw3_ctx* ctx, int32_t id, double* total_ms, int64_t* laun
 }
 
 }  // extern "C"
+
+// Device address of this device's coefficient array, looked up once per
+// device with that device current.
+static hipError_t scratch_address(int dev, double** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+int32_t mw3_cheby_spectrum_noalloc(const double* d_autoc, int32_t kmax, double emindlt,
+                                   double delta2, double hbr, double xkgam, const double* d_e,
+                                   int32_t ne, double* d_spectrum, void* stream) {
+  if (!d_autoc || !d_e || !d_spectrum || kmax < 0 || ne < 0)
+    return seterr(nullptr, MW3_EINVAL, "mw3_cheby_spectrum_noalloc: bad arguments");
+  if (!(delta2 > 0.0))
+    return seterr(nullptr, MW3_EINVAL, "mw3_cheby_spectrum_noalloc: delta2 must be positive");
+  if (kmax >= kSpecScratch) {
+    char msg[160];
+    snprintf(msg, sizeof msg,
+             "mw3_cheby_spectrum_noalloc: kmax = %d exceeds the largest supported, %d",
+             (int)kmax, kSpecScratch - 1);
+    return seterr(nullptr, MW3_EINVAL, msg);
+  }
+  if (ne == 0) return MW3_OK;
+  int dev = -1;
+  hipError_t err = hipGetDevice(&dev);
+  if (err != hipSuccess)
+    return seterr(nullptr, MW3_EDEVICE, "mw3_cheby_spectrum_noalloc: no HIP device", err);
+  if (dev < 0 || dev >= kMaxDevices)
+    return seterr(nullptr, MW3_EUNSUP, "mw3_cheby_spectrum_noalloc: device index above 63");
+  double* coef = nullptr;
+  err = scratch_address(dev, &coef);
+  if (err != hipSuccess)
+    return seterr(nullptr, MW3_EDEVICE, "mw3_cheby_spectrum_noalloc: coefficient array", err);
+  // the coefficient kernel writes all kmax + 1 entries before the sum reads
+  // them (one stream), so nothing of an earlier call needs clearing
+  err = mw3::launch_spectrum(static_cast<hipStream_t>(stream), d_autoc, coef, kmax, hbr, xkgam,
+                             d_e, ne, emindlt, delta2, d_spectrum);
+  if (err != hipSuccess)
+    return seterr(nullptr, MW3_EDEVICE, "mw3_cheby_spectrum_noalloc: launch", err);
+  return MW3_OK;
+}
 
 int32_t mw3_cheby_spectrum(const double* autoc, int32_t kmax, double emindlt, double delta2,
                            double hbr, double xkgam, const double* e, int32_t ne,

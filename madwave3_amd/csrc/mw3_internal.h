@@ -301,6 +301,9 @@ hipError_t launch_prod_step(Ctx* c, int slot, hipStream_t s);
 hipError_t launch_prod_flush(Ctx* c);
 
 constexpr int kUpdRows = 32;  // rows per update work item
+// most channels of one product-state context (LDS of the per-step product
+// reduction: 64 doubles per channel)
+constexpr int kProdMaxCh = 64;
 
 // GEMM passes of the step (template tag: one kernel symbol per pass)
 enum GemmRole {

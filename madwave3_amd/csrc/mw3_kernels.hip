@@ -2037,11 +2037,17 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
 // S(r, k1) from the low-mode table L[k1][r] of size nn1 (forward) and its
 // transpose LT[r][k1] (back; within each 16-column block of FP64 tables the
 // columns run k1 = 16 m + 4 rr + g at position 16 m + 4 g + rr, so that a
-// lane's four k1 of one step group are one read).  t2 stays in the MFMA
-// accumulators: their layout is the back product's B operand (as in
+// lane's four k1 of one step group are one read).  Both tables are padded
+// with zeros to 16 NK1 modes and to whole 16-row blocks of r, over every k1
+// at or past k1lo and every r at or past nn1: the products then need no
+// masks on loaded operands (a select on a freshly loaded value made the
+// compiler wait for every outstanding load, the next block's prefetch
+// included).  The t1 rows read past nn1 (below n1) are finite: the forward
+// R pass never writes them and the slab starts zeroed.  t2 stays in the
+// MFMA accumulators: their layout is the back product's B operand (as in
 // k_ang_fused).  Items: int4 {channel * na + angle, column block | k2lo << 16,
 // nn1 | k1lo << 16, size index}; sizes: int4 {L offset, L row stride,
-// LT offset, 0}.
+// LT offset, 0}, offsets into the one table buffer.
 // ---------------------------------------------------------------------------
 template <typename T, int NK1>
 __global__ __launch_bounds__(256) void k_rad_mid(T* __restrict__ t1, const T* __restrict__ tab,
@@ -2063,9 +2069,10 @@ __global__ __launch_bounds__(256) void k_rad_mid(T* __restrict__ t1, const T* __
   const int4 sz = sizes[it.w];
   const int ia = cg % na;
   T* tb = t1 + (int64_t)cg * n1 * n2;
-  const T* L = tab + sz.x;
+  const T* L = tabT + sz.x;
   const int ld = sz.y;
   const T* LT = tabT + sz.z;
+  (void)tab;
   const int col = 16 * cc + i;
   const int colc = col < n2 ? col : n2 - 1;
   const int nrb = (nn + 15) >> 4;
@@ -2075,44 +2082,34 @@ __global__ __launch_bounds__(256) void k_rad_mid(T* __restrict__ t1, const T* __
 #pragma unroll
   for (int m = 0; m < NK1; m++) acc[m] = (V){0, 0, 0, 0};
   T a[2][NK1][4], b[2][4];
+  (void)k1l;
   auto load = [&](int rb, int q) {
     const int r0 = 16 * rb + 4 * g;
-    const int c0 = r0 + 4 <= ld ? r0 : ld - 4;  // groups past the row are masked below
 #pragma unroll
-    for (int m = 0; m < NK1; m++) {
-      const int k1 = 16 * m + i;
-      ld4(L + (int64_t)(k1 < k1l ? k1 : 0) * ld + c0, a[q][m]);
-    }
+    for (int m = 0; m < NK1; m++) ld4(L + (int64_t)(16 * m + i) * ld + r0, a[q][m]);
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int r = r0 + s;
-      b[q][s] = tb[(int64_t)(r < nn ? r : nn - 1) * n2 + colc];
+      b[q][s] = tb[(int64_t)(r < n1 ? r : n1 - 1) * n2 + colc];
     }
   };
-  auto compute = [&](int rb, int q) {
-    const int r0 = 16 * rb + 4 * g;
+  auto compute = [&](int q) {
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const bool rin = r0 + s < nn;
-      const T bv = rin ? b[q][s] : (T)0;
+    for (int s = 0; s < 4; s++)
 #pragma unroll
-      for (int m = 0; m < NK1; m++) {
-        const T av = (rin && 16 * m + i < k1l) ? a[q][m][s] : (T)0;
-        acc[m] = M::mma(av, bv, acc[m]);
-      }
-    }
+      for (int m = 0; m < NK1; m++) acc[m] = M::mma(a[q][m][s], b[q][s], acc[m]);
   };
   // this wave's blocks w, w + 4, ..; a missing second block repeats the
-  // first with its rows masked (rin false past nn)
+  // first and is not computed
   const int nmine = nrb > w ? (nrb - w + 3) / 4 : 0;
   if (nmine > 0) {
     load(w, 0);
     for (int m2 = 0; m2 < nmine; m2 += 2) {
       const int rb0 = w + 4 * m2, rb1 = rb0 + 4;
       load(m2 + 1 < nmine ? rb1 : rb0, 1);
-      compute(rb0, 0);
+      compute(0);
       load(m2 + 2 < nmine ? rb1 + 4 : rb0, 0);
-      if (m2 + 1 < nmine) compute(rb1, 1);
+      if (m2 + 1 < nmine) compute(1);
     }
   }
 #pragma unroll
@@ -2204,10 +2201,15 @@ static hipError_t build_rad_mid(Ctx* c) {
     for (int ja = 0; ja < t.na; ja++)
       if (t.nn1[ja] == n) kcap = std::max(kcap, t.k1lo[ja]);
     qrec[q] = (int)sizes.size() / 4;
-    // a size with no low r modes has no table rows: its (fully masked) reads
-    // go to the start of the tables instead
-    sizes.push_back(kcap > 0 ? (int32_t)t.low_off[q] : 0);
-    sizes.push_back(ld);
+    // forward table Lp[k1][r], 16 nk1 rows of ldp = 16 ceil(n1/16) columns,
+    // zero at k1 >= k1lo of the size and at r >= nn1
+    const int ldp = (t.n1 + 15) / 16 * 16;
+    sizes.push_back((int32_t)lt.size());
+    sizes.push_back(ldp);
+    for (int k1 = 0; k1 < ktp; k1++)
+      for (int r = 0; r < ldp; r++)
+        lt.push_back(r < n && k1 < kcap ? t.dst_tables[(size_t)t.low_off[q] + (size_t)k1 * ld + r]
+                                        : 0.0);
     sizes.push_back((int32_t)lt.size());
     sizes.push_back(0);
     for (int r = 0; r < rows; r++)
@@ -2959,7 +2961,6 @@ hipError_t launch_psie(Ctx* c, int k) {
 // lane reads back only its own column of f, so no barrier is needed; f is
 // sized by the launch to the channel count (nc 64-lane rows), so that the
 // block holds only the LDS it uses beside the kernels of the other streams.
-constexpr int kProdMaxCh = 64;
 __global__ __launch_bounds__(64) void k_prod_fun(const double* __restrict__ ppart,
                                                  const int32_t* __restrict__ gfirst,
                                                  const double* __restrict__ dtrans,
@@ -3008,8 +3009,8 @@ hipError_t launch_prod_step(Ctx* c, int slot, hipStream_t s) {
   const HostTables& t = c->t;
   const int rc0 = c->pr_n2p0 / 64, rc1 = (c->pr_n2p0 + c->pr_n2p - 1) / 64;
   const int nrcp = rc1 - rc0 + 1;
+  if (t.nc > kProdMaxCh) return hipErrorInvalidValue;  // mw3_prod_attach rejects it first
   timer_begin(c, K_PROD, s);
-  if (t.nc > kProdMaxCh) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_prod_fun, dim3((unsigned)(c->pr_nap * nrcp)), dim3(64),
                      sizeof(double) * 64 * t.nc, s, c->d_ppart,
                      c->d_uw_gfirst, c->d_dtrans, c->d_ch_elec,

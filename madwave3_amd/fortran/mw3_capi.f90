@@ -1,7 +1,7 @@
 ! mw3_capi.f90 -- Fortran 2003 ISO_C_BINDING interface to libmw3 (include/mw3.h).
 !
 ! Module mw3_capi gives a Fortran host (MADWAVE3's main program or the
-! mod_Hphi_01y2 replacement in INTEGRATION.md) the C ABI of libmw3:
+! mod_Hphi_01y2 replacement in fortran/reference) the C ABI of libmw3:
 !   type(mw3_setup)        the C struct mw3_setup, field for field (bind(C));
 !   type(mw3_prod_setup)   the C struct mw3_prod_setup (product projection);
 !   mw3_create ... mw3_kernel_name   one interface per C entry point;
@@ -152,6 +152,19 @@ module mw3_capi
       real(c_double), intent(out) :: spectrum(*)
       integer(c_int32_t) :: rc
     end function mw3_cheby_spectrum
+
+    ! device buffers and a hipStream_t as c_ptr (no allocation in the library)
+    function mw3_cheby_spectrum_noalloc(d_autoc, kmax, emindlt, delta2, hbr, xkgam, d_e, ne, &
+        d_spectrum, stream) bind(C, name="mw3_cheby_spectrum_noalloc") result(rc)
+      import :: c_ptr, c_int32_t, c_double
+      type(c_ptr), value :: d_autoc
+      integer(c_int32_t), value :: kmax
+      real(c_double), value :: emindlt, delta2, hbr, xkgam
+      type(c_ptr), value :: d_e
+      integer(c_int32_t), value :: ne
+      type(c_ptr), value :: d_spectrum, stream
+      integer(c_int32_t) :: rc
+    end function mw3_cheby_spectrum_noalloc
 
     function mw3_cheby_k(ctx) bind(C, name="mw3_cheby_k") result(k)
       import :: c_ptr, c_int32_t

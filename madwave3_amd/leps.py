@@ -31,10 +31,14 @@ __all__ = ["LEPS", "diatomic_states", "product_states", "make_inputs", "config"]
 
 
 class LEPS:
-    """Symmetric LEPS surface of three identical Morse pairs (hartree, bohr)."""
+    """LEPS surface of three Morse pairs (hartree, bohr): the 0-1 and 0-2 pairs
+    share De, a, re; the 1-2 pair has the same a, re and its own well depth
+    (De12_eV, default De_eV: the symmetric surface).  A shallow 1-2 well
+    closes the 1-2 + 0 arrangement."""
 
-    def __init__(self, De_eV=4.75, a_invA=1.94, re_A=0.742, sato=0.17):
+    def __init__(self, De_eV=4.75, a_invA=1.94, re_A=0.742, sato=0.17, De12_eV=None):
         self.D = De_eV / C.au2eV
+        self.D12 = (De_eV if De12_eV is None else De12_eV) / C.au2eV
         self.a = a_invA * C.convl          # 1/bohr
         self.re = re_A / C.convl           # bohr
         self.s = sato
@@ -43,17 +47,17 @@ class LEPS:
         e = np.exp(-self.a * (r - self.re))
         return self.D * (e * e - 2.0 * e)
 
-    def _qj(self, r):
+    def _qj(self, r, D):
         e = np.exp(-self.a * (r - self.re))
-        D, s = self.D, self.s
+        s = self.s
         q = 0.25 * D * ((3.0 + s) * e * e - (2.0 + 6.0 * s) * e)
         j = 0.25 * D * ((1.0 + 3.0 * s) * e * e - (6.0 + 2.0 * s) * e)
         return q, j
 
     def __call__(self, r01, r02, r12):
-        q1, j1 = self._qj(r01)
-        q2, j2 = self._qj(r02)
-        q3, j3 = self._qj(r12)
+        q1, j1 = self._qj(r01, self.D)
+        q2, j2 = self._qj(r02, self.D)
+        q3, j3 = self._qj(r12, self.D12)
         x = j1 * j1 + j2 * j2 + j3 * j3 - j1 * j2 - j2 * j3 - j3 * j1
         return (q1 + q2 + q3 - np.sqrt(np.maximum(x, 0.0))) / (1.0 + self.s)
 
@@ -146,14 +150,24 @@ def config(cfg_id, scale=1.0, J=0):
         base.update(npun1=256, rfin1_bohr=8.0, npun2=512, rfin2_bohr=15.0,
                     nangu=120, ntimes=500, nloop=79, r1flux_bohr=4.0, iprod=2)
     elif cfg_id == 5:
+        # photodissociation (iphoto = 1): a seeded bound state of J = 1,
+        # parity -1 on the ground surface times a linear transition dipole
+        # to the three coupled states (photo_packet)
         base.update(npun1=256, rfin1_bohr=8.0, npun2=384, rfin2_bohr=15.0,
                     nangu=96, ntimes=500, nloop=119, r1flux_bohr=4.0, nelec=3,
                     state_offsets_eV=[0.0, 0.3, 0.6], coupling_eV=0.05,
-                    coupling_r_bohr=1.4, coupling_width_bohr=0.3)
+                    coupling_r_bohr=1.4, coupling_width_bohr=0.3,
+                    iphoto=1, Jtotini=1, iparini=-1, nvbound=0, bnd_R_bohr=4.5,
+                    bnd_width_bohr=0.4, bnd_energy_eV=-0.5, bnd_jmax=2,
+                    dip_strength=[1.0, 0.7, 0.4], dip_slope_r=0.5, dip_slope_Rcos=0.2,
+                    seed=20261202)
     elif cfg_id == 3:
+        # initial packet: a seeded superposition of the v = 0 packets of
+        # j = 0, 1, 2 (every Omega <= j, as set_colini builds them with iomref < 0)
         base.update(npun1=192, rfin1_bohr=8.0, npun2=384, rfin2_bohr=15.0,
                     nangu=96, ntimes=500, nloop=59, r1flux_bohr=4.0,
-                    Jtot=12, iparity=1, iommin=0, iommax=6, iomref=-1)
+                    Jtot=12, iparity=1, iommin=0, iommax=6, iomref=-1,
+                    jmix=(0, 1, 2), seed=20261202)
     elif cfg_id == 4:
         # one total J of the batched workload J = 0..23 (Jtot set by the caller;
         # Omega = 0..min(J, 6); parity +1 has no Omega = 0 block at odd J)
@@ -173,6 +187,92 @@ def config(cfg_id, scale=1.0, J=0):
     return base
 
 
+def jmix_weights(p):
+    """Seeded amplitudes of the initial rotational states p["jmix"]: standard
+    normal draws from numpy's default generator seeded with p["seed"],
+    normalised to unit sum of squares."""
+    rng = np.random.default_rng(int(p["seed"]))
+    w = rng.standard_normal(len(p["jmix"]))
+    return dict(zip(p["jmix"], w / math.sqrt(float(np.sum(w * w)))))
+
+
+def photo_packet(p, c):
+    """Synthetic inputs of the photodissociation run (configuration 5): the
+    bound state that dip_bndgrid reads from ../bndJxx{p,m} and the transition
+    dipole of ../dip (see photo.py), as (BoundState, dipol).
+
+    Bound state (J = Jtotini, Omega = 0, one electronic state): on the kept
+    points of the propagation grid,
+        fd_{v=0,j=0}(r) exp(-(R - R_b)^2 / (2 w^2)) sum_j a_j P~_j(cos gamma)
+    with P~_j the normalised Legendre functions times sqrt(GL weight), the
+    a_j (j = 0..bnd_jmax) seeded standard normal draws, normalised to 1.
+    Transition dipole, linear in the coordinates (atomic units, along the
+    body-fixed z axis, so only its q = 0 component is non-zero):
+        d_e(r, R, gamma) = s_e (1 + a (r - r_e) + b R cos(gamma)),  r, R in bohr,
+    s_e = dip_strength[e], a = dip_slope_r, b = dip_slope_Rcos."""
+    from . import photo
+    from .numerics import dwigner_vec
+
+    rng = np.random.default_rng(int(p["seed"]) + 1)
+    nj = int(p["bnd_jmax"]) + 1
+    a = rng.standard_normal(nj)
+    conv = C.convl
+    gam = np.arccos(c.cgamma)
+    leg = dwigner_vec(2 * (nj - 1), 0, 0, gam)[0::2]            # [j][iang]
+    ang = np.zeros(c.nangu)
+    for j in range(nj):
+        ang += a[j] * leg[j] * math.sqrt(j + 0.5) * np.sqrt(c.wreal)
+    fd0 = c.fd[0, 0 - c.jini, 0 - c.nvini]                      # v = 0, j = 0, state 1
+    ir1s, ir2s, ias, vals, dipol = [], [], [], [], []
+    re_b = p["re_A"] / conv
+    for ia in range(c.nangu):
+        n = int(c.npunreal[ia])
+        i1 = c.indr1[ia, :n].astype(np.int64)
+        i2 = c.indr2[ia, :n].astype(np.int64)
+        r_b = (c.rmis1 + (i1 - 1) * c.ah1) / conv
+        R_b = (c.rmis2 + (i2 - 1) * c.ah2) / conv
+        v = fd0[i1 - 1] * np.exp(-((R_b - p["bnd_R_bohr"]) ** 2) / (2.0 * p["bnd_width_bohr"] ** 2))
+        v = v * ang[ia]
+        keep = np.abs(v) > 0.0
+        ir1s.append(i1[keep])
+        ir2s.append(i2[keep])
+        ias.append(np.full(int(keep.sum()), ia + 1))
+        vals.append(v[keep])
+        lin = 1.0 + p["dip_slope_r"] * (r_b - re_b) + p["dip_slope_Rcos"] * R_b * c.cgamma[ia]
+        d = np.zeros((n, 3, c.nelecmax, 1))
+        for e in range(c.nelecmax):
+            d[:, 1, e, 0] = p["dip_strength"][e] * lin
+        dipol.append(d)
+    val = np.concatenate(vals)
+    val /= math.sqrt(float(np.sum(val * val)))
+    npt = val.size
+    grid = (c.rmis1, c.rfin1, c.npun1, c.rmis2, c.rfin2, c.npun2, c.nangu, c.inc)
+    bs = photo.BoundState(np.concatenate(ir1s), np.concatenate(ir2s), np.concatenate(ias),
+                          np.zeros(npt, dtype=np.int64), np.ones(npt, dtype=np.int64), val,
+                          p["nvbound"], p["bnd_energy_eV"], grid)
+    return bs, dipol
+
+
+def build(p):
+    """Case of parameter dict p (see config) with its initial packet: the
+    collision packet of set_colini, a seeded superposition of them over
+    p["jmix"], or the photodissociation packet of dip_bnd (iphoto > 0)."""
+    from . import model
+    from . import photo
+
+    text, pot, fdat = make_inputs(p)
+    c = model.build_case(NL.parse(text), pot, fdat)
+    if int(p.get("iphoto", 0)) > 0:
+        bs, dipol = photo_packet(p, c)
+        photo.dip_bnd(c, bs, dipol, int(p["iphoto"]), int(p["Jtotini"]), int(p["iparini"]))
+    elif p.get("jmix"):
+        w = jmix_weights(p)
+        c.rpaq0 = sum(wj * model.colini_packet(c, jref=j) for j, wj in w.items())
+        c.norm_initial = float(np.sum(c.rpaq0 * c.rpaq0))
+        c.jmix_weights = w
+    return c
+
+
 def make_inputs(p):
     """(input.dat text, PotData, FuncData) for parameter dict p (see config)."""
     conv = C.convl
@@ -184,7 +284,7 @@ def make_inputs(p):
     xm0, xm1, xm2 = p["xm0"], p["xm1"], p["xm2"]
     mu1 = xm0 * xm1 / (xm0 + xm1)
     mu2 = xm2 * (xm0 + xm1) / (xm0 + xm1 + xm2)
-    pes = LEPS(p["De_eV"], p["a_invA"], p["re_A"], p["sato"])
+    pes = LEPS(p["De_eV"], p["a_invA"], p["re_A"], p["sato"], p.get("De12_eV"))
     h2z = 1.0 / C.conve * C.conve1          # hartree -> zot
     vcut = C.ev_to_zot(p["vcut_eV"])
 
@@ -277,7 +377,12 @@ def make_inputs(p):
         IOMMIN=p["iommin"], IOMMAX=p["iommax"], J0=0, INC=1,
         JINI=p["jini"], JMAX=p["jmax"], NVINI=p["nvini"], NVMAX=p["nvmax"],
         NVREF=p["nvref"], JREF=p["jref"], IOMREF=p["iomref"], IELECREF=1))
-    text += NL.dump_group("inputprocess", dict(iphoto=0))
+    text += NL.dump_group("inputprocess", dict(iphoto=int(p.get("iphoto", 0))))
+    if int(p.get("iphoto", 0)) > 0:
+        text += NL.dump_group("inputbnd", dict(
+            Jtotini=p["Jtotini"], iparini=p["iparini"], nvbound=p["nvbound"],
+            nprocbnd=int(p.get("nprocbnd", 1)),
+            maxbnddim=npun1 * npun2 * nangu, igridbnd=1))
     text += NL.dump_group("inputabsorcion", dict(
         absalp1=p["absalp"], absr1=absr1, n1expo=p["nexpo"], ninterv1=1,
         absalp2=p["absalp"], absr2=absr2, n2expo=p["nexpo"], ninterv2=1))

@@ -284,6 +284,10 @@ def load_library(path=None):
     lib.mw3_cheby_spectrum.argtypes = [ct.c_void_p, ct.c_int32, ct.c_double, ct.c_double,
                                        ct.c_double, ct.c_double, ct.c_void_p, ct.c_int32,
                                        ct.c_void_p]
+    lib.mw3_cheby_spectrum_noalloc.restype = ct.c_int32
+    lib.mw3_cheby_spectrum_noalloc.argtypes = [ct.c_void_p, ct.c_int32, ct.c_double, ct.c_double,
+                                               ct.c_double, ct.c_double, ct.c_void_p, ct.c_int32,
+                                               ct.c_void_p, ct.c_void_p]
     lib.mw3_cheby_steps_batch.restype = ct.c_int32
     lib.mw3_cheby_steps_batch.argtypes = [ct.POINTER(ct.c_void_p), ct.c_int32, ct.c_int32,
                                           ct.c_void_p]

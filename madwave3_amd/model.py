@@ -82,6 +82,7 @@ def _grid(case, nl):
     c.nangproj0, c.nangproj1 = int(pr["nangproj0"]), int(pr["nangproj1"])
     pc = NL.group(nl, "inputprocess", {"nelec_bnd": 1})
     c.iphoto = int(pc["iphoto"])
+    c.nelec_bnd = int(pc["nelec_bnd"])
 
     # radial steps (SRC/mod_gridYpara_01y2.f:213-225)
     c.ah2 = (c.rfin2 - c.rmis2) / float(c.npun2 - 1) if c.npun2 > 1 else 0.0
@@ -372,9 +373,19 @@ def colini_params(c, nl):
 
 def _colini(c, nl):
     """set_colini + colini_reacJacobi (SRC/mod_colini_01y2.f:25-177)."""
-    xr2 = colini_params(c, nl)
+    c.xr2colini = colini_params(c, nl)
     if c.iprod == 1:
         raise InputError("iprod=1 (product Jacobi propagation) is not on this path")
+    c.rpaq0 = colini_packet(c)
+    c.norm_initial = float(np.sum(c.rpaq0 * c.rpaq0))
+
+
+def colini_packet(c, jref=None):
+    """The collision packet of colini_reacJacobi (SRC/mod_colini_01y2.f:86-177)
+    for the reference state (nvref, jref, iomref) of the case, or for another
+    rotational state jref of the same v and Omega choice."""
+    xr2 = c.xr2colini
+    jref = c.jref if jref is None else int(jref)
     sl, ch, ir, ir1, ir2 = point_index(c)
     iang = c.indangreal[sl]
     ielec = c.nelebas[ch]
@@ -382,18 +393,16 @@ def _colini(c, nl):
     angpart = np.zeros(c.ntot)
     if c.iomref >= 0:
         m = (ielec == c.ielecref) & (iom == c.iomref)
-        angpart[m] = c.Djmmp[iom[m], ielec[m] - 1, c.jref, iang[m] - 1]
+        angpart[m] = c.Djmmp[iom[m], ielec[m] - 1, jref, iang[m] - 1]
     else:
-        m = (ielec == c.ielecref) & (iom <= c.jref)
-        angpart[m] = ((-1.0) ** iom[m]) * c.Djmmp[iom[m], ielec[m] - 1, c.jref, iang[m] - 1]
+        m = (ielec == c.ielecref) & (iom <= jref)
+        angpart[m] = ((-1.0) ** iom[m]) * c.Djmmp[iom[m], ielec[m] - 1, jref, iang[m] - 1]
     if c.npun1 == 1:
         fdi = np.ones(c.ntot)
     else:
-        fdi = c.fd[c.ielecref - 1, c.jref - c.jini, c.nvref - c.nvini, ir1 - 1]
+        fdi = c.fd[c.ielecref - 1, jref - c.jini, c.nvref - c.nvini, ir1 - 1]
     xr = xr2[ir2 - 1]
-    psi = np.where(ielec == c.ielecref, -xr * angpart * 1.0 * fdi * math.sqrt(c.ah2), 0.0)
-    c.rpaq0 = psi
-    c.norm_initial = float(np.sum(psi * psi))
+    return np.where(ielec == c.ielecref, -xr * angpart * 1.0 * fdi * math.sqrt(c.ah2), 0.0)
 
 
 def flux_params(c, nl):
@@ -548,7 +557,22 @@ def load_case(case_dir):
         fdat.prod = refio.read_prodwf(fdir, int(pr["jiniprod"]), int(pr["jmaxprod"]),
                                       int(pr["nviniprod"]), int(pr["nvmaxprod"]),
                                       min(int(pr["n2prod1"]), int(g["npun2"])))
-    return build_case(nl, pot, fdat, base_dir=case_dir)
+    c = build_case(nl, pot, fdat, base_dir=case_dir)
+    if c.iphoto in (1, 2):
+        # read_trans_dipole + dip_bnd (SRC/main_madwave3.f:177-181): ../dip and
+        # ../bndJxx{p,m} beside the case directory
+        from . import photo
+        b = NL.group(nl, "inputbnd", {"nprocbnd": 1, "maxbnddim": None, "igridbnd": 1})
+        if int(b["igridbnd"]) != 1:
+            raise InputError("only igridbnd = 1 (bound state on the grid) is built")
+        up = os.path.join(case_dir, "..")
+        dipol = photo.read_dip(os.path.join(up, "dip"), c, c.nelec_bnd) if c.iphoto == 1 else None
+        bs = photo.read_bnd(up, c, int(b["jtotini"]), int(b["iparini"]), int(b["nvbound"]),
+                            int(b["nprocbnd"]), b["maxbnddim"] and int(b["maxbnddim"]))
+        photo.dip_bnd(c, bs, dipol, c.iphoto, int(b["jtotini"]), int(b["iparini"]))
+    elif c.iphoto == 3:
+        raise InputError("iphoto = 3 (V_elec times the bound state) is not built")
+    return c
 
 
 def write_case(case_dir, nl_text, pot, fdat):

@@ -98,3 +98,12 @@ def diagon(h):
     e = np.zeros(n)
     lib().diagon_(_p(a), _i(n), _i(n), _p(t), _p(e))
     return e
+
+
+def tresj(j1, j2, j3, m1, m2, m3):
+    """tresj of the reference (3j symbol); fills its factorial table first."""
+    L = lib()
+    L.factorial_()
+    out = ct.c_double()
+    L.tresj_(_i(j1), _i(j2), _i(j3), _i(m1), _i(m2), _i(m3), ct.byref(out))
+    return out.value

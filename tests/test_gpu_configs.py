@@ -1,0 +1,113 @@
+"""The benchmark configurations at their full grid sizes: the FP64 GPU
+propagator against the FP64 CPU oracle over the first steps of each run.
+
+This is the check that makes the FP64 GPU path the long-run reference of the
+full-length mixed-precision runs (`bench.py --accuracy`): after K steps the
+state psi_K agrees with the oracle's to ||dpsi||/||psi|| <= 1e-12, the norms
+to 1e-12, and the energy-resolved accumulators (whose terms cancel) to 1e-9
+relative to their largest entry.  The step counts keep the oracle's share to
+seconds on the host.
+
+  configuration 2  J = 0, 256 r x 512 R x 120 gamma, product states (iprod = 2)
+  configuration 3  J = 12, Omega = 0..6 (7 Coriolis-coupled blocks), 192 x 384 x 96,
+                   seeded j = 0, 1, 2 initial packet
+  configuration 5  three coupled states, 256 x 384 x 96, photodissociation packet
+                   (bound state x linear transition dipole, iphoto = 1)
+  configuration 4  J = 12 (7 blocks of 160 x 320 x 80) advanced inside the batch
+                   of all 24 J values (mw3_cheby_steps_batch)
+"""
+import functools
+
+import numpy as np
+import pytest
+
+from madwave3_amd import leps
+from madwave3_amd.lib import MW3_FP64, Propagator, steps_batch
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+K_STEPS = {2: 12, 3: 6, 5: 12}
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def state_rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@functools.lru_cache(maxsize=None)
+def full_case(cfg, J=0):
+    return leps.build(leps.config(cfg, J=J))
+
+
+def compare(g, o, c, nsteps, label):
+    st_o = o.begin(c.rpaq0)
+    st_g = g.begin(c.rpaq0)
+    assert rel(st_g, st_o) < 1e-11, label
+    n_o, a_o, _ = o.steps(nsteps)
+    n_g, a_g = g.steps(nsteps, want_auto=True)
+    print(label, "norms", n_g[-1], "rel err %.2e" % rel(n_g, n_o), flush=True)
+    assert rel(n_g, n_o) < 1e-12, label
+    if c.iphoto > 0:
+        assert rel(a_g, a_o) < 1e-12, label
+    pm_o, p_o, k_o = o.state()
+    pm_g, p_g, k_g = g.state()
+    assert k_o == k_g == 1 + nsteps
+    e_p, e_pm = state_rel(p_g, p_o), state_rel(pm_g, pm_o)
+    print(label, "state rel err %.2e / %.2e" % (e_p, e_pm), flush=True)
+    assert e_p < 1e-12 and e_pm < 1e-12, label
+    za_o, zb_o = o.flux_accum()
+    za_g, zb_g = g.flux_accum()
+    assert rel(za_g, za_o) < 1e-9 and rel(zb_g, zb_o) < 1e-9, label
+    assert rel(g.zs2(), o.zs2()) < 1e-9, label
+    assert rel(g.flux_result()[0], o.flux_result()[0]) < 1e-9, label
+    if c.iprod == 2:
+        assert rel(g.zs2prod(), o.zs2prod()) < 1e-9, label
+
+
+@pytest.mark.parametrize("cfg", [2, 3, 5])
+def test_full_size_fp64_matches_oracle(cfg):
+    c = full_case(cfg)
+    label = "configuration %d %s" % (cfg, c)
+    print(label, flush=True)
+    g = Propagator(c, MW3_FP64)
+    o = Oracle(c)
+    compare(g, o, c, K_STEPS[cfg], label)
+    g.close()
+
+
+def test_config4_batch_member_matches_oracle():
+    """J = 12 of configuration 4, stepped by mw3_cheby_steps_batch together
+    with the other 23 J values, against the oracle run of J = 12 alone; the
+    other members must stay finite and keep the norms a lone run gives."""
+    nsteps = 4
+    cases = [full_case(4, J) for J in range(24)]
+    print("configuration 4:", sum(c.ncan for c in cases), "Omega blocks", flush=True)
+    props = [Propagator(c, MW3_FP64) for c in cases]
+    for g, c in zip(props, cases):
+        g.begin(c.rpaq0)
+    norms = steps_batch(props, nsteps)
+    assert np.all(np.isfinite(norms))
+    c = cases[12]
+    o = Oracle(c)
+    o.begin(c.rpaq0)
+    n_o, _, _ = o.steps(nsteps)
+    assert rel(norms[12], n_o) < 1e-12
+    pm_o, p_o, _ = o.state()
+    pm_g, p_g, _ = props[12].state()
+    e = state_rel(p_g, p_o)
+    print("configuration 4 J=12 in the batch: state rel err %.2e" % e, flush=True)
+    assert e < 1e-12 and state_rel(pm_g, pm_o) < 1e-12
+    assert rel(props[12].flux_result()[0], o.flux_result()[0]) < 1e-9
+    # a member alone gives its batch norms bit for bit (J = 5: 5 blocks)
+    alone = Propagator(cases[5], MW3_FP64)
+    alone.begin(cases[5].rpaq0)
+    n_alone, _ = alone.steps(nsteps)
+    assert np.array_equal(n_alone, norms[5])
+    for g in props + [alone]:
+        g.close()

@@ -13,7 +13,7 @@ from tests.cases import case
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["j0", "coriol", "states", "odd"]
+CASES = ["j0", "coriol", "states", "odd", "states5", "states2J"]
 
 
 def rel(a, b):

@@ -85,6 +85,7 @@ ekinmin_eV=0.1d0,ekinmax_eV=1.5d0      ! comment
 
 def test_reference_formats_roundtrip(tmp_path):
     p = leps.config(5, scale=12)
+    p.update(iphoto=0)   # the collision packet (the photo files: tests/test_photo.py)
     text, pot, fdat = leps.make_inputs(p)
     model.write_case(str(tmp_path), text, pot, fdat)
     c1 = model.build_case(NL.parse(text), pot, fdat)
