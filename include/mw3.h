@@ -267,6 +267,45 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
                                    double delta2, double hbr, double xkgam, const double *e,
                                    int32_t ne, double *spectrum);
 
+/* Execution options of a context (no counterpart in the reference; results
+   do not depend on them beyond rounding order where stated).
+   MW3_OPT_GRAPHS  1 (default): Chebyshev steps run as captured HIP graphs,
+                   one per step, whenever kernel timing is off and nofe = 0;
+                   0: kernel by kernel.  Same kernels, same order of every
+                   sum: results are bitwise identical.
+   MW3_OPT_RADIAL  1 (default): the radial kinetic correction by the
+                   persistent low-mode kernels (R passes with the sine table
+                   held in registers, the two r passes in one kernel) where
+                   the grid allows them; 0: as batched GEMMs.
+   MW3_OPT_ANGULAR 0 (default): the angular term by the parity-folded
+                   one-pass kernel; 1: the unfolded one-pass kernel; 2: two
+                   GEMMs with the FBR coefficients between them.  A variant
+                   not built for the context falls back to the next one.
+   MW3_OPT_SIDE_STREAM  1: the angular term on its own stream beside the
+                   radial passes (default in MW3_MIXED); 0: one stream
+                   (default in MW3_FP64).
+   The radial and angular variants sum in different orders: results agree
+   to rounding.  mw3_get_option returns the variant in effect (after
+   fallbacks).  mw3_set_option finishes the steps in flight first. */
+#define MW3_OPT_GRAPHS 0
+#define MW3_OPT_RADIAL 1
+#define MW3_OPT_ANGULAR 2
+#define MW3_OPT_SIDE_STREAM 3
+#define MW3_NOPTIONS 4
+MW3_API int32_t mw3_set_option(mw3_ctx *ctx, int32_t option, int32_t value);
+MW3_API int32_t mw3_get_option(const mw3_ctx *ctx, int32_t option, int32_t *value);
+
+/* The autotuner (no counterpart in the reference): times trial_steps
+   (2..254, rounded down to even) Chebyshev steps of the context under every
+   combination of the options above that the context can run, from its
+   current state, and keeps the fastest.  The state, k and every accumulator
+   are as before the call (the trial steps are discarded: no ring flush, no
+   psi(E) sum); steps already buffered are accumulated first.  chosen (may be
+   NULL) receives the MW3_NOPTIONS option values in effect, ms_per_step (may
+   be NULL) the best trial time per step. */
+MW3_API int32_t mw3_autotune(mw3_ctx *ctx, int32_t trial_steps, int32_t *chosen,
+                             double *ms_per_step);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
    memory -- code under HIP graph capture, serving loops, harnesses that
    forbid hipMalloc: the sum of mw3_cheby_spectrum on device buffers the

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -152,6 +153,13 @@ static void free_all(Ctx* c) {
     (void)hipStreamSynchronize(c->side);
     (void)hipStreamDestroy(c->side);
   }
+  for (auto& g : c->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  if (c->d_slot) (void)hipFree(c->d_slot);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_slice) (void)hipEventDestroy(c->ev_slice);
   if (c->ev_cvj) (void)hipEventDestroy(c->ev_cvj);
   if (c->ev_psi) (void)hipEventDestroy(c->ev_psi);
@@ -199,6 +207,9 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   }
   c->s = *setup;
   c->precision = precision;
+  // the angular term beside the radial passes on its own stream in mixed
+  // precision (measured faster there; the FP64 pair ran slower side by side)
+  c->opt_side = precision == MW3_MIXED ? 1 : 0;
   (void)hipGetDevice(&c->device);
   const HostTables& t = c->t;
   const size_t nd = (size_t)t.nc * t.nslab;
@@ -225,6 +236,8 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "mw3_create: stream");
   CK(hipEventCreateWithFlags(&c->ev_psi, evf), "mw3_create: event");
   CK(hipEventCreateWithFlags(&c->ev_ang, evf), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_fork, evf), "mw3_create: event");
+  CK(alloc_zero((void**)&c->d_slot, sizeof(int32_t)), "mw3_create: ring slot");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
 
@@ -235,6 +248,7 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(upload_raw(&c->d_absfr1, setup->absfr1, (size_t)t.n1), "mw3_create: absfr1");
   CK(upload_raw(&c->d_absfr2, setup->absfr2, (size_t)t.n2), "mw3_create: absfr2");
   CK(alloc_zero((void**)&c->d_psi, nd * sizeof(double)), "mw3_create: psi");
+  c->graph_psi0 = c->d_psi;
   CK(alloc_zero((void**)&c->d_psim1, nd * sizeof(double)), "mw3_create: psi_m1");
   CK(alloc_zero((void**)&c->d_psi0, nd * sizeof(double)), "mw3_create: psi0");
   CK(alloc_zero((void**)&c->d_hout, nd * sizeof(double)), "mw3_create: hout");
@@ -493,7 +507,11 @@ int32_t mw3_cheby_begin(mw3_ctx* ctx, const double* psi0, double* stats3) {
   return MW3_OK;
 }
 
+static int run_deferred(Ctx* c);
+
 static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
+  int rd = run_deferred(c);
+  if (rd) return rd;
   if (c->cvj_pending) {  // the reactant-state sums of the ring are done first
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "ring flush: wait");
     c->cvj_pending = false;
@@ -515,30 +533,119 @@ static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
   return MW3_OK;
 }
 
+// Steps run as captured graphs when the context allows it: graph mode on,
+// not a step whose kernels are timed (timer events stay out of graphs; with
+// sampled timing the other steps still run as graphs) and no psi(E)
+// accumulation (its kernel takes the step index k as an argument).
+static bool graphs_usable(const Ctx* c) {
+  return c->graph_mode && !(c->timing && c->timing_step) && c->s.nofe <= 0;
+}
+
+// Captures one step as graph q (psi in buffer q): on the second stream the
+// projection sums of the step before (ring slot from the device counter,
+// which they advance), beside this step's H psi; then, once both are done,
+// the update (it rewrites the partials those sums read).  Every stream the
+// capture forks joins the main stream again before the update.
+static hipError_t capture_step(Ctx* c, int q) {
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  c->graph_slot_dev = true;
+  c->slice_fresh = false;
+  auto body = [&]() -> hipError_t {
+    hipError_t r;
+    if ((r = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return r;
+    if ((r = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return r;
+    if ((r = launch_tail_sums(c, 0, c->aux)) != hipSuccess) return r;
+    if ((r = launch_slot_next(c, c->aux)) != hipSuccess) return r;
+    if ((r = hipEventRecord(c->ev_cvj, c->aux)) != hipSuccess) return r;
+    c->cvj_pending = true;
+    if ((r = launch_hpsi(c, c->d_psi)) != hipSuccess) return r;
+    if (!c->cvj_joined && (r = hipStreamWaitEvent(c->stream, c->ev_cvj, 0)) != hipSuccess)
+      return r;
+    return launch_update(c, 2, c->d_psi, true);
+  };
+  hipError_t eb = body();
+  hipGraph_t g = nullptr;
+  e = hipStreamEndCapture(c->stream, &g);
+  c->graph_slot_dev = false;
+  c->cvj_pending = false;  // the graph's second-stream work ends inside the graph
+  c->cvj_joined = false;
+  c->slice_fresh = false;
+  if (eb != hipSuccess) e = eb;
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->graph[q], g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  return e;
+}
+
+void drop_graphs(Ctx* c) {
+  for (auto& g : c->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// the projection sums of the last step, when a captured step left them
+// pending, enqueued the ordinary way
+static int run_deferred(Ctx* c) {
+  if (c->deferred_slot < 0) return MW3_OK;
+  const int slot = c->deferred_slot;
+  c->deferred_slot = -1;
+  HIPCHK(c, launch_projections(c, slot), "mw3_cheby_steps: projections");
+  return MW3_OK;
+}
+
 // Enqueues one Chebyshev step of c (H psi_k, update into psim1 = psi_{k+1},
 // projections, swap) and flushes the ring when it is full or when this is step
-// nsteps of the call; *done counts the call's steps (output index).
+// nsteps of the call; *done counts the call's steps (output index).  In graph
+// mode a step's projection sums run inside the next step's graph (or before
+// the flush), on the second stream as before.
 static int step_enqueue(Ctx* c, int32_t* done, int32_t nsteps, double* norms, double* autocorr) {
   if (c->ring_fill == 0) c->ring_k0 = c->k + 1;
   int base = *done - c->ring_fill;  // output index of ring slot 0
   c->timing_step = (c->k % c->timing_every) == 0;
-  HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
-  // the update rewrites the slice the last k_cvj reads (already waited for
-  // through ev_ang when the angular term ran on the third stream)
-  if (c->cvj_pending && !c->cvj_joined)
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
-  HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
-  HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
-  c->slice_fresh = true;  // nothing below writes psi before the next H psi
+  const bool graphs = graphs_usable(c);
+  if (graphs && c->deferred_slot >= 0) {
+    const int q = c->d_psi == c->graph_psi0 ? 0 : 1;
+    if (!c->graph[q]) HIPCHK(c, capture_step(c, q), "mw3_cheby_steps: step capture");
+    HIPCHK(c, hipGraphLaunch(c->graph[q], c->stream), "mw3_cheby_steps: step graph");
+  } else {
+    int rd = run_deferred(c);  // a graph step before this one left its sums pending
+    if (rd) return rd;
+    HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
+    // the update rewrites the slice the last k_cvj reads (already waited for
+    // through ev_ang when the angular term ran on the third stream)
+    if (c->cvj_pending && !c->cvj_joined)
+      HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
+    HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
+    if (graphs) {  // the next step's graph takes this step's sums (slot counter set here)
+      HIPCHK(c, launch_slot_set(c, c->ring_fill, c->stream), "mw3_cheby_steps: slot");
+    } else {
+      HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
+      c->slice_fresh = true;  // nothing below writes psi before the next H psi
+    }
+  }
+  if (graphs) c->deferred_slot = c->ring_fill;
   c->k += 1;
-  HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
+  if (!c->trial) HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
   std::swap(c->d_psi, c->d_psim1);
   std::swap(c->d_psi32, c->d_psim132);
   c->ring_fill += 1;
   *done += 1;
+  if (c->trial && (c->ring_fill == kRing || *done == nsteps)) {
+    // autotuner trial: the step's sums end, the ring is discarded unread
+    int rc = run_deferred(c);
+    if (rc) return rc;
+    if (c->cvj_pending) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "trial: wait");
+    c->cvj_pending = false;
+    c->ring_fill = 0;
+    return MW3_OK;
+  }
   if (c->ring_fill == kRing || *done == nsteps) {
+    int rc = run_deferred(c);
+    if (rc) return rc;
     c->timing_step = true;  // the energy sums of every ring are timed
-    int rc = flush_ring(c, norms ? norms + base : nullptr, autocorr ? autocorr + base : nullptr);
+    rc = flush_ring(c, norms ? norms + base : nullptr, autocorr ? autocorr + base : nullptr);
     if (rc) return rc;
   }
   c->timing_step = true;
@@ -739,6 +846,7 @@ int32_t mw3_prod_attach(mw3_ctx* ctx, const mw3_prod_setup* ps) {
   // steps already in the ring are accumulated without the product projection
   int rc = flush_ring(c, nullptr, nullptr);
   if (rc) return rc;
+  drop_graphs(c);  // captured steps hold the kernels of the old set-up
   free_prod(c);
   c->pr_n2p0 = ps->n2prod0 - 1;
   c->pr_nap0 = ps->nangproj0 - 1;
@@ -907,6 +1015,173 @@ int32_t mw3_state_put(mw3_ctx* ctx, const double* psi_km1, const double* psi_k, 
   HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_state_put");
   c->k = k;
   c->begun = true;
+  return MW3_OK;
+}
+
+int32_t mw3_set_option(mw3_ctx* ctx, int32_t option, int32_t value) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  int rc = flush_ring(c, nullptr, nullptr);  // no step of the old setting left pending
+  if (rc) return rc;
+  switch (option) {
+    case MW3_OPT_GRAPHS:
+      if (value != 0 && value != 1) return seterr(c, MW3_EINVAL, "mw3_set_option: graphs 0 or 1");
+      c->graph_mode = value;
+      return MW3_OK;
+    case MW3_OPT_RADIAL:
+      if (value != 0 && value != 1) return seterr(c, MW3_EINVAL, "mw3_set_option: radial 0 or 1");
+      if (value != c->opt_radial) drop_graphs(c);
+      c->opt_radial = value;
+      return MW3_OK;
+    case MW3_OPT_ANGULAR:
+      if (value < 0 || value > 2) return seterr(c, MW3_EINVAL, "mw3_set_option: angular 0..2");
+      if (value != c->opt_angular) drop_graphs(c);
+      c->opt_angular = value;
+      return MW3_OK;
+    case MW3_OPT_SIDE_STREAM:
+      if (value != 0 && value != 1)
+        return seterr(c, MW3_EINVAL, "mw3_set_option: side stream 0 or 1");
+      if (value != c->opt_side) drop_graphs(c);
+      c->opt_side = value;
+      return MW3_OK;
+    default:
+      return seterr(c, MW3_EINVAL, "mw3_set_option: unknown option");
+  }
+}
+
+// the variant each option value selects in this context (the fallback where
+// the kernel of that value is not built): radial 1 needs the persistent R
+// passes or the fused r passes; angular 0 the folded kernel, 1 a one-pass one
+static void effective_options(const Ctx* c, int32_t* out) {
+  out[MW3_OPT_GRAPHS] = c->graph_mode;
+  out[MW3_OPT_RADIAL] = c->opt_radial && (c->rad_nitems || c->mid_nitems) ? 1 : 0;
+  int a = c->opt_angular;
+  if (a == 0 && !c->ang_fold) a = 1;
+  if (a == 1 && !c->ang_nt && !c->ang_fold) a = 2;
+  if (a == 1 && !c->ang_nt) a = 0;
+  out[MW3_OPT_ANGULAR] = a;
+  out[MW3_OPT_SIDE_STREAM] = c->opt_side;
+}
+
+int32_t mw3_get_option(const mw3_ctx* ctx, int32_t option, int32_t* value) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(ctx);
+  if (!c || !value) return MW3_EINVAL;
+  if (option < 0 || option >= MW3_NOPTIONS) return MW3_EINVAL;
+  int32_t all[MW3_NOPTIONS];
+  effective_options(c, all);
+  *value = all[option];
+  return MW3_OK;
+}
+
+// One autotuner trial: the context's current options over 2 + n steps from
+// the current state, timed over the last n with events on the main stream
+// once every stream has joined it.  No ring flush, no psi(E) sum.
+static int trial_ms(Ctx* c, int n, hipEvent_t e0, hipEvent_t e1, double* ms) {
+  int32_t done = 0;
+  c->trial = true;
+  int rc = MW3_OK;
+  while (done < 2 && rc == MW3_OK) rc = step_enqueue(c, &done, 2, nullptr, nullptr);
+  if (rc == MW3_OK) {
+    HIPCHK(c, hipEventRecord(e0, c->stream), "mw3_autotune: event");
+    done = 0;
+    while (done < n && rc == MW3_OK) rc = step_enqueue(c, &done, n, nullptr, nullptr);
+  }
+  c->trial = false;
+  if (rc) return rc;
+  HIPCHK(c, hipEventRecord(e1, c->stream), "mw3_autotune: event");
+  HIPCHK(c, hipEventSynchronize(e1), "mw3_autotune: sync");
+  float t = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&t, e0, e1), "mw3_autotune: elapsed");
+  *ms = (double)t / n;
+  return MW3_OK;
+}
+
+int32_t mw3_autotune(mw3_ctx* ctx, int32_t trial_steps, int32_t* chosen, double* ms_per_step) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  if (trial_steps < 2 || trial_steps > kRing - 2)
+    return seterr(c, MW3_EINVAL, "mw3_autotune: trial_steps must be in 2..254");
+  const int n = trial_steps & ~1;  // even: the psi buffers end where they began
+  int rc = flush_ring(c, nullptr, nullptr);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_autotune: sync");
+  const bool timing = c->timing;
+  c->timing = false;
+  // the state is saved in the two spare dense slabs and restored at the end
+  const size_t bytes = sizeof(double) * (size_t)c->t.nc * c->t.nslab;
+  double* const p0 = c->d_psi;
+  double* const pm0 = c->d_psim1;
+  HIPCHK(c, hipMemcpyAsync(c->d_hout, c->d_psi, bytes, hipMemcpyDeviceToDevice, c->stream),
+         "mw3_autotune: save");
+  HIPCHK(c, hipMemcpyAsync(c->d_scratch, c->d_psim1, bytes, hipMemcpyDeviceToDevice, c->stream),
+         "mw3_autotune: save");
+  const int k0 = c->k, rk0 = c->ring_k0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(c, hipEventCreate(&e0), "mw3_autotune: event");
+  hipError_t he = hipEventCreate(&e1);
+  if (he != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return seterr(c, MW3_EDEVICE, "mw3_autotune: event", he);
+  }
+  // candidates: every combination of the options, skipping values that fall
+  // back to a combination already listed
+  const int32_t saved[4] = {c->graph_mode, c->opt_radial, c->opt_angular, c->opt_side};
+  std::vector<std::array<int32_t, 4>> cands;
+  for (int g = 0; g <= 1; g++)
+    for (int r = 0; r <= 1; r++)
+      for (int a = 0; a <= 2; a++)
+        for (int sd = 0; sd <= 1; sd++) {
+          c->graph_mode = g;
+          c->opt_radial = r;
+          c->opt_angular = a;
+          c->opt_side = sd;
+          int32_t eff[MW3_NOPTIONS];
+          effective_options(c, eff);
+          std::array<int32_t, 4> key = {eff[0], eff[1], eff[2], eff[3]};
+          if (std::find(cands.begin(), cands.end(), key) == cands.end()) cands.push_back(key);
+        }
+  double best = 1e30;
+  std::array<int32_t, 4> best_key = {saved[0], saved[1], saved[2], saved[3]};
+  for (const auto& key : cands) {
+    drop_graphs(c);
+    c->graph_mode = key[0];
+    c->opt_radial = key[1];
+    c->opt_angular = key[2];
+    c->opt_side = key[3];
+    double ms = 0;
+    rc = trial_ms(c, n, e0, e1, &ms);
+    if (rc) break;
+    if (ms < best) {
+      best = ms;
+      best_key = key;
+    }
+  }
+  drop_graphs(c);
+  c->graph_mode = best_key[0];
+  c->opt_radial = best_key[1];
+  c->opt_angular = best_key[2];
+  c->opt_side = best_key[3];
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->timing = timing;
+  // restore the state (an even number of trial steps leaves the buffers in place)
+  c->d_psi = p0;
+  c->d_psim1 = pm0;
+  c->k = k0;
+  c->ring_k0 = rk0;
+  c->ring_fill = 0;
+  c->deferred_slot = -1;
+  c->slice_fresh = false;
+  HIPCHK(c, hipMemcpyAsync(c->d_psi, c->d_hout, bytes, hipMemcpyDeviceToDevice, c->stream),
+         "mw3_autotune: restore");
+  HIPCHK(c, hipMemcpyAsync(c->d_psim1, c->d_scratch, bytes, hipMemcpyDeviceToDevice, c->stream),
+         "mw3_autotune: restore");
+  HIPCHK(c, launch_to_f32(c, c->d_psi, c->d_psi32), "mw3_autotune: FP32 copy");
+  HIPCHK(c, launch_to_f32(c, c->d_psim1, c->d_psim132), "mw3_autotune: FP32 copy");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_autotune: sync");
+  if (rc) return rc;
+  if (chosen) effective_options(c, chosen);
+  if (ms_per_step) *ms_per_step = best;
   return MW3_OK;
 }
 

@@ -264,6 +264,25 @@ struct Ctx {
   int k = 0;
   bool begun = false;
 
+  // captured steps (hipGraph): graph[q] is one step -- the projection sums of
+  // the step before on the second stream beside this step's H psi, then the
+  // update -- for psi in buffer q (0: the buffer d_psi held at creation);
+  // d_slot is the ring slot of those projection sums, advanced by the graph
+  int graph_mode = 1;             // 0: off; 1: on when the context allows it
+  // execution options (include/mw3.h MW3_OPT_*): radial passes as the
+  // persistent low-mode kernels (1) or as GEMMs (0); angular term by the
+  // parity-folded one-pass kernel (0), the unfolded one-pass kernel (1) or
+  // GEMMs (2), each falling back to the next where not built; the angular
+  // term on the third stream (1) or the main one (0)
+  int opt_radial = 1, opt_angular = 0, opt_side = 0;
+  bool trial = false;             // autotuner trial steps: no ring flush, no psi(E)
+  bool graph_slot_dev = false;    // kernels being enqueued read their slot from d_slot
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  double* graph_psi0 = nullptr;   // d_psi at creation (buffer 0)
+  int32_t* d_slot = nullptr;
+  int deferred_slot = -1;         // ring slot whose projection sums are not yet enqueued
+  hipEvent_t ev_fork = nullptr;
+
   // timing
   bool timing = false;
   int timing_every = 1;     // time every N-th step
@@ -289,6 +308,7 @@ hipError_t launch_spectrum(hipStream_t s, const double* d_auto, double* d_a, int
 // 0: H psi into d_hout, 1: first step, 2: damped step (psi_new into d_psim1)
 hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj = false);
 hipError_t launch_projections(Ctx* c, int slot);
+hipError_t launch_tail_sums(Ctx* c, int slot, hipStream_t st);
 hipError_t launch_energy_accum(Ctx* c);
 hipError_t launch_psie(Ctx* c, int k);
 hipError_t launch_zero(Ctx* c, void* p, size_t bytes);
@@ -331,6 +351,8 @@ enum KernelId {
 };
 
 hipError_t launch_update_tail(Ctx* c, int mode, int slot, bool proj, hipStream_t st);
+hipError_t launch_slot_set(Ctx* c, int v, hipStream_t st);
+hipError_t launch_slot_next(Ctx* c, hipStream_t st);
 void timer_begin(Ctx* c, int id, hipStream_t s = nullptr);
 void timer_end(Ctx* c, int id, hipStream_t s = nullptr);
 

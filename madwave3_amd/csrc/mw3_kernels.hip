@@ -829,11 +829,46 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
 
 // flux columns of each (channel, angle, 64-R chunk) from the partials of its
 // work items (in item order); chunks with no listed rows write zeros
+// ring slot of a per-step projection kernel: `s`, or, inside a captured step
+// (one graph serves every slot), the value a device counter holds when the
+// kernel runs; the counter advances after the step's last projection kernel
+struct RingSlot {
+  const int32_t* d;
+  int32_t s;
+  __device__ int64_t get() const { return d ? (int64_t)*d : (int64_t)s; }
+};
+
+static RingSlot ring_slot(const Ctx* c, int slot) {
+  return RingSlot{c->graph_slot_dev ? c->d_slot : nullptr, slot};
+}
+
+__global__ void k_slot_set(int32_t* d, int32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d = v;
+}
+
+__global__ void k_slot_next(int32_t* d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *d = *d + 1;
+}
+
+hipError_t launch_slot_set(Ctx* c, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_slot_set, dim3(1), dim3(64), 0, st, c->d_slot, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_slot_next(Ctx* c, hipStream_t st) {
+  hipLaunchKernelGGL(k_slot_next, dim3(1), dim3(64), 0, st, c->d_slot);
+  return hipGetLastError();
+}
+
 __global__ __launch_bounds__(64) void k_update_flux(const double* __restrict__ upart,
                                                     const int32_t* __restrict__ gfirst,
                                                     const int32_t* __restrict__ fmap,
-                                                    double* __restrict__ zr,
-                                                    double* __restrict__ zdr, int n2) {
+                                                    double* __restrict__ zr_ring,
+                                                    double* __restrict__ zdr_ring, int n2,
+                                                    int64_t nflux, RingSlot rs) {
+  const int64_t slot = rs.get();
+  double* __restrict__ zr = zr_ring + slot * nflux;
+  double* __restrict__ zdr = zdr_ring + 2 * slot * nflux;
   const int g = blockIdx.x, lane = threadIdx.x;
   const int nrc = (n2 + 63) / 64;
   const int rc = g % nrc, cia = g / nrc;
@@ -885,7 +920,9 @@ static void dispatch_update(Ctx* c, unsigned nb, const double* psi, double* out,
           : launch_update_kernel<T, 2, SINGLE, false, false, false, CPL4>(c, nb, psi, out, a);
 }
 
-__global__ void k_reduce_parts(const double* __restrict__ part, int nb, double* __restrict__ out) {
+__global__ void k_reduce_parts(const double* __restrict__ part, int nb,
+                               double* __restrict__ out_ring, RingSlot rs) {
+  double* __restrict__ out = out_ring + 2 * (int64_t)rs.get();
   __shared__ double sh[2][256];
   // four independent partial sums per thread, so that four loads are in
   // flight at a time instead of one
@@ -933,8 +970,10 @@ constexpr int CVJ_WAVES = 16;
 __global__ __launch_bounds__(64 * CVJ_WAVES) void k_cvj(const double* __restrict__ slice,
                                                         const double* __restrict__ dcvj,
                                                         const double* __restrict__ fdv,
-                                                        double* __restrict__ cvj, int nj, int nv,
-                                                        int na, int n1) {
+                                                        double* __restrict__ cvj_ring, int nj,
+                                                        int nv, int na, int n1, int64_t ncvj,
+                                                        RingSlot rs) {
+  double* __restrict__ cvj = cvj_ring + (int64_t)rs.get() * ncvj;
   const int c = blockIdx.x / nj, jj = blockIdx.x % nj;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   extern __shared__ double xj[];  // [n1]
@@ -2646,12 +2685,17 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // measured 3.5% faster per step than one stream at configuration 2, while
   // the FP64 pair ran 2% slower side by side (its angular kernel holds a
   // whole CU's LDS)
-  hipStream_t sa = c->precision == MW3_MIXED ? c->side : s;
+  hipStream_t sa = c->opt_side ? c->side : s;
+  // kernel variants of this context's options (the autotuner's choices)
+  const bool use_rad = c->rad_nitems && c->opt_radial;
+  const bool use_mid = c->mid_nitems && c->opt_radial;
+  const bool use_fold = c->ang_fold && c->opt_angular == 0;
+  const bool use_one = c->ang_nt && c->opt_angular <= 1 && !use_fold;
   // the forward R pass first (measured 1.3% faster per FP64 step than after
   // the angular term)
   timer_begin(c, K_DST_R);
-  e = c->rad_nitems ? run_rad<T>(c, psi, t1, t2, 0)
-                    : run_gemm<G_DST_R, T, T, T, T>(s, psi, tab, t1, c->p_dstR);
+  e = use_rad ? run_rad<T>(c, psi, t1, t2, 0)
+              : run_gemm<G_DST_R, T, T, T, T>(s, psi, tab, t1, c->p_dstR);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_R);
   if (sa != s) {
@@ -2665,10 +2709,10 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   c->slice_fresh = false;
   c->cvj_joined = false;
   timer_begin(c, K_ANG_FWD, sa);
-  if (c->ang_fold) {
+  if (use_fold) {
     if ((e = launch_ang_fold<T>(c, psi, sa)) != hipSuccess) return e;
     timer_end(c, K_ANG_FWD, sa);
-  } else if (c->ang_nt) {
+  } else if (use_one) {
     if ((e = launch_ang_fused<T>(c, psi, sa)) != hipSuccess) return e;
     timer_end(c, K_ANG_FWD, sa);
   } else if (c->ang_fused) {
@@ -2713,7 +2757,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     if ((e = hipGetLastError()) != hipSuccess) return e;
     timer_end(c, K_CORIOLIS, sa);
   }
-  if (!c->ang_nt && !c->ang_fold) {
+  if (!use_one && !use_fold) {
     timer_begin(c, K_ANG_BWD, sa);
     e = run_gemm<G_ANG_BWD, T, T, T, T>(sa, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang,
                                         c->p_angB);
@@ -2734,7 +2778,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // update forms T psi = radcut psi - d_t2 on rows r < nn1
   KinParams kp{c->d_nn1, c->d_p2r1, c->d_p2r2, c->d_divi, t.radcut, t.n1};
   timer_begin(c, K_DST_r);
-  if (c->mid_nitems) {
+  if (use_mid) {
     if ((e = launch_rad_mid<T>(c)) != hipSuccess) return e;
   } else {
     e = run_gemm<G_DST_r_KIN, T, T, T, T>(s, tab, t1, t2, c->p_dstr, kp);
@@ -2744,8 +2788,8 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   }
   timer_end(c, K_DST_r);
   timer_begin(c, K_DST_RINV);
-  e = c->rad_nitems ? run_rad<T>(c, psi, t1, t2, 1)
-                    : run_gemm<G_DST_R_INV, T, T, T, T>(s, t1, tab, t2, c->p_dstRi);
+  e = use_rad ? run_rad<T>(c, psi, t1, t2, 1)
+              : run_gemm<G_DST_R_INV, T, T, T, T>(s, t1, tab, t2, c->p_dstRi);
   if (e != hipSuccess) return e;
   timer_end(c, K_DST_RINV);
   return sa != s ? hipStreamWaitEvent(s, c->ev_ang, 0) : hipSuccess;
@@ -2855,14 +2899,14 @@ hipError_t launch_update_tail(Ctx* c, int mode, int slot, bool proj, hipStream_t
   if (mode == 2 && proj && t.nflux > 0 && c->d_fmap) {
     const int ng = t.nc * t.na * ((t.n2 + 63) / 64);
     hipLaunchKernelGGL(k_update_flux, dim3(ng), dim3(64), 0, st, c->d_uw_part, c->d_uw_gfirst,
-                       c->d_fmap, c->d_ring_zr + (int64_t)slot * t.nflux,
-                       c->d_ring_zdr + (int64_t)2 * slot * t.nflux, t.n2);
+                       c->d_fmap, c->d_ring_zr, c->d_ring_zdr, t.n2, t.nflux,
+                       ring_slot(c, slot));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   if (mode != 0)
     hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(256), 0, st, c->d_norm_part,
-                       c->nnorm_blocks, c->d_ring_norm + 2 * slot);
+                       c->nnorm_blocks, c->d_ring_norm, ring_slot(c, slot));
   return hipGetLastError();
 }
 
@@ -2872,33 +2916,39 @@ hipError_t launch_update(Ctx* c, int mode, const double* psi, bool proj) {
   return update_impl<double>(c, mode, slot, psi, proj);
 }
 
+// the per-step sums of the update's partials and slice on stream st: flux
+// columns and norm, reactant-state and product-state sums, into ring slot
+hipError_t launch_tail_sums(Ctx* c, int slot, hipStream_t st) {
+  const HostTables& t = c->t;
+  hipError_t e;
+  timer_begin(c, K_UPDATE_TAIL, st);
+  if ((e = launch_update_tail(c, 2, slot, true, st)) != hipSuccess) return e;
+  timer_end(c, K_UPDATE_TAIL, st);
+  if (t.ncvj > 0) {
+    timer_begin(c, K_PROJ, st);
+    int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
+    hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(64 * CVJ_WAVES), sizeof(double) * t.n1, st,
+                       c->d_slice, c->d_dcvj, c->d_fdv, c->d_ring_cvj, nj, nv, t.na, t.n1,
+                       (int64_t)t.ncvj, ring_slot(c, slot));
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    timer_end(c, K_PROJ, st);
+  }
+  if (c->prod_on && (e = launch_prod_step(c, slot, st)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
 // reactant-state amplitudes of the slice the update wrote, into ring slot
 hipError_t launch_projections(Ctx* c, int slot) {
-  const HostTables& t = c->t;
-  hipError_t e = hipSuccess;
-  {
-    // on the second stream, after the update that wrote the slice and the
-    // partials: the flux columns and the norm of the step, then the
-    // reactant- and product-state sums
-    if ((e = hipEventRecord(c->ev_slice, c->stream)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(c->aux, c->ev_slice, 0)) != hipSuccess) return e;
-    timer_begin(c, K_UPDATE_TAIL, c->aux);
-    if ((e = launch_update_tail(c, 2, slot, true, c->aux)) != hipSuccess) return e;
-    timer_end(c, K_UPDATE_TAIL, c->aux);
-    if (t.ncvj > 0) {
-      timer_begin(c, K_PROJ, c->aux);
-      int nj = c->s.jmax - c->s.jini + 1, nv = c->s.nvmax - c->s.nvini + 1;
-      hipLaunchKernelGGL(k_cvj, dim3(t.nc * nj), dim3(64 * CVJ_WAVES), sizeof(double) * t.n1,
-                         c->aux, c->d_slice, c->d_dcvj, c->d_fdv,
-                         c->d_ring_cvj + (int64_t)slot * t.ncvj, nj, nv, t.na, t.n1);
-      if ((e = hipGetLastError()) != hipSuccess) return e;
-      timer_end(c, K_PROJ, c->aux);
-    }
-    if (c->prod_on && (e = launch_prod_step(c, slot, c->aux)) != hipSuccess) return e;
-    if ((e = hipEventRecord(c->ev_cvj, c->aux)) != hipSuccess) return e;
-    c->cvj_pending = true;
-  }
-  return e;
+  hipError_t e;
+  // on the second stream, after the update that wrote the slice and the
+  // partials: the flux columns and the norm of the step, then the
+  // reactant- and product-state sums
+  if ((e = hipEventRecord(c->ev_slice, c->stream)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(c->aux, c->ev_slice, 0)) != hipSuccess) return e;
+  if ((e = launch_tail_sums(c, slot, c->aux)) != hipSuccess) return e;
+  if ((e = hipEventRecord(c->ev_cvj, c->aux)) != hipSuccess) return e;
+  c->cvj_pending = true;
+  return hipSuccess;
 }
 
 hipError_t launch_energy_accum(Ctx* c) {
@@ -2965,9 +3015,11 @@ __global__ __launch_bounds__(64) void k_prod_fun(const double* __restrict__ ppar
                                                  const int32_t* __restrict__ gfirst,
                                                  const double* __restrict__ dtrans,
                                                  const int32_t* __restrict__ ch_elec,
-                                                 double* __restrict__ ring, int nc, int ne, int na,
-                                                 int n2, int nap0, int n2p0, int n2p, int nomb,
-                                                 int rc0, int nrcp, int64_t L) {
+                                                 double* __restrict__ ring_all, int nc, int ne,
+                                                 int na, int n2, int nap0, int n2p0, int n2p,
+                                                 int nomb, int rc0, int nrcp, int64_t L,
+                                                 RingSlot rs) {
+  double* __restrict__ ring = ring_all + (int64_t)rs.get() * nomb * ne * L;
   extern __shared__ double f[];  // [nc][64]
   const int lane = threadIdx.x;
   const int a = blockIdx.x / nrcp, rc = rc0 + blockIdx.x % nrcp;
@@ -3013,9 +3065,9 @@ hipError_t launch_prod_step(Ctx* c, int slot, hipStream_t s) {
   timer_begin(c, K_PROD, s);
   hipLaunchKernelGGL(k_prod_fun, dim3((unsigned)(c->pr_nap * nrcp)), dim3(64),
                      sizeof(double) * 64 * t.nc, s, c->d_ppart,
-                     c->d_uw_gfirst, c->d_dtrans, c->d_ch_elec,
-                     c->d_ring_prod + (int64_t)slot * c->pr_nomb * t.ne * c->pr_L, t.nc, t.ne,
-                     t.na, t.n2, c->pr_nap0, c->pr_n2p0, c->pr_n2p, c->pr_nomb, rc0, nrcp, c->pr_L);
+                     c->d_uw_gfirst, c->d_dtrans, c->d_ch_elec, c->d_ring_prod, t.nc, t.ne,
+                     t.na, t.n2, c->pr_nap0, c->pr_n2p0, c->pr_n2p, c->pr_nomb, rc0, nrcp, c->pr_L,
+                     ring_slot(c, slot));
   hipError_t e = hipGetLastError();
   timer_end(c, K_PROD, s);
   return e;

@@ -13,12 +13,17 @@ import numpy as np
 
 __all__ = ["Mw3Setup", "Mw3ProdSetup", "make_setup", "make_prod_setup", "load_library",
            "Propagator", "Mw3Error",
-           "MW3_FP64", "MW3_MIXED", "LIB_PATH"]
+           "MW3_FP64", "MW3_MIXED", "MW3_OPT_GRAPHS", "MW3_OPT_RADIAL", "MW3_OPT_ANGULAR",
+           "MW3_OPT_SIDE_STREAM", "OPTION_NAMES", "LIB_PATH"]
 
 MW3_ABI_VERSION = 1
 MW3_NKERNELS = 11  # include/mw3.h
 MW3_FP64 = 0
 MW3_MIXED = 1
+# include/mw3.h execution options
+MW3_OPT_GRAPHS, MW3_OPT_RADIAL, MW3_OPT_ANGULAR, MW3_OPT_SIDE_STREAM = 0, 1, 2, 3
+MW3_NOPTIONS = 4
+OPTION_NAMES = ("graphs", "radial", "angular", "side_stream")
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmw3.so")
 
 _pi = ct.POINTER(ct.c_int32)
@@ -284,6 +289,9 @@ def load_library(path=None):
     lib.mw3_cheby_spectrum.argtypes = [ct.c_void_p, ct.c_int32, ct.c_double, ct.c_double,
                                        ct.c_double, ct.c_double, ct.c_void_p, ct.c_int32,
                                        ct.c_void_p]
+    lib.mw3_autotune.argtypes = [ct.c_void_p, ct.c_int32, _pi, _pd]
+    lib.mw3_set_option.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32]
+    lib.mw3_get_option.argtypes = [ct.c_void_p, ct.c_int32, _pi]
     lib.mw3_cheby_spectrum_noalloc.restype = ct.c_int32
     lib.mw3_cheby_spectrum_noalloc.argtypes = [ct.c_void_p, ct.c_int32, ct.c_double, ct.c_double,
                                                ct.c_double, ct.c_double, ct.c_void_p, ct.c_int32,
@@ -431,6 +439,26 @@ class Propagator:
         p0 = None if psi0 is None else np.ascontiguousarray(psi0, dtype=np.float64)
         self._check(self.lib.mw3_state_put(self.h, _dp(psim1), _dp(psi), k, _dp(p0)),
                     "mw3_state_put")
+
+    def set_option(self, option, value):
+        self._check(self.lib.mw3_set_option(self.h, int(option), int(value)), "mw3_set_option")
+
+    def options(self):
+        """{name: value} of the execution options in effect."""
+        return {n: self.get_option(i) for i, n in enumerate(OPTION_NAMES)}
+
+    def autotune(self, steps=16):
+        """mw3_autotune: times every option combination over `steps` trial
+        steps and keeps the fastest; returns ({name: value}, ms per step)."""
+        ch = (ct.c_int32 * MW3_NOPTIONS)()
+        ms = ct.c_double()
+        self._check(self.lib.mw3_autotune(self.h, int(steps), ch, ct.byref(ms)), "mw3_autotune")
+        return dict(zip(OPTION_NAMES, list(ch))), ms.value
+
+    def get_option(self, option):
+        v = ct.c_int32()
+        self._check(self.lib.mw3_get_option(self.h, int(option), ct.byref(v)), "mw3_get_option")
+        return v.value
 
     def timing(self, on=True, every=1):
         """HIP-event timing of the kernels; every > 1 times every N-th step only."""
