@@ -92,11 +92,17 @@ def per_kernel(times, steps):
             for k, (ms, cnt) in times.items()}
 
 
-def run_gpu(case, precision, steps, warmup, timing):
+def run_gpu(case, precision, steps, warmup, timing, tune=0):
     from madwave3_amd.lib import Propagator
 
     g = Propagator(case, precision)
     g.begin(case.rpaq0)
+    tuned = None
+    if tune:
+        # the autotuner picks the kernel variants and stream/graph layout from
+        # timed trial steps; the state is left as it was
+        opts, ms = g.autotune(tune)
+        tuned = {"options": opts, "trial_ms_per_step": ms}
     if warmup:
         g.steps(warmup)
     if timing:
@@ -109,8 +115,10 @@ def run_gpu(case, precision, steps, warmup, timing):
     times = g.kernel_times() if timing else {}
     P = g.flux_result()[0]
     nflux = g.nflux
+    if tuned is None:
+        tuned = {"options": g.options()}
     g.close()
-    return dt, times, P, nflux
+    return dt, times, P, nflux, tuned
 
 
 def build_case_j(J):
@@ -312,6 +320,17 @@ def load_traffic(kernel):
         return None
 
 
+def load_accuracy(cfg):
+    """Full-length mixed vs FP64 differences of configuration cfg from the
+    committed summary of `bench.py --accuracy` runs, or None."""
+    path = os.path.join(ROOT, "profiles", "accuracy.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(str(cfg))
+    except (OSError, ValueError):
+        return None
+
+
 def cpu_baseline(case, nsteps):
     from oracle.oracle import Oracle, set_threads
 
@@ -338,6 +357,8 @@ def main():
     ap.add_argument("--no-timing", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run a copy of this many MiB first (byte-counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default execution options instead of autotuning them")
     ap.add_argument("--accuracy", action="store_true",
                     help="full-length mixed vs FP64 run of --config instead of the timing")
     ap.add_argument("--full-steps", type=int, default=0,
@@ -361,11 +382,16 @@ def main():
     case = build_case(a.config)
     print("# case %s built in %.1fs" % (case, time.perf_counter() - t0), file=sys.stderr, flush=True)
     N = case.ncan * case.nangu * case.npun1 * case.npun2
-    dt64, tim64, P64, nflux = run_gpu(case, MW3_FP64, a.steps, a.warmup, not a.no_timing)
-    print("# fp64: %.3f ms/step" % (1e3 * dt64 / a.steps), file=sys.stderr, flush=True)
-    dtmx, timmx, Pmx, _ = run_gpu(case, MW3_MIXED, a.steps, a.warmup, not a.no_timing)
-    print("# mixed: %.3f ms/step" % (1e3 * dtmx / a.steps), file=sys.stderr, flush=True)
+    tune = 0 if a.no_autotune else 16
+    dt64, tim64, P64, nflux, tuned64 = run_gpu(case, MW3_FP64, a.steps, a.warmup,
+                                               not a.no_timing, tune)
+    print("# fp64: %.3f ms/step %s" % (1e3 * dt64 / a.steps, tuned64), file=sys.stderr, flush=True)
+    dtmx, timmx, Pmx, _, tunedmx = run_gpu(case, MW3_MIXED, a.steps, a.warmup,
+                                           not a.no_timing, tune)
+    print("# mixed: %.3f ms/step %s" % (1e3 * dtmx / a.steps, tunedmx), file=sys.stderr,
+          flush=True)
     dP = float(np.max(np.abs(Pmx - P64)))
+    acc = load_accuracy(a.config)
 
     # roofline of the dominant kernel group of the headline (mixed) run: the
     # largest per-step time among the groups on the library's main stream
@@ -448,8 +474,14 @@ def main():
                    "parallelism": "single GPU"},
         "fp64_steps_per_s": a.steps / dt64,
         "fp64_ms_per_step": 1e3 * dt64 / a.steps,
-        "mixed_vs_fp64_max_abs_dP": dP,
-        "dP_steps": a.warmup + a.steps,
+        # the metric's error bound, measured over the configuration's full
+        # length (bench.py --accuracy; committed in profiles/accuracy.json)
+        "mixed_vs_fp64_max_abs_dP": acc.get("max_abs_dP") if acc else None,
+        "accuracy_full_run": acc,
+        # the same difference after this run's warmup + timed steps only
+        "short_run_max_abs_dP": dP,
+        "short_run_steps": a.warmup + a.steps,
+        "autotune": {"mixed": tunedmx, "fp64": tuned64},
         "roofline": roof,
         "kernels": kern,
         "kernels_fp64": {k: {"ms_per_step": v} for k, v in kernel_groups(tim64, a.steps).items()},

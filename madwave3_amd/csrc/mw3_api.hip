@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +15,7 @@
 using namespace mw3;
 
 static thread_local std::string g_create_err;
+static std::atomic<uint64_t> g_next_uid{1};
 
 // Device array of mw3_cheby_spectrum_noalloc: the Chebyshev coefficients of
 // one call (kmax + 1 doubles).  Every device the library runs on holds its
@@ -159,7 +161,14 @@ static void free_all(Ctx* c) {
       g = nullptr;
     }
   if (c->d_slot) (void)hipFree(c->d_slot);
+  for (auto& g : c->batch_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_bfork) (void)hipEventDestroy(c->ev_bfork);
+  if (c->ev_bjoin) (void)hipEventDestroy(c->ev_bjoin);
   if (c->ev_slice) (void)hipEventDestroy(c->ev_slice);
   if (c->ev_cvj) (void)hipEventDestroy(c->ev_cvj);
   if (c->ev_psi) (void)hipEventDestroy(c->ev_psi);
@@ -207,6 +216,7 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   }
   c->s = *setup;
   c->precision = precision;
+  c->uid = g_next_uid.fetch_add(1);
   // the angular term beside the radial passes on its own stream in mixed
   // precision (measured faster there; the FP64 pair ran slower side by side)
   c->opt_side = precision == MW3_MIXED ? 1 : 0;
@@ -237,6 +247,8 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(hipEventCreateWithFlags(&c->ev_psi, evf), "mw3_create: event");
   CK(hipEventCreateWithFlags(&c->ev_ang, evf), "mw3_create: event");
   CK(hipEventCreateWithFlags(&c->ev_fork, evf), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_bfork, evf), "mw3_create: event");
+  CK(hipEventCreateWithFlags(&c->ev_bjoin, evf), "mw3_create: event");
   CK(alloc_zero((void**)&c->d_slot, sizeof(int32_t)), "mw3_create: ring slot");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
@@ -541,14 +553,12 @@ static bool graphs_usable(const Ctx* c) {
   return c->graph_mode && !(c->timing && c->timing_step) && c->s.nofe <= 0;
 }
 
-// Captures one step as graph q (psi in buffer q): on the second stream the
+// The captured step body of c on its own streams: on the second stream the
 // projection sums of the step before (ring slot from the device counter,
 // which they advance), beside this step's H psi; then, once both are done,
-// the update (it rewrites the partials those sums read).  Every stream the
-// capture forks joins the main stream again before the update.
-static hipError_t capture_step(Ctx* c, int q) {
-  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return e;
+// the update (it rewrites the partials those sums read).  Every stream
+// forked from c's main stream joins it again before the update.
+static hipError_t capture_body(Ctx* c) {
   c->graph_slot_dev = true;
   c->slice_fresh = false;
   auto body = [&]() -> hipError_t {
@@ -564,17 +574,28 @@ static hipError_t capture_step(Ctx* c, int q) {
       return r;
     return launch_update(c, 2, c->d_psi, true);
   };
-  hipError_t eb = body();
-  hipGraph_t g = nullptr;
-  e = hipStreamEndCapture(c->stream, &g);
+  hipError_t e = body();
   c->graph_slot_dev = false;
   c->cvj_pending = false;  // the graph's second-stream work ends inside the graph
   c->cvj_joined = false;
   c->slice_fresh = false;
+  return e;
+}
+
+static hipError_t end_capture(hipStream_t s, hipError_t eb, hipGraphExec_t* out) {
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
   if (eb != hipSuccess) e = eb;
-  if (e == hipSuccess) e = hipGraphInstantiate(&c->graph[q], g, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   if (g) (void)hipGraphDestroy(g);
   return e;
+}
+
+// Captures one step of c as graph q (psi in buffer q).
+static hipError_t capture_step(Ctx* c, int q) {
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  return end_capture(c->stream, capture_body(c), &c->graph[q]);
 }
 
 void drop_graphs(Ctx* c) {
@@ -583,6 +604,40 @@ void drop_graphs(Ctx* c) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
+  for (auto& g : c->batch_graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  c->batch_of.clear();
+  c->graph_gen += 1;
+}
+
+// Captures one step of every member of a batch as one graph, held by the
+// first member: each member's body on its own streams, forked from and
+// joined back to the first member's main stream.
+static hipError_t capture_batch(Ctx** m, int n, int q) {
+  Ctx* c0 = m[0];
+  hipError_t e = hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  hipError_t eb = hipEventRecord(c0->ev_bfork, c0->stream);
+  for (int i = 0; i < n && eb == hipSuccess; i++) {
+    Ctx* c = m[i];
+    if (i > 0) eb = hipStreamWaitEvent(c->stream, c0->ev_bfork, 0);
+    if (eb == hipSuccess) eb = capture_body(c);
+    if (i > 0 && eb == hipSuccess) eb = hipEventRecord(c->ev_bjoin, c->stream);
+    if (i > 0 && eb == hipSuccess) eb = hipStreamWaitEvent(c0->stream, c->ev_bjoin, 0);
+  }
+  e = end_capture(c0->stream, eb, &c0->batch_graph[q]);
+  if (e == hipSuccess) {
+    c0->batch_of.resize(n);
+    c0->batch_gen.resize(n);
+    for (int i = 0; i < n; i++) {
+      c0->batch_of[i] = m[i]->uid;
+      c0->batch_gen[i] = m[i]->graph_gen;
+    }
+  }
+  return e;
 }
 
 // the projection sums of the last step, when a captured step left them
@@ -595,36 +650,45 @@ static int run_deferred(Ctx* c) {
   return MW3_OK;
 }
 
-// Enqueues one Chebyshev step of c (H psi_k, update into psim1 = psi_{k+1},
-// projections, swap) and flushes the ring when it is full or when this is step
-// nsteps of the call; *done counts the call's steps (output index).  In graph
-// mode a step's projection sums run inside the next step's graph (or before
-// the flush), on the second stream as before.
-static int step_enqueue(Ctx* c, int32_t* done, int32_t nsteps, double* norms, double* autocorr) {
+// Step of c, first part: ring bookkeeping and whether the step may run as a
+// graph.
+static bool step_pre(Ctx* c) {
   if (c->ring_fill == 0) c->ring_k0 = c->k + 1;
-  int base = *done - c->ring_fill;  // output index of ring slot 0
   c->timing_step = (c->k % c->timing_every) == 0;
-  const bool graphs = graphs_usable(c);
+  return graphs_usable(c);
+}
+
+// Step of c, second part: the kernels, as a graph or one by one.
+static int step_body(Ctx* c, bool graphs) {
   if (graphs && c->deferred_slot >= 0) {
     const int q = c->d_psi == c->graph_psi0 ? 0 : 1;
     if (!c->graph[q]) HIPCHK(c, capture_step(c, q), "mw3_cheby_steps: step capture");
     HIPCHK(c, hipGraphLaunch(c->graph[q], c->stream), "mw3_cheby_steps: step graph");
-  } else {
-    int rd = run_deferred(c);  // a graph step before this one left its sums pending
-    if (rd) return rd;
-    HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
-    // the update rewrites the slice the last k_cvj reads (already waited for
-    // through ev_ang when the angular term ran on the third stream)
-    if (c->cvj_pending && !c->cvj_joined)
-      HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
-    HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
-    if (graphs) {  // the next step's graph takes this step's sums (slot counter set here)
-      HIPCHK(c, launch_slot_set(c, c->ring_fill, c->stream), "mw3_cheby_steps: slot");
-    } else {
-      HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
-      c->slice_fresh = true;  // nothing below writes psi before the next H psi
-    }
+    return MW3_OK;
   }
+  int rd = run_deferred(c);  // a graph step before this one left its sums pending
+  if (rd) return rd;
+  HIPCHK(c, launch_hpsi(c, c->d_psi), "mw3_cheby_steps: H psi");
+  // the update rewrites the slice the last k_cvj reads (already waited for
+  // through ev_ang when the angular term ran on the third stream)
+  if (c->cvj_pending && !c->cvj_joined)
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_cvj, 0), "mw3_cheby_steps: wait");
+  HIPCHK(c, launch_update(c, 2, c->d_psi, true), "mw3_cheby_steps: update");
+  if (graphs) {  // the next step's graph takes this step's sums (slot counter set here)
+    HIPCHK(c, launch_slot_set(c, c->ring_fill, c->stream), "mw3_cheby_steps: slot");
+  } else {
+    HIPCHK(c, launch_projections(c, c->ring_fill), "mw3_cheby_steps: projections");
+    c->slice_fresh = true;  // nothing below writes psi before the next H psi
+  }
+  return MW3_OK;
+}
+
+// Step of c, last part: k, the psi buffers, and the ring flush when the ring
+// is full or when this is step nsteps of the call; *done counts the call's
+// steps (output index).
+static int step_post(Ctx* c, bool graphs, int32_t* done, int32_t nsteps, double* norms,
+                     double* autocorr) {
+  const int base = *done - c->ring_fill;  // output index of ring slot 0
   if (graphs) c->deferred_slot = c->ring_fill;
   c->k += 1;
   if (!c->trial) HIPCHK(c, launch_psie(c, c->k), "mw3_cheby_steps: psi(E)");
@@ -650,6 +714,16 @@ static int step_enqueue(Ctx* c, int32_t* done, int32_t nsteps, double* norms, do
   }
   c->timing_step = true;
   return MW3_OK;
+}
+
+// One Chebyshev step of c (H psi_k, update into psim1 = psi_{k+1},
+// projections, swap).  In graph mode a step's projection sums run inside the
+// next step's graph (or before the flush), on the second stream as before.
+static int step_enqueue(Ctx* c, int32_t* done, int32_t nsteps, double* norms, double* autocorr) {
+  const bool graphs = step_pre(c);
+  int rc = step_body(c, graphs);
+  if (rc) return rc;
+  return step_post(c, graphs, done, nsteps, norms, autocorr);
 }
 
 int32_t mw3_cheby_steps(mw3_ctx* ctx, int32_t nsteps, double* norms, double* autocorr) {
@@ -678,15 +752,52 @@ int32_t mw3_cheby_steps_batch(mw3_ctx** ctxs, int32_t n, int32_t nsteps, double*
       return seterr(c, MW3_EINVAL, "mw3_cheby_steps_batch: contexts on different devices");
   }
   std::vector<int32_t> done(n, 0);
+  std::vector<Ctx*> m(n);
+  for (int i = 0; i < n; i++) m[i] = reinterpret_cast<Ctx*>(ctxs[i]);
+  Ctx* c0 = m[0];
+  std::vector<char> gflag(n);
   // step-major order: every context's step s is enqueued before any step s+1,
-  // so the streams of all contexts hold work at the same time
+  // so the streams of all contexts hold work at the same time.  When every
+  // member can run the step as a graph with psi in the same buffer, the whole
+  // batch step is one graph (held by the first member, captured once per
+  // buffer and member list).
   int rc = MW3_OK;
-  for (int32_t st = 0; st < nsteps && rc == MW3_OK; st++)
-    for (int i = 0; i < n && rc == MW3_OK; i++) {
-      Ctx* c = reinterpret_cast<Ctx*>(ctxs[i]);
-      rc = step_enqueue(c, &done[i], nsteps, norms ? norms + (size_t)i * nsteps : nullptr,
-                        nullptr);
+  for (int32_t st = 0; st < nsteps && rc == MW3_OK; st++) {
+    bool whole = n > 1;
+    const bool q0 = c0->d_psi == c0->graph_psi0;
+    for (int i = 0; i < n; i++) {
+      gflag[i] = step_pre(m[i]);
+      whole = whole && gflag[i] && m[i]->deferred_slot >= 0 &&
+              (m[i]->d_psi == m[i]->graph_psi0) == q0;
     }
+    if (whole) {
+      const int q = q0 ? 0 : 1;
+      bool same = c0->batch_of.size() == (size_t)n;
+      for (int i = 0; i < n && same; i++)
+        same = c0->batch_of[i] == m[i]->uid && c0->batch_gen[i] == m[i]->graph_gen;
+      if (!same) {
+        for (auto& g : c0->batch_graph)
+          if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+          }
+        c0->batch_of.clear();
+      }
+      if (!c0->batch_graph[q])
+        HIPCHK(c0, capture_batch(m.data(), n, q), "mw3_cheby_steps_batch: step capture");
+      HIPCHK(c0, hipGraphLaunch(c0->batch_graph[q], c0->stream), "mw3_cheby_steps_batch: graph");
+      // the members' streams follow the batch graph
+      HIPCHK(c0, hipEventRecord(c0->ev_bfork, c0->stream), "mw3_cheby_steps_batch: event");
+      for (int i = 1; i < n; i++)
+        HIPCHK(m[i], hipStreamWaitEvent(m[i]->stream, c0->ev_bfork, 0),
+               "mw3_cheby_steps_batch: wait");
+    } else {
+      for (int i = 0; i < n && rc == MW3_OK; i++) rc = step_body(m[i], gflag[i]);
+    }
+    for (int i = 0; i < n && rc == MW3_OK; i++)
+      rc = step_post(m[i], gflag[i], &done[i], nsteps,
+                     norms ? norms + (size_t)i * nsteps : nullptr, nullptr);
+  }
   for (int i = 0; i < n; i++)  // the caller may write psi before the next H psi
     reinterpret_cast<Ctx*>(ctxs[i])->slice_fresh = false;
   return rc;
