@@ -1155,6 +1155,12 @@ int32_t mw3_set_option(mw3_ctx* ctx, int32_t option, int32_t value) {
       if (value != c->opt_side) drop_graphs(c);
       c->opt_side = value;
       return MW3_OK;
+    case MW3_OPT_RADIAL_GRID:
+      if (value != 1 && value != 2 && value != 4)
+        return seterr(c, MW3_EINVAL, "mw3_set_option: radial grid divisor 1, 2 or 4");
+      if (value != c->opt_rad_grid) drop_graphs(c);
+      c->opt_rad_grid = value;
+      return MW3_OK;
     default:
       return seterr(c, MW3_EINVAL, "mw3_set_option: unknown option");
   }
@@ -1172,6 +1178,7 @@ static void effective_options(const Ctx* c, int32_t* out) {
   if (a == 1 && !c->ang_nt) a = 0;
   out[MW3_OPT_ANGULAR] = a;
   out[MW3_OPT_SIDE_STREAM] = c->opt_side;
+  out[MW3_OPT_RADIAL_GRID] = c->rad_nitems && c->opt_radial ? c->opt_rad_grid : 1;
 }
 
 int32_t mw3_get_option(const mw3_ctx* ctx, int32_t option, int32_t* value) {
@@ -1236,29 +1243,37 @@ int32_t mw3_autotune(mw3_ctx* ctx, int32_t trial_steps, int32_t* chosen, double*
   }
   // candidates: every combination of the options, skipping values that fall
   // back to a combination already listed
-  const int32_t saved[4] = {c->graph_mode, c->opt_radial, c->opt_angular, c->opt_side};
-  std::vector<std::array<int32_t, 4>> cands;
+  typedef std::array<int32_t, MW3_NOPTIONS> Key;
+  auto apply = [&](const Key& k) {
+    c->graph_mode = k[MW3_OPT_GRAPHS];
+    c->opt_radial = k[MW3_OPT_RADIAL];
+    c->opt_angular = k[MW3_OPT_ANGULAR];
+    c->opt_side = k[MW3_OPT_SIDE_STREAM];
+    c->opt_rad_grid = k[MW3_OPT_RADIAL_GRID];
+  };
+  Key best_key;
+  effective_options(c, best_key.data());
+  std::vector<Key> cands;
   for (int g = 0; g <= 1; g++)
     for (int r = 0; r <= 1; r++)
       for (int a = 0; a <= 2; a++)
-        for (int sd = 0; sd <= 1; sd++) {
-          c->graph_mode = g;
-          c->opt_radial = r;
-          c->opt_angular = a;
-          c->opt_side = sd;
-          int32_t eff[MW3_NOPTIONS];
-          effective_options(c, eff);
-          std::array<int32_t, 4> key = {eff[0], eff[1], eff[2], eff[3]};
-          if (std::find(cands.begin(), cands.end(), key) == cands.end()) cands.push_back(key);
-        }
+        for (int sd = 0; sd <= 1; sd++)
+          for (int gd = 1; gd <= 4; gd *= 2) {
+            Key k;
+            k[MW3_OPT_GRAPHS] = g;
+            k[MW3_OPT_RADIAL] = r;
+            k[MW3_OPT_ANGULAR] = a;
+            k[MW3_OPT_SIDE_STREAM] = sd;
+            k[MW3_OPT_RADIAL_GRID] = gd;
+            apply(k);
+            Key eff;
+            effective_options(c, eff.data());
+            if (std::find(cands.begin(), cands.end(), eff) == cands.end()) cands.push_back(eff);
+          }
   double best = 1e30;
-  std::array<int32_t, 4> best_key = {saved[0], saved[1], saved[2], saved[3]};
   for (const auto& key : cands) {
     drop_graphs(c);
-    c->graph_mode = key[0];
-    c->opt_radial = key[1];
-    c->opt_angular = key[2];
-    c->opt_side = key[3];
+    apply(key);
     double ms = 0;
     rc = trial_ms(c, n, e0, e1, &ms);
     if (rc) break;
@@ -1268,10 +1283,7 @@ int32_t mw3_autotune(mw3_ctx* ctx, int32_t trial_steps, int32_t* chosen, double*
     }
   }
   drop_graphs(c);
-  c->graph_mode = best_key[0];
-  c->opt_radial = best_key[1];
-  c->opt_angular = best_key[2];
-  c->opt_side = best_key[3];
+  apply(best_key);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->timing = timing;

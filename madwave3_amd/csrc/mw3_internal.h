@@ -275,6 +275,7 @@ struct Ctx {
   // GEMMs (2), each falling back to the next where not built; the angular
   // term on the third stream (1) or the main one (0)
   int opt_radial = 1, opt_angular = 0, opt_side = 0;
+  int opt_rad_grid = 1;           // persistent radial grids: resident blocks / this
   bool trial = false;             // autotuner trial steps: no ring flush, no psi(E)
   bool graph_slot_dev = false;    // kernels being enqueued read their slot from d_slot
   hipGraphExec_t graph[2] = {nullptr, nullptr};

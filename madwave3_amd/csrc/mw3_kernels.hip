@@ -2318,12 +2318,17 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
     return persistent_grid(k_rad_inv<T, NKB, HB>, 32 * HB, &c->rad_grid[1]);
   }
   const int n = c->rad_nitems;
+  // the grid: the resident blocks divided by the grid option (each block
+  // loads its waves' table slices once, so fewer blocks read fewer table
+  // bytes and walk more items each)
+  const int gdiv = c->opt_rad_grid > 0 ? c->opt_rad_grid : 1;
+  const int g0 = std::max(c->rad_grid[0] / gdiv, 1), g1 = std::max(c->rad_grid[1] / gdiv, 1);
   if (mode == 0) {
-    hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB>), dim3(n < c->rad_grid[0] ? n : c->rad_grid[0]),
-                       dim3(128 * NKB), 0, c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+    hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB>), dim3(n < g0 ? n : g0), dim3(128 * NKB), 0,
+                       c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
   } else {
-    hipLaunchKernelGGL((k_rad_inv<T, NKB, HB>), dim3(n < c->rad_grid[1] ? n : c->rad_grid[1]),
-                       dim3(32 * HB), 0, c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+    hipLaunchKernelGGL((k_rad_inv<T, NKB, HB>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
+                       c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
   }
   return hipGetLastError();
 }

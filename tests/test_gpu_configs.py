@@ -3,10 +3,11 @@ propagator against the FP64 CPU oracle over the first steps of each run.
 
 This is the check that makes the FP64 GPU path the long-run reference of the
 full-length mixed-precision runs (`bench.py --accuracy`): after K steps the
-state psi_K agrees with the oracle's to ||dpsi||/||psi|| <= 1e-12, the norms
-to 1e-12, and the energy-resolved accumulators (whose terms cancel) to 1e-9
-relative to their largest entry.  The step counts keep the oracle's share to
-seconds on the host.
+state psi_K agrees with the oracle's to ||dpsi||/||psi|| <= 1e-12 and so do
+the norms (1e-11 with the parity-folded angular kernel, see below), and the
+energy-resolved accumulators (whose terms cancel) to 1e-9 relative to their
+largest entry.  The step counts keep the oracle's share to seconds on the
+host.
 
   configuration 2  J = 0, 256 r x 512 R x 120 gamma, product states (iprod = 2)
   configuration 3  J = 12, Omega = 0..6 (7 Coriolis-coupled blocks), 192 x 384 x 96,
@@ -22,7 +23,7 @@ import numpy as np
 import pytest
 
 from madwave3_amd import leps
-from madwave3_amd.lib import MW3_FP64, Propagator, steps_batch
+from madwave3_amd.lib import MW3_FP64, MW3_OPT_ANGULAR, Propagator, steps_batch
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -45,40 +46,60 @@ def full_case(cfg, J=0):
     return leps.build(leps.config(cfg, J=J))
 
 
-def compare(g, o, c, nsteps, label):
-    st_o = o.begin(c.rpaq0)
-    st_g = g.begin(c.rpaq0)
-    assert rel(st_g, st_o) < 1e-11, label
-    n_o, a_o, _ = o.steps(nsteps)
-    n_g, a_g = g.steps(nsteps, want_auto=True)
-    print(label, "norms", n_g[-1], "rel err %.2e" % rel(n_g, n_o), flush=True)
-    assert rel(n_g, n_o) < 1e-12, label
-    if c.iphoto > 0:
-        assert rel(a_g, a_o) < 1e-12, label
-    pm_o, p_o, k_o = o.state()
-    pm_g, p_g, k_g = g.state()
-    assert k_o == k_g == 1 + nsteps
-    e_p, e_pm = state_rel(p_g, p_o), state_rel(pm_g, pm_o)
-    print(label, "state rel err %.2e / %.2e" % (e_p, e_pm), flush=True)
-    assert e_p < 1e-12 and e_pm < 1e-12, label
-    za_o, zb_o = o.flux_accum()
-    za_g, zb_g = g.flux_accum()
-    assert rel(za_g, za_o) < 1e-9 and rel(zb_g, zb_o) < 1e-9, label
-    assert rel(g.zs2(), o.zs2()) < 1e-9, label
-    assert rel(g.flux_result()[0], o.flux_result()[0]) < 1e-9, label
+def oracle_run(c, nsteps):
+    o = Oracle(c)
+    r = {"stats": o.begin(c.rpaq0)}
+    r["norms"], r["auto"], _ = o.steps(nsteps)
+    r["psim1"], r["psi"], r["k"] = o.state()
+    r["zcr"], r["zcdr"] = o.flux_accum()
+    r["zs2"] = o.zs2()
+    r["P"] = o.flux_result()[0]
     if c.iprod == 2:
-        assert rel(g.zs2prod(), o.zs2prod()) < 1e-9, label
+        r["zs2prod"] = o.zs2prod()
+    return r
+
+
+def compare(g, r, c, nsteps, label, tol):
+    """GPU run against the oracle's results r; tol bounds the state and the
+    norms (relative), the accumulators get 1e-9."""
+    st_g = g.begin(c.rpaq0)
+    assert rel(st_g, r["stats"]) < 1e-11, label
+    n_g, a_g = g.steps(nsteps, want_auto=True)
+    e_n = rel(n_g, r["norms"])
+    pm_g, p_g, k_g = g.state()
+    e_p, e_pm = state_rel(p_g, r["psi"]), state_rel(pm_g, r["psim1"])
+    print(label, "norm rel err %.2e, state rel err %.2e / %.2e (bar %.0e)" % (e_n, e_p, e_pm, tol),
+          flush=True)
+    assert e_n < tol, label
+    if c.iphoto > 0:
+        assert rel(a_g, r["auto"]) < tol, label
+    assert k_g == r["k"] == 1 + nsteps
+    assert e_p < tol and e_pm < tol, label
+    za_g, zb_g = g.flux_accum()
+    assert rel(za_g, r["zcr"]) < 1e-9 and rel(zb_g, r["zcdr"]) < 1e-9, label
+    assert rel(g.zs2(), r["zs2"]) < 1e-9, label
+    assert rel(g.flux_result()[0], r["P"]) < 1e-9, label
+    if c.iprod == 2:
+        assert rel(g.zs2prod(), r["zs2prod"]) < 1e-9, label
 
 
 @pytest.mark.parametrize("cfg", [2, 3, 5])
 def test_full_size_fp64_matches_oracle(cfg):
+    """The unfolded angular kernel (or the GEMMs where Coriolis coupling
+    needs them) to 1e-12; the default parity-folded angular kernel to 1e-11:
+    it uses D(n_gamma-1-gamma, j) = +-D(gamma, j), which the Gauss-Legendre
+    tables hold only to their own rounding (4e-14 of the largest entry at
+    configuration 2), and that difference adds up over the steps."""
     c = full_case(cfg)
     label = "configuration %d %s" % (cfg, c)
     print(label, flush=True)
-    g = Propagator(c, MW3_FP64)
-    o = Oracle(c)
-    compare(g, o, c, K_STEPS[cfg], label)
-    g.close()
+    r = oracle_run(c, K_STEPS[cfg])
+    for angular, tol in ((1, 1e-12), (0, 1e-11)):
+        g = Propagator(c, MW3_FP64)
+        g.set_option(MW3_OPT_ANGULAR, angular)
+        compare(g, r, c, K_STEPS[cfg], "%s angular variant %d" % (label, g.get_option(MW3_OPT_ANGULAR)),
+                tol)
+        g.close()
 
 
 def test_config4_batch_member_matches_oracle():
@@ -97,12 +118,12 @@ def test_config4_batch_member_matches_oracle():
     o = Oracle(c)
     o.begin(c.rpaq0)
     n_o, _, _ = o.steps(nsteps)
-    assert rel(norms[12], n_o) < 1e-12
+    assert rel(norms[12], n_o) < 1e-11
     pm_o, p_o, _ = o.state()
     pm_g, p_g, _ = props[12].state()
     e = state_rel(p_g, p_o)
     print("configuration 4 J=12 in the batch: state rel err %.2e" % e, flush=True)
-    assert e < 1e-12 and state_rel(pm_g, pm_o) < 1e-12
+    assert e < 1e-11 and state_rel(pm_g, pm_o) < 1e-11
     assert rel(props[12].flux_result()[0], o.flux_result()[0]) < 1e-9
     # a member alone gives its batch norms bit for bit (J = 5: 5 blocks)
     alone = Propagator(cases[5], MW3_FP64)

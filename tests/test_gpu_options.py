@@ -12,13 +12,15 @@ import numpy as np
 import pytest
 
 from madwave3_amd.lib import (MW3_FP64, MW3_MIXED, MW3_OPT_ANGULAR, MW3_OPT_RADIAL,
-                              MW3_OPT_SIDE_STREAM, Propagator)
+                              MW3_OPT_RADIAL_GRID, MW3_OPT_SIDE_STREAM, Propagator)
 from oracle.oracle import Oracle
 from tests.cases import case
 
 pytestmark = pytest.mark.gpu
 
-COMBOS = list(itertools.product([0, 1], [0, 1, 2], [0, 1]))   # radial, angular, side
+# radial, angular, side stream, radial grid divisor
+COMBOS = [k + (1,) for k in itertools.product([0, 1], [0, 1, 2], [0, 1])] + \
+    [(1, 0, 1, 2), (1, 0, 0, 4)]
 
 
 def rel(a, b):
@@ -37,17 +39,18 @@ def test_every_option_combination_matches_oracle(name):
     n_ref, _, _ = o.steps(60)
     P_ref = o.flux_result()[0]
     seen = set()
-    for radial, angular, side in COMBOS:
+    for radial, angular, side, grid in COMBOS:
         for prec in (MW3_FP64, MW3_MIXED):
             g = Propagator(c, prec)
             g.set_option(MW3_OPT_RADIAL, radial)
             g.set_option(MW3_OPT_ANGULAR, angular)
             g.set_option(MW3_OPT_SIDE_STREAM, side)
+            g.set_option(MW3_OPT_RADIAL_GRID, grid)
             eff = tuple(g.get_option(k) for k in (MW3_OPT_RADIAL, MW3_OPT_ANGULAR,
-                                                  MW3_OPT_SIDE_STREAM))
+                                                  MW3_OPT_SIDE_STREAM, MW3_OPT_RADIAL_GRID))
             seen.add(eff)
-            label = "%s prec %d options %s (in effect %s)" % (name, prec, (radial, angular, side),
-                                                             eff)
+            label = "%s prec %d options %s (in effect %s)" % (
+                name, prec, (radial, angular, side, grid), eff)
             e_h = rel(g.hphi(psi), h_ref)
             if prec == MW3_FP64:
                 g.begin(c.rpaq0)
