@@ -3,7 +3,6 @@
 
 #include <algorithm>
 #include <array>
-#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -15,7 +14,6 @@
 using namespace mw3;
 
 static thread_local std::string g_create_err;
-static std::atomic<uint64_t> g_next_uid{1};
 
 // Device array of mw3_cheby_spectrum_noalloc: the Chebyshev coefficients of
 // one call (kmax + 1 doubles).  Every device the library runs on holds its
@@ -161,14 +159,7 @@ static void free_all(Ctx* c) {
       g = nullptr;
     }
   if (c->d_slot) (void)hipFree(c->d_slot);
-  for (auto& g : c->batch_graph)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-  if (c->ev_bfork) (void)hipEventDestroy(c->ev_bfork);
-  if (c->ev_bjoin) (void)hipEventDestroy(c->ev_bjoin);
   if (c->ev_slice) (void)hipEventDestroy(c->ev_slice);
   if (c->ev_cvj) (void)hipEventDestroy(c->ev_cvj);
   if (c->ev_psi) (void)hipEventDestroy(c->ev_psi);
@@ -216,7 +207,6 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   }
   c->s = *setup;
   c->precision = precision;
-  c->uid = g_next_uid.fetch_add(1);
   // the angular term beside the radial passes on its own stream in mixed
   // precision (measured faster there; the FP64 pair ran slower side by side)
   c->opt_side = precision == MW3_MIXED ? 1 : 0;
@@ -247,8 +237,6 @@ int32_t mw3_create(const mw3_setup* setup, int32_t precision, mw3_ctx** out) {
   CK(hipEventCreateWithFlags(&c->ev_psi, evf), "mw3_create: event");
   CK(hipEventCreateWithFlags(&c->ev_ang, evf), "mw3_create: event");
   CK(hipEventCreateWithFlags(&c->ev_fork, evf), "mw3_create: event");
-  CK(hipEventCreateWithFlags(&c->ev_bfork, evf), "mw3_create: event");
-  CK(hipEventCreateWithFlags(&c->ev_bjoin, evf), "mw3_create: event");
   CK(alloc_zero((void**)&c->d_slot, sizeof(int32_t)), "mw3_create: ring slot");
   CK(upload_raw(&c->d_pt_dense, t.pt_dense.data(), t.pt_dense.size()), "mw3_create: index map");
   CK(upload_raw(&c->d_mask, t.mask.data(), t.mask.size()), "mw3_create: mask");
@@ -604,40 +592,6 @@ void drop_graphs(Ctx* c) {
       (void)hipGraphExecDestroy(g);
       g = nullptr;
     }
-  for (auto& g : c->batch_graph)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
-  c->batch_of.clear();
-  c->graph_gen += 1;
-}
-
-// Captures one step of every member of a batch as one graph, held by the
-// first member: each member's body on its own streams, forked from and
-// joined back to the first member's main stream.
-static hipError_t capture_batch(Ctx** m, int n, int q) {
-  Ctx* c0 = m[0];
-  hipError_t e = hipStreamBeginCapture(c0->stream, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return e;
-  hipError_t eb = hipEventRecord(c0->ev_bfork, c0->stream);
-  for (int i = 0; i < n && eb == hipSuccess; i++) {
-    Ctx* c = m[i];
-    if (i > 0) eb = hipStreamWaitEvent(c->stream, c0->ev_bfork, 0);
-    if (eb == hipSuccess) eb = capture_body(c);
-    if (i > 0 && eb == hipSuccess) eb = hipEventRecord(c->ev_bjoin, c->stream);
-    if (i > 0 && eb == hipSuccess) eb = hipStreamWaitEvent(c0->stream, c->ev_bjoin, 0);
-  }
-  e = end_capture(c0->stream, eb, &c0->batch_graph[q]);
-  if (e == hipSuccess) {
-    c0->batch_of.resize(n);
-    c0->batch_gen.resize(n);
-    for (int i = 0; i < n; i++) {
-      c0->batch_of[i] = m[i]->uid;
-      c0->batch_gen[i] = m[i]->graph_gen;
-    }
-  }
-  return e;
 }
 
 // the projection sums of the last step, when a captured step left them
@@ -752,52 +706,16 @@ int32_t mw3_cheby_steps_batch(mw3_ctx** ctxs, int32_t n, int32_t nsteps, double*
       return seterr(c, MW3_EINVAL, "mw3_cheby_steps_batch: contexts on different devices");
   }
   std::vector<int32_t> done(n, 0);
-  std::vector<Ctx*> m(n);
-  for (int i = 0; i < n; i++) m[i] = reinterpret_cast<Ctx*>(ctxs[i]);
-  Ctx* c0 = m[0];
-  std::vector<char> gflag(n);
   // step-major order: every context's step s is enqueued before any step s+1,
-  // so the streams of all contexts hold work at the same time.  When every
-  // member can run the step as a graph with psi in the same buffer, the whole
-  // batch step is one graph (held by the first member, captured once per
-  // buffer and member list).
+  // so the streams of all contexts hold work at the same time (each member
+  // runs its steps as its own captured graphs where it can)
   int rc = MW3_OK;
-  for (int32_t st = 0; st < nsteps && rc == MW3_OK; st++) {
-    bool whole = n > 1;
-    const bool q0 = c0->d_psi == c0->graph_psi0;
-    for (int i = 0; i < n; i++) {
-      gflag[i] = step_pre(m[i]);
-      whole = whole && gflag[i] && m[i]->deferred_slot >= 0 &&
-              (m[i]->d_psi == m[i]->graph_psi0) == q0;
+  for (int32_t st = 0; st < nsteps && rc == MW3_OK; st++)
+    for (int i = 0; i < n && rc == MW3_OK; i++) {
+      Ctx* c = reinterpret_cast<Ctx*>(ctxs[i]);
+      rc = step_enqueue(c, &done[i], nsteps, norms ? norms + (size_t)i * nsteps : nullptr,
+                        nullptr);
     }
-    if (whole) {
-      const int q = q0 ? 0 : 1;
-      bool same = c0->batch_of.size() == (size_t)n;
-      for (int i = 0; i < n && same; i++)
-        same = c0->batch_of[i] == m[i]->uid && c0->batch_gen[i] == m[i]->graph_gen;
-      if (!same) {
-        for (auto& g : c0->batch_graph)
-          if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-          }
-        c0->batch_of.clear();
-      }
-      if (!c0->batch_graph[q])
-        HIPCHK(c0, capture_batch(m.data(), n, q), "mw3_cheby_steps_batch: step capture");
-      HIPCHK(c0, hipGraphLaunch(c0->batch_graph[q], c0->stream), "mw3_cheby_steps_batch: graph");
-      // the members' streams follow the batch graph
-      HIPCHK(c0, hipEventRecord(c0->ev_bfork, c0->stream), "mw3_cheby_steps_batch: event");
-      for (int i = 1; i < n; i++)
-        HIPCHK(m[i], hipStreamWaitEvent(m[i]->stream, c0->ev_bfork, 0),
-               "mw3_cheby_steps_batch: wait");
-    } else {
-      for (int i = 0; i < n && rc == MW3_OK; i++) rc = step_body(m[i], gflag[i]);
-    }
-    for (int i = 0; i < n && rc == MW3_OK; i++)
-      rc = step_post(m[i], gflag[i], &done[i], nsteps,
-                     norms ? norms + (size_t)i * nsteps : nullptr, nullptr);
-  }
   for (int i = 0; i < n; i++)  // the caller may write psi before the next H psi
     reinterpret_cast<Ctx*>(ctxs[i])->slice_fresh = false;
   return rc;

@@ -283,15 +283,6 @@ struct Ctx {
   int32_t* d_slot = nullptr;
   int deferred_slot = -1;         // ring slot whose projection sums are not yet enqueued
   hipEvent_t ev_fork = nullptr;
-  // a batch's whole step as one graph (held by the batch's first member): the
-  // member list and each member's graph generation at capture; ev_bfork /
-  // ev_bjoin fork the members' streams from the first member's and join them
-  uint64_t uid = 0;                // unique over the process (never reused)
-  uint64_t graph_gen = 0;          // advanced whenever this context's graphs are dropped
-  hipGraphExec_t batch_graph[2] = {nullptr, nullptr};
-  std::vector<uint64_t> batch_of;  // (uid, graph_gen) of each member at capture
-  std::vector<uint64_t> batch_gen;
-  hipEvent_t ev_bfork = nullptr, ev_bjoin = nullptr;
 
   // timing
   bool timing = false;
