@@ -3,10 +3,10 @@ propagator against the FP64 CPU oracle over the first steps of each run.
 
 This is the check that makes the FP64 GPU path the long-run reference of the
 full-length mixed-precision runs (`bench.py --accuracy`): after K steps the
-state psi_K agrees with the oracle's to ||dpsi||/||psi|| <= 1e-12 and so do
-the norms (1e-11 with the parity-folded angular kernel, see below), and the
-energy-resolved accumulators (whose terms cancel) to 1e-9 relative to their
-largest entry.  The step counts keep the oracle's share to seconds on the
+state psi_K agrees with the oracle's to ||dpsi||/||psi|| <= 1e-12 (1e-11
+with the parity-folded angular kernel, see below), the norms (sums over the
+whole grid, added in another order) to 1e-11, and the energy-resolved
+accumulators (whose terms cancel) to 1e-9 relative to their largest entry.  The step counts keep the oracle's share to seconds on the
 host.
 
   configuration 2  J = 0, 256 r x 512 R x 120 gamma, product states (iprod = 2)
@@ -60,8 +60,10 @@ def oracle_run(c, nsteps):
 
 
 def compare(g, r, c, nsteps, label, tol):
-    """GPU run against the oracle's results r; tol bounds the state and the
-    norms (relative), the accumulators get 1e-9."""
+    """GPU run against the oracle's results r; tol bounds the state
+    (||dpsi||/||psi||); the norms and autocorrelations, sums of 5-17 million
+    terms that the GPU and the oracle add in different orders, get 1e-11; the
+    energy-resolved accumulators 1e-9."""
     st_g = g.begin(c.rpaq0)
     assert rel(st_g, r["stats"]) < 1e-11, label
     n_g, a_g = g.steps(nsteps, want_auto=True)
@@ -70,9 +72,9 @@ def compare(g, r, c, nsteps, label, tol):
     e_p, e_pm = state_rel(p_g, r["psi"]), state_rel(pm_g, r["psim1"])
     print(label, "norm rel err %.2e, state rel err %.2e / %.2e (bar %.0e)" % (e_n, e_p, e_pm, tol),
           flush=True)
-    assert e_n < tol, label
+    assert e_n < 1e-11, label
     if c.iphoto > 0:
-        assert rel(a_g, r["auto"]) < tol, label
+        assert rel(a_g, r["auto"]) < 1e-11, label
     assert k_g == r["k"] == 1 + nsteps
     assert e_p < tol and e_pm < tol, label
     za_g, zb_g = g.flux_accum()
