@@ -7,6 +7,7 @@ the reference tree) and stores the numbers they hold as data:
   hd_h_j000_angles.txt     the 140 Gauss-Legendre angles (degrees)
   hd_h_j000_norms.txt      |psi_k|^2 for k = 1..4001 (check)
   hd_h_j000_S2prod.txt     the 500-energy probability table
+  hd_h_input.dat           the example's input.dat
 Run only where the reference tree exists; the outputs are committed.
 """
 import json
@@ -70,6 +71,11 @@ def main():
     with open(os.path.join(HERE, "hd_h_j000_norms.txt"), "w") as f:
         for k, v in norms:
             f.write("%d %s\n" % (k, v))
+
+    # the example's input file (the namelists of the run the log belongs to)
+    src_in = os.path.join(REF, "EXAMPLES", "HD+H_collision", "input.dat")
+    with open(src_in) as f, open(os.path.join(HERE, "hd_h_input.dat"), "w") as g:
+        g.write(f.read())
 
     src = os.path.join(D, "S2prod.v00.J000.k00008")
     with open(src) as f, open(os.path.join(HERE, "hd_h_j000_S2prod.txt"), "w") as g:
