@@ -229,15 +229,16 @@ def run_full(case, precision, nsteps, label, chunk=5120):
     return out
 
 
-def spectrum_of(case, autoc, ne=400):
+def spectrum_of(case, autoc, ne=2000):
     """cheby-spectra's sum over the autocorrelation (auto[0] = 1) on ne
-    energies spanning the run's energy window, in the reference's units
+    energies across the propagator's spectral range (98% of it, where the
+    Chebyshev weight 1/sqrt(1 - Es^2) stays finite), in the reference's units
     before the cross-section factor (SRC/photo/cheby-spectra.f:58-97)."""
     from madwave3_amd import consts as C
     from madwave3_amd.lib import cheby_spectrum
 
     a = np.concatenate([[1.0], autoc])
-    e = np.linspace(float(case.etotS2[0]), float(case.etotS2[-1]), ne)
+    e = case.emindlt + case.delta2 * np.linspace(-0.98, 0.98, ne)
     return cheby_spectrum(a, case.emindlt, case.delta2, C.hbr, 0.0, e)
 
 

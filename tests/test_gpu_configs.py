@@ -6,7 +6,8 @@ full-length mixed-precision runs (`bench.py --accuracy`): after K steps the
 state psi_K agrees with the oracle's to ||dpsi||/||psi|| <= 1e-12 (1e-11
 with the parity-folded angular kernel, see below), the norms (sums over the
 whole grid, added in another order) to 1e-11, and the energy-resolved
-accumulators (whose terms cancel) to 1e-9 relative to their largest entry.  The step counts keep the oracle's share to seconds on the
+accumulators (whose terms cancel) to 1e-9 relative to their largest entry
+plus 1e-13.  The step counts keep the oracle's share to seconds on the
 host.
 
   configuration 2  J = 0, 256 r x 512 R x 120 gamma, product states (iprod = 2)
@@ -35,6 +36,16 @@ def rel(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def close(a, b, rtol=1e-9, atol=1e-13):
+    """max |a - b| within rtol of the largest |b| plus atol: the energy
+    accumulators of a run whose packet has not reached the analysis line
+    (the photodissociation packet at configuration 5) hold rounding-level
+    values only."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b))) <= rtol * float(np.max(np.abs(b))) + atol
 
 
 def state_rel(a, b):
@@ -78,11 +89,14 @@ def compare(g, r, c, nsteps, label, tol):
     assert k_g == r["k"] == 1 + nsteps
     assert e_p < tol and e_pm < tol, label
     za_g, zb_g = g.flux_accum()
-    assert rel(za_g, r["zcr"]) < 1e-9 and rel(zb_g, r["zcdr"]) < 1e-9, label
-    assert rel(g.zs2(), r["zs2"]) < 1e-9, label
-    assert rel(g.flux_result()[0], r["P"]) < 1e-9, label
+    zs = g.zs2()
+    print(label, "accumulators: zS2 max %.2e err %.2e" % (np.max(np.abs(r["zs2"])),
+                                                         np.max(np.abs(zs - r["zs2"]))), flush=True)
+    assert close(za_g, r["zcr"]) and close(zb_g, r["zcdr"]), label
+    assert close(zs, r["zs2"]), label
+    assert close(g.flux_result()[0], r["P"]), label
     if c.iprod == 2:
-        assert rel(g.zs2prod(), r["zs2prod"]) < 1e-9, label
+        assert close(g.zs2prod(), r["zs2prod"]), label
 
 
 @pytest.mark.parametrize("cfg", [2, 3, 5])
@@ -126,7 +140,7 @@ def test_config4_batch_member_matches_oracle():
     e = state_rel(p_g, p_o)
     print("configuration 4 J=12 in the batch: state rel err %.2e" % e, flush=True)
     assert e < 1e-11 and state_rel(pm_g, pm_o) < 1e-11
-    assert rel(props[12].flux_result()[0], o.flux_result()[0]) < 1e-9
+    assert close(props[12].flux_result()[0], o.flux_result()[0])
     # a member alone gives its batch norms bit for bit (J = 5: 5 blocks)
     alone = Propagator(cases[5], MW3_FP64)
     alone.begin(cases[5].rpaq0)
