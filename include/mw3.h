@@ -189,14 +189,23 @@ MW3_API int32_t mw3_cheby_k(const mw3_ctx *ctx);
    S2prodelectot(netot,nelecmax) as [nelecmax][netot]; either may be NULL. */
 MW3_API int32_t mw3_flux_result(mw3_ctx *ctx, double *S2prodtot, double *S2prodelec);
 
+/* The E > V(flux point) test of totalflux_k (SRC/mod_flux_01y2.f:640):
+   on = 1 (default) keeps a flux point's terms only at the energies above its
+   potential, as the reference's current source does; on = 0 keeps them at
+   every energy, as the earlier version of the reference that made the
+   HD+H example's published probability table did.  The test is applied when
+   the sums are read (mw3_flux_result, mw3_flux_accum_get), so it may be set
+   at any time and applies to every step. */
+MW3_API int32_t mw3_flux_energy_test(mw3_ctx *ctx, int32_t on);
+
 /* zS2(netot,nvini:nvmax,jini:jmax,ncan) of Cvjflux_k
    (SRC/mod_flux_01y2.f:663-775), complex as interleaved (re, im):
    [ncan][jmax-jini+1][nvmax-nvini+1][netot][2]. */
 MW3_API int32_t mw3_zs2_get(mw3_ctx *ctx, double *zS2);
 
 /* zCR, zCdR(netot,nflux) of totalflux_k (SRC/mod_flux_01y2.f:585-660), complex
-   interleaved: [nflux][netot][2] each.  The E > V(flux point) mask of the
-   reference is applied.  Either may be NULL.  _put restores them (restart,
+   interleaved: [nflux][netot][2] each.  The E > V(flux point) test is
+   applied unless mw3_flux_energy_test turned it off.  Either may be NULL.  _put restores them (restart,
    SRC/mod_flux_01y2.f:298-322). */
 MW3_API int32_t mw3_flux_accum_get(mw3_ctx *ctx, double *zCR, double *zCdR);
 MW3_API int32_t mw3_flux_accum_put(mw3_ctx *ctx, const double *zCR, const double *zCdR,

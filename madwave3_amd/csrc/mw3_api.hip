@@ -750,6 +750,14 @@ int32_t mw3_flux_result(mw3_ctx* ctx, double* S2prodtot, double* S2prodelec) {
   return MW3_OK;
 }
 
+int32_t mw3_flux_energy_test(mw3_ctx* ctx, int32_t on) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  if (on != 0 && on != 1) return seterr(c, MW3_EINVAL, "mw3_flux_energy_test: on must be 0 or 1");
+  c->flux_etest = on;
+  return MW3_OK;
+}
+
 int32_t mw3_zs2_get(mw3_ctx* ctx, double* zS2) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c || !zS2) return MW3_EINVAL;
@@ -777,7 +785,7 @@ int32_t mw3_flux_accum_get(mw3_ctx* ctx, double* zCR, double* zCdR) {
            "mw3_flux_accum_get");
   HIPCHK(c, hipStreamSynchronize(c->stream), "mw3_flux_accum_get");
   // the reference accumulates only where E > V(flux point)
-  for (int64_t f = 0; f < c->t.nflux; f++)
+  for (int64_t f = 0; f < (c->flux_etest ? c->t.nflux : 0); f++)
     for (int e = 0; e < netot; e++)
       if (!(c->etot_host[e] > c->t.potiflux[f])) {
         if (zCR) zCR[2 * (f * netot + e)] = zCR[2 * (f * netot + e) + 1] = 0.0;

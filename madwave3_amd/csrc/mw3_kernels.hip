@@ -3121,7 +3121,8 @@ hipError_t build_prod_plan(Ctx* c) {
 }
 
 // P(E) and per-state sums (totalflux, SRC/mod_flux_01y2.f:778-816):
-// out[0][E] = sum_f [E > V_f] Im(conj(zCR) zCdR), out[1+e][E] per state e.
+// out[0][E] = sum_f [E > V_f] Im(conj(zCR) zCdR), out[1+e][E] per state e;
+// potf = nullptr drops the E > V_f test.
 __global__ void k_flux_sum(const double* __restrict__ zcr, const double* __restrict__ zcdr,
                            const double* __restrict__ potf, const int32_t* __restrict__ felec,
                            const double* __restrict__ etot, double* __restrict__ out,
@@ -3132,7 +3133,7 @@ __global__ void k_flux_sum(const double* __restrict__ zcr, const double* __restr
   for (int q = 0; q <= ne && q < 9; q++) acc[q] = 0;
   double E = etot[e];
   for (int64_t f = threadIdx.x; f < nflux; f += blockDim.x) {
-    if (!(E > potf[f])) continue;
+    if (potf && !(E > potf[f])) continue;
     double ar = zcr[2 * (f * netot + e)], ai = zcr[2 * (f * netot + e) + 1];
     double br = zcdr[2 * (f * netot + e)], bi = zcdr[2 * (f * netot + e) + 1];
     double v = ar * bi - ai * br;
@@ -3153,7 +3154,8 @@ __global__ void k_flux_sum(const double* __restrict__ zcr, const double* __restr
 hipError_t launch_flux_result(Ctx* c, double* d_out) {
   int ne = c->t.ne;
   hipLaunchKernelGGL(k_flux_sum, dim3(c->s.netot), dim3(256), sizeof(double) * 256 * (1 + ne),
-                     c->stream, c->d_zcr, c->d_zcdr, c->d_potiflux, c->d_flux_elec, c->d_etot,
+                     c->stream, c->d_zcr, c->d_zcdr, c->flux_etest ? c->d_potiflux : nullptr,
+                     c->d_flux_elec, c->d_etot,
                      d_out, c->t.nflux, c->s.netot, ne);
   return hipGetLastError();
 }

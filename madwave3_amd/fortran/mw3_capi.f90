@@ -180,6 +180,13 @@ module mw3_capi
       integer(c_int32_t) :: rc
     end function mw3_flux_result
 
+    function mw3_flux_energy_test(ctx, on) bind(C, name="mw3_flux_energy_test") result(rc)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+      integer(c_int32_t) :: rc
+    end function mw3_flux_energy_test
+
     function mw3_zs2_get(ctx, zs2) bind(C, name="mw3_zs2_get") result(rc)
       import :: c_ptr, c_double, c_int32_t
       type(c_ptr), value :: ctx

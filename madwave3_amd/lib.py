@@ -279,6 +279,7 @@ def load_library(path=None):
     lib.mw3_cheby_k.argtypes = [ct.c_void_p]
     lib.mw3_flux_result.argtypes = [ct.c_void_p, _pd, _pd]
     lib.mw3_zs2_get.argtypes = [ct.c_void_p, _pd]
+    lib.mw3_flux_energy_test.argtypes = [ct.c_void_p, ct.c_int32]
     lib.mw3_flux_accum_get.argtypes = [ct.c_void_p, _pd, _pd]
     lib.mw3_flux_accum_put.argtypes = [ct.c_void_p, _pd, _pd, _pd]
     lib.mw3_psie_get.argtypes = [ct.c_void_p, _pd]
@@ -381,6 +382,11 @@ class Propagator:
         el = np.zeros((c.nelecmax, c.netot))
         self._check(self.lib.mw3_flux_result(self.h, _dp(tot), _dp(el)), "mw3_flux_result")
         return tot, el
+
+    def flux_energy_test(self, on):
+        """Keep (1, default) or drop (0) the E > V(flux point) test of the
+        flux sums (mw3_flux_energy_test)."""
+        self._check(self.lib.mw3_flux_energy_test(self.h, int(on)), "mw3_flux_energy_test")
 
     def zs2(self):
         c = self.case

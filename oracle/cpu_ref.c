@@ -105,6 +105,7 @@ typedef struct oref_ctx {
   double *psi, *psim1, *hpsi, *psi0;
   int k;
   int begun;
+  int flux_etest; /* E > V(flux point) test of totalflux_k (1) or none (0) */
   char err[256];
 } oref_ctx;
 
@@ -372,6 +373,7 @@ ORF_EXPORT int oref_create(const mw3_setup *s, oref_ctx **out) {
   c->iphoto = s->iphoto;
   c->iprod = s->iprod;
   c->kminelastic = s->kminelastic;
+  c->flux_etest = 1;
   c->icanref = s->icanref;
   c->nvref = s->nvref;
   c->jref = s->jref;
@@ -895,7 +897,7 @@ static void totalflux_k(oref_ctx *c, int k) {
     cplx zdR = zf[((size_t)ic * na + sl) * n2 + (c->fl_ir2[f] - 1)];
     for (int ie = 0; ie < c->netot; ie++) {
       cplx ze = zeroall ? zero : c->zCk[ie];
-      if (c->etots2[ie] > c->potiflux[f]) {
+      if (!c->flux_etest || c->etots2[ie] > c->potiflux[f]) {
         cplx *a = &c->zCR[(size_t)f * c->netot + ie];
         cplx *b = &c->zCdR[(size_t)f * c->netot + ie];
         a->re += ze.re * zR;
@@ -1224,6 +1226,14 @@ ORF_EXPORT int oref_flux_result(oref_ctx *c, double *S2prodtot,
       for (int e = 0; e < c->nelec; e++)
         S2prodelec[(size_t)e * c->netot + ie] = el[e] * c->reacfct[ie] / c->ah1;
   }
+  return MW3_OK;
+}
+
+/* Keep (1) or drop (0) the E > V(flux point) test of totalflux_k
+   (SRC/mod_flux_01y2.f:640); applied as the steps accumulate, so set it
+   before the first step. */
+ORF_EXPORT int oref_flux_energy_test(oref_ctx *c, int on) {
+  c->flux_etest = on ? 1 : 0;
   return MW3_OK;
 }
 

@@ -35,6 +35,7 @@ def lib():
         L.oref_cheby_steps.argtypes = [ct.c_void_p, ct.c_int, _pd, _pd, _pd]
         L.oref_flux_result.argtypes = [ct.c_void_p, _pd, _pd]
         L.oref_zs2_get.argtypes = [ct.c_void_p, _pd]
+        L.oref_flux_energy_test.argtypes = [ct.c_void_p, ct.c_int]
         L.oref_flux_accum_get.argtypes = [ct.c_void_p, _pd, _pd]
         L.oref_psie_get.argtypes = [ct.c_void_p, _pd]
         L.oref_cheby_spectrum.restype = None
@@ -118,6 +119,11 @@ class Oracle:
         el = np.zeros((c.nelecmax, c.netot))
         self.L.oref_flux_result(self.h, _dp(tot), _dp(el))
         return tot, el
+
+    def flux_energy_test(self, on):
+        """Keep (1) or drop (0) the E > V(flux point) test; before the first
+        step."""
+        self.L.oref_flux_energy_test(self.h, int(on))
 
     def zs2(self):
         c = self.case

@@ -7,6 +7,8 @@ the reference tree) and stores the numbers they hold as data:
   hd_h_j000_angles.txt     the 140 Gauss-Legendre angles (degrees)
   hd_h_j000_norms.txt      |psi_k|^2 for k = 1..4001 (check)
   hd_h_j000_S2prod.txt     the 500-energy probability table
+  hd_h_j000_reac_levels.txt  reactant (0-1) levels: ielec j v E - eref (eV)
+  hd_h_j000_prod_levels.txt  product (0-2) levels: j v E - eref (eV)
   hd_h_input.dat           the example's input.dat
 Run only where the reference tree exists; the outputs are committed.
 """
@@ -76,6 +78,28 @@ def main():
     src_in = os.path.join(REF, "EXAMPLES", "HD+H_collision", "input.dat")
     with open(src_in) as f, open(os.path.join(HERE, "hd_h_input.dat"), "w") as g:
         g.write(f.read())
+
+    # diatomic levels, as radial_functions01_read and product_radialf_read
+    # print them after subtracting the reference energy
+    def levels(header, ncols, name):
+        i0 = next(i for i, l in enumerate(lines) if re.match(header, l))
+        rows = []
+        for l in lines[i0 + 2:]:
+            t = l.split()
+            if not t and not rows:
+                continue
+            if len(t) == ncols and all(re.match(r"^-?\d+$", x) for x in t[:-1]):
+                rows.append(t)
+            else:
+                break
+        with open(os.path.join(HERE, name), "w") as f:
+            for t in rows:
+                f.write(" ".join(t) + "\n")
+        return len(rows)
+
+    nr = levels(r"\s*ie\s+j\s+v\s+E\(eV\)", 4, "hd_h_j000_reac_levels.txt")
+    npr = levels(r"\s*j\s+v\s+E\(eV\)", 3, "hd_h_j000_prod_levels.txt")
+    print("levels:", nr, npr)
 
     src = os.path.join(D, "S2prod.v00.J000.k00008")
     with open(src) as f, open(os.path.join(HERE, "hd_h_j000_S2prod.txt"), "w") as g:

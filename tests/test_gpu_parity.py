@@ -78,6 +78,35 @@ def test_recursion_and_projections_fp64(name):
     assert rel(a_g, a_o) < 1e-9 and rel(b_g, b_o) < 1e-9
 
 
+def test_flux_without_the_energy_test_matches_oracle():
+    """mw3_flux_energy_test(0) against the oracle run without the test; the
+    switch acts when the sums are read, so turning it back on gives the
+    default result again."""
+    c = case("states")
+    o = Oracle(c)
+    o.flux_energy_test(0)
+    g = Propagator(c, MW3_FP64)
+    o.begin(c.rpaq0)
+    g.begin(c.rpaq0)
+    o.steps(300)
+    g.steps(300)
+    p_on = g.flux_result()[0]
+    g.flux_energy_test(0)
+    t_o, e_o = o.flux_result()
+    t_g, e_g = g.flux_result()
+    a_o, b_o = o.flux_accum()
+    a_g, b_g = g.flux_accum()
+    print("P without the test: max %.3e, with it: max %.3e" % (np.max(np.abs(t_g)),
+                                                              np.max(np.abs(p_on))), flush=True)
+    assert rel(t_g, t_o) < 1e-9 and rel(e_g, e_o) < 1e-9
+    assert rel(a_g, a_o) < 1e-9 and rel(b_g, b_o) < 1e-9
+    assert np.max(np.abs(t_g - p_on)) > 1e-6 * np.max(np.abs(t_g))   # the test drops terms here
+    g.flux_energy_test(1)
+    assert np.array_equal(g.flux_result()[0], p_on)
+    with pytest.raises(RuntimeError):
+        g.flux_energy_test(2)
+
+
 def test_mixed_state_put_between_step_calls():
     """Mixed precision runs the angular term on its own stream, ordered after
     the last write of psi on the main stream.  A state put between two step
