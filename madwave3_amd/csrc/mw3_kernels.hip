@@ -1878,7 +1878,8 @@ static hipError_t ang_fused_setup(Ctx* c, int nt) {
 // registers for the whole launch (the forward pass: one 16-wide k2 tile of
 // one parity, all j; the inverse pass: one pair of 16-wide R tiles, all
 // k2), and the block walks the work items with the item's 16-row tile staged
-// in LDS; the next item's tile loads into registers during the MFMAs.
+// in LDS (the forward pass stages it folded, psi[j] +- psi[n2-1-j]); the
+// next items' tiles load into registers during the MFMAs.
 // MFMA k index of lane group g at step s: j = 16 kb + 4 g + s (forward) and
 // k2 = 32 kb + 8 g + 2 s + parity (inverse), so a lane's operands for four
 // steps are one 16-byte read.
@@ -1894,7 +1895,7 @@ struct RadPad<double> {
   static constexpr int v = 2;
 };
 
-template <typename T, int NKB, int HB>
+template <typename T, int NKB, int HB, bool FULL>
 __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi,
                                                        const T* __restrict__ Tt,
                                                        T* __restrict__ t1,
@@ -1903,9 +1904,12 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
   typedef MfmaAcc<T> M;
   typedef typename M::V V;
   constexpr int N2 = 32 * HB, H = N2 / 2, NT = 128 * NKB;  // threads
-  constexpr int LP = N2 + RadPad<T>::v;
-  constexpr int NV4 = 16 * N2 / 4, PER = (NV4 + NT - 1) / NT;
-  __shared__ __align__(16) T tile[16 * LP];
+  constexpr int LP = H + RadPad<T>::v;
+  // the tile is staged folded: fold[0][row][j] = psi[j] + psi[n2-1-j] and
+  // fold[1][row][j] = psi[j] - psi[n2-1-j], j < h; a thread loads the
+  // vectors at j and at its mirror and folds them as it stores
+  constexpr int NP = 16 * H / 4, PER = (NP + NT - 1) / NT;
+  __shared__ __align__(16) T fold[2][2][16 * LP];  // [buffer][parity]
   const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, wave = threadIdx.x >> 6;
   const int p = wave / NKB, t = wave - p * NKB;  // parity and k' tile of this wave
   const int64_t plane = (int64_t)n1 * N2;
@@ -1916,59 +1920,113 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
 #pragma unroll
     for (int kb = 0; kb < HB; kb++) ld4(trow + 16 * kb, b[kb]);
   }
-  T pf[PER][4];
-  auto fetch = [&](const int4& it) {
+  auto fetch = [&](const int4& it, T(&pf)[PER][2][4]) {
 #pragma unroll
     for (int l = 0; l < PER; l++) {
       int v = threadIdx.x + NT * l;
-      v = v < NV4 ? v : NV4 - 1;  // spare threads repeat the last vector
-      const int row = v / (N2 / 4), c4 = v - row * (N2 / 4);
+      v = v < NP ? v : NP - 1;  // spare threads repeat the last pair
+      const int row = v / (H / 4), c4 = v - row * (H / 4);
       int r = 16 * it.y + row;
       r = r < it.z ? r : it.z - 1;
-      ld4(psi + it.x * plane + (int64_t)r * N2 + 4 * c4, pf[l]);
+      const T* src = psi + it.x * plane + (int64_t)r * N2;
+      ld4(src + 4 * c4, pf[l][0]);               // psi[j .. j+3], j = 4 c4
+      ld4(src + (N2 - 4 - 4 * c4), pf[l][1]);    // psi[n2-4-j .. n2-1-j]
     }
   };
-  auto put = [&]() {
+  auto put = [&](const T(&pf)[PER][2][4], int buf) {
 #pragma unroll
     for (int l = 0; l < PER; l++) {
       int v = threadIdx.x + NT * l;
-      v = v < NV4 ? v : NV4 - 1;
-      const int row = v / (N2 / 4), c4 = v - row * (N2 / 4);
-      st4(&tile[row * LP + 4 * c4], pf[l]);
+      v = v < NP ? v : NP - 1;
+      const int row = v / (H / 4), c4 = v - row * (H / 4);
+      T e[4], o[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        e[q] = pf[l][0][q] + pf[l][1][3 - q];
+        o[q] = pf[l][0][q] - pf[l][1][3 - q];
+      }
+      st4(&fold[buf][0][row * LP + 4 * c4], e);
+      st4(&fold[buf][1][row * LP + 4 * c4], o);
     }
   };
-  int n = blockIdx.x;
-  if (n >= nitems) return;  // block-uniform
-  int4 cur = items[n];
-  fetch(cur);
-  for (; n < nitems; n += gridDim.x) {
-    const int4 nxt = items[n + (int)gridDim.x < nitems ? n + (int)gridDim.x : n];
-    __syncthreads();  // the previous tile is no longer read
-    put();
-    __syncthreads();
-    fetch(nxt);  // the last item fetches itself again, unused
-    V acc = (V){0, 0, 0, 0};
-    const T* trow = &tile[i * LP + 4 * g];
+  auto mult = [&](int buf) {
+    // two accumulator chains (even and odd j blocks), summed at the end
+    V acc0 = (V){0, 0, 0, 0}, acc1 = (V){0, 0, 0, 0};
+    const T* frow = &fold[buf][p][i * LP + 4 * g];
 #pragma unroll
-    for (int kb = 0; kb < HB; kb++) {
-      T lo[4], hi[4];
-      ld4(trow + 16 * kb, lo);
-      ld4(trow + (N2 - 4 - 8 * g) - 16 * kb, hi);  // psi[n2-4-j .. n2-1-j], j = 16 kb + 4 g
+    for (int kb = 0; kb < HB; kb += 2) {
+      T x0[4], x1[4];
+      ld4(frow + 16 * kb, x0);
+      ld4(frow + 16 * (kb + 1), x1);
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        const T x = p ? lo[s] - hi[3 - s] : lo[s] + hi[3 - s];
-        acc = M::mma(x, b[kb][s], acc);
+        acc0 = M::mma(x0[s], b[kb][s], acc0);
+        acc1 = M::mma(x1[s], b[kb + 1][s], acc1);
       }
     }
-    T* orow = t1 + cur.x * plane;
+    V acc;
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) acc[rr] = acc0[rr] + acc1[rr];
+    return acc;
+  };
+  // With FULL (n1 a multiple of 16) every lane stores, with no branch:
+  // rows nn1 .. 16 ceil(nn1 / 16) - 1 (< n1) and columns k2lo .. 32 NKB - 1
+  // then hold finite values that no later pass uses (the r-pass tables are
+  // zero on those rows, the r-pass factor (radcut - E)_+ is zero on those
+  // columns and the inverse pass reads them as zero).  Otherwise rows past
+  // n1 are not stored.
+  auto store = [&](const V& acc, const int4& it) {
+    T* orow = t1 + it.x * plane;
     const int k2 = 2 * (16 * t + i) + p;
 #pragma unroll
     for (int rr = 0; rr < 4; rr++) {
-      const int row = 16 * cur.y + M::row(lane, rr);
-      if (row < cur.z && k2 < (cur.w & 0xffff)) orow[(int64_t)row * N2 + k2] = acc[rr];
+      const int row = 16 * it.y + M::row(lane, rr);
+      if (FULL || row < n1) orow[(int64_t)row * N2 + k2] = acc[rr];
     }
+  };
+  const int G = (int)gridDim.x;
+  int n = blockIdx.x;
+  if (n >= nitems) return;  // block-uniform
+  auto at = [&](int m) { return items[m < nitems ? m : nitems - 1]; };
+  // Two LDS buffers, one barrier per item: while a wave multiplies item n
+  // out of one buffer it has stored item n + G (loaded into registers during
+  // item n - G) into the other and issued the loads of item n + 2 G.  The
+  // barrier at the end of an item frees its buffer and completes the other.
+  // Each item's results are stored at the start of the next one, before the
+  // tile is staged: the wait for the staged loads then leaves those stores
+  // in flight.  A prefetch past the last item re-reads the last item,
+  // unused; the scheduling barriers keep this order of issue.
+  T pf[PER][2][4];
+  int4 cur = at(n);
+  fetch(cur, pf);
+  put(pf, 0);
+  int4 nxt = at(n + G);
+  fetch(nxt, pf);
+  __syncthreads();
+  int4 nn = at(n + 2 * G);
+  put(pf, 1);
+  fetch(nn, pf);
+  __builtin_amdgcn_sched_barrier(0);
+  V acc = mult(0);
+  __syncthreads();
+  int4 prev = cur;
+  cur = nxt;
+  nxt = nn;
+  n += G;
+  for (int buf = 1; n < nitems; n += G, buf ^= 1) {
+    store(acc, prev);
+    __builtin_amdgcn_sched_barrier(0);
+    put(pf, buf ^ 1);
+    nn = at(n + 2 * G);
+    fetch(nn, pf);
+    __builtin_amdgcn_sched_barrier(0);
+    acc = mult(buf);
+    __syncthreads();
+    prev = cur;
     cur = nxt;
+    nxt = nn;
   }
+  store(acc, prev);
 }
 
 template <typename T, int NKB, int HB>
@@ -2313,7 +2371,9 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   const T* tab = (const T*)c->d_dst_tables;
   const int4* items = (const int4*)c->d_rad_items;
   if (mode == 2) {
-    hipError_t e = persistent_grid(k_rad_fwd<T, NKB, HB>, 128 * NKB, &c->rad_grid[0]);
+    hipError_t e = t.n1 % 16 == 0
+                       ? persistent_grid(k_rad_fwd<T, NKB, HB, true>, 128 * NKB, &c->rad_grid[0])
+                       : persistent_grid(k_rad_fwd<T, NKB, HB, false>, 128 * NKB, &c->rad_grid[0]);
     if (e != hipSuccess) return e;
     return persistent_grid(k_rad_inv<T, NKB, HB>, 32 * HB, &c->rad_grid[1]);
   }
@@ -2324,8 +2384,13 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   const int gdiv = c->opt_rad_grid > 0 ? c->opt_rad_grid : 1;
   const int g0 = std::max(c->rad_grid[0] / gdiv, 1), g1 = std::max(c->rad_grid[1] / gdiv, 1);
   if (mode == 0) {
-    hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB>), dim3(n < g0 ? n : g0), dim3(128 * NKB), 0,
-                       c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+    if (t.n1 % 16 == 0)
+      hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, true>), dim3(n < g0 ? n : g0), dim3(128 * NKB), 0,
+                         c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+    else
+      hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, false>), dim3(n < g0 ? n : g0),
+                         dim3(128 * NKB), 0, c->stream, psi, tab + t.rad_tt_off, t1, items, n,
+                         t.n1);
   } else {
     hipLaunchKernelGGL((k_rad_inv<T, NKB, HB>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
                        c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
