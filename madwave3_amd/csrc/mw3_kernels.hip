@@ -1895,30 +1895,42 @@ struct RadPad<double> {
   static constexpr int v = 2;
 };
 
+// waves of k_rad_fwd per (parity, k' tile): two in FP32 when HB is a
+// multiple of 4, each over half the j range (the block then has 12 waves for
+// three k' tiles, three per SIMD); one otherwise, and in FP64, whose table
+// slice leaves no room for more waves
+template <typename T, int HB>
+struct RadSplit {
+  static constexpr int v = sizeof(T) == 4 && HB % 4 == 0 ? 2 : 1;
+};
+
 template <typename T, int NKB, int HB, bool FULL>
-__global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi,
-                                                       const T* __restrict__ Tt,
-                                                       T* __restrict__ t1,
-                                                       const int4* __restrict__ items, int nitems,
-                                                       int n1) {
+__global__ __launch_bounds__(128 * NKB * (RadSplit<T, HB>::v)) void k_rad_fwd(
+    const T* __restrict__ psi, const T* __restrict__ Tt, T* __restrict__ t1,
+    const int4* __restrict__ items, int nitems, int n1) {
   typedef MfmaAcc<T> M;
   typedef typename M::V V;
-  constexpr int N2 = 32 * HB, H = N2 / 2, NT = 128 * NKB;  // threads
+  constexpr int KS = RadSplit<T, HB>::v, HBW = HB / KS;  // j blocks per wave (even)
+  constexpr int N2 = 32 * HB, H = N2 / 2, NT = 128 * NKB * KS;  // threads
   constexpr int LP = H + RadPad<T>::v;
   // the tile is staged folded: fold[0][row][j] = psi[j] + psi[n2-1-j] and
   // fold[1][row][j] = psi[j] - psi[n2-1-j], j < h; a thread loads the
   // vectors at j and at its mirror and folds them as it stores
   constexpr int NP = 16 * H / 4, PER = (NP + NT - 1) / NT;
   __shared__ __align__(16) T fold[2][2][16 * LP];  // [buffer][parity]
+  // partial tiles of the second j half, by buffer (KS = 2)
+  __shared__ V red[KS == 2 ? 2 : 1][KS == 2 ? 2 * NKB : 1][64];
   const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, wave = threadIdx.x >> 6;
-  const int p = wave / NKB, t = wave - p * NKB;  // parity and k' tile of this wave
+  const int kh = wave / (2 * NKB), w2 = wave - kh * 2 * NKB;  // j half, tile index
+  const int p = w2 / NKB, t = w2 - p * NKB;  // parity and k' tile of this wave
   const int64_t plane = (int64_t)n1 * N2;
-  // b[kb][s] = S(16 kb + 4 g + s, 2 (16 t + i) + p) for every j block
-  T b[HB][4];
+  // b[kb][s] = S(16 (kh HBW + kb) + 4 g + s, 2 (16 t + i) + p) over this
+  // wave's j blocks
+  T b[HBW][4];
   {
-    const T* trow = Tt + (int64_t)(2 * (16 * t + i) + p) * H + 4 * g;
+    const T* trow = Tt + (int64_t)(2 * (16 * t + i) + p) * H + 16 * kh * HBW + 4 * g;
 #pragma unroll
-    for (int kb = 0; kb < HB; kb++) ld4(trow + 16 * kb, b[kb]);
+    for (int kb = 0; kb < HBW; kb++) ld4(trow + 16 * kb, b[kb]);
   }
   auto fetch = [&](const int4& it, T(&pf)[PER][2][4]) {
 #pragma unroll
@@ -1950,18 +1962,27 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
     }
   };
   auto mult = [&](int buf) {
-    // two accumulator chains (even and odd j blocks), summed at the end
+    // two accumulator chains (even and odd j blocks), summed at the end;
+    // the operands of the next pair of j blocks are read from LDS before the
+    // MFMAs of this pair are issued (the scheduling barrier keeps that
+    // order), so the LDS latency hides behind the matrix work
     V acc0 = (V){0, 0, 0, 0}, acc1 = (V){0, 0, 0, 0};
-    const T* frow = &fold[buf][p][i * LP + 4 * g];
+    const T* frow = &fold[buf][p][i * LP + 16 * kh * HBW + 4 * g];
+    T x[2][2][4];
+    ld4(frow, x[0][0]);
+    ld4(frow + 16, x[0][1]);
 #pragma unroll
-    for (int kb = 0; kb < HB; kb += 2) {
-      T x0[4], x1[4];
-      ld4(frow + 16 * kb, x0);
-      ld4(frow + 16 * (kb + 1), x1);
+    for (int kb = 0; kb < HBW; kb += 2) {
+      const int c = (kb >> 1) & 1;
+      if (kb + 2 < HBW) {
+        ld4(frow + 16 * (kb + 2), x[c ^ 1][0]);
+        ld4(frow + 16 * (kb + 3), x[c ^ 1][1]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        acc0 = M::mma(x0[s], b[kb][s], acc0);
-        acc1 = M::mma(x1[s], b[kb + 1][s], acc1);
+        acc0 = M::mma(x[c][0][s], b[kb][s], acc0);
+        acc1 = M::mma(x[c][1][s], b[kb + 1][s], acc1);
       }
     }
     V acc;
@@ -1974,8 +1995,14 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
   // then hold finite values that no later pass uses (the r-pass tables are
   // zero on those rows, the r-pass factor (radcut - E)_+ is zero on those
   // columns and the inverse pass reads them as zero).  Otherwise rows past
-  // n1 are not stored.
-  auto store = [&](const V& acc, const int4& it) {
+  // n1 are not stored.  With KS = 2 the first-half waves add the second
+  // half's tile (left in LDS by the previous item) and store.
+  auto store = [&](V acc, const int4& it, int buf) {
+    if (KS == 2) {
+      if (kh) return;  // wave-uniform
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) acc[rr] += red[buf][w2][lane][rr];
+    }
     T* orow = t1 + it.x * plane;
     const int k2 = 2 * (16 * t + i) + p;
 #pragma unroll
@@ -1984,6 +2011,9 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
       if (FULL || row < n1) orow[(int64_t)row * N2 + k2] = acc[rr];
     }
   };
+  auto keep = [&](const V& acc, int buf) {  // the second half's partial tile
+    if (KS == 2 && kh) red[buf][w2][lane] = acc;
+  };
   const int G = (int)gridDim.x;
   int n = blockIdx.x;
   if (n >= nitems) return;  // block-uniform
@@ -1991,11 +2021,12 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
   // Two LDS buffers, one barrier per item: while a wave multiplies item n
   // out of one buffer it has stored item n + G (loaded into registers during
   // item n - G) into the other and issued the loads of item n + 2 G.  The
-  // barrier at the end of an item frees its buffer and completes the other.
-  // Each item's results are stored at the start of the next one, before the
-  // tile is staged: the wait for the staged loads then leaves those stores
-  // in flight.  A prefetch past the last item re-reads the last item,
-  // unused; the scheduling barriers keep this order of issue.
+  // barrier at the end of an item frees its buffer and completes the other
+  // (and the partial tiles of that item).  Each item's results are stored at
+  // the start of the next one, before the tile is staged: the wait for the
+  // staged loads then leaves those stores in flight.  A prefetch past the
+  // last item re-reads the last item, unused; the scheduling barriers keep
+  // this order of issue.
   T pf[PER][2][4];
   int4 cur = at(n);
   fetch(cur, pf);
@@ -2008,25 +2039,28 @@ __global__ __launch_bounds__(128 * NKB) void k_rad_fwd(const T* __restrict__ psi
   fetch(nn, pf);
   __builtin_amdgcn_sched_barrier(0);
   V acc = mult(0);
+  keep(acc, 0);
   __syncthreads();
   int4 prev = cur;
   cur = nxt;
   nxt = nn;
   n += G;
-  for (int buf = 1; n < nitems; n += G, buf ^= 1) {
-    store(acc, prev);
+  int buf = 1;
+  for (; n < nitems; n += G, buf ^= 1) {
+    store(acc, prev, buf ^ 1);
     __builtin_amdgcn_sched_barrier(0);
     put(pf, buf ^ 1);
     nn = at(n + 2 * G);
     fetch(nn, pf);
     __builtin_amdgcn_sched_barrier(0);
     acc = mult(buf);
+    keep(acc, buf);
     __syncthreads();
     prev = cur;
     cur = nxt;
     nxt = nn;
   }
-  store(acc, prev);
+  store(acc, prev, buf ^ 1);
 }
 
 template <typename T, int NKB, int HB>
@@ -2372,8 +2406,10 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   const int4* items = (const int4*)c->d_rad_items;
   if (mode == 2) {
     hipError_t e = t.n1 % 16 == 0
-                       ? persistent_grid(k_rad_fwd<T, NKB, HB, true>, 128 * NKB, &c->rad_grid[0])
-                       : persistent_grid(k_rad_fwd<T, NKB, HB, false>, 128 * NKB, &c->rad_grid[0]);
+                       ? persistent_grid(k_rad_fwd<T, NKB, HB, true>, 128 * NKB * RadSplit<T, HB>::v,
+                                         &c->rad_grid[0])
+                       : persistent_grid(k_rad_fwd<T, NKB, HB, false>, 128 * NKB * RadSplit<T, HB>::v,
+                                         &c->rad_grid[0]);
     if (e != hipSuccess) return e;
     return persistent_grid(k_rad_inv<T, NKB, HB>, 32 * HB, &c->rad_grid[1]);
   }
@@ -2384,13 +2420,13 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   const int gdiv = c->opt_rad_grid > 0 ? c->opt_rad_grid : 1;
   const int g0 = std::max(c->rad_grid[0] / gdiv, 1), g1 = std::max(c->rad_grid[1] / gdiv, 1);
   if (mode == 0) {
+    const int nt = 128 * NKB * RadSplit<T, HB>::v;
     if (t.n1 % 16 == 0)
-      hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, true>), dim3(n < g0 ? n : g0), dim3(128 * NKB), 0,
+      hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, true>), dim3(n < g0 ? n : g0), dim3(nt), 0,
                          c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
     else
-      hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, false>), dim3(n < g0 ? n : g0),
-                         dim3(128 * NKB), 0, c->stream, psi, tab + t.rad_tt_off, t1, items, n,
-                         t.n1);
+      hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, false>), dim3(n < g0 ? n : g0), dim3(nt), 0,
+                         c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
   } else {
     hipLaunchKernelGGL((k_rad_inv<T, NKB, HB>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
                        c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
