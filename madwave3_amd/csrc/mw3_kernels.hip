@@ -2063,7 +2063,7 @@ __global__ __launch_bounds__(128 * NKB * (RadSplit<T, HB>::v)) void k_rad_fwd(
   store(acc, prev, buf ^ 1);
 }
 
-template <typename T, int NKB, int HB>
+template <typename T, int NKB, int HB, bool FULL>
 __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
                                                      const T* __restrict__ Ti,
                                                      T* __restrict__ corr,
@@ -2074,7 +2074,7 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
   constexpr int N2 = 32 * HB, NT = 32 * HB;  // one wave per pair of R tiles of the first half
   constexpr int K2P = 32 * NKB, LP = K2P + RadPad<T>::v;
   constexpr int NV4 = 16 * K2P / 4, PER = (NV4 + NT - 1) / NT;
-  __shared__ __align__(16) T tile[16 * LP];
+  __shared__ __align__(16) T tile[2][16 * LP];
   const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, pair = threadIdx.x >> 6;
   const int64_t plane = (int64_t)n1 * N2;
   // b[q][kb][e] = S(R, 32 kb + 8 g + e), R = 16 (2 pair + q) + i
@@ -2100,7 +2100,7 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
       ld4(t1 + it.x * plane + (int64_t)r * N2 + 4 * c4, pf[l]);
     }
   };
-  auto put = [&](const int4& it) {  // columns k2 >= k2lo enter as zero
+  auto put = [&](const int4& it, int buf) {  // columns k2 >= k2lo enter as zero
 #pragma unroll
     for (int l = 0; l < PER; l++) {
       int v = threadIdx.x + NT * l;
@@ -2109,55 +2109,83 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
 #pragma unroll
       for (int e = 0; e < 4; e++)
         if (4 * c4 + e >= (it.w & 0xffff)) pf[l][e] = 0;
-      st4(&tile[row * LP + 4 * c4], pf[l]);
+      st4(&tile[buf][row * LP + 4 * c4], pf[l]);
     }
   };
-  int n = blockIdx.x;
-  if (n >= nitems) return;  // block-uniform
-  int4 cur = items[n];
-  fetch(cur);
-  for (; n < nitems; n += gridDim.x) {
-    const int4 nxt = items[n + (int)gridDim.x < nitems ? n + (int)gridDim.x : n];
-    __syncthreads();
-    put(cur);
-    __syncthreads();
-    fetch(nxt);
-    // pairs whose 16 rows hold no compressed-grid point in their columns or
-    // the mirrored ones: the update reads none of their output
-    if (!((cur.w >> (16 + pair)) & 1)) {
-      cur = nxt;
-      continue;
-    }
+  // pairs whose 16 rows hold no compressed-grid point in their columns or
+  // the mirrored ones: the update reads none of their output
+  auto active = [&](const int4& it) { return (it.w >> (16 + pair)) & 1; };
+  V E[2], O[2];
+  auto mult = [&](int buf) {
     T a[NKB][2][4];
 #pragma unroll
     for (int kb = 0; kb < NKB; kb++) {
-      ld4(&tile[i * LP + 32 * kb + 8 * g], a[kb][0]);
-      ld4(&tile[i * LP + 32 * kb + 8 * g + 4], a[kb][1]);
+      ld4(&tile[buf][i * LP + 32 * kb + 8 * g], a[kb][0]);
+      ld4(&tile[buf][i * LP + 32 * kb + 8 * g + 4], a[kb][1]);
     }
-    T* ob = corr + cur.x * plane;
 #pragma unroll
     for (int q = 0; q < 2; q++) {
-      V E = (V){0, 0, 0, 0}, O = (V){0, 0, 0, 0};
+      E[q] = (V){0, 0, 0, 0};
+      O[q] = (V){0, 0, 0, 0};
 #pragma unroll
       for (int kb = 0; kb < NKB; kb++)
 #pragma unroll
         for (int s = 0; s < 4; s++) {
           const int e0 = 2 * s, e1 = 2 * s + 1;
-          E = M::mma(a[kb][e0 >> 2][e0 & 3], b[q][kb][e0 >> 2][e0 & 3], E);
-          O = M::mma(a[kb][e1 >> 2][e1 & 3], b[q][kb][e1 >> 2][e1 & 3], O);
+          E[q] = M::mma(a[kb][e0 >> 2][e0 & 3], b[q][kb][e0 >> 2][e0 & 3], E[q]);
+          O[q] = M::mma(a[kb][e1 >> 2][e1 & 3], b[q][kb][e1 >> 2][e1 & 3], O[q]);
         }
+    }
+  };
+  // With FULL (n1 a multiple of 16) rows nn1 .. 16 ceil(nn1 / 16) - 1 are
+  // stored too (below n1; the update reads no row past nn1), with no
+  // per-lane branch; otherwise rows past nn1 are not stored.
+  auto store = [&](const int4& it) {
+    T* ob = corr + it.x * plane;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
       const int R = 16 * (2 * pair + q) + i;
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int row = 16 * cur.y + M::row(lane, rr);
-        if (row < cur.z) {
-          ob[(int64_t)row * N2 + R] = E[rr] + O[rr];
-          ob[(int64_t)row * N2 + (N2 - 1 - R)] = E[rr] - O[rr];
+        const int row = 16 * it.y + M::row(lane, rr);
+        if (FULL || row < it.z) {
+          ob[(int64_t)row * N2 + R] = E[q][rr] + O[q][rr];
+          ob[(int64_t)row * N2 + (N2 - 1 - R)] = E[q][rr] - O[q][rr];
         }
       }
     }
+  };
+  const int G = (int)gridDim.x;
+  int n = blockIdx.x;
+  if (n >= nitems) return;  // block-uniform
+  auto at = [&](int m) { return items[m < nitems ? m : nitems - 1]; };
+  // Two LDS buffers, one barrier per item, as in k_rad_fwd: item n is
+  // multiplied out of one buffer after item n + G has been staged into the
+  // other and the loads of item n + 2 G issued; its results are stored at
+  // the start of the next item, after the staging.  A prefetch past the
+  // last item re-reads the last item, unused.
+  int4 cur = at(n);
+  fetch(cur);
+  put(cur, 0);
+  int4 nxt = at(n + G);
+  fetch(nxt);
+  __syncthreads();
+  int4 prev = cur;
+  bool have = false;  // results of prev in E, O
+  for (int buf = 0; n < nitems; n += G, buf ^= 1) {
+    put(nxt, buf ^ 1);
+    if (have && active(prev)) store(prev);
+    const int4 nn = at(n + 2 * G);
+    fetch(nn);
+    __builtin_amdgcn_sched_barrier(0);  // issue the loads before the MFMAs
+    if (active(cur)) mult(buf);
+    have = true;
+    __syncthreads();
+    prev = cur;
     cur = nxt;
+    nxt = nn;
   }
+  if (active(prev)) store(prev);
 }
 
 // ---------------------------------------------------------------------------
@@ -2411,7 +2439,8 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
                        : persistent_grid(k_rad_fwd<T, NKB, HB, false>, 128 * NKB * RadSplit<T, HB>::v,
                                          &c->rad_grid[0]);
     if (e != hipSuccess) return e;
-    return persistent_grid(k_rad_inv<T, NKB, HB>, 32 * HB, &c->rad_grid[1]);
+    return t.n1 % 16 == 0 ? persistent_grid(k_rad_inv<T, NKB, HB, true>, 32 * HB, &c->rad_grid[1])
+                          : persistent_grid(k_rad_inv<T, NKB, HB, false>, 32 * HB, &c->rad_grid[1]);
   }
   const int n = c->rad_nitems;
   // the grid: the resident blocks divided by the grid option (each block
@@ -2428,8 +2457,12 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
       hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, false>), dim3(n < g0 ? n : g0), dim3(nt), 0,
                          c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
   } else {
-    hipLaunchKernelGGL((k_rad_inv<T, NKB, HB>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
-                       c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+    if (t.n1 % 16 == 0)
+      hipLaunchKernelGGL((k_rad_inv<T, NKB, HB, true>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
+                         c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+    else
+      hipLaunchKernelGGL((k_rad_inv<T, NKB, HB, false>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
+                         c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
   }
   return hipGetLastError();
 }
