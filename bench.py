@@ -92,11 +92,13 @@ def per_kernel(times, steps):
             for k, (ms, cnt) in times.items()}
 
 
-def run_gpu(case, precision, steps, warmup, timing, tune=0):
-    from madwave3_amd.lib import Propagator
+def run_gpu(case, precision, steps, warmup, timing, tune=0, options=None):
+    from madwave3_amd.lib import OPTION_NAMES, Propagator
 
     g = Propagator(case, precision)
     g.begin(case.rpaq0)
+    for name, value in (options or {}).items():
+        g.set_option(OPTION_NAMES.index(name), value)
     tuned = None
     if tune:
         # the autotuner picks the kernel variants and stream/graph layout from
@@ -360,6 +362,9 @@ def main():
                     help="run a copy of this many MiB first (byte-counter calibration)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default execution options instead of autotuning them")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="set an execution option (graphs, radial, angular, side_stream, "
+                         "radial_grid) in both propagators instead of autotuning; repeatable")
     ap.add_argument("--accuracy", action="store_true",
                     help="full-length mixed vs FP64 run of --config instead of the timing")
     ap.add_argument("--full-steps", type=int, default=0,
@@ -383,12 +388,13 @@ def main():
     case = build_case(a.config)
     print("# case %s built in %.1fs" % (case, time.perf_counter() - t0), file=sys.stderr, flush=True)
     N = case.ncan * case.nangu * case.npun1 * case.npun2
-    tune = 0 if a.no_autotune else 16
+    opts = dict((kv.split("=")[0], int(kv.split("=")[1])) for kv in a.option)
+    tune = 0 if (a.no_autotune or opts) else 16
     dt64, tim64, P64, nflux, tuned64 = run_gpu(case, MW3_FP64, a.steps, a.warmup,
-                                               not a.no_timing, tune)
+                                               not a.no_timing, tune, opts)
     print("# fp64: %.3f ms/step %s" % (1e3 * dt64 / a.steps, tuned64), file=sys.stderr, flush=True)
     dtmx, timmx, Pmx, _, tunedmx = run_gpu(case, MW3_MIXED, a.steps, a.warmup,
-                                           not a.no_timing, tune)
+                                           not a.no_timing, tune, opts)
     print("# mixed: %.3f ms/step %s" % (1e3 * dtmx / a.steps, tunedmx), file=sys.stderr,
           flush=True)
     dP = float(np.max(np.abs(Pmx - P64)))
