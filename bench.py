@@ -389,7 +389,7 @@ def main():
     print("# case %s built in %.1fs" % (case, time.perf_counter() - t0), file=sys.stderr, flush=True)
     N = case.ncan * case.nangu * case.npun1 * case.npun2
     opts = dict((kv.split("=")[0], int(kv.split("=")[1])) for kv in a.option)
-    tune = 0 if (a.no_autotune or opts) else 16
+    tune = 0 if (a.no_autotune or opts) else 48
     dt64, tim64, P64, nflux, tuned64 = run_gpu(case, MW3_FP64, a.steps, a.warmup,
                                                not a.no_timing, tune, opts)
     print("# fp64: %.3f ms/step %s" % (1e3 * dt64 / a.steps, tuned64), file=sys.stderr, flush=True)
