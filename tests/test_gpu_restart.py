@@ -30,7 +30,7 @@ def test_continue_from_files(tmp_path, name):
     g2 = Propagator(c, MW3_FP64)
     assert restart.resume(g2, str(tmp_path)) == (301, 1, 1)
     n2, a2 = g2.steps(200, want_auto=c.iphoto > 0)
-    assert g2.k() == g1.k() == 501
+    assert g2.k == g1.k == 501
     print("norms %.1e P %.1e" % (rel(n2, n1), rel(g2.flux_result()[0], g1.flux_result()[0])),
           flush=True)
     assert rel(n2, n1) < 1e-13
