@@ -736,8 +736,14 @@ __global__ __launch_bounds__(256) void k_update(const double* __restrict__ psi,
     for (int u = 0; u < 4; u++) {
       const int idx = b + u < cnt ? b + u : cnt - 1;
       rr[u] = a.urow[off + idx];
-      act[u] = b + u < cnt && inR && ((a.uword[off + idx] >> lane) & 1ull);
-      pl[u] = (int64_t)ia * plane + (int64_t)rr[u] * a.n2 + Ra;
+      const uint64_t w = a.uword[off + idx];
+      const bool kept = (w >> lane) & 1ull;
+      act[u] = b + u < cnt && inR && kept;
+      // lanes without a kept point read the row's first kept point instead
+      // of their own: the row's loads then touch only the cache lines that
+      // hold kept points (the values they load are not used)
+      const int Ru = kept ? Ra : rc * 64 + (w ? (int)__builtin_ctzll(w) : 0);
+      pl[u] = (int64_t)ia * plane + (int64_t)rr[u] * a.n2 + Ru;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
