@@ -364,7 +364,8 @@ def main():
                     help="keep the default execution options instead of autotuning them")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
                     help="set an execution option (graphs, radial, angular, side_stream, "
-                         "radial_grid) in both propagators instead of autotuning; repeatable")
+                         "radial_grid, angular_grid) in both propagators instead of autotuning; "
+                         "repeatable")
     ap.add_argument("--accuracy", action="store_true",
                     help="full-length mixed vs FP64 run of --config instead of the timing")
     ap.add_argument("--full-steps", type=int, default=0,

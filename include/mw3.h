@@ -293,6 +293,8 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
    MW3_OPT_SIDE_STREAM  1: the angular term on its own stream beside the
                    radial passes (default in MW3_MIXED); 0: one stream
                    (default in MW3_FP64).
+   MW3_OPT_ANGULAR_GRID 1 (default) or 2: the folded angular kernel runs
+                   its blocks per channel divided by this value.
    MW3_OPT_RADIAL_GRID  1 (default), 2 or 4: the persistent radial kernels
                    run the resident block count divided by this value (each
                    block loads its waves' sine-table slices once).
@@ -304,7 +306,8 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
 #define MW3_OPT_ANGULAR 2
 #define MW3_OPT_SIDE_STREAM 3
 #define MW3_OPT_RADIAL_GRID 4
-#define MW3_NOPTIONS 5
+#define MW3_OPT_ANGULAR_GRID 5
+#define MW3_NOPTIONS 6
 MW3_API int32_t mw3_set_option(mw3_ctx *ctx, int32_t option, int32_t value);
 MW3_API int32_t mw3_get_option(const mw3_ctx *ctx, int32_t option, int32_t *value);
 

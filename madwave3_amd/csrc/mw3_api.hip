@@ -1081,6 +1081,12 @@ int32_t mw3_set_option(mw3_ctx* ctx, int32_t option, int32_t value) {
       if (value != c->opt_side) drop_graphs(c);
       c->opt_side = value;
       return MW3_OK;
+    case MW3_OPT_ANGULAR_GRID:
+      if (value != 1 && value != 2)
+        return seterr(c, MW3_EINVAL, "mw3_set_option: angular grid divisor 1 or 2");
+      if (value != c->opt_ang_grid) drop_graphs(c);
+      c->opt_ang_grid = value;
+      return MW3_OK;
     case MW3_OPT_RADIAL_GRID:
       if (value != 1 && value != 2 && value != 4)
         return seterr(c, MW3_EINVAL, "mw3_set_option: radial grid divisor 1, 2 or 4");
@@ -1105,6 +1111,7 @@ static void effective_options(const Ctx* c, int32_t* out) {
   out[MW3_OPT_ANGULAR] = a;
   out[MW3_OPT_SIDE_STREAM] = c->opt_side;
   out[MW3_OPT_RADIAL_GRID] = c->rad_nitems && c->opt_radial ? c->opt_rad_grid : 1;
+  out[MW3_OPT_ANGULAR_GRID] = c->ang_fold && out[MW3_OPT_ANGULAR] == 0 ? c->opt_ang_grid : 1;
 }
 
 int32_t mw3_get_option(const mw3_ctx* ctx, int32_t option, int32_t* value) {
@@ -1176,6 +1183,7 @@ int32_t mw3_autotune(mw3_ctx* ctx, int32_t trial_steps, int32_t* chosen, double*
     c->opt_angular = k[MW3_OPT_ANGULAR];
     c->opt_side = k[MW3_OPT_SIDE_STREAM];
     c->opt_rad_grid = k[MW3_OPT_RADIAL_GRID];
+    c->opt_ang_grid = k[MW3_OPT_ANGULAR_GRID];
   };
   Key best_key;
   effective_options(c, best_key.data());
@@ -1184,18 +1192,20 @@ int32_t mw3_autotune(mw3_ctx* ctx, int32_t trial_steps, int32_t* chosen, double*
     for (int r = 0; r <= 1; r++)
       for (int a = 0; a <= 2; a++)
         for (int sd = 0; sd <= 1; sd++)
-          for (int gd = 1; gd <= 4; gd *= 2) {
-            Key k;
-            k[MW3_OPT_GRAPHS] = g;
-            k[MW3_OPT_RADIAL] = r;
-            k[MW3_OPT_ANGULAR] = a;
-            k[MW3_OPT_SIDE_STREAM] = sd;
-            k[MW3_OPT_RADIAL_GRID] = gd;
-            apply(k);
-            Key eff;
-            effective_options(c, eff.data());
-            if (std::find(cands.begin(), cands.end(), eff) == cands.end()) cands.push_back(eff);
-          }
+          for (int gd = 1; gd <= 4; gd *= 2)
+            for (int ag = 1; ag <= 2; ag++) {
+              Key k;
+              k[MW3_OPT_GRAPHS] = g;
+              k[MW3_OPT_RADIAL] = r;
+              k[MW3_OPT_ANGULAR] = a;
+              k[MW3_OPT_SIDE_STREAM] = sd;
+              k[MW3_OPT_RADIAL_GRID] = gd;
+              k[MW3_OPT_ANGULAR_GRID] = ag;
+              apply(k);
+              Key eff;
+              effective_options(c, eff.data());
+              if (std::find(cands.begin(), cands.end(), eff) == cands.end()) cands.push_back(eff);
+            }
   double best = 1e30;
   for (const auto& key : cands) {
     drop_graphs(c);

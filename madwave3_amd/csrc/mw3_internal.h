@@ -276,6 +276,7 @@ struct Ctx {
   // term on the third stream (1) or the main one (0)
   int opt_radial = 1, opt_angular = 0, opt_side = 0;
   int opt_rad_grid = 1;           // persistent radial grids: resident blocks / this
+  int opt_ang_grid = 1;           // folded angular kernel: blocks per channel / this
   int flux_etest = 1;             // E > V(flux point) test when the flux sums are read
   bool trial = false;             // autotuner trial steps: no ring flush, no psi(E)
   bool graph_slot_dev = false;    // kernels being enqueued read their slot from d_slot

@@ -1708,11 +1708,13 @@ template <typename T, int NT>
 static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   const HostTables& t = c->t;
   if (c->ang_ngroups == 0) return;
-  hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3((unsigned)(c->ang_fold_nbc * t.nc)),
-                     dim3(64 * ANG_WAVES), 0, st, psi, (const T*)c->d_fold_D,
-                     (const T*)c->d_fold_a, (const T*)c->d_fold_b, (T*)c->d_ang,
-                     c->d_ang_groups, c->ang_ngroups,
-                     c->ang_fold_nbc, t.na, t.n1, t.n2);
+  // fewer blocks per channel (the grid option) walk more groups each and
+  // leave CUs to the radial passes beside them
+  const int nbc = std::max(1, c->ang_fold_nbc / (c->opt_ang_grid > 0 ? c->opt_ang_grid : 1));
+  hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
+                     st, psi, (const T*)c->d_fold_D, (const T*)c->d_fold_a,
+                     (const T*)c->d_fold_b, (T*)c->d_ang, c->d_ang_groups, c->ang_ngroups, nbc,
+                     t.na, t.n1, t.n2);
 }
 
 template <typename T>

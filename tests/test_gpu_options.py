@@ -11,16 +11,16 @@ import itertools
 import numpy as np
 import pytest
 
-from madwave3_amd.lib import (MW3_FP64, MW3_MIXED, MW3_OPT_ANGULAR, MW3_OPT_RADIAL,
-                              MW3_OPT_RADIAL_GRID, MW3_OPT_SIDE_STREAM, Propagator)
+from madwave3_amd.lib import (MW3_FP64, MW3_MIXED, MW3_OPT_ANGULAR, MW3_OPT_ANGULAR_GRID,
+                              MW3_OPT_RADIAL, MW3_OPT_RADIAL_GRID, MW3_OPT_SIDE_STREAM, Propagator)
 from oracle.oracle import Oracle
 from tests.cases import case
 
 pytestmark = pytest.mark.gpu
 
-# radial, angular, side stream, radial grid divisor
-COMBOS = [k + (1,) for k in itertools.product([0, 1], [0, 1, 2], [0, 1])] + \
-    [(1, 0, 1, 2), (1, 0, 0, 4)]
+# radial, angular, side stream, radial grid divisor, angular grid divisor
+COMBOS = [k + (1, 1) for k in itertools.product([0, 1], [0, 1, 2], [0, 1])] + \
+    [(1, 0, 1, 2, 1), (1, 0, 0, 4, 1), (1, 0, 1, 2, 2), (0, 0, 0, 1, 2)]
 
 
 def rel(a, b):
@@ -39,18 +39,20 @@ def test_every_option_combination_matches_oracle(name):
     n_ref, _, _ = o.steps(60)
     P_ref = o.flux_result()[0]
     seen = set()
-    for radial, angular, side, grid in COMBOS:
+    for radial, angular, side, grid, agrid in COMBOS:
         for prec in (MW3_FP64, MW3_MIXED):
             g = Propagator(c, prec)
             g.set_option(MW3_OPT_RADIAL, radial)
             g.set_option(MW3_OPT_ANGULAR, angular)
             g.set_option(MW3_OPT_SIDE_STREAM, side)
             g.set_option(MW3_OPT_RADIAL_GRID, grid)
+            g.set_option(MW3_OPT_ANGULAR_GRID, agrid)
             eff = tuple(g.get_option(k) for k in (MW3_OPT_RADIAL, MW3_OPT_ANGULAR,
-                                                  MW3_OPT_SIDE_STREAM, MW3_OPT_RADIAL_GRID))
+                                                  MW3_OPT_SIDE_STREAM, MW3_OPT_RADIAL_GRID,
+                                                  MW3_OPT_ANGULAR_GRID))
             seen.add(eff)
             label = "%s prec %d options %s (in effect %s)" % (
-                name, prec, (radial, angular, side, grid), eff)
+                name, prec, (radial, angular, side, grid, agrid), eff)
             e_h = rel(g.hphi(psi), h_ref)
             if prec == MW3_FP64:
                 g.begin(c.rpaq0)
