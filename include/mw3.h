@@ -198,6 +198,24 @@ MW3_API int32_t mw3_flux_result(mw3_ctx *ctx, double *S2prodtot, double *S2prode
    at any time and applies to every step. */
 MW3_API int32_t mw3_flux_energy_test(mw3_ctx *ctx, int32_t on);
 
+/* The per-step records the reference writes to Cvj.NNNN and Cvjprod.NNNN
+   ("write(15) it,Cvj" and "write(16) it,Cvjprod", SRC/main_madwave3.f:353-365):
+   with on = 1, every ring flush (at the end of every mw3_cheby_steps call and
+   every 256 steps inside one) appends one record per step it holds to a list
+   in host memory: the Chebyshev index k, Cvj(nvini:nvmax, jini:jmax, ncan)
+   of Cvjflux_k (SRC/mod_flux_01y2.f:663-733) as [ncan][nj][nv], and, with a
+   product projection attached, Cvjprod as mw3_cvjprod_get lays it out.
+   on = 0 stops recording and drops the held records; so do
+   mw3_cheby_begin and mw3_prod_attach. */
+MW3_API int32_t mw3_records(mw3_ctx *ctx, int32_t on);
+
+/* Moves the oldest min(maxrec, held) records out, oldest first: it[i],
+   cvj[i][ncan*nj*nv], cvjprod[i][nomb*nj'*nv'] (it, cvj or cvjprod may be
+   NULL: not copied; cvjprod is left untouched without a product
+   projection); *nrec = the number moved. */
+MW3_API int32_t mw3_records_get(mw3_ctx *ctx, int32_t maxrec, int32_t *it, double *cvj,
+                                double *cvjprod, int32_t *nrec);
+
 /* zS2(netot,nvini:nvmax,jini:jmax,ncan) of Cvjflux_k
    (SRC/mod_flux_01y2.f:663-775), complex as interleaved (re, im):
    [ncan][jmax-jini+1][nvmax-nvini+1][netot][2]. */

@@ -496,6 +496,9 @@ int32_t mw3_cheby_begin(mw3_ctx* ctx, const double* psi0, double* stats3) {
     stats3[2] = x2;
   }
   c->ring_fill = 0;
+  c->rec_it.clear();
+  c->rec_cvj.clear();
+  c->rec_cvjp.clear();
   HIPCHK(c, launch_update(c, 1, c->d_psi), "mw3_cheby_begin: first step");
   std::swap(c->d_psi, c->d_psim1);
   std::swap(c->d_psi32, c->d_psim132);
@@ -508,6 +511,12 @@ int32_t mw3_cheby_begin(mw3_ctx* ctx, const double* psi0, double* stats3) {
 }
 
 static int run_deferred(Ctx* c);
+
+// Values of one step's Cvjprod record: the product projection's rows when one
+// is attached and the energy sums run (netot > 0), else none.
+static size_t rec_nprod(const Ctx* c) {
+  return (c->prod_on && c->s.netot > 0) ? (size_t)c->pr_nomb * c->pr_nvj : 0;
+}
 
 static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
   int rd = run_deferred(c);
@@ -522,6 +531,22 @@ static int flush_ring(Ctx* c, double* norms_dst, double* auto_dst) {
   HIPCHK(c, hipMemcpyAsync(nb.data(), c->d_ring_norm, nb.size() * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream),
          "norms");
+  if (c->rec_on) {  // the ring's Cvj (and Cvjprod, filled by the accumulation above) rows
+    const size_t ns = (size_t)c->ring_fill, ncv = (size_t)c->t.ncvj;
+    const size_t ncp = rec_nprod(c);
+    const size_t o1 = c->rec_cvj.size(), o2 = c->rec_cvjp.size();
+    c->rec_cvj.resize(o1 + ns * ncv);
+    c->rec_cvjp.resize(o2 + ns * ncp);
+    if (ncv)
+      HIPCHK(c, hipMemcpyAsync(c->rec_cvj.data() + o1, c->d_ring_cvj, ns * ncv * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream),
+             "records");
+    if (ncp)
+      HIPCHK(c, hipMemcpyAsync(c->rec_cvjp.data() + o2, c->d_ring_cvjp, ns * ncp * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream),
+             "records");
+    for (size_t i = 0; i < ns; i++) c->rec_it.push_back(c->ring_k0 + (int32_t)i);
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
   for (int i = 0; i < c->ring_fill; i++) {
     if (norms_dst) norms_dst[i] = nb[2 * i];
@@ -758,6 +783,36 @@ int32_t mw3_flux_energy_test(mw3_ctx* ctx, int32_t on) {
   return MW3_OK;
 }
 
+int32_t mw3_records(mw3_ctx* ctx, int32_t on) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  if (on != 0 && on != 1) return seterr(c, MW3_EINVAL, "mw3_records: on must be 0 or 1");
+  c->rec_on = on;
+  if (!on) {
+    c->rec_it.clear();
+    c->rec_cvj.clear();
+    c->rec_cvjp.clear();
+  }
+  return MW3_OK;
+}
+
+int32_t mw3_records_get(mw3_ctx* ctx, int32_t maxrec, int32_t* it, double* cvj, double* cvjprod,
+                        int32_t* nrec) {
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (!c) return MW3_EINVAL;
+  if (maxrec < 0 || !nrec) return seterr(c, MW3_EINVAL, "mw3_records_get: maxrec < 0 or nrec NULL");
+  const size_t n = std::min((size_t)maxrec, c->rec_it.size());
+  const size_t ncv = (size_t)c->t.ncvj, ncp = rec_nprod(c);
+  if (it) std::copy(c->rec_it.begin(), c->rec_it.begin() + n, it);
+  if (cvj && ncv) std::copy(c->rec_cvj.begin(), c->rec_cvj.begin() + n * ncv, cvj);
+  if (cvjprod && ncp) std::copy(c->rec_cvjp.begin(), c->rec_cvjp.begin() + n * ncp, cvjprod);
+  c->rec_it.erase(c->rec_it.begin(), c->rec_it.begin() + n);
+  c->rec_cvj.erase(c->rec_cvj.begin(), c->rec_cvj.begin() + n * ncv);
+  c->rec_cvjp.erase(c->rec_cvjp.begin(), c->rec_cvjp.begin() + n * ncp);
+  *nrec = (int32_t)n;
+  return MW3_OK;
+}
+
 int32_t mw3_zs2_get(mw3_ctx* ctx, double* zS2) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (!c || !zS2) return MW3_EINVAL;
@@ -885,6 +940,9 @@ int32_t mw3_prod_attach(mw3_ctx* ctx, const mw3_prod_setup* ps) {
   if (rc) return rc;
   drop_graphs(c);  // captured steps hold the kernels of the old set-up
   free_prod(c);
+  c->rec_it.clear();  // held records have the old Cvjprod shape
+  c->rec_cvj.clear();
+  c->rec_cvjp.clear();
   c->pr_n2p0 = ps->n2prod0 - 1;
   c->pr_nap0 = ps->nangproj0 - 1;
   c->pr_n2p = ps->n2prod1 - ps->n2prod0 + 1;

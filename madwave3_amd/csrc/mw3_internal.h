@@ -195,6 +195,11 @@ struct Ctx {
   int ring_fill = 0;             // steps buffered in the ring
   int ring_k0 = 0;               // k of ring slot 0
   std::vector<double> host_norms;
+  // per-step records (mw3_records): every ring flush appends its steps
+  int rec_on = 0;
+  std::vector<int32_t> rec_it;   // [nrec] k of each record
+  std::vector<double> rec_cvj;   // [nrec][ncvj]
+  std::vector<double> rec_cvjp;  // [nrec][nomb * nvj] (product projection attached)
 
   // GEMM passes: R rows, r columns, angular forward, angular back
   GemmPlan p_dstR, p_dstr, p_dstr2, p_dstRi, p_angF, p_angB;

@@ -187,6 +187,24 @@ module mw3_capi
       integer(c_int32_t) :: rc
     end function mw3_flux_energy_test
 
+    function mw3_records(ctx, on) bind(C, name="mw3_records") result(rc)
+      import :: c_ptr, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: on
+      integer(c_int32_t) :: rc
+    end function mw3_records
+
+    function mw3_records_get(ctx, maxrec, it, cvj, cvjprod, nrec) &
+        bind(C, name="mw3_records_get") result(rc)
+      import :: c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: maxrec
+      integer(c_int32_t), intent(out) :: it(*)
+      real(c_double), intent(out) :: cvj(*), cvjprod(*)
+      integer(c_int32_t), intent(out) :: nrec
+      integer(c_int32_t) :: rc
+    end function mw3_records_get
+
     function mw3_zs2_get(ctx, zs2) bind(C, name="mw3_zs2_get") result(rc)
       import :: c_ptr, c_double, c_int32_t
       type(c_ptr), value :: ctx

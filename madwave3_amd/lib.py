@@ -282,6 +282,8 @@ def load_library(path=None):
     lib.mw3_flux_result.argtypes = [ct.c_void_p, _pd, _pd]
     lib.mw3_zs2_get.argtypes = [ct.c_void_p, _pd]
     lib.mw3_flux_energy_test.argtypes = [ct.c_void_p, ct.c_int32]
+    lib.mw3_records.argtypes = [ct.c_void_p, ct.c_int32]
+    lib.mw3_records_get.argtypes = [ct.c_void_p, ct.c_int32, _pi, _pd, _pd, _pi]
     lib.mw3_flux_accum_get.argtypes = [ct.c_void_p, _pd, _pd]
     lib.mw3_flux_accum_put.argtypes = [ct.c_void_p, _pd, _pd, _pd]
     lib.mw3_psie_get.argtypes = [ct.c_void_p, _pd]
@@ -389,6 +391,38 @@ class Propagator:
         """Keep (1, default) or drop (0) the E > V(flux point) test of the
         flux sums (mw3_flux_energy_test)."""
         self._check(self.lib.mw3_flux_energy_test(self.h, int(on)), "mw3_flux_energy_test")
+
+    def records(self, on=True):
+        """Start (on) or stop recording the per-step Cvj / Cvjprod records
+        (mw3_records); stopping drops what is held."""
+        self._check(self.lib.mw3_records(self.h, int(bool(on))), "mw3_records")
+
+    def take_records(self, maxrec=None):
+        """The held records, oldest first, moved out of the context:
+        (it[n], cvj[n, ncan, nj, nv], cvjprod[n, nomb, nj', nv'] or None)."""
+        c = self.case
+        big = 1 << 30 if maxrec is None else int(maxrec)
+        shp = (c.ncan, c.jmax - c.jini + 1, c.nvmax - c.nvini + 1) if c.netot > 0 else (0,)
+        pshape = getattr(self, "_prod_shape", None)
+        its, cvs, cps = [], [], []
+        while big > 0:
+            m = min(big, 4096)
+            it = np.zeros(m, dtype=np.int32)
+            cv = np.zeros((m,) + shp)
+            cp = np.zeros((m,) + pshape) if pshape else None
+            n = ct.c_int32(0)
+            self._check(self.lib.mw3_records_get(self.h, m, it.ctypes.data_as(_pi), _dp(cv),
+                                                 _dp(cp), ct.byref(n)), "mw3_records_get")
+            n = n.value
+            its.append(it[:n])
+            cvs.append(cv[:n])
+            if cp is not None:
+                cps.append(cp[:n])
+            big -= n
+            if n < m:
+                break
+        return (np.concatenate(its), np.concatenate(cvs),
+                np.concatenate(cps) if pshape else None)
 
     def zs2(self):
         c = self.case

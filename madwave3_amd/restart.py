@@ -9,6 +9,14 @@ mod_coortrans01_02.f:180-196):
                    point), zCdR(netot, flux point)
   contNN.S2prod    unformatted (iprod > 1): zS2prod(netot, v', j', Omega')
 
+and the per-step records of every time loop (main_madwave3.f:311-322,
+353-365, 388-394), written by rank 0:
+
+  Cvj.LLLL         unformatted, one record "it, Cvj" per step (iphoto >= 1,
+                   iprod = 1 or iwrt_reac_distri = 2, and xm2 > 1e-3)
+  Cvjprod.LLLL     unformatted, one record "it, Cvjprod" per step (iprod = 2)
+
+with LLLL the loop number iloop.
 NN is the rank; one process holds the whole grid here, so the files are
 those of rank 0 of a one-rank run (the state in the A0 layout of
 nproc = 1, the flux points in its order).  Unformatted sequential files
@@ -17,13 +25,17 @@ before and after each record.  wvpcont writes the second line of cont.data
 from a variable it never sets (abstot); 0 is written here.
 
 write_cont(dir, prop, iloop) and resume(prop, dir) move a Propagator's state
-and accumulators through these files; read_cont returns their contents."""
+and accumulators through these files; read_cont returns their contents.
+time_loop(prop, dir, iloop, ntimes) runs one loop of the reference's main
+program on a Propagator: ntimes steps, the loop's record files and the
+continuation files."""
 import os
 import struct
 
 import numpy as np
 
-__all__ = ["write_records", "read_records", "write_cont", "read_cont", "resume"]
+__all__ = ["write_records", "read_records", "write_cont", "read_cont", "resume",
+           "write_step_records", "read_step_records", "record_kinds", "time_loop"]
 
 _MAXREC = 2 ** 31 - 1
 
@@ -134,3 +146,74 @@ def resume(prop, dirpath, idproc=0):
     if d["zs2prod"] is not None:
         prop.put_zs2prod(d["zs2prod"])
     return d["it0"], d["indt"], d["iloop"]
+
+
+def write_step_records(path, its, values):
+    """A Cvj.LLLL / Cvjprod.LLLL file: per step i one unformatted record
+    holding the 4-byte integer its[i] and then values[i] (float64, its
+    dimensions in C order reversed: Fortran order).  Opened as the reference
+    opens it (status='new'): an existing file is an error."""
+    its = np.asarray(its, dtype=np.int32)
+    values = np.ascontiguousarray(values, dtype=np.float64).reshape(len(its), -1)
+    n = 4 + 8 * values.shape[1]
+    if n > _MAXREC:
+        raise ValueError("record of %d bytes exceeds one Fortran record" % n)
+    mark = struct.pack("<i", n)
+    with open(path, "xb") as f:
+        for i in range(len(its)):
+            f.write(mark)
+            f.write(struct.pack("<i", int(its[i])))
+            f.write(values[i].tobytes())
+            f.write(mark)
+
+
+def read_step_records(path, shape):
+    """(its, values[n, *shape]) of a file written as write_step_records
+    writes it."""
+    m = int(np.prod(shape))
+    its, vals = [], []
+    for rec in read_records(path):
+        if len(rec) != 4 + 8 * m:
+            raise ValueError("%s: record of %d bytes, expected %d" % (path, len(rec), 4 + 8 * m))
+        its.append(struct.unpack_from("<i", rec, 0)[0])
+        vals.append(np.frombuffer(rec, dtype=np.float64, offset=4))
+    v = np.array(vals).reshape((len(its),) + tuple(shape))
+    return np.array(its, dtype=np.int32), v
+
+
+def record_kinds(case):
+    """(Cvj, Cvjprod): whether the main program writes each record file for
+    this case (main_madwave3.f:311-322)."""
+    cvj = ((getattr(case, "iphoto", 0) >= 1 or getattr(case, "iprod", 0) == 1
+            or getattr(case, "iwrt_reac_distri", 0) == 2) and getattr(case, "xm2", 1.0) > 1e-3)
+    return cvj, getattr(case, "iprod", 0) == 2
+
+
+def time_loop(prop, dirpath, iloop, ntimes, indt=0, idproc=0):
+    """One pass of the main program's time loop (main_madwave3.f:308-394)
+    on Propagator prop, begun or resumed: ntimes Chebyshev steps with their
+    per-step records into Cvj.LLLL / Cvjprod.LLLL (as the case asks for
+    them), then the continuation files.  Returns the steps' norms."""
+    case = prop.case
+    want_cvj, want_prod = record_kinds(case)
+    want_cvj = want_cvj and case.netot > 0
+    os.makedirs(dirpath, exist_ok=True)
+    names = [os.path.join(dirpath, "%s.%04d" % (w, iloop))
+             for w, on in (("Cvj", want_cvj), ("Cvjprod", want_prod)) if on]
+    for n in names:
+        if os.path.exists(n):
+            raise FileExistsError(n)
+    prop.records(want_cvj or want_prod)
+    try:
+        norms = prop.steps(ntimes)[0]
+        its, cvj, cvjprod = prop.take_records() if (want_cvj or want_prod) else (None,) * 3
+    finally:
+        prop.records(False)
+    if want_prod and cvjprod is None:
+        raise ValueError("iprod = 2: the context has no product tables attached")
+    if want_cvj:
+        write_step_records(os.path.join(dirpath, "Cvj.%04d" % iloop), its, cvj)
+    if want_prod:
+        write_step_records(os.path.join(dirpath, "Cvjprod.%04d" % iloop), its, cvjprod)
+    write_cont(dirpath, prop, iloop, indt, idproc)
+    return norms
