@@ -213,6 +213,7 @@ struct Ctx {
   int32_t* d_mid_sizes = nullptr;  // its per-size table records (int4)
   int mid_nitems = 0;              // 0: the r passes run as two GEMMs
   int mid_nk1 = 0;                 // 16-row tiles of the low r modes
+  int rad_ncu = 1;                 // compute units (the LDS-table forward R pass)
   int ang_grid = 0;        // its persistent blocks
   int32_t* d_ang_tiles = nullptr;  // tiles of the one-pass kernels with compressed-grid points
   int ang_ntiles = 0;

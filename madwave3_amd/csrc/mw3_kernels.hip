@@ -1436,7 +1436,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
       for (int r = 0; r < 4; r++)
         f[t][r] = (T)((double)f[t][r] * (ca[t & 1][r] + cb[t & 1][r]));
     };
-    T a2[3][4];
+    T a2[3][4], a4[4][4];
     auto rd2 = [&](int q, T (&a)[4]) {
       const int h = q / (NT * UN), t = (q - h * NT * UN) / UN;
       const int u = h * UN + (q - h * NT * UN - t * UN);
@@ -1448,8 +1448,13 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
         for (int r = 0; r < 4; r++) a[r] = drow[M::row(lane, r)];
       }
     };
-    rd2(0, a2[0]);
-    if (NQ > 1) rd2(1, a2[1]);
+    if constexpr (UN % 2 == 0) {
+      rd2(0, a4[0]);
+      rd2(1, a4[1]);
+    } else {
+      rd2(0, a2[0]);
+      if (NQ > 1) rd2(1, a2[1]);
+    }
     scale(0);
     constexpr int UH = UN / 2 > 0 ? UN / 2 - 1 : 0;
     char* ob = (char*)(ang + (int64_t)ch * na * pl);
@@ -1458,22 +1463,50 @@ __global__ __launch_bounds__(64 * ANG_WAVES) void k_ang_fused(
       V o[UN];
 #pragma unroll
       for (int uu = 0; uu < UN; uu++) o[uu] = (V){0, 0, 0, 0};
+      if constexpr (UN % 2 == 0) {
+        // stages (h, t, uu) and (h, t, uu + 1) together, their MFMAs
+        // alternating between the two output tiles: no MFMA waits on the one
+        // just before (stage q read into slot q % 4, two stages ahead)
 #pragma unroll
-      for (int t = 0; t < NT; t++) {
+        for (int t = 0; t < NT; t++) {
 #pragma unroll
-        for (int uu = 0; uu < UN; uu++) {
-          const int q = (h * NT + t) * UN + uu;
-          if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
-          if (h == 0 && uu == 0 && t + 2 < NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
-          if (uu == 0 && t == 0 && h == NH - 1) {
+          for (int uu = 0; uu < UN; uu += 2) {
+            const int q = (h * NT + t) * UN + uu;
+            if (q + 2 < NQ) rd2(q + 2, a4[(q + 2) % 4]);
+            if (q + 3 < NQ) rd2(q + 3, a4[(q + 3) % 4]);
+            if (h == 0 && uu == 0 && t + 2 < NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
+            if (uu == 0 && t == 0 && h == NH - 1) {
 #pragma unroll
-            for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+              for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+              o[uu] = M::mma(a4[q % 4][r], f[t][r], o[uu]);
+              o[uu + 1] = M::mma(a4[(q + 1) % 4][r], f[t][r], o[uu + 1]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (h == 0 && uu <= UH && UH < uu + 2 && t + 1 < NT) scale(t + 1);
           }
-          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else {
 #pragma unroll
-          for (int r = 0; r < 4; r++) o[uu] = M::mma(a2[q % 3][r], f[t][r], o[uu]);
-          __builtin_amdgcn_sched_barrier(0);
-          if (h == 0 && uu == UH && t + 1 < NT) scale(t + 1);
+        for (int t = 0; t < NT; t++) {
+#pragma unroll
+          for (int uu = 0; uu < UN; uu++) {
+            const int q = (h * NT + t) * UN + uu;
+            if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
+            if (h == 0 && uu == 0 && t + 2 < NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
+            if (uu == 0 && t == 0 && h == NH - 1) {
+#pragma unroll
+              for (int c = 0; c < XA && c < NT; c++) ldx(c, tnext);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < 4; r++) o[uu] = M::mma(a2[q % 3][r], f[t][r], o[uu]);
+            __builtin_amdgcn_sched_barrier(0);
+            if (h == 0 && uu == UH && t + 1 < NT) scale(t + 1);
+          }
         }
       }
       const int ll = lane_opaque(lane), pp = lane_opaque(p);
@@ -1648,27 +1681,59 @@ __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_
         for (int r = 0; r < 4; r++) a[r] = drow[M::row(lane, r)];
       }
     };
-    rd2(0, a2[0]);
-    if (NQ > 1) rd2(1, a2[1]);
-    scale(0);
     constexpr int UH = NT / 2 > 0 ? NT / 2 - 1 : 0;
+    if constexpr (NT % 2 == 0) {
+      // stages (t, u) and (t, u + 1) together, their MFMAs alternating
+      // between the two output tiles: no MFMA waits on the one just before
+      T a4[4][4];  // stage q in slot q % 4, read two stages ahead
+      rd2(0, a4[0]);
+      rd2(1, a4[1]);
+      scale(0);
 #pragma unroll
-    for (int t = 0; t < 2 * NT; t++) {
+      for (int t = 0; t < 2 * NT; t++) {
 #pragma unroll
-      for (int u = 0; u < NT; u++) {
-        const int q = t * NT + u;
-        if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
-        if (u == 0 && t + 2 < 2 * NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
-        if (u == 0 && t == NT) {
+        for (int u = 0; u < NT; u += 2) {
+          const int q = t * NT + u;
+          if (q + 2 < NQ) rd2(q + 2, a4[(q + 2) % 4]);
+          if (q + 3 < NQ) rd2(q + 3, a4[(q + 3) % 4]);
+          if (u == 0 && t + 2 < 2 * NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
+          if (u == 0 && t == NT) {
 #pragma unroll
-          for (int c = 0; c < XA && c < NT; c++) ldx(c, cnext);
+            for (int c = 0; c < XA && c < NT; c++) ldx(c, cnext);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          const int ou = (t / NT) * NT + u;
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            o[ou] = M::mma(a4[q % 4][r], f[t][r], o[ou]);
+            o[ou + 1] = M::mma(a4[(q + 1) % 4][r], f[t][r], o[ou + 1]);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          if (u <= UH && UH < u + 2 && t + 1 < 2 * NT) scale(t + 1);
         }
-        __builtin_amdgcn_sched_barrier(0);
-        const int ou = (t / NT) * NT + u;
+      }
+    } else {
+      rd2(0, a2[0]);
+      if (NQ > 1) rd2(1, a2[1]);
+      scale(0);
 #pragma unroll
-        for (int r = 0; r < 4; r++) o[ou] = M::mma(a2[q % 3][r], f[t][r], o[ou]);
-        __builtin_amdgcn_sched_barrier(0);
-        if (u == UH && t + 1 < 2 * NT) scale(t + 1);
+      for (int t = 0; t < 2 * NT; t++) {
+#pragma unroll
+        for (int u = 0; u < NT; u++) {
+          const int q = t * NT + u;
+          if (q + 2 < NQ) rd2(q + 2, a2[(q + 2) % 3]);
+          if (u == 0 && t + 2 < 2 * NT) ldcoef(t + 2, ca[t & 1], cb[t & 1]);
+          if (u == 0 && t == NT) {
+#pragma unroll
+            for (int c = 0; c < XA && c < NT; c++) ldx(c, cnext);
+          }
+          __builtin_amdgcn_sched_barrier(0);
+          const int ou = (t / NT) * NT + u;
+#pragma unroll
+          for (int r = 0; r < 4; r++) o[ou] = M::mma(a2[q % 3][r], f[t][r], o[ou]);
+          __builtin_amdgcn_sched_barrier(0);
+          if (u == UH && t + 1 < 2 * NT) scale(t + 1);
+        }
       }
     }
     const int ll = lane_opaque(lane), pp = lane_opaque(p);
@@ -1912,6 +1977,17 @@ struct RadSplit {
   static constexpr int v = sizeof(T) == 4 && HB % 4 == 0 ? 2 : 1;
 };
 
+// work items per block of k_rad_fwd / k_rad_inv (the grid is raised to keep
+// each block's list within it)
+constexpr int kRadMaxItems = 256;
+
+// items of k_rad_fwd whose loads are in flight while a wave multiplies: two
+// in FP32; one in FP64, whose registers hold no second set
+template <typename T>
+struct RadDepth {
+  static constexpr int v = sizeof(T) == 4 ? 2 : 1;
+};
+
 template <typename T, int NKB, int HB, bool FULL>
 __global__ __launch_bounds__(128 * NKB * (RadSplit<T, HB>::v)) void k_rad_fwd(
     const T* __restrict__ psi, const T* __restrict__ Tt, T* __restrict__ t1,
@@ -1926,8 +2002,8 @@ __global__ __launch_bounds__(128 * NKB * (RadSplit<T, HB>::v)) void k_rad_fwd(
   // vectors at j and at its mirror and folds them as it stores
   constexpr int NP = 16 * H / 4, PER = (NP + NT - 1) / NT;
   __shared__ __align__(16) T fold[2][2][16 * LP];  // [buffer][parity]
-  // partial tiles of the second j half, by buffer (KS = 2)
-  __shared__ V red[KS == 2 ? 2 : 1][KS == 2 ? 2 * NKB : 1][64];
+  // partial tiles of both j halves, by buffer (KS = 2)
+  __shared__ V red[KS == 2 ? 2 : 1][KS == 2 ? 2 : 1][KS == 2 ? 2 * NKB : 1][64];
   const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, wave = threadIdx.x >> 6;
   const int kh = wave / (2 * NKB), w2 = wave - kh * 2 * NKB;  // j half, tile index
   const int p = w2 / NKB, t = w2 - p * NKB;  // parity and k' tile of this wave
@@ -2003,72 +2079,333 @@ __global__ __launch_bounds__(128 * NKB * (RadSplit<T, HB>::v)) void k_rad_fwd(
   // then hold finite values that no later pass uses (the r-pass tables are
   // zero on those rows, the r-pass factor (radcut - E)_+ is zero on those
   // columns and the inverse pass reads them as zero).  Otherwise rows past
-  // n1 are not stored.  With KS = 2 the first-half waves add the second
-  // half's tile (left in LDS by the previous item) and store.
-  auto store = [&](V acc, const int4& it, int buf) {
-    if (KS == 2) {
-      if (kh) return;  // wave-uniform
-#pragma unroll
-      for (int rr = 0; rr < 4; rr++) acc[rr] += red[buf][w2][lane][rr];
-    }
+  // n1 are not stored.  With KS = 2 the two halves' tiles meet in LDS (left
+  // there by the previous item) and each half stores two of a lane's four
+  // rows: every wave runs the same stores, so no branch leaves the compiler
+  // unsure how many stores are in flight when it waits for the next loads.
+  auto store = [&](const V& acc, const int4& it, int buf) {
     T* orow = t1 + it.x * plane;
     const int k2 = 2 * (16 * t + i) + p;
+    if constexpr (KS == 2) {
+      const V o = red[buf][kh ^ 1][w2][lane];
+      const T v0 = kh ? acc[2] + o[2] : acc[0] + o[0];
+      const T v1 = kh ? acc[3] + o[3] : acc[1] + o[1];
+      const int r0 = 16 * it.y + M::row(lane, 2 * kh), r1 = 16 * it.y + M::row(lane, 2 * kh + 1);
+      if (FULL || r0 < n1) orow[(int64_t)r0 * N2 + k2] = v0;
+      if (FULL || r1 < n1) orow[(int64_t)r1 * N2 + k2] = v1;
+    } else {
 #pragma unroll
-    for (int rr = 0; rr < 4; rr++) {
-      const int row = 16 * it.y + M::row(lane, rr);
-      if (FULL || row < n1) orow[(int64_t)row * N2 + k2] = acc[rr];
+      for (int rr = 0; rr < 4; rr++) {
+        const int row = 16 * it.y + M::row(lane, rr);
+        if (FULL || row < n1) orow[(int64_t)row * N2 + k2] = acc[rr];
+      }
     }
   };
-  auto keep = [&](const V& acc, int buf) {  // the second half's partial tile
-    if (KS == 2 && kh) red[buf][w2][lane] = acc;
+  auto keep = [&](const V& acc, int buf) {  // this half's partial tile
+    if constexpr (KS == 2) red[buf][kh][w2][lane] = acc;
   };
   const int G = (int)gridDim.x;
-  int n = blockIdx.x;
-  if (n >= nitems) return;  // block-uniform
-  auto at = [&](int m) { return items[m < nitems ? m : nitems - 1]; };
-  // Two LDS buffers, one barrier per item: while a wave multiplies item n
-  // out of one buffer it has stored item n + G (loaded into registers during
-  // item n - G) into the other and issued the loads of item n + 2 G.  The
-  // barrier at the end of an item frees its buffer and completes the other
-  // (and the partial tiles of that item).  Each item's results are stored at
-  // the start of the next one, before the tile is staged: the wait for the
-  // staged loads then leaves those stores in flight.  A prefetch past the
-  // last item re-reads the last item, unused; the scheduling barriers keep
-  // this order of issue.
-  T pf[PER][2][4];
-  int4 cur = at(n);
-  fetch(cur, pf);
-  put(pf, 0);
-  int4 nxt = at(n + G);
-  fetch(nxt, pf);
+  if ((int)blockIdx.x >= nitems) return;  // block-uniform
+  // this block's items blockIdx.x + k G, k < K, copied to LDS once: read
+  // from there they cost an in-order LDS read, where a scalar load would make
+  // every later LDS wait a full one
+  __shared__ int4 its[kRadMaxItems];
+  const int K = (nitems - 1 - (int)blockIdx.x) / G + 1;  // <= kRadMaxItems (host)
+  for (int k = threadIdx.x; k < K; k += NT) its[k] = items[blockIdx.x + k * G];
   __syncthreads();
-  int4 nn = at(n + 2 * G);
-  put(pf, 1);
-  fetch(nn, pf);
-  __builtin_amdgcn_sched_barrier(0);
-  V acc = mult(0);
-  keep(acc, 0);
-  __syncthreads();
-  int4 prev = cur;
-  cur = nxt;
-  nxt = nn;
-  n += G;
-  int buf = 1;
-  for (; n < nitems; n += G, buf ^= 1) {
-    store(acc, prev, buf ^ 1);
-    __builtin_amdgcn_sched_barrier(0);
-    put(pf, buf ^ 1);
-    nn = at(n + 2 * G);
-    fetch(nn, pf);
-    __builtin_amdgcn_sched_barrier(0);
-    acc = mult(buf);
-    keep(acc, buf);
+  auto at = [&](int k) { return its[k < K ? k : K - 1]; };
+  // Two LDS buffers, one barrier per item: while a wave multiplies item k
+  // out of one buffer it has stored item k + 1 (loaded into registers during
+  // an earlier item) into the other and issued the loads of a later item.
+  // The barrier at the end of an item frees its buffer and completes the
+  // other (and the partial tiles of that item).  Each item's results are
+  // stored at the start of the next one, before the tile is staged: the wait
+  // for the staged loads then leaves those stores in flight.  A prefetch past
+  // the last item re-reads the last item, unused; the scheduling barriers
+  // keep this order of issue.
+  if constexpr (RadDepth<T>::v == 2) {
+    // two register sets: the loads of item k are issued two items before its
+    // tile is staged, so two items' loads are in flight while a wave
+    // multiplies (the loop is unrolled by two so that each set has fixed
+    // registers)
+    T pa[PER][2][4], pb[PER][2][4];
+    fetch(at(0), pa);
+    put(pa, 0);
+    fetch(at(1), pa);
+    fetch(at(2), pb);
     __syncthreads();
-    prev = cur;
-    cur = nxt;
-    nxt = nn;
+    put(pa, 1);
+    fetch(at(3), pa);
+    __builtin_amdgcn_sched_barrier(0);
+    V acc = mult(0);
+    keep(acc, 0);
+    __syncthreads();
+    // item k is staged in buffer b; p holds item k + 1
+    auto body = [&](T(&p)[PER][2][4], int k, int b) {
+      store(acc, at(k - 1), b ^ 1);
+      __builtin_amdgcn_sched_barrier(0);
+      put(p, b ^ 1);
+      fetch(at(k + 3), p);
+      __builtin_amdgcn_sched_barrier(0);
+      acc = mult(b);
+      keep(acc, b);
+      __syncthreads();
+    };
+    int k = 1, last = 0;  // last: buffer of the last item multiplied
+    while (k < K) {
+      body(pb, k, 1);
+      last = 1;
+      if (++k >= K) break;
+      body(pa, k, 0);
+      last = 0;
+      ++k;
+    }
+    store(acc, at(k - 1), last);
+  } else {
+    T pf[PER][2][4];
+    fetch(at(0), pf);
+    put(pf, 0);
+    fetch(at(1), pf);
+    __syncthreads();
+    put(pf, 1);
+    fetch(at(2), pf);
+    __builtin_amdgcn_sched_barrier(0);
+    V acc = mult(0);
+    keep(acc, 0);
+    __syncthreads();
+    int k = 1, buf = 1;
+    for (; k < K; k++, buf ^= 1) {
+      store(acc, at(k - 1), buf ^ 1);
+      __builtin_amdgcn_sched_barrier(0);
+      put(pf, buf ^ 1);
+      fetch(at(k + 2), pf);
+      __builtin_amdgcn_sched_barrier(0);
+      acc = mult(buf);
+      keep(acc, buf);
+      __syncthreads();
+    }
+    store(acc, at(k - 1), buf ^ 1);
   }
-  store(acc, prev, buf ^ 1);
+}
+
+// ---------------------------------------------------------------------------
+// The forward R pass in FP32 with the sine table in LDS: the same sums as
+// k_rad_fwd (t1[r][k2] = sum_{j<h} f_p(r, j) S(j, k2) with the fold f_0 =
+// psi[j] + psi[n2-1-j] for even k2, f_1 = psi[j] - psi[n2-1-j] for odd k2),
+// but each wave owns whole work items: it streams its 16-row tile from HBM,
+// folds it in registers and multiplies it by every low mode, reading the
+// table from LDS.  The block copies the table into LDS once (even modes,
+// then odd ones, rows padded by 4 floats) and its waves then run with no
+// barrier, so one wave's loads overlap another's matrix work.
+// Lane (g, i) holds row i of the tile and, at step s of j block kb,
+// j = 16 kb + 4 g + s (one 16-byte load at j and one at its mirror); the
+// table read for mode tile t of parity p is the 16-byte row segment
+// S(16 kb + 4 g .. + 3, 2 (16 t + i) + p).  Loads run RD j blocks ahead.
+// Items are spread over the waves round robin by block first, so that every
+// SIMD gets about the same number.  Stores as k_rad_fwd: rows past nn1 of
+// the last row tile hold finite values no later pass reads (FULL), and the
+// two parities of a mode pair are one 8-byte store.
+// ---------------------------------------------------------------------------
+constexpr int kFwdLdsWaves = 8;
+
+template <int NKB, int HB, bool FULL>
+__global__ __launch_bounds__(64 * kFwdLdsWaves) void k_rad_fwd_lds(
+    const float* __restrict__ psi, const float* __restrict__ Tt, float* __restrict__ t1,
+    const int4* __restrict__ items, int nitems, int n1) {
+  typedef MfmaAcc<float> M;
+  typedef M::V V;
+  constexpr int N2 = 32 * HB, H = N2 / 2, NB = HB;  // j blocks of 16
+  constexpr int LP = H + 4, NM = 16 * NKB;           // modes per parity
+  constexpr int RD = 4;                              // j blocks loaded ahead
+  __shared__ __align__(16) float tab[2][NM * LP];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  {
+    // every load of the copy in flight at once (a thread past the end
+    // repeats the last vector)
+    constexpr int NV = 2 * NM * (H / 4), PT = (NV + 64 * kFwdLdsWaves - 1) / (64 * kFwdLdsWaves);
+    float x[PT][4];
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kFwdLdsWaves * l;
+      v = v < NV ? v : NV - 1;
+      ld4(Tt + (int64_t)(v / (H / 4)) * H + 4 * (v % (H / 4)), x[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kFwdLdsWaves * l;
+      v = v < NV ? v : NV - 1;
+      const int row = v / (H / 4), c4 = v % (H / 4);  // row = k2
+      st4(&tab[row & 1][(row >> 1) * LP + 4 * c4], x[l]);
+    }
+  }
+  __syncthreads();
+  const int64_t plane = (int64_t)n1 * N2;
+  const int nb = (int)gridDim.x;
+  for (int k = (int)blockIdx.x + nb * w; k < nitems; k += nb * kFwdLdsWaves) {
+    const int4 it = items[k];
+    int r = 16 * it.y + i;
+    r = r < it.z ? r : it.z - 1;
+    const float* src = psi + it.x * plane + (int64_t)r * N2 + 4 * g;
+    const float* mir = psi + it.x * plane + (int64_t)r * N2 + (N2 - 4 - 4 * g);
+    float a[RD][4], m[RD][4];
+#pragma unroll
+    for (int d = 0; d < RD; d++) {
+      ld4(src + 16 * d, a[d]);
+      ld4(mir - 16 * d, m[d]);
+    }
+    V acc[2][NKB];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int t = 0; t < NKB; t++) acc[p][t] = (V){0, 0, 0, 0};
+    float b[2][2][NKB][4];  // [j block parity][mode parity][tile]
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int t = 0; t < NKB; t++) ld4(&tab[p][(16 * t + i) * LP + 4 * g], b[0][p][t]);
+#pragma unroll
+    for (int kb = 0; kb < NB; kb++) {
+      const int sl = kb % RD, cb = kb & 1;
+      float e[4], o[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        e[q] = a[sl][q] + m[sl][3 - q];
+        o[q] = a[sl][q] - m[sl][3 - q];
+      }
+      if (kb + RD < NB) {
+        ld4(src + 16 * (kb + RD), a[sl]);
+        ld4(mir - 16 * (kb + RD), m[sl]);
+      }
+      if (kb + 1 < NB) {
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+          for (int t = 0; t < NKB; t++)
+            ld4(&tab[p][(16 * t + i) * LP + 16 * (kb + 1) + 4 * g], b[cb ^ 1][p][t]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      // one step of every accumulator between two scheduling barriers: no
+      // MFMA waits on the one issued just before it
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+#pragma unroll
+        for (int t = 0; t < NKB; t++) {
+          acc[0][t] = M::mma(e[s], b[cb][0][t][s], acc[0][t]);
+          acc[1][t] = M::mma(o[s], b[cb][1][t][s], acc[1][t]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    float* orow = t1 + it.x * plane;
+#pragma unroll
+    for (int t = 0; t < NKB; t++)
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int row = 16 * it.y + M::row(lane, rr);
+        typedef float F2 __attribute__((ext_vector_type(2)));
+        if (FULL || row < n1)
+          *(F2*)&orow[(int64_t)row * N2 + 2 * (16 * t + i)] = (F2){acc[0][t][rr], acc[1][t][rr]};
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The inverse R pass in FP32 with the sine table in LDS: the sums of
+// k_rad_inv (corr[r][R] = E + O, corr[r][n2-1-R] = E - O for R < h, E / O
+// over the even / odd modes k2 < k2lo) with each wave owning whole work
+// items: it loads its 16 rows of t1 once (columns k2 >= k2lo read as zero),
+// then walks the 16-wide R tiles of the first half, reading the table
+// S(R, k2) from LDS one tile ahead, and skips the tile pairs whose rows hold
+// no compressed-grid point in those columns or their mirrors (the update
+// reads none of that output).  The block copies the table (rows R < h,
+// padded by 4 floats) into LDS once; the waves then run with no barrier.
+// Lane (g, i) holds row i and modes 32 kb + 8 g + e (e < 8: steps s take
+// e = 2 s + parity), and reads the table row R = 16 tq + i at the same modes.
+// ---------------------------------------------------------------------------
+constexpr int kInvLdsWaves = 8;
+
+template <int NKB, int HB, bool FULL>
+__global__ __launch_bounds__(64 * kInvLdsWaves) void k_rad_inv_lds(
+    const float* __restrict__ t1, const float* __restrict__ Ti, float* __restrict__ corr,
+    const int4* __restrict__ items, int nitems, int n1) {
+  typedef MfmaAcc<float> M;
+  typedef M::V V;
+  constexpr int N2 = 32 * HB, H = N2 / 2, NTQ = H / 16;  // R tiles of the first half
+  constexpr int K2P = 32 * NKB, LP = K2P + 4;
+  __shared__ __align__(16) float tab[H * LP];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  {
+    constexpr int NV = H * (K2P / 4), PT = (NV + 64 * kInvLdsWaves - 1) / (64 * kInvLdsWaves);
+    float x[PT][4];
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kInvLdsWaves * l;
+      v = v < NV ? v : NV - 1;
+      ld4(Ti + (int64_t)(v / (K2P / 4)) * K2P + 4 * (v % (K2P / 4)), x[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kInvLdsWaves * l;
+      v = v < NV ? v : NV - 1;
+      st4(&tab[(v / (K2P / 4)) * LP + 4 * (v % (K2P / 4))], x[l]);
+    }
+  }
+  __syncthreads();
+  const int64_t plane = (int64_t)n1 * N2;
+  const int nb = (int)gridDim.x;
+  for (int k = (int)blockIdx.x + nb * w; k < nitems; k += nb * kInvLdsWaves) {
+    const int4 it = items[k];
+    int r = 16 * it.y + i;
+    r = r < it.z ? r : it.z - 1;
+    const float* src = t1 + it.x * plane + (int64_t)r * N2 + 8 * g;
+    float a[NKB][2][4];
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+      ld4(src + 32 * kb, a[kb][0]);
+      ld4(src + 32 * kb + 4, a[kb][1]);
+    }
+    float b[2][NKB][2][4];
+    auto rdb = [&](int tq, float (&bb)[NKB][2][4]) {
+      const float* trow = &tab[(16 * tq + i) * LP + 8 * g];
+#pragma unroll
+      for (int kb = 0; kb < NKB; kb++) {
+        ld4(trow + 32 * kb, bb[kb][0]);
+        ld4(trow + 32 * kb + 4, bb[kb][1]);
+      }
+    };
+    rdb(0, b[0]);
+    const int kl = it.w & 0xffff;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (32 * kb + 8 * g + e >= kl) a[kb][e >> 2][e & 3] = 0.0f;
+    float* ob = corr + it.x * plane;
+#pragma unroll
+    for (int tq = 0; tq < NTQ; tq++) {
+      if (tq + 1 < NTQ) rdb(tq + 1, b[(tq + 1) & 1]);
+      if ((it.w >> (16 + (tq >> 1))) & 1) {  // wave-uniform
+        V E = (V){0, 0, 0, 0}, O = (V){0, 0, 0, 0};
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            const int e0 = 2 * s, e1 = 2 * s + 1;
+            E = M::mma(a[kb][e0 >> 2][e0 & 3], b[tq & 1][kb][e0 >> 2][e0 & 3], E);
+            O = M::mma(a[kb][e1 >> 2][e1 & 3], b[tq & 1][kb][e1 >> 2][e1 & 3], O);
+          }
+        const int R = 16 * tq + i;
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+          const int row = 16 * it.y + M::row(lane, rr);
+          if (FULL || row < it.z) {
+            ob[(int64_t)row * N2 + R] = E[rr] + O[rr];
+            ob[(int64_t)row * N2 + (N2 - 1 - R)] = E[rr] - O[rr];
+          }
+        }
+      }
+    }
+  }
 }
 
 template <typename T, int NKB, int HB, bool FULL>
@@ -2164,26 +2501,30 @@ __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
     }
   };
   const int G = (int)gridDim.x;
-  int n = blockIdx.x;
-  if (n >= nitems) return;  // block-uniform
-  auto at = [&](int m) { return items[m < nitems ? m : nitems - 1]; };
-  // Two LDS buffers, one barrier per item, as in k_rad_fwd: item n is
-  // multiplied out of one buffer after item n + G has been staged into the
-  // other and the loads of item n + 2 G issued; its results are stored at
-  // the start of the next item, after the staging.  A prefetch past the
-  // last item re-reads the last item, unused.
-  int4 cur = at(n);
+  if ((int)blockIdx.x >= nitems) return;  // block-uniform
+  // this block's items in LDS, as in k_rad_fwd
+  __shared__ int4 its[kRadMaxItems];
+  const int K = (nitems - 1 - (int)blockIdx.x) / G + 1;  // <= kRadMaxItems (host)
+  for (int k = threadIdx.x; k < K; k += NT) its[k] = items[blockIdx.x + k * G];
+  __syncthreads();
+  auto at = [&](int k) { return its[k < K ? k : K - 1]; };
+  // Two LDS buffers, one barrier per item, as in k_rad_fwd: item k is
+  // multiplied out of one buffer after item k + 1 has been staged into the
+  // other and the loads of item k + 2 issued; its results are stored at the
+  // start of the next item, after the staging.  A prefetch past the last
+  // item re-reads the last item, unused.
+  int4 cur = at(0);
   fetch(cur);
   put(cur, 0);
-  int4 nxt = at(n + G);
+  int4 nxt = at(1);
   fetch(nxt);
   __syncthreads();
   int4 prev = cur;
   bool have = false;  // results of prev in E, O
-  for (int buf = 0; n < nitems; n += G, buf ^= 1) {
+  for (int k = 0, buf = 0; k < K; k++, buf ^= 1) {
     put(nxt, buf ^ 1);
     if (have && active(prev)) store(prev);
-    const int4 nn = at(n + 2 * G);
+    const int4 nn = at(k + 2);
     fetch(nn);
     __builtin_amdgcn_sched_barrier(0);  // issue the loads before the MFMAs
     if (active(cur)) mult(buf);
@@ -2230,7 +2571,10 @@ __global__ __launch_bounds__(256) void k_rad_mid(T* __restrict__ t1, const T* __
   constexpr int KTP = 16 * NK1;  // row stride of LT
   __shared__ V red[4][NK1][64];
   const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
-  const int4 it = items[blockIdx.x];
+  // XCD-aware: XCD x (blocks x, x + 8, ..) runs items [x q8, (x + 1) q8),
+  // so the items of one table, which are consecutive, share that XCD's L2
+  const int hw = blockIdx.x, q8 = (int)(gridDim.x >> 3);
+  const int4 it = items[hw < 8 * q8 ? (hw & 7) * q8 + (hw >> 3) : hw];
   const int cg = it.x, cc = it.y & 0xffff, k2l = it.y >> 16, nn = it.z & 0xffff,
             k1l = it.z >> 16;
   const int4 sz = sizes[it.w];
@@ -2389,17 +2733,19 @@ static hipError_t build_rad_mid(Ctx* c) {
       }
   }
   if (lt.size() >= (size_t)1 << 31 || t.dst_tables.size() >= (size_t)1 << 31) return hipSuccess;
+  // items in table order (size, angle, channel, column block): the kernel
+  // runs consecutive items on one XCD
   std::vector<int32_t> items;
-  for (int ch = 0; ch < t.nc; ch++)
+  for (size_t q = 0; q < t.dst_sizes.size(); q++)
     for (int ia = 0; ia < t.na; ia++) {
-      int q = 0;
-      while (t.dst_sizes[q] != t.nn1[ia]) q++;
-      for (int cc = 0; 16 * cc < t.k2lo[ia]; cc++) {
-        items.push_back(ch * t.na + ia);
-        items.push_back(cc | (t.k2lo[ia] << 16));
-        items.push_back(t.nn1[ia] | (t.k1lo[ia] << 16));
-        items.push_back(qrec[q]);
-      }
+      if (t.dst_sizes[q] != t.nn1[ia]) continue;
+      for (int ch = 0; ch < t.nc; ch++)
+        for (int cc = 0; 16 * cc < t.k2lo[ia]; cc++) {
+          items.push_back(ch * t.na + ia);
+          items.push_back(cc | (t.k2lo[ia] << 16));
+          items.push_back(t.nn1[ia] | (t.k1lo[ia] << 16));
+          items.push_back(qrec[q]);
+        }
     }
   if (items.empty()) return hipSuccess;
   hipError_t e;
@@ -2441,7 +2787,11 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   const T* tab = (const T*)c->d_dst_tables;
   const int4* items = (const int4*)c->d_rad_items;
   if (mode == 2) {
-    hipError_t e = t.n1 % 16 == 0
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->rad_ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    e = t.n1 % 16 == 0
                        ? persistent_grid(k_rad_fwd<T, NKB, HB, true>, 128 * NKB * RadSplit<T, HB>::v,
                                          &c->rad_grid[0])
                        : persistent_grid(k_rad_fwd<T, NKB, HB, false>, 128 * NKB * RadSplit<T, HB>::v,
@@ -2455,8 +2805,20 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   // loads its waves' table slices once, so fewer blocks read fewer table
   // bytes and walk more items each)
   const int gdiv = c->opt_rad_grid > 0 ? c->opt_rad_grid : 1;
-  const int g0 = std::max(c->rad_grid[0] / gdiv, 1), g1 = std::max(c->rad_grid[1] / gdiv, 1);
+  const int gmin = (n + kRadMaxItems - 1) / kRadMaxItems;  // each block's list fits its LDS copy
+  const int g0 = std::max(std::max(c->rad_grid[0] / gdiv, 1), gmin);
+  const int g1 = std::max(std::max(c->rad_grid[1] / gdiv, 1), gmin);
   if (mode == 0) {
+    if constexpr (sizeof(T) == 4) {  // the table in LDS, one block per CU
+      const int nb = std::max(1, std::min(c->rad_ncu, n));
+      if (t.n1 % 16 == 0)
+        hipLaunchKernelGGL((k_rad_fwd_lds<NKB, HB, true>), dim3(nb), dim3(64 * kFwdLdsWaves), 0,
+                           c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+      else
+        hipLaunchKernelGGL((k_rad_fwd_lds<NKB, HB, false>), dim3(nb), dim3(64 * kFwdLdsWaves), 0,
+                           c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+      return hipGetLastError();
+    }
     const int nt = 128 * NKB * RadSplit<T, HB>::v;
     if (t.n1 % 16 == 0)
       hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, true>), dim3(n < g0 ? n : g0), dim3(nt), 0,
@@ -2465,6 +2827,16 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
       hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, false>), dim3(n < g0 ? n : g0), dim3(nt), 0,
                          c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
   } else {
+    if constexpr (sizeof(T) == 4) {  // the table in LDS, one block per CU
+      const int nb = std::max(1, std::min(c->rad_ncu, n));
+      if (t.n1 % 16 == 0)
+        hipLaunchKernelGGL((k_rad_inv_lds<NKB, HB, true>), dim3(nb), dim3(64 * kInvLdsWaves), 0,
+                           c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+      else
+        hipLaunchKernelGGL((k_rad_inv_lds<NKB, HB, false>), dim3(nb), dim3(64 * kInvLdsWaves), 0,
+                           c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+      return hipGetLastError();
+    }
     if (t.n1 % 16 == 0)
       hipLaunchKernelGGL((k_rad_inv<T, NKB, HB, true>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
                          c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
