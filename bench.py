@@ -92,6 +92,25 @@ def per_kernel(times, steps):
             for k, (ms, cnt) in times.items()}
 
 
+def angular_flops(case, variant, N):
+    """Algorithmic flops of the angular term per step: 4 n_gamma N for the
+    unfolded contraction (two gamma sums of 2 n_gamma flops per grid value,
+    n_j = n_gamma).  The parity-folded kernel (variant 0) sums over half the
+    angles in each class, 2 n_gamma flops per value, and computes only the
+    16-column groups of the (r, R) plane that hold a compressed-grid point
+    at some angle; it is credited with those columns only."""
+    if variant != 0:
+        return 4.0 * case.nangu * N
+    n1, n2 = case.npun1, case.npun2
+    live = np.zeros(n1 * n2, dtype=bool)
+    for ia in range(case.nangu):
+        n = int(case.npunreal[ia])
+        live[(case.indr1[ia, :n].astype(np.int64) - 1) * n2 + case.indr2[ia, :n] - 1] = True
+    ngroups = int(np.count_nonzero(np.pad(live, (0, (-live.size) % 16)).reshape(-1, 16).any(1)))
+    cols = min(16 * ngroups, n1 * n2)
+    return 2.0 * case.nangu * case.nangu * cols * case.ncan
+
+
 def run_gpu(case, precision, steps, warmup, timing, tune=0, options=None):
     from madwave3_amd.lib import OPTION_NAMES, Propagator
 
@@ -419,8 +438,10 @@ def main():
         work = {
             # (a) radial kinetic: 2 s N bytes per step
             "radial_dst": ("hbm", 2.0 * s_op * N, "GB/s"),
-            # (b) angular: 4 n_gamma N flops per step
-            "angular": ("mfma", 4.0 * case.nangu * N, "TFLOP/s"),
+            # (b) angular: 4 n_gamma N flops per step; the parity-folded
+            # kernel does half of them, on the columns it computes
+            "angular": ("mfma", angular_flops(case, tunedmx["options"].get("angular", 1), N),
+                        "TFLOP/s"),
             # (c) fused V + absorption + update: 8 + 8 + s + (V) + 8 bytes per
             # state-point on the N_c compressed points, V being the unique
             # entries of the symmetric n_e x n_e matrix per spatial point
