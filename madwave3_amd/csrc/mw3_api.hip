@@ -133,7 +133,8 @@ static void free_all(Ctx* c) {
                   c->d_uw_item, c->d_uw_gfirst, c->d_uw_part, c->d_tprod, c->d_ppart,
                   c->d_dtrans, c->d_ring_prod, c->d_prodwf, c->d_prod_part,
                   c->d_ring_cvjp, c->d_cvjp_last, c->d_zs2prod, c->d_rad_items, c->d_fold_D, c->d_fold_a, c->d_fold_b,
-                  c->d_lowT, c->d_mid_items, c->d_mid_sizes, c->d_ang_tiles,
+                  c->d_lowT, c->d_mid_items, c->d_mid_sizes, c->d_mid_items2, c->d_mid_wts,
+                  c->d_ang_tiles,
                   c->d_ang_groups};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

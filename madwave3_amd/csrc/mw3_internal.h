@@ -211,6 +211,9 @@ struct Ctx {
   void* d_lowT = nullptr;          // transposed low-mode r tables of k_rad_mid
   int32_t* d_mid_items = nullptr;  // its work items (int4)
   int32_t* d_mid_sizes = nullptr;  // its per-size table records (int4)
+  int32_t* d_mid_items2 = nullptr; // packed items of the FP32 kernel (2 int4 each)
+  float* d_mid_wts = nullptr;      // its weights (radcut - E)_+ divi [angle][k2][k1]
+  int mid_rbw = 0;                 // r blocks per wave of the FP32 kernel
   int mid_nitems = 0;              // 0: the r passes run as two GEMMs
   int mid_nk1 = 0;                 // 16-row tiles of the low r modes
   int rad_ncu = 1;                 // compute units (the LDS-table forward R pass)

@@ -2200,7 +2200,9 @@ __global__ __launch_bounds__(128 * NKB * (RadSplit<T, HB>::v)) void k_rad_fwd(
 // Lane (g, i) holds row i of the tile and, at step s of j block kb,
 // j = 16 kb + 4 g + s (one 16-byte load at j and one at its mirror); the
 // table read for mode tile t of parity p is the 16-byte row segment
-// S(16 kb + 4 g .. + 3, 2 (16 t + i) + p).  Loads run RD j blocks ahead.
+// S(16 kb + 4 g .. + 3, 2 (16 t + i) + p).  Loads run RD j blocks ahead;
+// the first item's go out before the table copy, and each next item's before
+// the stores of the one before.
 // Items are spread over the waves round robin by block first, so that every
 // SIMD gets about the same number.  Stores as k_rad_fwd: rows past nn1 of
 // the last row tile hold finite values no later pass reads (FULL), and the
@@ -2219,6 +2221,25 @@ __global__ __launch_bounds__(64 * kFwdLdsWaves) void k_rad_fwd_lds(
   constexpr int RD = 4;                              // j blocks loaded ahead
   __shared__ __align__(16) float tab[2][NM * LP];
   const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  if (nitems <= 0) return;  // grid-uniform
+  const int64_t plane = (int64_t)n1 * N2;
+  const int nb = (int)gridDim.x, stride = nb * kFwdLdsWaves;
+  int k = (int)blockIdx.x + nb * w;
+  int4 it = items[k < nitems ? k : nitems - 1];
+  const float *src, *mir;
+  float a[RD][4], m[RD][4];
+  auto start = [&]() {  // the item's first RD j blocks
+    int r = 16 * it.y + i;
+    r = r < it.z ? r : it.z - 1;
+    src = psi + it.x * plane + (int64_t)r * N2 + 4 * g;
+    mir = psi + it.x * plane + (int64_t)r * N2 + (N2 - 4 - 4 * g);
+#pragma unroll
+    for (int d = 0; d < RD; d++) {
+      ld4(src + 16 * d, a[d]);
+      ld4(mir - 16 * d, m[d]);
+    }
+  };
+  start();  // in flight during the table copy
   {
     // every load of the copy in flight at once (a thread past the end
     // repeats the last vector)
@@ -2239,20 +2260,7 @@ __global__ __launch_bounds__(64 * kFwdLdsWaves) void k_rad_fwd_lds(
     }
   }
   __syncthreads();
-  const int64_t plane = (int64_t)n1 * N2;
-  const int nb = (int)gridDim.x;
-  for (int k = (int)blockIdx.x + nb * w; k < nitems; k += nb * kFwdLdsWaves) {
-    const int4 it = items[k];
-    int r = 16 * it.y + i;
-    r = r < it.z ? r : it.z - 1;
-    const float* src = psi + it.x * plane + (int64_t)r * N2 + 4 * g;
-    const float* mir = psi + it.x * plane + (int64_t)r * N2 + (N2 - 4 - 4 * g);
-    float a[RD][4], m[RD][4];
-#pragma unroll
-    for (int d = 0; d < RD; d++) {
-      ld4(src + 16 * d, a[d]);
-      ld4(mir - 16 * d, m[d]);
-    }
+  while (k < nitems) {  // wave-uniform
     V acc[2][NKB];
 #pragma unroll
     for (int p = 0; p < 2; p++)
@@ -2297,11 +2305,17 @@ __global__ __launch_bounds__(64 * kFwdLdsWaves) void k_rad_fwd_lds(
       }
     }
     float* orow = t1 + it.x * plane;
+    const int y0 = 16 * it.y;
+    k += stride;
+    if (k < nitems) {  // the next item's first loads go out before the stores
+      it = items[k];
+      start();
+    }
 #pragma unroll
     for (int t = 0; t < NKB; t++)
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const int row = 16 * it.y + M::row(lane, rr);
+        const int row = y0 + M::row(lane, rr);
         typedef float F2 __attribute__((ext_vector_type(2)));
         if (FULL || row < n1)
           *(F2*)&orow[(int64_t)row * N2 + 2 * (16 * t + i)] = (F2){acc[0][t][rr], acc[1][t][rr]};
@@ -2320,7 +2334,10 @@ __global__ __launch_bounds__(64 * kFwdLdsWaves) void k_rad_fwd_lds(
 // reads none of that output).  The block copies the table (rows R < h,
 // padded by 4 floats) into LDS once; the waves then run with no barrier.
 // Lane (g, i) holds row i and modes 32 kb + 8 g + e (e < 8: steps s take
-// e = 2 s + parity), and reads the table row R = 16 tq + i at the same modes.
+// e = 2 s + parity), and reads the table row R = 16 tq + i at the same modes;
+// with the table as the MFMA's A operand, the lane's results are
+// R = 16 tq + 4 g .. + 3 of row i: two 16-byte stores per tile.  The first
+// item's loads are issued before the table copy, so the two overlap.
 // ---------------------------------------------------------------------------
 constexpr int kInvLdsWaves = 8;
 
@@ -2334,6 +2351,24 @@ __global__ __launch_bounds__(64 * kInvLdsWaves) void k_rad_inv_lds(
   constexpr int K2P = 32 * NKB, LP = K2P + 4;
   __shared__ __align__(16) float tab[H * LP];
   const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  if (nitems <= 0) return;  // grid-uniform
+  const int64_t plane = (int64_t)n1 * N2;
+  const int nb = (int)gridDim.x, stride = nb * kInvLdsWaves;
+  int k = (int)blockIdx.x + nb * w;
+  // the item's 16 rows of t1: lane (g, i) row i, modes 32 kb + 8 g .. + 7
+  int4 it = items[k < nitems ? k : nitems - 1];
+  float a[NKB][2][4];
+  auto load_a = [&]() {
+    int r = 16 * it.y + i;
+    r = r < it.z ? r : it.z - 1;
+    const float* src = t1 + it.x * plane + (int64_t)r * N2 + 8 * g;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+      ld4(src + 32 * kb, a[kb][0]);
+      ld4(src + 32 * kb + 4, a[kb][1]);
+    }
+  };
+  load_a();  // in flight during the table copy
   {
     constexpr int NV = H * (K2P / 4), PT = (NV + 64 * kInvLdsWaves - 1) / (64 * kInvLdsWaves);
     float x[PT][4];
@@ -2351,28 +2386,16 @@ __global__ __launch_bounds__(64 * kInvLdsWaves) void k_rad_inv_lds(
     }
   }
   __syncthreads();
-  const int64_t plane = (int64_t)n1 * N2;
-  const int nb = (int)gridDim.x;
-  for (int k = (int)blockIdx.x + nb * w; k < nitems; k += nb * kInvLdsWaves) {
-    const int4 it = items[k];
-    int r = 16 * it.y + i;
-    r = r < it.z ? r : it.z - 1;
-    const float* src = t1 + it.x * plane + (int64_t)r * N2 + 8 * g;
-    float a[NKB][2][4];
+  float b[2][NKB][2][4];
+  auto rdb = [&](int tq, float (&bb)[NKB][2][4]) {
+    const float* trow = &tab[(16 * tq + i) * LP + 8 * g];
 #pragma unroll
     for (int kb = 0; kb < NKB; kb++) {
-      ld4(src + 32 * kb, a[kb][0]);
-      ld4(src + 32 * kb + 4, a[kb][1]);
+      ld4(trow + 32 * kb, bb[kb][0]);
+      ld4(trow + 32 * kb + 4, bb[kb][1]);
     }
-    float b[2][NKB][2][4];
-    auto rdb = [&](int tq, float (&bb)[NKB][2][4]) {
-      const float* trow = &tab[(16 * tq + i) * LP + 8 * g];
-#pragma unroll
-      for (int kb = 0; kb < NKB; kb++) {
-        ld4(trow + 32 * kb, bb[kb][0]);
-        ld4(trow + 32 * kb + 4, bb[kb][1]);
-      }
-    };
+  };
+  while (k < nitems) {  // wave-uniform
     rdb(0, b[0]);
     const int kl = it.w & 0xffff;
 #pragma unroll
@@ -2380,30 +2403,40 @@ __global__ __launch_bounds__(64 * kInvLdsWaves) void k_rad_inv_lds(
 #pragma unroll
       for (int e = 0; e < 8; e++)
         if (32 * kb + 8 * g + e >= kl) a[kb][e >> 2][e & 3] = 0.0f;
-    float* ob = corr + it.x * plane;
+    float* orow = corr + it.x * plane + (int64_t)(16 * it.y + i) * N2;
+    const bool live = FULL || 16 * it.y + i < it.z;
 #pragma unroll
     for (int tq = 0; tq < NTQ; tq++) {
       if (tq + 1 < NTQ) rdb(tq + 1, b[(tq + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);  // one tile ahead, no further
       if ((it.w >> (16 + (tq >> 1))) & 1) {  // wave-uniform
+        // table as the A operand: the result's rows are R = 16 tq + 4 g + rr
+        // and its columns the t1 rows, so a lane stores four consecutive R
         V E = (V){0, 0, 0, 0}, O = (V){0, 0, 0, 0};
 #pragma unroll
         for (int kb = 0; kb < NKB; kb++)
 #pragma unroll
           for (int s = 0; s < 4; s++) {
             const int e0 = 2 * s, e1 = 2 * s + 1;
-            E = M::mma(a[kb][e0 >> 2][e0 & 3], b[tq & 1][kb][e0 >> 2][e0 & 3], E);
-            O = M::mma(a[kb][e1 >> 2][e1 & 3], b[tq & 1][kb][e1 >> 2][e1 & 3], O);
+            E = M::mma(b[tq & 1][kb][e0 >> 2][e0 & 3], a[kb][e0 >> 2][e0 & 3], E);
+            O = M::mma(b[tq & 1][kb][e1 >> 2][e1 & 3], a[kb][e1 >> 2][e1 & 3], O);
           }
-        const int R = 16 * tq + i;
+        float ep[4], em[4];
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
-          const int row = 16 * it.y + M::row(lane, rr);
-          if (FULL || row < it.z) {
-            ob[(int64_t)row * N2 + R] = E[rr] + O[rr];
-            ob[(int64_t)row * N2 + (N2 - 1 - R)] = E[rr] - O[rr];
-          }
+          ep[rr] = E[rr] + O[rr];
+          em[3 - rr] = E[rr] - O[rr];
+        }
+        if (live) {
+          st4(orow + 16 * tq + 4 * g, ep);
+          st4(orow + (N2 - 4 - 16 * tq - 4 * g), em);
         }
       }
+    }
+    k += stride;
+    if (k < nitems) {
+      it = items[k];
+      load_a();
     }
   }
 }
@@ -2664,8 +2697,117 @@ __global__ __launch_bounds__(256) void k_rad_mid(T* __restrict__ t1, const T* __
   }
 }
 
+// ---------------------------------------------------------------------------
+// The two r passes in FP32, every load of an item issued before its first
+// product: one block of four waves per item (channel-angle, 16 columns k2),
+// wave w owning the r blocks w, w + 4, .. (RBW of them) in both products.
+// An item is two int4: {channel * na + angle, column block | k2lo << 16,
+// nn1 | k1lo << 16, L offset}, {L row stride, LT offset, weight offset, 0};
+// the weights (radcut - E)_+ divi come from a table W[angle][k2][k1] (k1
+// padded to 16 NK1), so the item needs no other read.  Tables and operand
+// layouts as k_rad_mid; the back product keeps two accumulators (steps of
+// even and odd index), summed at the end.
+// ---------------------------------------------------------------------------
+template <int NK1, int RBW>
+__global__ __launch_bounds__(256) void k_rad_mid_f(float* __restrict__ t1,
+                                                   const float* __restrict__ tabT,
+                                                   const float* __restrict__ wts,
+                                                   const int4* __restrict__ items, int n1, int n2) {
+  typedef MfmaAcc<float> M;
+  typedef M::V V;
+  constexpr int KTP = 16 * NK1;
+  __shared__ V red[4][NK1][64];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  // XCD-aware: XCD x (blocks x, x + 8, ..) runs items [x q8, (x + 1) q8),
+  // so the items of one table, which are consecutive, share that XCD's L2
+  const int hw = blockIdx.x, q8 = (int)(gridDim.x >> 3);
+  const int it = hw < 8 * q8 ? (hw & 7) * q8 + (hw >> 3) : hw;
+  const int4 i0 = items[2 * it], i1 = items[2 * it + 1];
+  const int cg = i0.x, cc = i0.y & 0xffff, k2l = i0.y >> 16, nn = i0.z & 0xffff;
+  float* tb = t1 + (int64_t)cg * n1 * n2;
+  const float* L = tabT + i0.w;
+  const int ld = i1.x;
+  const float* LT = tabT + i1.y;
+  const int col = 16 * cc + i;
+  const int colc = col < n2 ? col : n2 - 1;
+  const int nrb = (nn + 15) >> 4;  // <= 4 RBW (host)
+  float a[RBW][NK1][4], b[RBW][4], at[RBW][NK1][4], wk[NK1][4];
+#pragma unroll
+  for (int q = 0; q < RBW; q++) {
+    const int rb = w + 4 * q < nrb ? w + 4 * q : nrb - 1;
+    const int r0 = 16 * rb + 4 * g;
+#pragma unroll
+    for (int m = 0; m < NK1; m++) ld4(L + (int64_t)(16 * m + i) * ld + r0, a[q][m]);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int r = r0 + s;
+      b[q][s] = tb[(int64_t)(r < n1 ? r : n1 - 1) * n2 + colc];
+    }
+#pragma unroll
+    for (int m = 0; m < NK1; m++) ld4(LT + (int64_t)(16 * rb + i) * KTP + 16 * m + 4 * g, at[q][m]);
+  }
+#pragma unroll
+  for (int m = 0; m < NK1; m++) ld4(wts + i1.z + (int64_t)col * KTP + 16 * m + 4 * g, wk[m]);
+  V acc[NK1];
+#pragma unroll
+  for (int m = 0; m < NK1; m++) acc[m] = (V){0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < RBW; q++)
+    if (w + 4 * q < nrb)  // wave-uniform
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int m = 0; m < NK1; m++) acc[m] = M::mma(a[q][m][s], b[q][s], acc[m]);
+#pragma unroll
+  for (int m = 0; m < NK1; m++) red[w][m][lane] = acc[m];
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < NK1; m++) {
+    acc[m] = red[0][m][lane];
+#pragma unroll
+    for (int q = 1; q < 4; q++) acc[m] += red[q][m][lane];
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) acc[m][rr] *= wk[m][rr];
+  }
+#pragma unroll
+  for (int q = 0; q < RBW; q++) {
+    const int mt = w + 4 * q;
+    if (mt >= nrb) break;  // wave-uniform
+    V o[2] = {(V){0, 0, 0, 0}, (V){0, 0, 0, 0}};
+#pragma unroll
+    for (int m = 0; m < NK1; m++)
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) o[rr & 1] = M::mma(at[q][m][rr], acc[m][rr], o[rr & 1]);
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const int r = 16 * mt + M::row(lane, rr);
+      if (r < nn && col < k2l) tb[(int64_t)r * n2 + col] = o[0][rr] + o[1][rr];
+    }
+  }
+}
+
+template <int NK1, int RBW>
+static hipError_t launch_rad_mid_f(Ctx* c) {
+  hipLaunchKernelGGL((k_rad_mid_f<NK1, RBW>), dim3((unsigned)c->mid_nitems), dim3(256), 0,
+                     c->stream, (float*)c->d_t1, (const float*)c->d_lowT,
+                     (const float*)c->d_mid_wts, (const int4*)c->d_mid_items2, c->t.n1, c->t.n2);
+  return hipGetLastError();
+}
+
 template <typename T>
 static hipError_t launch_rad_mid(Ctx* c) {
+  if constexpr (sizeof(T) == 4) {
+    if (c->d_mid_items2) {
+#define MW3_MIDF(NK, RB) \
+  if (c->mid_nk1 == NK && c->mid_rbw == RB) return launch_rad_mid_f<NK, RB>(c);
+      MW3_MIDF(1, 1) MW3_MIDF(1, 2) MW3_MIDF(1, 4)
+      MW3_MIDF(2, 1) MW3_MIDF(2, 2) MW3_MIDF(2, 4)
+      MW3_MIDF(3, 1) MW3_MIDF(3, 2) MW3_MIDF(3, 4)
+      MW3_MIDF(4, 1) MW3_MIDF(4, 2) MW3_MIDF(4, 4)
+#undef MW3_MIDF
+      return hipErrorInvalidValue;
+    }
+  }
   const HostTables& t = c->t;
   const unsigned nblk = (unsigned)c->mid_nitems;  // one block of 4 waves per item
   const T* tab = (const T*)c->d_dst_tables;
@@ -2765,6 +2907,37 @@ static hipError_t build_rad_mid(Ctx* c) {
   if ((e = up((void**)&c->d_mid_sizes, sizes.data(), sizes.size() * 4)) != hipSuccess) return e;
   c->mid_nitems = (int)(items.size() / 4);
   c->mid_nk1 = nk1;
+  // FP32: k_rad_mid_f's packed items and weight table W[angle][k2][k1] =
+  // (radcut - p2r1(k1) - p2r2(k2))_+ divi, rounded to FP32 (k1 clamped to
+  // n1 - 1 and k2 to n2 - 1 as k_rad_mid does), when every wave's r blocks
+  // fit 4 per wave
+  const int nrbmax = (t.n1 + 15) / 16;
+  const int rbw = nrbmax <= 4 ? 1 : nrbmax <= 8 ? 2 : nrbmax <= 16 ? 4 : 0;
+  if (!f64 && rbw) {
+    int ncb = 0;
+    for (int ia = 0; ia < t.na; ia++) ncb = std::max(ncb, (t.k2lo[ia] + 15) / 16);
+    const int ncol = 16 * ncb;
+    std::vector<float> wts((size_t)t.na * ncol * ktp);
+    for (int ia = 0; ia < t.na; ia++)
+      for (int k2 = 0; k2 < ncol; k2++)
+        for (int k1 = 0; k1 < ktp; k1++) {
+          const double x = t.radcut - (t.p2r1[(size_t)ia * t.n1 + std::min(k1, t.n1 - 1)] +
+                                       t.p2r2[std::min(k2, t.n2 - 1)]);
+          wts[((size_t)ia * ncol + k2) * ktp + k1] = (float)(x > 0.0 ? x * t.divi[ia] : 0.0);
+        }
+    std::vector<int32_t> it2;
+    const size_t ni = items.size() / 4;
+    for (size_t k = 0; k < ni; k++) {
+      const int32_t* r = &items[4 * k];
+      const int32_t* z = &sizes[4 * (size_t)r[3]];
+      const int ia = r[0] % t.na;
+      it2.insert(it2.end(), {r[0], r[1], r[2], z[0], z[1], z[2], (int32_t)((size_t)ia * ncol * ktp), 0});
+    }
+    if (wts.size() >= (size_t)1 << 31) return hipSuccess;
+    if ((e = up((void**)&c->d_mid_wts, wts.data(), wts.size() * sizeof(float))) != hipSuccess) return e;
+    if ((e = up((void**)&c->d_mid_items2, it2.data(), it2.size() * 4)) != hipSuccess) return e;
+    c->mid_rbw = rbw;
+  }
   return hipSuccess;
 }
 
