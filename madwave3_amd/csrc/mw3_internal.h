@@ -216,7 +216,8 @@ struct Ctx {
   int mid_rbw = 0;                 // r blocks per wave of the FP32 kernel
   int mid_nitems = 0;              // 0: the r passes run as two GEMMs
   int mid_nk1 = 0;                 // 16-row tiles of the low r modes
-  int rad_ncu = 1;                 // compute units (the LDS-table forward R pass)
+  int rad_ncu = 1;                 // compute units (the LDS-table R passes)
+  bool rad_lds64 = false;          // FP64 R passes with the table split over block pairs
   int ang_grid = 0;        // its persistent blocks
   int32_t* d_ang_tiles = nullptr;  // tiles of the one-pass kernels with compressed-grid points
   int ang_ntiles = 0;

@@ -2441,6 +2441,218 @@ __global__ __launch_bounds__(64 * kInvLdsWaves) void k_rad_inv_lds(
   }
 }
 
+// ---------------------------------------------------------------------------
+// The FP64 R passes with the sine table in LDS.  The FP64 tables (about 200
+// KB) do not fit one block's LDS, so each is split over a pair of blocks on
+// one XCD (blocks b and b + 8, as workgroups go to the XCDs round robin by
+// linear id): in the forward pass the two mode parities, in the inverse pass
+// the two halves of the R range.  Both blocks of a pair walk the same work
+// items with the same waves, so the second block's reads of an item's rows
+// hit the L2 the first one filled.  Otherwise as k_rad_fwd_lds /
+// k_rad_inv_lds: the table copied into LDS once, then one wave per item with
+// no barrier.  The grid is a multiple of 16 blocks.
+// ---------------------------------------------------------------------------
+constexpr int kRadLds64Waves = 8;
+
+__device__ __forceinline__ void lds64_pair(int& part, int& pair, int& npairs) {
+  const int b = (int)blockIdx.x;
+  part = (b >> 3) & 1;
+  pair = (b >> 4) * 8 + (b & 7);
+  npairs = (int)gridDim.x / 2;
+}
+
+template <int NKB, int HB, bool FULL>
+__global__ __launch_bounds__(64 * kRadLds64Waves) void k_rad_fwd_lds64(
+    const double* __restrict__ psi, const double* __restrict__ Tt, double* __restrict__ t1,
+    const int4* __restrict__ items, int nitems, int n1) {
+  typedef MfmaAcc<double> M;
+  typedef M::V V;
+  constexpr int N2 = 32 * HB, H = N2 / 2, NB = HB;  // j blocks of 16
+  constexpr int LP = H + 2, NM = 16 * NKB;           // modes of one parity
+  constexpr int RD = 3;                              // j blocks loaded ahead
+  __shared__ __align__(16) double tab[NM * LP];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  if (nitems <= 0) return;  // grid-uniform
+  int p, pair, npairs;
+  lds64_pair(p, pair, npairs);
+  const int64_t plane = (int64_t)n1 * N2;
+  const int stride = npairs * kRadLds64Waves;
+  int k = pair + npairs * w;
+  int4 it = items[k < nitems ? k : nitems - 1];
+  const double *src, *mir;
+  double a[RD][4], m[RD][4];
+  auto start = [&]() {
+    int r = 16 * it.y + i;
+    r = r < it.z ? r : it.z - 1;
+    src = psi + it.x * plane + (int64_t)r * N2 + 4 * g;
+    mir = psi + it.x * plane + (int64_t)r * N2 + (N2 - 4 - 4 * g);
+#pragma unroll
+    for (int d = 0; d < RD; d++) {
+      ld4(src + 16 * d, a[d]);
+      ld4(mir - 16 * d, m[d]);
+    }
+  };
+  start();  // in flight during the table copy
+  {
+    // rows k2 = 2 m + p of the table, every load in flight at once
+    constexpr int NV = NM * (H / 4), PT = (NV + 64 * kRadLds64Waves - 1) / (64 * kRadLds64Waves);
+    double x[PT][4];
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kRadLds64Waves * l;
+      v = v < NV ? v : NV - 1;
+      ld4(Tt + (int64_t)(2 * (v / (H / 4)) + p) * H + 4 * (v % (H / 4)), x[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kRadLds64Waves * l;
+      v = v < NV ? v : NV - 1;
+      st4(&tab[(v / (H / 4)) * LP + 4 * (v % (H / 4))], x[l]);
+    }
+  }
+  __syncthreads();
+  const double sg = p ? -1.0 : 1.0;
+  while (k < nitems) {  // wave-uniform
+    V acc[NKB];
+#pragma unroll
+    for (int t = 0; t < NKB; t++) acc[t] = (V){0, 0, 0, 0};
+    double b[2][NKB][4];
+#pragma unroll
+    for (int t = 0; t < NKB; t++) ld4(&tab[(16 * t + i) * LP + 4 * g], b[0][t]);
+#pragma unroll
+    for (int kb = 0; kb < NB; kb++) {
+      const int sl = kb % RD, cb = kb & 1;
+      double f[4];  // psi[j] + psi[n2-1-j] (even modes) or psi[j] - psi[n2-1-j] (odd)
+#pragma unroll
+      for (int q = 0; q < 4; q++) f[q] = a[sl][q] + sg * m[sl][3 - q];
+      if (kb + RD < NB) {
+        ld4(src + 16 * (kb + RD), a[sl]);
+        ld4(mir - 16 * (kb + RD), m[sl]);
+      }
+      if (kb + 1 < NB) {
+#pragma unroll
+        for (int t = 0; t < NKB; t++) ld4(&tab[(16 * t + i) * LP + 16 * (kb + 1) + 4 * g], b[cb ^ 1][t]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+#pragma unroll
+        for (int t = 0; t < NKB; t++) acc[t] = M::mma(f[s], b[cb][t][s], acc[t]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    double* orow = t1 + it.x * plane;
+    const int y0 = 16 * it.y;
+    k += stride;
+    if (k < nitems) {  // the next item's first loads go out before the stores
+      it = items[k];
+      start();
+    }
+#pragma unroll
+    for (int t = 0; t < NKB; t++)
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const int row = y0 + M::row(lane, rr);
+        if (FULL || row < n1) orow[(int64_t)row * N2 + 2 * (16 * t + i) + p] = acc[t][rr];
+      }
+  }
+}
+
+template <int NKB, int HB, bool FULL>
+__global__ __launch_bounds__(64 * kRadLds64Waves) void k_rad_inv_lds64(
+    const double* __restrict__ t1, const double* __restrict__ Ti, double* __restrict__ corr,
+    const int4* __restrict__ items, int nitems, int n1) {
+  typedef MfmaAcc<double> M;
+  typedef M::V V;
+  constexpr int N2 = 32 * HB, H = N2 / 2, NTQ = H / 16, NTH = NTQ / 2;  // R tiles per half
+  constexpr int K2P = 32 * NKB, LP = K2P + 2;
+  __shared__ __align__(16) double tab[16 * NTH * LP];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  if (nitems <= 0) return;  // grid-uniform
+  int h, pair, npairs;
+  lds64_pair(h, pair, npairs);
+  const int64_t plane = (int64_t)n1 * N2;
+  const int stride = npairs * kRadLds64Waves;
+  int k = pair + npairs * w;
+  int4 it = items[k < nitems ? k : nitems - 1];
+  double a[NKB][2][4];
+  auto load_a = [&]() {
+    int r = 16 * it.y + i;
+    r = r < it.z ? r : it.z - 1;
+    const double* src = t1 + it.x * plane + (int64_t)r * N2 + 8 * g;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+      ld4(src + 32 * kb, a[kb][0]);
+      ld4(src + 32 * kb + 4, a[kb][1]);
+    }
+  };
+  load_a();  // in flight during the table copy
+  {
+    // rows R = 16 NTH h .. of the table
+    constexpr int NV = 16 * NTH * (K2P / 4);
+    constexpr int PT = (NV + 64 * kRadLds64Waves - 1) / (64 * kRadLds64Waves);
+    double x[PT][4];
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kRadLds64Waves * l;
+      v = v < NV ? v : NV - 1;
+      ld4(Ti + (int64_t)(16 * NTH * h + v / (K2P / 4)) * K2P + 4 * (v % (K2P / 4)), x[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kRadLds64Waves * l;
+      v = v < NV ? v : NV - 1;
+      st4(&tab[(v / (K2P / 4)) * LP + 4 * (v % (K2P / 4))], x[l]);
+    }
+  }
+  __syncthreads();
+  while (k < nitems) {  // wave-uniform
+    const int kl = it.w & 0xffff;
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+      for (int e = 0; e < 8; e++)
+        if (32 * kb + 8 * g + e >= kl) a[kb][e >> 2][e & 3] = 0.0;
+    double* ob = corr + it.x * plane;
+#pragma unroll
+    for (int tl = 0; tl < NTH; tl++) {
+      const int tq = NTH * h + tl;
+      if ((it.w >> (16 + (tq >> 1))) & 1) {  // wave-uniform
+        double b[NKB][2][4];
+        const double* trow = &tab[(16 * tl + i) * LP + 8 * g];
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++) {
+          ld4(trow + 32 * kb, b[kb][0]);
+          ld4(trow + 32 * kb + 4, b[kb][1]);
+        }
+        V E = (V){0, 0, 0, 0}, O = (V){0, 0, 0, 0};
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+          for (int s = 0; s < 4; s++) {
+            const int e0 = 2 * s, e1 = 2 * s + 1;
+            E = M::mma(a[kb][e0 >> 2][e0 & 3], b[kb][e0 >> 2][e0 & 3], E);
+            O = M::mma(a[kb][e1 >> 2][e1 & 3], b[kb][e1 >> 2][e1 & 3], O);
+          }
+        const int R = 16 * tq + i;
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+          const int row = 16 * it.y + M::row(lane, rr);
+          if (FULL || row < it.z) {
+            ob[(int64_t)row * N2 + R] = E[rr] + O[rr];
+            ob[(int64_t)row * N2 + (N2 - 1 - R)] = E[rr] - O[rr];
+          }
+        }
+      }
+    }
+    k += stride;
+    if (k < nitems) {
+      it = items[k];
+      load_a();
+    }
+  }
+}
+
 template <typename T, int NKB, int HB, bool FULL>
 __global__ __launch_bounds__(32 * HB) void k_rad_inv(const T* __restrict__ t1,
                                                      const T* __restrict__ Ti,
@@ -2964,6 +3176,9 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->rad_ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
+    // FP64 passes with the table split over block pairs (whole 16-block
+    // groups; the HB R tiles of the first half split in two)
+    c->rad_lds64 = sizeof(T) == 8 && c->rad_ncu >= 16 && HB % 2 == 0;
     e = t.n1 % 16 == 0
                        ? persistent_grid(k_rad_fwd<T, NKB, HB, true>, 128 * NKB * RadSplit<T, HB>::v,
                                          &c->rad_grid[0])
@@ -2992,6 +3207,18 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
                            c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
       return hipGetLastError();
     }
+    if constexpr (sizeof(T) == 8) {
+      if (c->rad_lds64) {  // pairs of blocks split the table (16-block groups)
+        const int nb = c->rad_ncu / 16 * 16;
+        if (t.n1 % 16 == 0)
+          hipLaunchKernelGGL((k_rad_fwd_lds64<NKB, HB, true>), dim3(nb), dim3(64 * kRadLds64Waves),
+                             0, c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+        else
+          hipLaunchKernelGGL((k_rad_fwd_lds64<NKB, HB, false>), dim3(nb), dim3(64 * kRadLds64Waves),
+                             0, c->stream, psi, tab + t.rad_tt_off, t1, items, n, t.n1);
+        return hipGetLastError();
+      }
+    }
     const int nt = 128 * NKB * RadSplit<T, HB>::v;
     if (t.n1 % 16 == 0)
       hipLaunchKernelGGL((k_rad_fwd<T, NKB, HB, true>), dim3(n < g0 ? n : g0), dim3(nt), 0,
@@ -3009,6 +3236,18 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
         hipLaunchKernelGGL((k_rad_inv_lds<NKB, HB, false>), dim3(nb), dim3(64 * kInvLdsWaves), 0,
                            c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
       return hipGetLastError();
+    }
+    if constexpr (sizeof(T) == 8) {
+      if (c->rad_lds64) {
+        const int nb = c->rad_ncu / 16 * 16;
+        if (t.n1 % 16 == 0)
+          hipLaunchKernelGGL((k_rad_inv_lds64<NKB, HB, true>), dim3(nb), dim3(64 * kRadLds64Waves),
+                             0, c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+        else
+          hipLaunchKernelGGL((k_rad_inv_lds64<NKB, HB, false>), dim3(nb), dim3(64 * kRadLds64Waves),
+                             0, c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+        return hipGetLastError();
+      }
     }
     if (t.n1 % 16 == 0)
       hipLaunchKernelGGL((k_rad_inv<T, NKB, HB, true>), dim3(n < g1 ? n : g1), dim3(32 * HB), 0,
