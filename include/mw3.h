@@ -313,11 +313,13 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
                    (default in MW3_FP64).
    MW3_OPT_ANGULAR_GRID 1 (default) or 2: the folded angular kernel runs
                    its blocks per channel divided by this value.
-   MW3_OPT_RADIAL_GRID  1 (default), 2 or 4: the persistent FP64 R-pass
+   MW3_OPT_RADIAL_GRID  1 (default), 2 or 4: the staged persistent R-pass
                    kernels run the resident block count divided by this
                    value (each block loads its waves' sine-table slices
-                   once); the FP32 R passes, one block per compute unit with
-                   the table in LDS, do not use it.
+                   once).  The R passes with the sine table in LDS (FP32,
+                   and FP64 on devices of 16 or more compute units) run a
+                   grid of their own and ignore it; mw3_get_option then
+                   reports 1.
    The radial and angular variants sum in different orders: results agree
    to rounding.  mw3_get_option returns the variant in effect (after
    fallbacks).  mw3_set_option finishes the steps in flight first. */

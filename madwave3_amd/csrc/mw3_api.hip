@@ -1169,7 +1169,11 @@ static void effective_options(const Ctx* c, int32_t* out) {
   if (a == 1 && !c->ang_nt) a = 0;
   out[MW3_OPT_ANGULAR] = a;
   out[MW3_OPT_SIDE_STREAM] = c->opt_side;
-  out[MW3_OPT_RADIAL_GRID] = c->rad_nitems && c->opt_radial ? c->opt_rad_grid : 1;
+  // the grid divisor applies to the staged R-pass kernels only: the FP32
+  // passes, and the FP64 ones with their table split over block pairs, run
+  // with the table in LDS on a grid of their own
+  const bool staged = c->precision != MW3_MIXED && !c->rad_lds64;
+  out[MW3_OPT_RADIAL_GRID] = c->rad_nitems && c->opt_radial && staged ? c->opt_rad_grid : 1;
   out[MW3_OPT_ANGULAR_GRID] = c->ang_fold && out[MW3_OPT_ANGULAR] == 0 ? c->opt_ang_grid : 1;
 }
 
