@@ -2442,6 +2442,170 @@ __global__ __launch_bounds__(64 * kInvLdsWaves) void k_rad_inv_lds(
 }
 
 // ---------------------------------------------------------------------------
+// The inverse R pass of k_rad_inv_lds with its products on the FP16 matrix
+// cores, compensated: each FP32 operand x is split into x_h = fp16(x) and
+// x_l = fp16(x - x_h) (22 significant bits), and the sums take
+// A_h B_h + A_h B_l + A_l B_h, accumulated in FP32.  The term dropped,
+// A_l B_l, and the splits are each about 2^-22 of |A| |B|, the size of FP32
+// rounding over these sums.  The table (|S| <= 2) is split in the LDS copy;
+// each t1 row is first scaled by a power of two so that its largest entry
+// lies in [2^14, 2^15) (the FP16 range is 2^-24 .. 65504), and the results of
+// that row are scaled back exactly.  v_mfma_f32_16x16x32_f16 does 8192
+// multiply-adds per issue against 1024 for v_mfma_f32_16x16x4_f32 at
+// half the cycles, so the three products cost 3/16 of the FP32 ones.
+// Per parity p the modes 2 k' + p, k' < 16 NKB, run as ceil(NKB / 2)
+// products of depth 32; lane (g, i) holds k' = 32 m + 8 g .. + 7 of row i
+// (the same k' slots in the table operand, whose rows are R = 16 tq + i), so
+// a lane's t1 values are the 16 consecutive modes 64 m + 16 g .. + 15.
+// Modes at or past k2lo are zero; the table slots past 16 NKB read the last
+// 8 of the row instead (finite, times zero).  Stores, tile skipping and item
+// order as k_rad_inv_lds.
+// ---------------------------------------------------------------------------
+template <int NKB, int HB, bool FULL>
+__global__ __launch_bounds__(64 * kInvLdsWaves) void k_rad_inv_h(
+    const float* __restrict__ t1, const float* __restrict__ Ti, float* __restrict__ corr,
+    const int4* __restrict__ items, int nitems, int n1) {
+  typedef _Float16 H8 __attribute__((ext_vector_type(8)));
+  typedef _Float16 H2 __attribute__((ext_vector_type(2)));
+  typedef float V __attribute__((ext_vector_type(4)));
+  constexpr int N2 = 32 * HB, H = N2 / 2, NTQ = H / 16;
+  constexpr int K2P = 32 * NKB, KH = 16 * NKB, LPH = KH + 8, MK = (NKB + 1) / 2;
+  constexpr int PART = H * LPH;  // halves per table part: E hi, E lo, O hi, O lo
+  __shared__ __align__(16) _Float16 tab[4 * PART];
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15, w = threadIdx.x >> 6;
+  if (nitems <= 0) return;  // grid-uniform
+  const int64_t plane = (int64_t)n1 * N2;
+  const int nb = (int)gridDim.x, stride = nb * kInvLdsWaves;
+  int k = (int)blockIdx.x + nb * w;
+  int4 it = items[k < nitems ? k : nitems - 1];
+  float a[MK][16];  // row i of t1, modes 64 m + 16 g + c
+  auto load_a = [&]() {
+    int r = 16 * it.y + i;
+    r = r < it.z ? r : it.z - 1;
+    const float* src = t1 + it.x * plane + (int64_t)r * N2 + 16 * g;
+#pragma unroll
+    for (int m = 0; m < MK; m++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        float x[4];
+        ld4(src + 64 * m + 4 * q, x);
+#pragma unroll
+        for (int c = 0; c < 4; c++) a[m][4 * q + c] = x[c];
+      }
+  };
+  load_a();  // in flight during the table copy
+  {
+    constexpr int NV = H * (K2P / 4), PT = (NV + 64 * kInvLdsWaves - 1) / (64 * kInvLdsWaves);
+    float x[PT][4];
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kInvLdsWaves * l;
+      v = v < NV ? v : NV - 1;
+      ld4(Ti + (int64_t)(v / (K2P / 4)) * K2P + 4 * (v % (K2P / 4)), x[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < PT; l++) {
+      int v = threadIdx.x + 64 * kInvLdsWaves * l;
+      v = v < NV ? v : NV - 1;
+      // modes 4 c4 .. + 3 of row R: k' = 2 c4, 2 c4 + 1 of each parity
+      const int R = v / (K2P / 4), c4 = v % (K2P / 4), o = R * LPH + 2 * c4;
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        const float s0 = x[l][p], s1 = x[l][p + 2];
+        const _Float16 h0 = (_Float16)s0, h1 = (_Float16)s1;
+        *(H2*)&tab[(2 * p) * PART + o] = (H2){h0, h1};
+        *(H2*)&tab[(2 * p + 1) * PART + o] =
+            (H2){(_Float16)(s0 - (float)h0), (_Float16)(s1 - (float)h1)};
+      }
+    }
+  }
+  __syncthreads();
+  H8 ta[2][2][MK][2];  // [buffer][parity][m][hi, lo]
+  auto rdb = [&](int tq, H8 (&bb)[2][MK][2]) {
+    const int row = (16 * tq + i) * LPH;
+#pragma unroll
+    for (int m = 0; m < MK; m++) {
+      int kk = 32 * m + 8 * g;
+      kk = kk + 8 <= KH ? kk : KH - 8;
+#pragma unroll
+      for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) bb[p][m][h] = *(const H8*)&tab[(2 * p + h) * PART + row + kk];
+    }
+  };
+  while (k < nitems) {  // wave-uniform
+    rdb(0, ta[0]);
+    // zero the modes at or past k2lo, scale the row, split it
+    const int kl = it.w & 0xffff;
+    float mx = 0.0f;
+#pragma unroll
+    for (int m = 0; m < MK; m++)
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        if (64 * m + 16 * g + c >= kl) a[m][c] = 0.0f;
+        mx = fmaxf(mx, fabsf(a[m][c]));
+      }
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    int ex = 0;
+    if (mx > 0.0f) {
+      frexpf(mx, &ex);
+      ex = 15 - ex;  // mx 2^ex in [2^14, 2^15)
+    }
+    H8 tb[2][MK][2];  // [parity][m][hi, lo]
+#pragma unroll
+    for (int m = 0; m < MK; m++)
+#pragma unroll
+      for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const float xs = ldexpf(a[m][2 * j + p], ex);
+          const _Float16 hx = (_Float16)xs;
+          tb[p][m][0][j] = hx;
+          tb[p][m][1][j] = (_Float16)(xs - (float)hx);
+        }
+    float* orow = corr + it.x * plane + (int64_t)(16 * it.y + i) * N2;
+    const bool live = FULL || 16 * it.y + i < it.z;
+#pragma unroll
+    for (int tq = 0; tq < NTQ; tq++) {
+      if (tq + 1 < NTQ) rdb(tq + 1, ta[(tq + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);  // one tile ahead, no further
+      if ((it.w >> (16 + (tq >> 1))) & 1) {  // wave-uniform
+        // table as the A operand: rows R = 16 tq + 4 g + rr, columns the t1
+        // rows; the cross terms sum apart from the main ones
+        V E0 = (V){0, 0, 0, 0}, E1 = E0, O0 = E0, O1 = E0;
+        const auto& A = ta[tq & 1];
+#pragma unroll
+        for (int m = 0; m < MK; m++) {
+          E0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][m][0], tb[0][m][0], E0, 0, 0, 0);
+          O0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][m][0], tb[1][m][0], O0, 0, 0, 0);
+          E1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][m][0], tb[0][m][1], E1, 0, 0, 0);
+          O1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][m][0], tb[1][m][1], O1, 0, 0, 0);
+          E1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][m][1], tb[0][m][0], E1, 0, 0, 0);
+          O1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][m][1], tb[1][m][0], O1, 0, 0, 0);
+        }
+        float ep[4], em[4];
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++) {
+          const float e = E0[rr] + E1[rr], o = O0[rr] + O1[rr];
+          ep[rr] = ldexpf(e + o, -ex);
+          em[3 - rr] = ldexpf(e - o, -ex);
+        }
+        if (live) {
+          st4(orow + 16 * tq + 4 * g, ep);
+          st4(orow + (N2 - 4 - 16 * tq - 4 * g), em);
+        }
+      }
+    }
+    k += stride;
+    if (k < nitems) {
+      it = items[k];
+      load_a();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // The FP64 R passes with the sine table in LDS.  The FP64 tables (about 200
 // KB) do not fit one block's LDS, so each is split over a pair of blocks on
 // one XCD (blocks b and b + 8, as workgroups go to the XCDs round robin by
@@ -3179,6 +3343,7 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
     // FP64 passes with the table split over block pairs (whole 16-block
     // groups; the HB R tiles of the first half split in two)
     c->rad_lds64 = sizeof(T) == 8 && c->rad_ncu >= 16 && HB % 2 == 0;
+    c->rad_inv_h = sizeof(T) == 4;
     e = t.n1 % 16 == 0
                        ? persistent_grid(k_rad_fwd<T, NKB, HB, true>, 128 * NKB * RadSplit<T, HB>::v,
                                          &c->rad_grid[0])
@@ -3229,6 +3394,15 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   } else {
     if constexpr (sizeof(T) == 4) {  // the table in LDS, one block per CU
       const int nb = std::max(1, std::min(c->rad_ncu, n));
+      if (c->rad_inv_h) {  // products on the FP16 matrix cores, compensated
+        if (t.n1 % 16 == 0)
+          hipLaunchKernelGGL((k_rad_inv_h<NKB, HB, true>), dim3(nb), dim3(64 * kInvLdsWaves), 0,
+                             c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+        else
+          hipLaunchKernelGGL((k_rad_inv_h<NKB, HB, false>), dim3(nb), dim3(64 * kInvLdsWaves), 0,
+                             c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
+        return hipGetLastError();
+      }
       if (t.n1 % 16 == 0)
         hipLaunchKernelGGL((k_rad_inv_lds<NKB, HB, true>), dim3(nb), dim3(64 * kInvLdsWaves), 0,
                            c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);
