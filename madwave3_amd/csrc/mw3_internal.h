@@ -219,6 +219,7 @@ struct Ctx {
   int rad_ncu = 1;                 // compute units (the LDS-table R passes)
   bool rad_lds64 = false;          // FP64 R passes with the table split over block pairs
   bool rad_inv_h = false;          // FP32 inverse R pass with compensated FP16 products
+  bool ang_fold_h = false;         // FP32 folded angular kernel with compensated FP16 products
   int ang_grid = 0;        // its persistent blocks
   int32_t* d_ang_tiles = nullptr;  // tiles of the one-pass kernels with compressed-grid points
   int ang_ntiles = 0;

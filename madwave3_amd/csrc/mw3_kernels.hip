@@ -1751,13 +1751,268 @@ __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_
   }
 }
 
+// ---------------------------------------------------------------------------
+// k_ang_fold for FP32 data with both products on the FP16 matrix cores,
+// compensated as in k_rad_inv_h: operands split into hi = fp16(x) and
+// lo = fp16(x - hi), sums of hi hi + hi lo + lo hi in FP32, each product's
+// right operand scaled per column by a power of two so that its largest
+// entry lies in [2^14, 2^15) and the result scaled back exactly.  NT even:
+// the depth-32 products take 32 angles (forward) or 32 functions of one
+// class (back).
+//   forward: F[j][p] = sum_ia D(ia, j) w[ia][p], w = u (class +1) or v;
+//     lane (g, i) holds column i and angles 32 m + 8 g .. + 7, and reads the
+//     table operand from a j-major image Dj[j][ia] (16-byte reads);
+//   back: out[ia][p] = sum_j D(ia, j) G[j][p] per class; lane (g, i) holds
+//     rows 16 t + 4 g + r of F, so its 8 depth slots of chunk m are its own
+//     rows of tiles 2 m and 2 m + 1 (no exchange), and the angle-major image
+//     Di stores each class's columns permuted to that order (16-byte reads).
+// Both images are built from the FP32 table once per block.  Per group the
+// next group's psi values and coefficients are loaded while this one runs.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
+    const float* __restrict__ psi, const float* __restrict__ Dfold,
+    const float* __restrict__ afold, const float* __restrict__ bfold, float* __restrict__ ang,
+    const int32_t* __restrict__ groups, int ngroups, int nbc, int na, int n1, int n2) {
+  typedef _Float16 H8 __attribute__((ext_vector_type(8)));
+  typedef _Float16 H4 __attribute__((ext_vector_type(4)));
+  typedef float V __attribute__((ext_vector_type(4)));
+  static_assert(NT % 2 == 0, "depth-32 chunks");
+  constexpr int NR = 16 * NT, NC2 = 2 * NR, MK = NT / 2;
+  constexpr int PF = NR + 8, PB = NC2 + 8;  // image pitches (halves)
+  constexpr int SF = NC2 * PF, SB = NR * PB;
+  __shared__ __align__(16) _Float16 Dj[2 * SF];  // hi, lo
+  __shared__ __align__(16) _Float16 Di[2 * SB];
+  const int pl = n1 * n2, h = na >> 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, i = lane & 15;
+  const int ch = blockIdx.x / nbc, bl = blockIdx.x - ch * nbc;
+  const int stride = nbc * ANG_WAVES;
+  const char* pb = (const char*)(psi + (int64_t)ch * na * pl);
+  const char* ac = (const char*)(afold + (int64_t)ch * NC2 * n1);
+  const char* bc = (const char*)(bfold + (int64_t)ch * NC2 * n2);
+  char* ob = (char*)(ang + (int64_t)ch * na * pl);
+  // a group's inputs: psi at angles 32 m + 8 g + e and their mirrors (angles
+  // past h read angle h-1, against zero rows of D), the coefficients of rows
+  // j = 16 t + 4 g + r
+  float xl[MK][8], xh[MK][8], ca[2 * NT][4], cb[2 * NT][4];
+  auto ldall = [&](int col0) {
+    int pp = col0 + lane_opaque(i);
+    pp = pp < pl ? pp : pl - 1;
+    const int gg = lane_opaque(g), ll = lane_opaque(lane);
+#pragma unroll
+    for (int m = 0; m < MK; m++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        int ia = 32 * m + 8 * gg + e;
+        ia = ia < h ? ia : h - 1;
+        xl[m][e] = *(const float*)(pb + (uint32_t)(ia * pl + pp) * 4u);
+        xh[m][e] = *(const float*)(pb + (uint32_t)((na - 1 - ia) * pl + pp) * 4u);
+      }
+    const int r1 = pp / n2, R = pp - r1 * n2;
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = 16 * t + 4 * (ll >> 4) + r;
+        ca[t][r] = *(const float*)(ac + (uint32_t)(j * n1 + r1) * 4u);
+        cb[t][r] = *(const float*)(bc + (uint32_t)(j * n2 + R) * 4u);
+      }
+  };
+  int gi = 2 * (bl + nbc * (wave >> 1)) + (wave & 1);
+  const bool busy = gi < ngroups;
+  if (busy) ldall(16 * groups[gi]);  // in flight during the image copy
+  {
+    const float* Dc = Dfold + (int64_t)ch * NR * NC2;
+    constexpr int NV4 = NR * NC2 / 4;
+    constexpr int NV = (NV4 + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
+    constexpr int NG = NV < 4 ? NV : 4;
+#pragma unroll
+    for (int l0 = 0; l0 < NV; l0 += NG) {
+      float dv[NG][4];
+#pragma unroll
+      for (int l = 0; l < NG; l++) {
+        int v = threadIdx.x + 64 * ANG_WAVES * (l0 + l);
+        v = v < NV4 ? v : NV4 - 1;
+        ld4(Dc + 4 * (int64_t)v, dv[l]);
+      }
+#pragma unroll
+      for (int l = 0; l < NG; l++) {
+        int v = threadIdx.x + 64 * ANG_WAVES * (l0 + l);
+        v = v < NV4 ? v : NV4 - 1;
+        const int ia = v / (NC2 / 4), j0 = 4 * (v - ia * (NC2 / 4));
+        H4 hv, lv;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          hv[r] = (_Float16)dv[l][r];
+          lv[r] = (_Float16)(dv[l][r] - (float)hv[r]);
+        }
+        // class column jl = 16 (2 m + hh) + 4 q + r goes to 32 m + 8 q + 4 hh + r
+        const int cls = j0 / NR, jl = j0 - cls * NR;
+        const int cp = cls * NR + 32 * (jl >> 5) + 8 * ((jl >> 2) & 3) + 4 * ((jl >> 4) & 1);
+        *(H4*)&Di[ia * PB + cp] = hv;
+        *(H4*)&Di[SB + ia * PB + cp] = lv;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          Dj[(j0 + r) * PF + ia] = hv[r];
+          Dj[SF + (j0 + r) * PF + ia] = lv[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!busy) return;  // no block-wide barrier follows
+  auto maxcol = [](float x) {  // over the four lanes of one column
+    x = fmaxf(x, __shfl_xor(x, 16));
+    return fmaxf(x, __shfl_xor(x, 32));
+  };
+  auto expo = [](float mx) {  // mx 2^e in [2^14, 2^15)
+    int ex = 0;
+    if (mx > 0.0f) {
+      frexpf(mx, &ex);
+      ex = 15 - ex;
+    }
+    return ex;
+  };
+  for (; gi < ngroups; gi += stride) {
+    const int col = 16 * groups[gi];
+    const int cnext = 16 * groups[gi + stride < ngroups ? gi + stride : gi];
+    const int p = col + i;
+    // fold, scale per column and class, split
+    float mu = 0.0f, mv = 0.0f;
+#pragma unroll
+    for (int m = 0; m < MK; m++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float u = xl[m][e] + xh[m][e], v = xl[m][e] - xh[m][e];
+        xl[m][e] = u;
+        xh[m][e] = v;
+        mu = fmaxf(mu, fabsf(u));
+        mv = fmaxf(mv, fabsf(v));
+      }
+    const int eu = expo(maxcol(mu)), ev = expo(maxcol(mv));
+    H8 bw[2][MK][2];  // [class][m][hi, lo]
+#pragma unroll
+    for (int m = 0; m < MK; m++)
+#pragma unroll
+      for (int e = 0; e < 8; e++) {
+        const float su = ldexpf(xl[m][e], eu), sv = ldexpf(xh[m][e], ev);
+        const _Float16 hu = (_Float16)su, hv = (_Float16)sv;
+        bw[0][m][0][e] = hu;
+        bw[0][m][1][e] = (_Float16)(su - (float)hu);
+        bw[1][m][0][e] = hv;
+        bw[1][m][1][e] = (_Float16)(sv - (float)hv);
+      }
+    float cs[2 * NT][4];
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) cs[t][r] = ca[t][r] + cb[t][r];
+    ldall(cnext);  // the next group's inputs, in flight during this one
+    // forward: tile pairs (t, t + 1) of one class, MFMAs alternating between them
+    V f[2 * NT];
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++) f[t] = (V){0, 0, 0, 0};
+    constexpr int NQF = NT * MK;  // steps: tile pair tp, chunk m
+    H8 af[2][2][2];               // [buffer][tile of the pair][hi, lo]
+    auto rdf = [&](int q, H8 (&a)[2][2]) {
+      const int tp = q / MK, m = q - tp * MK;
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        const _Float16* s = &Dj[(16 * (2 * tp + d) + i) * PF + 32 * m + 8 * g];
+        a[d][0] = *(const H8*)s;
+        a[d][1] = *(const H8*)(s + SF);
+      }
+    };
+    rdf(0, af[0]);
+#pragma unroll
+    for (int q = 0; q < NQF; q++) {
+      if (q + 1 < NQF) rdf(q + 1, af[(q + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      const int tp = q / MK, m = q - tp * MK, t0 = 2 * tp, cl = t0 / NT;
+      const auto& A = af[q & 1];
+      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bw[cl][m][0], f[t0], 0, 0, 0);
+      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bw[cl][m][0], f[t0 + 1], 0, 0, 0);
+      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bw[cl][m][1], f[t0], 0, 0, 0);
+      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bw[cl][m][1], f[t0 + 1], 0, 0, 0);
+      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][1], bw[cl][m][0], f[t0], 0, 0, 0);
+      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][1], bw[cl][m][0], f[t0 + 1], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // G = (a + b) F, unscaled, then scaled per column and class and split
+    float mg[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        f[t][r] = ldexpf(f[t][r] * cs[t][r], t < NT ? -eu : -ev);
+        mg[t / NT] = fmaxf(mg[t / NT], fabsf(f[t][r]));
+      }
+    const int eg[2] = {expo(maxcol(mg[0])), expo(maxcol(mg[1]))};
+    H8 bg[2][MK][2];  // [class][m][hi, lo]: slot 4 hh + r = row r of tile 2 m + hh
+#pragma unroll
+    for (int c2 = 0; c2 < 2; c2++)
+#pragma unroll
+      for (int m = 0; m < MK; m++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const float s = ldexpf(f[c2 * NT + 2 * m + (e >> 2)][e & 3], eg[c2]);
+          const _Float16 hs = (_Float16)s;
+          bg[c2][m][0][e] = hs;
+          bg[c2][m][1][e] = (_Float16)(s - (float)hs);
+        }
+    // back: output tile pairs (u, u + 1) of one class
+    V o[2 * NT];
+#pragma unroll
+    for (int u = 0; u < 2 * NT; u++) o[u] = (V){0, 0, 0, 0};
+    constexpr int NQB = NT * MK;  // steps: class, tile pair, chunk
+    H8 ab[2][2][2];
+    auto rdb = [&](int q, H8 (&a)[2][2]) {
+      const int c2 = q / (NQB / 2), rem = q - c2 * (NQB / 2), up = rem / MK, m = rem - up * MK;
+#pragma unroll
+      for (int d = 0; d < 2; d++) {
+        const _Float16* s = &Di[(16 * (2 * up + d) + i) * PB + c2 * NR + 32 * m + 8 * g];
+        a[d][0] = *(const H8*)s;
+        a[d][1] = *(const H8*)(s + SB);
+      }
+    };
+    rdb(0, ab[0]);
+#pragma unroll
+    for (int q = 0; q < NQB; q++) {
+      if (q + 1 < NQB) rdb(q + 1, ab[(q + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+      const int c2 = q / (NQB / 2), rem = q - c2 * (NQB / 2), up = rem / MK, m = rem - up * MK;
+      const int u0 = c2 * NT + 2 * up;
+      const auto& A = ab[q & 1];
+      o[u0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bg[c2][m][0], o[u0], 0, 0, 0);
+      o[u0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bg[c2][m][0], o[u0 + 1], 0, 0, 0);
+      o[u0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bg[c2][m][1], o[u0], 0, 0, 0);
+      o[u0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bg[c2][m][1], o[u0 + 1], 0, 0, 0);
+      o[u0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][1], bg[c2][m][0], o[u0], 0, 0, 0);
+      o[u0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][1], bg[c2][m][0], o[u0 + 1], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    const int ll = lane_opaque(lane), pp = lane_opaque(p);
+#pragma unroll
+    for (int u = 0; u < NT; u++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int ia = 16 * u + 4 * (ll >> 4) + r;
+        if (ia < h && pp < pl) {
+          const float e = ldexpf(o[u][r], -eg[0]), od = ldexpf(o[NT + u][r], -eg[1]);
+          *(float*)(ob + (uint32_t)(ia * pl + pp) * 4u) = e + od;
+          *(float*)(ob + (uint32_t)((na - 1 - ia) * pl + pp) * 4u) = e - od;
+        }
+      }
+  }
+}
+
 // blocks per channel: the resident blocks shared evenly by the channels, and
 // no more than the group list can feed
-template <typename T, int NT>
-static hipError_t ang_fold_grid(int nc, int ngroups, int* nbc) {
+template <typename K>
+static hipError_t ang_fold_grid_k(K kernel, int nc, int ngroups, int* nbc) {
   int nb = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ang_fold<T, NT>,
-                                                              64 * ANG_WAVES, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 64 * ANG_WAVES, 0);
   if (e != hipSuccess) return e;
   int dev = 0, ncu = 0;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -1770,12 +2025,29 @@ static hipError_t ang_fold_grid(int nc, int ngroups, int* nbc) {
 }
 
 template <typename T, int NT>
+static hipError_t ang_fold_grid(Ctx* c, int nc, int ngroups, int* nbc) {
+  if constexpr (sizeof(T) == 4 && NT % 2 == 0) {
+    if (c->ang_fold_h) return ang_fold_grid_k(k_ang_fold_h<NT>, nc, ngroups, nbc);
+  }
+  return ang_fold_grid_k(k_ang_fold<T, NT>, nc, ngroups, nbc);
+}
+
+template <typename T, int NT>
 static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   const HostTables& t = c->t;
   if (c->ang_ngroups == 0) return;
   // fewer blocks per channel (the grid option) walk more groups each and
   // leave CUs to the radial passes beside them
   const int nbc = std::max(1, c->ang_fold_nbc / (c->opt_ang_grid > 0 ? c->opt_ang_grid : 1));
+  if constexpr (sizeof(T) == 4 && NT % 2 == 0) {
+    if (c->ang_fold_h) {  // products on the FP16 matrix cores, compensated
+      hipLaunchKernelGGL((k_ang_fold_h<NT>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
+                         st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
+                         (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups,
+                         c->ang_ngroups, nbc, t.na, t.n1, t.n2);
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
                      st, psi, (const T*)c->d_fold_D, (const T*)c->d_fold_a,
                      (const T*)c->d_fold_b, (T*)c->d_ang, c->d_ang_groups, c->ang_ngroups, nbc,
@@ -1797,11 +2069,12 @@ static hipError_t launch_ang_fold(Ctx* c, const T* psi, hipStream_t st) {
 template <typename T>
 static hipError_t ang_fold_setup(Ctx* c, int nt) {
   const int nc = c->t.nc, ng = c->ang_ngroups;
+  c->ang_fold_h = sizeof(T) == 4 && nt % 2 == 0;
   switch (nt) {
-    case 1: return ang_fold_grid<T, 1>(nc, ng, &c->ang_fold_nbc);
-    case 2: return ang_fold_grid<T, 2>(nc, ng, &c->ang_fold_nbc);
-    case 3: return ang_fold_grid<T, 3>(nc, ng, &c->ang_fold_nbc);
-    case 4: return ang_fold_grid<T, 4>(nc, ng, &c->ang_fold_nbc);
+    case 1: return ang_fold_grid<T, 1>(c, nc, ng, &c->ang_fold_nbc);
+    case 2: return ang_fold_grid<T, 2>(c, nc, ng, &c->ang_fold_nbc);
+    case 3: return ang_fold_grid<T, 3>(c, nc, ng, &c->ang_fold_nbc);
+    case 4: return ang_fold_grid<T, 4>(c, nc, ng, &c->ang_fold_nbc);
     default: return hipErrorInvalidValue;
   }
 }
