@@ -1756,9 +1756,11 @@ __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_
 // compensated as in k_rad_inv_h: operands split into hi = fp16(x) and
 // lo = fp16(x - hi), sums of hi hi + hi lo + lo hi in FP32, each product's
 // right operand scaled per column by a power of two so that its largest
-// entry lies in [2^14, 2^15) and the result scaled back exactly.  NT even:
-// the depth-32 products take 32 angles (forward) or 32 functions of one
-// class (back).
+// entry lies in [2^14, 2^15) and the result scaled back exactly.  The
+// depth-32 products take 32 angles (forward) or 32 functions of one class
+// (back); for odd NT both depths are padded to 32 ceil(NT / 2) with zero
+// rows and columns of the images, and the back products run an even number
+// of output tiles per class.
 //   forward: F[j][p] = sum_ia D(ia, j) w[ia][p], w = u (class +1) or v;
 //     lane (g, i) holds column i and angles 32 m + 8 g .. + 7, and reads the
 //     table operand from a j-major image Dj[j][ia] (16-byte reads);
@@ -1777,10 +1779,10 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   typedef _Float16 H8 __attribute__((ext_vector_type(8)));
   typedef _Float16 H4 __attribute__((ext_vector_type(4)));
   typedef float V __attribute__((ext_vector_type(4)));
-  static_assert(NT % 2 == 0, "depth-32 chunks");
-  constexpr int NR = 16 * NT, NC2 = 2 * NR, MK = NT / 2;
-  constexpr int PF = NR + 8, PB = NC2 + 8;  // image pitches (halves)
-  constexpr int SF = NC2 * PF, SB = NR * PB;
+  constexpr int NR = 16 * NT, NC2 = 2 * NR, MK = (NT + 1) / 2;
+  constexpr int KP = 32 * MK, NTP = 2 * MK;     // padded depth and output tiles per class
+  constexpr int PF = KP + 8, PB = 2 * KP + 8;  // image pitches (halves)
+  constexpr int SF = NC2 * PF, SB = KP * PB;
   __shared__ __align__(16) _Float16 Dj[2 * SF];  // hi, lo
   __shared__ __align__(16) _Float16 Di[2 * SB];
   const int pl = n1 * n2, h = na >> 1;
@@ -1822,6 +1824,11 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   int gi = 2 * (bl + nbc * (wave >> 1)) + (wave & 1);
   const bool busy = gi < ngroups;
   if (busy) ldall(16 * groups[gi]);  // in flight during the image copy
+  if constexpr (KP != NR) {  // padded depth: zero images first
+    for (int v = threadIdx.x; v < (SF + SB) / 4; v += 64 * ANG_WAVES)
+      *(H8*)(v < SF / 4 ? &Dj[8 * v] : &Di[8 * (v - SF / 4)]) = (H8){0, 0, 0, 0, 0, 0, 0, 0};
+    __syncthreads();
+  }
   {
     const float* Dc = Dfold + (int64_t)ch * NR * NC2;
     constexpr int NV4 = NR * NC2 / 4;
@@ -1849,7 +1856,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         }
         // class column jl = 16 (2 m + hh) + 4 q + r goes to 32 m + 8 q + 4 hh + r
         const int cls = j0 / NR, jl = j0 - cls * NR;
-        const int cp = cls * NR + 32 * (jl >> 5) + 8 * ((jl >> 2) & 3) + 4 * ((jl >> 4) & 1);
+        const int cp = cls * KP + 32 * (jl >> 5) + 8 * ((jl >> 2) & 3) + 4 * ((jl >> 4) & 1);
         *(H4*)&Di[ia * PB + cp] = hv;
         *(H4*)&Di[SB + ia * PB + cp] = lv;
 #pragma unroll
@@ -1929,14 +1936,15 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     for (int q = 0; q < NQF; q++) {
       if (q + 1 < NQF) rdf(q + 1, af[(q + 1) & 1]);
       __builtin_amdgcn_sched_barrier(0);
-      const int tp = q / MK, m = q - tp * MK, t0 = 2 * tp, cl = t0 / NT;
+      // the two tiles of a pair may be of two classes (odd NT)
+      const int tp = q / MK, m = q - tp * MK, t0 = 2 * tp, c0 = t0 / NT, c1 = (t0 + 1) / NT;
       const auto& A = af[q & 1];
-      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bw[cl][m][0], f[t0], 0, 0, 0);
-      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bw[cl][m][0], f[t0 + 1], 0, 0, 0);
-      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bw[cl][m][1], f[t0], 0, 0, 0);
-      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bw[cl][m][1], f[t0 + 1], 0, 0, 0);
-      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][1], bw[cl][m][0], f[t0], 0, 0, 0);
-      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][1], bw[cl][m][0], f[t0 + 1], 0, 0, 0);
+      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bw[c0][m][0], f[t0], 0, 0, 0);
+      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bw[c1][m][0], f[t0 + 1], 0, 0, 0);
+      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bw[c0][m][1], f[t0], 0, 0, 0);
+      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bw[c1][m][1], f[t0 + 1], 0, 0, 0);
+      f[t0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][1], bw[c0][m][0], f[t0], 0, 0, 0);
+      f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][1], bw[c1][m][0], f[t0 + 1], 0, 0, 0);
       __builtin_amdgcn_sched_barrier(0);
     }
     // G = (a + b) F, unscaled, then scaled per column and class and split
@@ -1956,22 +1964,23 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
       for (int m = 0; m < MK; m++)
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-          const float s = ldexpf(f[c2 * NT + 2 * m + (e >> 2)][e & 3], eg[c2]);
+          const int tt = 2 * m + (e >> 2);  // past NT: the zero padding of the depth
+          const float s = tt < NT ? ldexpf(f[c2 * NT + (tt < NT ? tt : 0)][e & 3], eg[c2]) : 0.0f;
           const _Float16 hs = (_Float16)s;
           bg[c2][m][0][e] = hs;
           bg[c2][m][1][e] = (_Float16)(s - (float)hs);
         }
     // back: output tile pairs (u, u + 1) of one class
-    V o[2 * NT];
+    V o[2 * NTP];
 #pragma unroll
-    for (int u = 0; u < 2 * NT; u++) o[u] = (V){0, 0, 0, 0};
-    constexpr int NQB = NT * MK;  // steps: class, tile pair, chunk
+    for (int u = 0; u < 2 * NTP; u++) o[u] = (V){0, 0, 0, 0};
+    constexpr int NQB = 2 * MK * MK;  // steps: class, tile pair, chunk
     H8 ab[2][2][2];
     auto rdb = [&](int q, H8 (&a)[2][2]) {
       const int c2 = q / (NQB / 2), rem = q - c2 * (NQB / 2), up = rem / MK, m = rem - up * MK;
 #pragma unroll
       for (int d = 0; d < 2; d++) {
-        const _Float16* s = &Di[(16 * (2 * up + d) + i) * PB + c2 * NR + 32 * m + 8 * g];
+        const _Float16* s = &Di[(16 * (2 * up + d) + i) * PB + c2 * KP + 32 * m + 8 * g];
         a[d][0] = *(const H8*)s;
         a[d][1] = *(const H8*)(s + SB);
       }
@@ -1982,7 +1991,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
       if (q + 1 < NQB) rdb(q + 1, ab[(q + 1) & 1]);
       __builtin_amdgcn_sched_barrier(0);
       const int c2 = q / (NQB / 2), rem = q - c2 * (NQB / 2), up = rem / MK, m = rem - up * MK;
-      const int u0 = c2 * NT + 2 * up;
+      const int u0 = c2 * NTP + 2 * up;
       const auto& A = ab[q & 1];
       o[u0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[0][0], bg[c2][m][0], o[u0], 0, 0, 0);
       o[u0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][0], bg[c2][m][0], o[u0 + 1], 0, 0, 0);
@@ -1999,7 +2008,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
       for (int r = 0; r < 4; r++) {
         const int ia = 16 * u + 4 * (ll >> 4) + r;
         if (ia < h && pp < pl) {
-          const float e = ldexpf(o[u][r], -eg[0]), od = ldexpf(o[NT + u][r], -eg[1]);
+          const float e = ldexpf(o[u][r], -eg[0]), od = ldexpf(o[NTP + u][r], -eg[1]);
           *(float*)(ob + (uint32_t)(ia * pl + pp) * 4u) = e + od;
           *(float*)(ob + (uint32_t)((na - 1 - ia) * pl + pp) * 4u) = e - od;
         }
@@ -2026,7 +2035,7 @@ static hipError_t ang_fold_grid_k(K kernel, int nc, int ngroups, int* nbc) {
 
 template <typename T, int NT>
 static hipError_t ang_fold_grid(Ctx* c, int nc, int ngroups, int* nbc) {
-  if constexpr (sizeof(T) == 4 && NT % 2 == 0) {
+  if constexpr (sizeof(T) == 4) {
     if (c->ang_fold_h) return ang_fold_grid_k(k_ang_fold_h<NT>, nc, ngroups, nbc);
   }
   return ang_fold_grid_k(k_ang_fold<T, NT>, nc, ngroups, nbc);
@@ -2039,7 +2048,7 @@ static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   // fewer blocks per channel (the grid option) walk more groups each and
   // leave CUs to the radial passes beside them
   const int nbc = std::max(1, c->ang_fold_nbc / (c->opt_ang_grid > 0 ? c->opt_ang_grid : 1));
-  if constexpr (sizeof(T) == 4 && NT % 2 == 0) {
+  if constexpr (sizeof(T) == 4) {
     if (c->ang_fold_h) {  // products on the FP16 matrix cores, compensated
       hipLaunchKernelGGL((k_ang_fold_h<NT>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
                          st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
@@ -2069,7 +2078,7 @@ static hipError_t launch_ang_fold(Ctx* c, const T* psi, hipStream_t st) {
 template <typename T>
 static hipError_t ang_fold_setup(Ctx* c, int nt) {
   const int nc = c->t.nc, ng = c->ang_ngroups;
-  c->ang_fold_h = sizeof(T) == 4 && nt % 2 == 0;
+  c->ang_fold_h = sizeof(T) == 4;
   switch (nt) {
     case 1: return ang_fold_grid<T, 1>(c, nc, ng, &c->ang_fold_nbc);
     case 2: return ang_fold_grid<T, 2>(c, nc, ng, &c->ang_fold_nbc);

@@ -24,7 +24,7 @@ import numpy as np
 import pytest
 
 from madwave3_amd import leps
-from madwave3_amd.lib import MW3_FP64, MW3_OPT_ANGULAR, Propagator, steps_batch
+from madwave3_amd.lib import MW3_FP64, MW3_MIXED, MW3_OPT_ANGULAR, Propagator, steps_batch
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -116,6 +116,30 @@ def test_full_size_fp64_matches_oracle(cfg):
         compare(g, r, c, K_STEPS[cfg], "%s angular variant %d" % (label, g.get_option(MW3_OPT_ANGULAR)),
                 tol)
         g.close()
+
+
+@pytest.mark.parametrize("cfg", [2, 3, 5])
+def test_full_size_hphi_mixed_matches_oracle(cfg):
+    """One H psi in mixed precision at the full grid size, so that the FP32
+    operator kernels run at their production shapes (the folded angular
+    kernel with 60 and 48 angle pairs, the R passes at 512 and 384 points,
+    seven Coriolis-coupled blocks), against the oracle on the
+    configuration's initial packet and on a random vector: within 2e-5 of the
+    largest entry, the bound of the small cases, for each angular variant."""
+    c = full_case(cfg)
+    o = Oracle(c)
+    rng = np.random.default_rng(cfg)
+    for name, psi in (("packet", c.rpaq0), ("random", rng.standard_normal(c.ntot))):
+        h_ref = o.hphi(psi)
+        for angular in (0, 1):
+            g = Propagator(c, MW3_MIXED)
+            g.set_option(MW3_OPT_ANGULAR, angular)
+            e = rel(g.hphi(psi), h_ref)
+            label = "configuration %d, %s, angular variant %d" % (cfg, name,
+                                                                  g.get_option(MW3_OPT_ANGULAR))
+            g.close()
+            print(label, "H psi rel err %.2e" % e, flush=True)
+            assert e < 2e-5, label
 
 
 def test_config4_batch_member_matches_oracle():
