@@ -320,6 +320,13 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
                    and FP64 on devices of 16 or more compute units) run a
                    grid of their own and ignore it; mw3_get_option then
                    reports 1.
+   MW3_OPT_FP16_PRODUCTS  MW3_MIXED only.  1 (default): the inverse R pass
+                   and the folded angular kernel multiply on the FP16 matrix
+                   cores, each FP32 operand split into FP16 hi and lo parts
+                   (22 significant bits, data scaled per row or column by a
+                   power of two) and hi hi + hi lo + lo hi summed in FP32;
+                   0: those products in FP32.  A precision choice: the
+                   autotuner keeps the value set.  Reported 0 in MW3_FP64.
    The radial and angular variants sum in different orders: results agree
    to rounding.  mw3_get_option returns the variant in effect (after
    fallbacks).  mw3_set_option finishes the steps in flight first. */
@@ -329,7 +336,8 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
 #define MW3_OPT_SIDE_STREAM 3
 #define MW3_OPT_RADIAL_GRID 4
 #define MW3_OPT_ANGULAR_GRID 5
-#define MW3_NOPTIONS 6
+#define MW3_OPT_FP16_PRODUCTS 6
+#define MW3_NOPTIONS 7
 MW3_API int32_t mw3_set_option(mw3_ctx *ctx, int32_t option, int32_t value);
 MW3_API int32_t mw3_get_option(const mw3_ctx *ctx, int32_t option, int32_t *value);
 

@@ -1152,6 +1152,12 @@ int32_t mw3_set_option(mw3_ctx* ctx, int32_t option, int32_t value) {
       if (value != c->opt_rad_grid) drop_graphs(c);
       c->opt_rad_grid = value;
       return MW3_OK;
+    case MW3_OPT_FP16_PRODUCTS:
+      if (value != 0 && value != 1)
+        return seterr(c, MW3_EINVAL, "mw3_set_option: fp16 products 0 or 1");
+      if (value != c->opt_fp16) drop_graphs(c);
+      c->opt_fp16 = value;
+      return MW3_OK;
     default:
       return seterr(c, MW3_EINVAL, "mw3_set_option: unknown option");
   }
@@ -1175,6 +1181,7 @@ static void effective_options(const Ctx* c, int32_t* out) {
   const bool staged = c->precision != MW3_MIXED && !c->rad_lds64;
   out[MW3_OPT_RADIAL_GRID] = c->rad_nitems && c->opt_radial && staged ? c->opt_rad_grid : 1;
   out[MW3_OPT_ANGULAR_GRID] = c->ang_fold && out[MW3_OPT_ANGULAR] == 0 ? c->opt_ang_grid : 1;
+  out[MW3_OPT_FP16_PRODUCTS] = c->precision == MW3_MIXED ? c->opt_fp16 : 0;
 }
 
 int32_t mw3_get_option(const mw3_ctx* ctx, int32_t option, int32_t* value) {
@@ -1247,6 +1254,7 @@ int32_t mw3_autotune(mw3_ctx* ctx, int32_t trial_steps, int32_t* chosen, double*
     c->opt_side = k[MW3_OPT_SIDE_STREAM];
     c->opt_rad_grid = k[MW3_OPT_RADIAL_GRID];
     c->opt_ang_grid = k[MW3_OPT_ANGULAR_GRID];
+    c->opt_fp16 = k[MW3_OPT_FP16_PRODUCTS];
   };
   Key best_key;
   effective_options(c, best_key.data());
@@ -1264,6 +1272,7 @@ int32_t mw3_autotune(mw3_ctx* ctx, int32_t trial_steps, int32_t* chosen, double*
               k[MW3_OPT_SIDE_STREAM] = sd;
               k[MW3_OPT_RADIAL_GRID] = gd;
               k[MW3_OPT_ANGULAR_GRID] = ag;
+              k[MW3_OPT_FP16_PRODUCTS] = best_key[MW3_OPT_FP16_PRODUCTS];  // kept as set
               apply(k);
               Key eff;
               effective_options(c, eff.data());

@@ -220,6 +220,8 @@ struct Ctx {
   bool rad_lds64 = false;          // FP64 R passes with the table split over block pairs
   bool rad_inv_h = false;          // FP32 inverse R pass with compensated FP16 products
   bool ang_fold_h = false;         // FP32 folded angular kernel with compensated FP16 products
+  int ang_fold_h_nbc = 0;          // its blocks per channel
+  int opt_fp16 = 1;                // MW3_OPT_FP16_PRODUCTS (mixed only)
   int ang_grid = 0;        // its persistent blocks
   int32_t* d_ang_tiles = nullptr;  // tiles of the one-pass kernels with compressed-grid points
   int ang_ntiles = 0;

@@ -2036,7 +2036,10 @@ static hipError_t ang_fold_grid_k(K kernel, int nc, int ngroups, int* nbc) {
 template <typename T, int NT>
 static hipError_t ang_fold_grid(Ctx* c, int nc, int ngroups, int* nbc) {
   if constexpr (sizeof(T) == 4) {
-    if (c->ang_fold_h) return ang_fold_grid_k(k_ang_fold_h<NT>, nc, ngroups, nbc);
+    if (c->ang_fold_h) {
+      hipError_t e = ang_fold_grid_k(k_ang_fold_h<NT>, nc, ngroups, &c->ang_fold_h_nbc);
+      if (e != hipSuccess) return e;
+    }
   }
   return ang_fold_grid_k(k_ang_fold<T, NT>, nc, ngroups, nbc);
 }
@@ -2047,16 +2050,18 @@ static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   if (c->ang_ngroups == 0) return;
   // fewer blocks per channel (the grid option) walk more groups each and
   // leave CUs to the radial passes beside them
-  const int nbc = std::max(1, c->ang_fold_nbc / (c->opt_ang_grid > 0 ? c->opt_ang_grid : 1));
+  const int gdiv = c->opt_ang_grid > 0 ? c->opt_ang_grid : 1;
   if constexpr (sizeof(T) == 4) {
-    if (c->ang_fold_h) {  // products on the FP16 matrix cores, compensated
-      hipLaunchKernelGGL((k_ang_fold_h<NT>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
+    if (c->ang_fold_h && c->opt_fp16) {  // products on the FP16 matrix cores, compensated
+      const int nbh = std::max(1, c->ang_fold_h_nbc / gdiv);
+      hipLaunchKernelGGL((k_ang_fold_h<NT>), dim3((unsigned)(nbh * t.nc)), dim3(64 * ANG_WAVES), 0,
                          st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
                          (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups,
-                         c->ang_ngroups, nbc, t.na, t.n1, t.n2);
+                         c->ang_ngroups, nbh, t.na, t.n1, t.n2);
       return;
     }
   }
+  const int nbc = std::max(1, c->ang_fold_nbc / gdiv);
   hipLaunchKernelGGL((k_ang_fold<T, NT>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
                      st, psi, (const T*)c->d_fold_D, (const T*)c->d_fold_a,
                      (const T*)c->d_fold_b, (T*)c->d_ang, c->d_ang_groups, c->ang_ngroups, nbc,
@@ -3676,7 +3681,7 @@ static hipError_t rad_nkb_hb(Ctx* c, const T* psi, T* t1, T* corr, int mode) {
   } else {
     if constexpr (sizeof(T) == 4) {  // the table in LDS, one block per CU
       const int nb = std::max(1, std::min(c->rad_ncu, n));
-      if (c->rad_inv_h) {  // products on the FP16 matrix cores, compensated
+      if (c->rad_inv_h && c->opt_fp16) {  // products on the FP16 matrix cores, compensated
         if (t.n1 % 16 == 0)
           hipLaunchKernelGGL((k_rad_inv_h<NKB, HB, true>), dim3(nb), dim3(64 * kInvLdsWaves), 0,
                              c->stream, t1, tab + t.rad_ti_off, corr, items, n, t.n1);

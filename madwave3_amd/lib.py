@@ -14,7 +14,8 @@ import numpy as np
 __all__ = ["Mw3Setup", "Mw3ProdSetup", "make_setup", "make_prod_setup", "load_library",
            "Propagator", "Mw3Error",
            "MW3_FP64", "MW3_MIXED", "MW3_OPT_GRAPHS", "MW3_OPT_RADIAL", "MW3_OPT_ANGULAR",
-           "MW3_OPT_SIDE_STREAM", "MW3_OPT_RADIAL_GRID", "MW3_OPT_ANGULAR_GRID", "OPTION_NAMES",
+           "MW3_OPT_SIDE_STREAM", "MW3_OPT_RADIAL_GRID", "MW3_OPT_ANGULAR_GRID",
+           "MW3_OPT_FP16_PRODUCTS", "OPTION_NAMES",
            "LIB_PATH"]
 
 MW3_ABI_VERSION = 1
@@ -25,8 +26,10 @@ MW3_MIXED = 1
 MW3_OPT_GRAPHS, MW3_OPT_RADIAL, MW3_OPT_ANGULAR, MW3_OPT_SIDE_STREAM = 0, 1, 2, 3
 MW3_OPT_RADIAL_GRID = 4
 MW3_OPT_ANGULAR_GRID = 5
-MW3_NOPTIONS = 6
-OPTION_NAMES = ("graphs", "radial", "angular", "side_stream", "radial_grid", "angular_grid")
+MW3_OPT_FP16_PRODUCTS = 6
+MW3_NOPTIONS = 7
+OPTION_NAMES = ("graphs", "radial", "angular", "side_stream", "radial_grid", "angular_grid",
+                "fp16_products")
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmw3.so")
 
 _pi = ct.POINTER(ct.c_int32)

@@ -12,7 +12,8 @@ import numpy as np
 import pytest
 
 from madwave3_amd.lib import (MW3_FP64, MW3_MIXED, MW3_OPT_ANGULAR, MW3_OPT_ANGULAR_GRID,
-                              MW3_OPT_RADIAL, MW3_OPT_RADIAL_GRID, MW3_OPT_SIDE_STREAM, Propagator)
+                              MW3_OPT_FP16_PRODUCTS, MW3_OPT_RADIAL, MW3_OPT_RADIAL_GRID,
+                              MW3_OPT_SIDE_STREAM, Propagator)
 from oracle.oracle import Oracle
 from tests.cases import case
 
@@ -95,3 +96,34 @@ def test_autotune_keeps_state_and_accumulators(prec):
     n2, _ = h.steps(40)
     assert rel(n1, n2) < (1e-11 if prec == MW3_FP64 else 1e-5)
     assert rel(g.flux_result()[0], h.flux_result()[0]) < (1e-9 if prec == MW3_FP64 else 1e-4)
+
+
+@pytest.mark.parametrize("name", ["j0", "states", "r512", "r384"])
+def test_fp16_products_option(name):
+    """MW3_OPT_FP16_PRODUCTS: mixed H psi with the inverse R pass and the
+    folded angular kernel multiplying as compensated FP16 pairs (default)
+    and in FP32, both within the mixed bar of the oracle; the two differ
+    (other kernels ran), the FP64 context reports 0, and the autotuner keeps
+    the value it was given."""
+    c = case(name)
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal(c.ntot)
+    h_ref = Oracle(c).hphi(psi)
+    out = {}
+    for fp16 in (1, 0):
+        g = Propagator(c, MW3_MIXED)
+        g.set_option(MW3_OPT_SIDE_STREAM, 0)
+        g.set_option(MW3_OPT_FP16_PRODUCTS, fp16)
+        assert g.get_option(MW3_OPT_FP16_PRODUCTS) == fp16
+        out[fp16] = g.hphi(psi)
+        e = rel(out[fp16], h_ref)
+        print(name, "fp16 products", fp16, "H psi rel err %.2e" % e, flush=True)
+        assert e < 2e-5
+        g.begin(c.rpaq0)
+        chosen, _ = g.autotune(4)
+        assert chosen["fp16_products"] == fp16
+        g.close()
+    assert not np.array_equal(out[0], out[1])
+    g = Propagator(c, MW3_FP64)
+    assert g.get_option(MW3_OPT_FP16_PRODUCTS) == 0
+    g.close()
