@@ -1778,6 +1778,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     const int32_t* __restrict__ groups, int ngroups, int nbc, int na, int n1, int n2) {
   typedef _Float16 H8 __attribute__((ext_vector_type(8)));
   typedef _Float16 H4 __attribute__((ext_vector_type(4)));
+  typedef _Float16 H2 __attribute__((ext_vector_type(2)));
   typedef float V __attribute__((ext_vector_type(4)));
   constexpr int NR = 16 * NT, NC2 = 2 * NR, MK = (NT + 1) / 2;
   constexpr int KP = 32 * MK, NTP = 2 * MK;     // padded depth and output tiles per class
@@ -1859,11 +1860,33 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         const int cp = cls * KP + 32 * (jl >> 5) + 8 * ((jl >> 2) & 3) + 4 * ((jl >> 4) & 1);
         *(H4*)&Di[ia * PB + cp] = hv;
         *(H4*)&Di[SB + ia * PB + cp] = lv;
+      }
+    }
+    // the j-major image in a second pass: a thread takes two consecutive
+    // angles of four functions, so that the lanes of a wave store
+    // consecutive 4-byte words of one image row
+    constexpr int NP = NR / 2 * (NC2 / 4);
+    constexpr int NVP = (NP + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
+    float dp[NVP][2][4];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          Dj[(j0 + r) * PF + ia] = hv[r];
-          Dj[SF + (j0 + r) * PF + ia] = lv[r];
-        }
+    for (int l = 0; l < NVP; l++) {
+      int v = threadIdx.x + 64 * ANG_WAVES * l;
+      v = v < NP ? v : NP - 1;
+      const int ia = 2 * (v % (NR / 2)), j0 = 4 * (v / (NR / 2));
+      ld4(Dc + (int64_t)ia * NC2 + j0, dp[l][0]);
+      ld4(Dc + (int64_t)(ia + 1) * NC2 + j0, dp[l][1]);
+    }
+#pragma unroll
+    for (int l = 0; l < NVP; l++) {
+      int v = threadIdx.x + 64 * ANG_WAVES * l;
+      v = v < NP ? v : NP - 1;
+      const int ia = 2 * (v % (NR / 2)), j0 = 4 * (v / (NR / 2));
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const _Float16 h0 = (_Float16)dp[l][0][r], h1 = (_Float16)dp[l][1][r];
+        *(H2*)&Dj[(j0 + r) * PF + ia] = (H2){h0, h1};
+        *(H2*)&Dj[SF + (j0 + r) * PF + ia] =
+            (H2){(_Float16)(dp[l][0][r] - (float)h0), (_Float16)(dp[l][1][r] - (float)h1)};
       }
     }
     __syncthreads();
