@@ -488,9 +488,9 @@ def main():
         roof = {"bound": work[dom][0], "achieved": d["achieved"], "peak": d["peak"],
                 "unit": d["unit"], "frac": d["frac"], "traffic": load_traffic(dom) if a.config == 2 else None,
                 "kernel": dom, "launches_per_unit": GROUP_LAUNCHES[dom],
-                "unit_of_work": "one Chebyshev step of the group (the scope table's per-step "
-                                "formula); traffic = HBM bytes per step from the committed "
-                                "byte-counter summary"}
+                "unit_of_work": "one Chebyshev step of the group (its per-step work formula, "
+                                "as in `kernels`); traffic = HBM bytes per step from the "
+                                "committed byte-counter summary"}
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(case, a.cpu_steps)
