@@ -134,7 +134,7 @@ static void free_all(Ctx* c) {
                   c->d_dtrans, c->d_ring_prod, c->d_prodwf, c->d_prod_part,
                   c->d_ring_cvjp, c->d_cvjp_last, c->d_zs2prod, c->d_rad_items, c->d_fold_D, c->d_fold_a, c->d_fold_b,
                   c->d_lowT, c->d_mid_items, c->d_mid_sizes, c->d_mid_items2, c->d_mid_wts,
-                  c->d_ang_tiles,
+                  c->d_ang_tiles, c->d_fold_jmap,
                   c->d_ang_groups};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1170,7 +1170,8 @@ static void effective_options(const Ctx* c, int32_t* out) {
   out[MW3_OPT_GRAPHS] = c->graph_mode;
   out[MW3_OPT_RADIAL] = c->opt_radial && (c->rad_nitems || c->mid_nitems) ? 1 : 0;
   int a = c->opt_angular;
-  if (a == 0 && !c->ang_fold) a = 1;
+  const bool cor_fold = c->ang_fold_cor && c->precision == MW3_MIXED && c->opt_fp16;
+  if (a == 0 && !c->ang_fold && !cor_fold) a = 1;
   if (a == 1 && !c->ang_nt && !c->ang_fold) a = 2;
   if (a == 1 && !c->ang_nt) a = 0;
   out[MW3_OPT_ANGULAR] = a;
@@ -1180,7 +1181,8 @@ static void effective_options(const Ctx* c, int32_t* out) {
   // with the table in LDS on a grid of their own
   const bool staged = c->precision != MW3_MIXED && !c->rad_lds64;
   out[MW3_OPT_RADIAL_GRID] = c->rad_nitems && c->opt_radial && staged ? c->opt_rad_grid : 1;
-  out[MW3_OPT_ANGULAR_GRID] = c->ang_fold && out[MW3_OPT_ANGULAR] == 0 ? c->opt_ang_grid : 1;
+  out[MW3_OPT_ANGULAR_GRID] =
+      (c->ang_fold || cor_fold) && out[MW3_OPT_ANGULAR] == 0 ? c->opt_ang_grid : 1;
   out[MW3_OPT_FP16_PRODUCTS] = c->precision == MW3_MIXED ? c->opt_fp16 : 0;
 }
 

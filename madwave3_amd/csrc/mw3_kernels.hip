@@ -1771,11 +1771,13 @@ __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_
 // Both images are built from the FP32 table once per block.  Per group the
 // next group's psi values and coefficients are loaded while this one runs.
 // ---------------------------------------------------------------------------
-template <int NT>
+template <int NT, int PART>
 __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     const float* __restrict__ psi, const float* __restrict__ Dfold,
     const float* __restrict__ afold, const float* __restrict__ bfold, float* __restrict__ ang,
-    const int32_t* __restrict__ groups, int ngroups, int nbc, int na, int n1, int n2) {
+    const int32_t* __restrict__ groups, int ngroups, int nbc, int na, int n1, int n2,
+    float* __restrict__ fbr, const float* __restrict__ phi, const int32_t* __restrict__ jmap,
+    int nJ) {
   typedef _Float16 H8 __attribute__((ext_vector_type(8)));
   typedef _Float16 H4 __attribute__((ext_vector_type(4)));
   typedef _Float16 H2 __attribute__((ext_vector_type(2)));
@@ -1784,8 +1786,8 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   constexpr int KP = 32 * MK, NTP = 2 * MK;     // padded depth and output tiles per class
   constexpr int PF = KP + 8, PB = 2 * KP + 8;  // image pitches (halves)
   constexpr int SF = NC2 * PF, SB = KP * PB;
-  __shared__ __align__(16) _Float16 Dj[2 * SF];  // hi, lo
-  __shared__ __align__(16) _Float16 Di[2 * SB];
+  __shared__ __align__(16) _Float16 Dj[PART == 2 ? 8 : 2 * SF];  // hi, lo
+  __shared__ __align__(16) _Float16 Di[PART == 1 ? 8 : 2 * SB];
   const int pl = n1 * n2, h = na >> 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, i = lane & 15;
@@ -1798,11 +1800,32 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   // a group's inputs: psi at angles 32 m + 8 g + e and their mirrors (angles
   // past h read angle h-1, against zero rows of D), the coefficients of rows
   // j = 16 t + 4 g + r
-  float xl[MK][8], xh[MK][8], ca[2 * NT][4], cb[2 * NT][4];
+  // (parts 1 and 2: the function j at position 16 t + 4 g + r of this
+  // channel's class order, -1 for padding; G read from the coefficient
+  // pass's slab at those functions)
+  float xl[MK][8], xh[MK][8], ca[2 * NT][4], cb[2 * NT][4], gv[2 * NT][4];
+  int jm[2 * NT][4];
+  if constexpr (PART != 0) {
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) jm[t][r] = jmap[ch * NC2 + 16 * t + 4 * g + r];
+  }
+  const float* phc = phi + (int64_t)ch * nJ * pl;
   auto ldall = [&](int col0) {
     int pp = col0 + lane_opaque(i);
     pp = pp < pl ? pp : pl - 1;
     const int gg = lane_opaque(g), ll = lane_opaque(lane);
+    if constexpr (PART == 2) {
+#pragma unroll
+      for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int j = jm[t][r] >= 0 ? jm[t][r] : 0;
+          gv[t][r] = phc[(int64_t)j * pl + pp];
+        }
+      return;
+    }
 #pragma unroll
     for (int m = 0; m < MK; m++)
 #pragma unroll
@@ -1812,6 +1835,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         xl[m][e] = *(const float*)(pb + (uint32_t)(ia * pl + pp) * 4u);
         xh[m][e] = *(const float*)(pb + (uint32_t)((na - 1 - ia) * pl + pp) * 4u);
       }
+    if constexpr (PART == 1) return;
     const int r1 = pp / n2, R = pp - r1 * n2;
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++)
@@ -1826,8 +1850,12 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   const bool busy = gi < ngroups;
   if (busy) ldall(16 * groups[gi]);  // in flight during the image copy
   if constexpr (KP != NR) {  // padded depth: zero images first
-    for (int v = threadIdx.x; v < (SF + SB) / 4; v += 64 * ANG_WAVES)
-      *(H8*)(v < SF / 4 ? &Dj[8 * v] : &Di[8 * (v - SF / 4)]) = (H8){0, 0, 0, 0, 0, 0, 0, 0};
+    if constexpr (PART != 2)
+      for (int v = threadIdx.x; v < SF / 4; v += 64 * ANG_WAVES)
+        *(H8*)&Dj[8 * v] = (H8){0, 0, 0, 0, 0, 0, 0, 0};
+    if constexpr (PART != 1)
+      for (int v = threadIdx.x; v < SB / 4; v += 64 * ANG_WAVES)
+        *(H8*)&Di[8 * v] = (H8){0, 0, 0, 0, 0, 0, 0, 0};
     __syncthreads();
   }
   {
@@ -1836,7 +1864,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     constexpr int NV = (NV4 + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
     constexpr int NG = NV < 4 ? NV : 4;
 #pragma unroll
-    for (int l0 = 0; l0 < NV; l0 += NG) {
+    for (int l0 = 0; l0 < (PART == 1 ? 0 : NV); l0 += NG) {
       float dv[NG][4];
 #pragma unroll
       for (int l = 0; l < NG; l++) {
@@ -1865,9 +1893,9 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     // the j-major image in a second pass: a thread takes two consecutive
     // angles of four functions, so that the lanes of a wave store
     // consecutive 4-byte words of one image row
-    constexpr int NP = NR / 2 * (NC2 / 4);
+    constexpr int NP = PART == 2 ? 0 : NR / 2 * (NC2 / 4);
     constexpr int NVP = (NP + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
-    float dp[NVP][2][4];
+    float dp[NVP > 0 ? NVP : 1][2][4];
 #pragma unroll
     for (int l = 0; l < NVP; l++) {
       int v = threadIdx.x + 64 * ANG_WAVES * l;
@@ -1908,6 +1936,15 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     const int col = 16 * groups[gi];
     const int cnext = 16 * groups[gi + stride < ngroups ? gi + stride : gi];
     const int p = col + i;
+    V f[2 * NT];
+    if constexpr (PART == 2) {
+      // G from the slab (padding positions zero), then the next group's
+#pragma unroll
+      for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) f[t][r] = jm[t][r] >= 0 ? gv[t][r] : 0.0f;
+      ldall(cnext);
+    } else {
     // fold, scale per column and class, split
     float mu = 0.0f, mv = 0.0f;
 #pragma unroll
@@ -1934,13 +1971,14 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         bw[1][m][1][e] = (_Float16)(sv - (float)hv);
       }
     float cs[2 * NT][4];
+    if constexpr (PART == 0) {
 #pragma unroll
-    for (int t = 0; t < 2 * NT; t++)
+      for (int t = 0; t < 2 * NT; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) cs[t][r] = ca[t][r] + cb[t][r];
+        for (int r = 0; r < 4; r++) cs[t][r] = ca[t][r] + cb[t][r];
+    }
     ldall(cnext);  // the next group's inputs, in flight during this one
     // forward: tile pairs (t, t + 1) of one class, MFMAs alternating between them
-    V f[2 * NT];
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++) f[t] = (V){0, 0, 0, 0};
     constexpr int NQF = NT * MK;  // steps: tile pair tp, chunk m
@@ -1970,15 +2008,30 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
       f[t0 + 1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(A[1][1], bw[c1][m][0], f[t0 + 1], 0, 0, 0);
       __builtin_amdgcn_sched_barrier(0);
     }
-    // G = (a + b) F, unscaled, then scaled per column and class and split
+    if constexpr (PART == 1) {
+      // F, unscaled, to the FBR slab rows of its functions
+      const int pp = lane_opaque(p);
+      float* fbc = fbr + (int64_t)ch * nJ * pl;
+#pragma unroll
+      for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (jm[t][r] >= 0 && pp < pl)
+            fbc[(int64_t)jm[t][r] * pl + pp] = ldexpf(f[t][r], t < NT ? -eu : -ev);
+      continue;
+    }
+    // G = (a + b) F, unscaled
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) f[t][r] = ldexpf(f[t][r] * cs[t][r], t < NT ? -eu : -ev);
+    }
+    // G scaled per column and class and split
     float mg[2] = {0.0f, 0.0f};
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++)
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        f[t][r] = ldexpf(f[t][r] * cs[t][r], t < NT ? -eu : -ev);
-        mg[t / NT] = fmaxf(mg[t / NT], fabsf(f[t][r]));
-      }
+      for (int r = 0; r < 4; r++) mg[t / NT] = fmaxf(mg[t / NT], fabsf(f[t][r]));
     const int eg[2] = {expo(maxcol(mg[0])), expo(maxcol(mg[1]))};
     H8 bg[2][MK][2];  // [class][m][hi, lo]: slot 4 hh + r = row r of tile 2 m + hh
 #pragma unroll
@@ -2060,7 +2113,7 @@ template <typename T, int NT>
 static hipError_t ang_fold_grid(Ctx* c, int nc, int ngroups, int* nbc) {
   if constexpr (sizeof(T) == 4) {
     if (c->ang_fold_h) {
-      hipError_t e = ang_fold_grid_k(k_ang_fold_h<NT>, nc, ngroups, &c->ang_fold_h_nbc);
+      hipError_t e = ang_fold_grid_k(k_ang_fold_h<NT, 0>, nc, ngroups, &c->ang_fold_h_nbc);
       if (e != hipSuccess) return e;
     }
   }
@@ -2077,10 +2130,10 @@ static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
   if constexpr (sizeof(T) == 4) {
     if (c->ang_fold_h && c->opt_fp16) {  // products on the FP16 matrix cores, compensated
       const int nbh = std::max(1, c->ang_fold_h_nbc / gdiv);
-      hipLaunchKernelGGL((k_ang_fold_h<NT>), dim3((unsigned)(nbh * t.nc)), dim3(64 * ANG_WAVES), 0,
-                         st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
+      hipLaunchKernelGGL((k_ang_fold_h<NT, 0>), dim3((unsigned)(nbh * t.nc)), dim3(64 * ANG_WAVES),
+                         0, st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
                          (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups,
-                         c->ang_ngroups, nbh, t.na, t.n1, t.n2);
+                         c->ang_ngroups, nbh, t.na, t.n1, t.n2, nullptr, nullptr, nullptr, 0);
       return;
     }
   }
@@ -2116,6 +2169,50 @@ static hipError_t ang_fold_setup(Ctx* c, int nt) {
   }
 }
 
+// the folded parts beside the Coriolis pass (FP32 data): their grids
+template <int NT>
+static hipError_t ang_cor_grid(Ctx* c) {
+  hipError_t e = ang_fold_grid_k(k_ang_fold_h<NT, 1>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[0]);
+  if (e != hipSuccess) return e;
+  return ang_fold_grid_k(k_ang_fold_h<NT, 2>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[1]);
+}
+
+static hipError_t ang_cor_setup(Ctx* c, int nt) {
+  switch (nt) {
+    case 1: return ang_cor_grid<1>(c);
+    case 2: return ang_cor_grid<2>(c);
+    case 3: return ang_cor_grid<3>(c);
+    case 4: return ang_cor_grid<4>(c);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// part 1: psi -> F into the FBR slab (function order); part 2: the
+// coefficient pass's output -> the angular term
+template <int NT, int PART>
+static void launch_ang_cor_nt(Ctx* c, const float* psi, hipStream_t st) {
+  const HostTables& t = c->t;
+  const int gdiv = c->opt_ang_grid > 0 ? c->opt_ang_grid : 1;
+  const int nbc = std::max(1, c->ang_cor_nbc[PART - 1] / gdiv);
+  hipLaunchKernelGGL((k_ang_fold_h<NT, PART>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
+                     st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
+                     (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups, c->ang_ngroups,
+                     nbc, t.na, t.n1, t.n2, (float*)c->d_fbr, (const float*)c->d_phi,
+                     c->d_fold_jmap, t.nJ);
+}
+
+template <int PART>
+static hipError_t launch_ang_cor(Ctx* c, const float* psi, hipStream_t st) {
+  switch (c->ang_fold_cor) {
+    case 1: launch_ang_cor_nt<1, PART>(c, psi, st); break;
+    case 2: launch_ang_cor_nt<2, PART>(c, psi, st); break;
+    case 3: launch_ang_cor_nt<3, PART>(c, psi, st); break;
+    case 4: launch_ang_cor_nt<4, PART>(c, psi, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // the folded angular tables when every channel's functions have the parity
 // symmetry; returns the kernel's tile count (0: not folded)
 static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
@@ -2146,10 +2243,12 @@ static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
   const int NR = 16 * nt, NC2 = 2 * NR;
   std::vector<double> Df((size_t)nc * NR * NC2, 0.0), af((size_t)nc * NC2 * t.n1, 0.0),
       bf((size_t)nc * NC2 * t.n2, 0.0);
+  std::vector<int32_t> jmap((size_t)nc * NC2, -1);
   for (int ch = 0; ch < nc; ch++) {
     int slot[2] = {0, NR};
     for (int j = 0; j < nJ; j++) {
       const int col = slot[cls[(size_t)ch * nJ + j] > 0 ? 0 : 1]++;
+      jmap[(size_t)ch * NC2 + col] = j;
       for (int ia = 0; ia < h; ia++)
         Df[((size_t)ch * NR + ia) * NC2 + col] = t.Dmat[((size_t)ch * na + ia) * t.nJp + j];
       for (int r = 0; r < t.n1; r++)
@@ -2186,6 +2285,10 @@ static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
   };
   if ((e = put_table(&c->d_fold_a, af)) != hipSuccess) return e;
   if ((e = put_table(&c->d_fold_b, bf)) != hipSuccess) return e;
+  if ((e = hipMalloc(&c->d_fold_jmap, jmap.size() * sizeof(int32_t))) != hipSuccess) return e;
+  if ((e = hipMemcpy(c->d_fold_jmap, jmap.data(), jmap.size() * sizeof(int32_t),
+                     hipMemcpyHostToDevice)) != hipSuccess)
+    return e;
   *nt_out = nt;
   return hipSuccess;
 }
@@ -4009,7 +4112,9 @@ hipError_t build_plans(Ctx* c) {
   // some angle (the mask is the same for every channel)
   const bool tiles_fit = (plane + 16 * ANG_WAVES - 1) / (16 * ANG_WAVES) * t.nc < (1 << 24);
   if (!tiles_fit) c->ang_nt = 0;  // tile entries keep the index in 24 bits
-  if (tiles_fit && (c->ang_nt || (c->ang_fused && offs32))) {
+  // (also for the folded parts beside the Coriolis pass: mixed precision)
+  const bool cor_try = !c->ang_fused && c->precision == MW3_MIXED && offs32;
+  if (tiles_fit && (c->ang_nt || (c->ang_fused && offs32) || cor_try)) {
     const int64_t tw = 16 * ANG_WAVES, tpc = (plane + tw - 1) / tw;
     std::vector<uint8_t> act((size_t)tpc, 0);  // bit w: wave w's 16 columns
     for (int ia = 0; ia < t.na; ia++)
@@ -4055,6 +4160,18 @@ hipError_t build_plans(Ctx* c) {
       e = c->precision == MW3_MIXED ? ang_fold_setup<float>(c, ntf) : ang_fold_setup<double>(c, ntf);
       if (e != hipSuccess) return e;
       c->ang_fold = ntf;
+    }
+  }
+  // with Coriolis coupling (mixed precision): the folded forward part into
+  // the FBR slab and the folded back part from the coefficient pass's output,
+  // the coefficient pass between them as in the GEMM variant
+  c->ang_fold_cor = 0;
+  if (cor_try && tiles_fit && c->ang_ngroups > 0) {
+    int ntf = 0;
+    if ((e = build_ang_fold(c, &ntf)) != hipSuccess) return e;
+    if (ntf > 0) {
+      if ((e = ang_cor_setup(c, ntf)) != hipSuccess) return e;
+      c->ang_fold_cor = ntf;
     }
   }
   return hipSuccess;
@@ -4106,6 +4223,9 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   const bool use_mid = c->mid_nitems && c->opt_radial;
   const bool use_fold = c->ang_fold && c->opt_angular == 0;
   const bool use_one = c->ang_nt && c->opt_angular <= 1 && !use_fold;
+  // folded parts beside the Coriolis pass (mixed precision, FP16 products)
+  const bool use_cor = sizeof(T) == 4 && c->ang_fold_cor && c->opt_angular == 0 && c->opt_fp16 &&
+                       !use_fold && !use_one && !c->ang_fused;
   // the forward R pass first (measured 1.3% faster per FP64 step than after
   // the angular term)
   timer_begin(c, K_DST_R);
@@ -4142,9 +4262,16 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     if (e != hipSuccess) return e;
     timer_end(c, K_ANG_FWD, sa);
   } else {
-    e = run_gemm<G_ANG_FWD, T, T, T, T>(sa, (const T*)c->d_Dmat, psi, (T*)c->d_fbr,
-                                        c->p_angF);
-    if (e != hipSuccess) return e;
+    if constexpr (sizeof(T) == 4) {
+      if (use_cor) {
+        if ((e = launch_ang_cor<1>(c, psi, sa)) != hipSuccess) return e;
+      }
+    }
+    if (!use_cor) {
+      e = run_gemm<G_ANG_FWD, T, T, T, T>(sa, (const T*)c->d_Dmat, psi, (T*)c->d_fbr,
+                                          c->p_angF);
+      if (e != hipSuccess) return e;
+    }
     timer_end(c, K_ANG_FWD, sa);
     timer_begin(c, K_CORIOLIS, sa);
     const unsigned nall = (unsigned)((int64_t)t.nJ * ((t.n1 + FBR_RB - 1) / FBR_RB));
@@ -4174,9 +4301,17 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   }
   if (!use_one && !use_fold) {
     timer_begin(c, K_ANG_BWD, sa);
-    e = run_gemm<G_ANG_BWD, T, T, T, T>(sa, (const T*)c->d_Dmat, (const T*)c->d_phi, (T*)c->d_ang,
-                                        c->p_angB);
-    if (e != hipSuccess) return e;
+    if constexpr (sizeof(T) == 4) {
+      if (use_cor) {
+        // (the psi argument is not read by the back part)
+        if ((e = launch_ang_cor<2>(c, (const float*)c->d_phi, sa)) != hipSuccess) return e;
+      }
+    }
+    if (!use_cor) {
+      e = run_gemm<G_ANG_BWD, T, T, T, T>(sa, (const T*)c->d_Dmat, (const T*)c->d_phi,
+                                          (T*)c->d_ang, c->p_angB);
+      if (e != hipSuccess) return e;
+    }
     timer_end(c, K_ANG_BWD, sa);
   }
   if (sa != s) {
