@@ -301,12 +301,15 @@ MW3_API int32_t mw3_cheby_spectrum(const double *autoc, int32_t kmax, double emi
                    0: kernel by kernel.  Same kernels, same order of every
                    sum: results are bitwise identical.
    MW3_OPT_RADIAL  1 (default): the radial kinetic correction by the
-                   persistent low-mode kernels (R passes with the sine table
-                   held in registers, the two r passes in one kernel) where
-                   the grid allows them; 0: as batched GEMMs.
+                   low-mode kernels (R passes with the sine table in LDS, one
+                   wave per 16-row work item; the two r passes in one kernel)
+                   where the grid allows them; 0: as batched GEMMs.
    MW3_OPT_ANGULAR 0 (default): the angular term by the parity-folded
-                   one-pass kernel; 1: the unfolded one-pass kernel; 2: two
-                   GEMMs with the FBR coefficients between them.  A variant
+                   one-pass kernel (with Coriolis-coupled channels, in
+                   MW3_MIXED with FP16 products: its forward and back parts
+                   around the coefficient pass); 1: the unfolded one-pass
+                   kernel; 2: two GEMMs with the FBR coefficients between
+                   them.  A variant
                    not built for the context falls back to the next one.
    MW3_OPT_SIDE_STREAM  1: the angular term on its own stream beside the
                    radial passes (default in MW3_MIXED); 0: one stream
