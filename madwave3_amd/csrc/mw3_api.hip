@@ -134,7 +134,7 @@ static void free_all(Ctx* c) {
                   c->d_dtrans, c->d_ring_prod, c->d_prodwf, c->d_prod_part,
                   c->d_ring_cvjp, c->d_cvjp_last, c->d_zs2prod, c->d_rad_items, c->d_fold_D, c->d_fold_a, c->d_fold_b,
                   c->d_lowT, c->d_mid_items, c->d_mid_sizes, c->d_mid_items2, c->d_mid_wts,
-                  c->d_ang_tiles, c->d_fold_jmap,
+                  c->d_ang_tiles, c->d_fold_jmap, c->d_fold_bm, c->d_fold_bp,
                   c->d_ang_groups};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

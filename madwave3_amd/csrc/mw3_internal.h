@@ -222,8 +222,10 @@ struct Ctx {
   bool ang_fold_h = false;         // FP32 folded angular kernel with compensated FP16 products
   int ang_fold_h_nbc = 0;          // its blocks per channel
   int ang_fold_cor = 0;            // tiles of the folded forward / back parts with Coriolis (mixed)
-  int ang_cor_nbc[2] = {0, 0};     // their blocks per channel
+  int ang_cor_nbc[3] = {0, 0, 0};  // their blocks per channel (forward, back, back with the stencil)
   int32_t* d_fold_jmap = nullptr;  // [nc][32 tiles] function j at each class-order position, or -1
+  void* d_fold_bm = nullptr;       // [nc][32 tiles][n2] b_j(R) of the Omega - 1 neighbour, class order
+  void* d_fold_bp = nullptr;       // the same for Omega + 1
   int opt_fp16 = 1;                // MW3_OPT_FP16_PRODUCTS (mixed only)
   int ang_grid = 0;        // its persistent blocks
   int32_t* d_ang_tiles = nullptr;  // tiles of the one-pass kernels with compressed-grid points

@@ -1777,7 +1777,8 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     const float* __restrict__ afold, const float* __restrict__ bfold, float* __restrict__ ang,
     const int32_t* __restrict__ groups, int ngroups, int nbc, int na, int n1, int n2,
     float* __restrict__ fbr, const float* __restrict__ phi, const int32_t* __restrict__ jmap,
-    int nJ) {
+    int nJ, const float* __restrict__ bmf, const float* __restrict__ bpf,
+    const int32_t* __restrict__ nbr) {
   typedef _Float16 H8 __attribute__((ext_vector_type(8)));
   typedef _Float16 H4 __attribute__((ext_vector_type(4)));
   typedef _Float16 H2 __attribute__((ext_vector_type(2)));
@@ -1786,7 +1787,8 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   constexpr int KP = 32 * MK, NTP = 2 * MK;     // padded depth and output tiles per class
   constexpr int PF = KP + 8, PB = 2 * KP + 8;  // image pitches (halves)
   constexpr int SF = NC2 * PF, SB = KP * PB;
-  __shared__ __align__(16) _Float16 Dj[PART == 2 ? 8 : 2 * SF];  // hi, lo
+  constexpr bool BACK = PART >= 2;  // back product only
+  __shared__ __align__(16) _Float16 Dj[BACK ? 8 : 2 * SF];  // hi, lo
   __shared__ __align__(16) _Float16 Di[PART == 1 ? 8 : 2 * SB];
   const int pl = n1 * n2, h = na >> 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1804,7 +1806,25 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   // channel's class order, -1 for padding; G read from the coefficient
   // pass's slab at those functions)
   float xl[MK][8], xh[MK][8], ca[2 * NT][4], cb[2 * NT][4], gv[2 * NT][4];
+  float fm[2 * NT][4], fp[2 * NT][4], cm[2 * NT][4], cq[2 * NT][4];
   int jm[2 * NT][4];
+  // part 3: G = (a + b) F + b_m F(Omega - 1) + b_p F(Omega + 1) formed here
+  // from the FBR slab (function order, so a neighbour's row is the same j);
+  // the term of an absent neighbour is dropped (its row reads the channel's
+  // own, times zero)
+  int sm = ch, sp = ch;
+  float wm = 0.0f, wp = 0.0f;
+  if constexpr (PART == 3) {
+    sm = nbr[ch * 3 + 0] >= 0 ? nbr[ch * 3 + 0] : ch;
+    sp = nbr[ch * 3 + 2] >= 0 ? nbr[ch * 3 + 2] : ch;
+    wm = nbr[ch * 3 + 0] >= 0 ? 1.0f : 0.0f;
+    wp = nbr[ch * 3 + 2] >= 0 ? 1.0f : 0.0f;
+  }
+  const float* fbo = fbr + (int64_t)ch * nJ * pl;
+  const float* fbm = fbr + (int64_t)sm * nJ * pl;
+  const float* fbp = fbr + (int64_t)sp * nJ * pl;
+  const char* mc = (const char*)(bmf + (int64_t)ch * NC2 * n2);
+  const char* qc = (const char*)(bpf + (int64_t)ch * NC2 * n2);
   if constexpr (PART != 0) {
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++)
@@ -1823,6 +1843,18 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         for (int r = 0; r < 4; r++) {
           const int j = jm[t][r] >= 0 ? jm[t][r] : 0;
           gv[t][r] = phc[(int64_t)j * pl + pp];
+        }
+      return;
+    }
+    if constexpr (PART == 3) {  // F only: the coefficients load per group (registers)
+#pragma unroll
+      for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int64_t o = (int64_t)(jm[t][r] >= 0 ? jm[t][r] : 0) * pl + pp;
+          gv[t][r] = fbo[o];
+          fm[t][r] = fbm[o];
+          fp[t][r] = fbp[o];
         }
       return;
     }
@@ -1846,11 +1878,27 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         cb[t][r] = *(const float*)(bc + (uint32_t)(j * n2 + R) * 4u);
       }
   };
+  // part 3: the group's four coefficient rows
+  auto ldcoef3 = [&](int col0) {
+    int pp = col0 + lane_opaque(i);
+    pp = pp < pl ? pp : pl - 1;
+    const int ll = lane_opaque(lane), r1 = pp / n2, R = pp - r1 * n2;
+#pragma unroll
+    for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int j = 16 * t + 4 * (ll >> 4) + r;
+        ca[t][r] = *(const float*)(ac + (uint32_t)(j * n1 + r1) * 4u);
+        cb[t][r] = *(const float*)(bc + (uint32_t)(j * n2 + R) * 4u);
+        cm[t][r] = *(const float*)(mc + (uint32_t)(j * n2 + R) * 4u);
+        cq[t][r] = *(const float*)(qc + (uint32_t)(j * n2 + R) * 4u);
+      }
+  };
   int gi = 2 * (bl + nbc * (wave >> 1)) + (wave & 1);
   const bool busy = gi < ngroups;
   if (busy) ldall(16 * groups[gi]);  // in flight during the image copy
   if constexpr (KP != NR) {  // padded depth: zero images first
-    if constexpr (PART != 2)
+    if constexpr (!BACK)
       for (int v = threadIdx.x; v < SF / 4; v += 64 * ANG_WAVES)
         *(H8*)&Dj[8 * v] = (H8){0, 0, 0, 0, 0, 0, 0, 0};
     if constexpr (PART != 1)
@@ -1893,7 +1941,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     // the j-major image in a second pass: a thread takes two consecutive
     // angles of four functions, so that the lanes of a wave store
     // consecutive 4-byte words of one image row
-    constexpr int NP = PART == 2 ? 0 : NR / 2 * (NC2 / 4);
+    constexpr int NP = BACK ? 0 : NR / 2 * (NC2 / 4);
     constexpr int NVP = (NP + 64 * ANG_WAVES - 1) / (64 * ANG_WAVES);
     float dp[NVP > 0 ? NVP : 1][2][4];
 #pragma unroll
@@ -1943,6 +1991,17 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
       for (int t = 0; t < 2 * NT; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) f[t][r] = jm[t][r] >= 0 ? gv[t][r] : 0.0f;
+      ldall(cnext);
+    } else if constexpr (PART == 3) {
+      ldcoef3(col);
+#pragma unroll
+      for (int t = 0; t < 2 * NT; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const float gq = (ca[t][r] + cb[t][r]) * gv[t][r] + wm * cm[t][r] * fm[t][r] +
+                           wp * cq[t][r] * fp[t][r];
+          f[t][r] = jm[t][r] >= 0 ? gq : 0.0f;
+        }
       ldall(cnext);
     } else {
     // fold, scale per column and class, split
@@ -2133,7 +2192,8 @@ static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
       hipLaunchKernelGGL((k_ang_fold_h<NT, 0>), dim3((unsigned)(nbh * t.nc)), dim3(64 * ANG_WAVES),
                          0, st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
                          (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups,
-                         c->ang_ngroups, nbh, t.na, t.n1, t.n2, nullptr, nullptr, nullptr, 0);
+                         c->ang_ngroups, nbh, t.na, t.n1, t.n2, nullptr, nullptr, nullptr, 0,
+                         nullptr, nullptr, nullptr);
       return;
     }
   }
@@ -2174,7 +2234,12 @@ template <int NT>
 static hipError_t ang_cor_grid(Ctx* c) {
   hipError_t e = ang_fold_grid_k(k_ang_fold_h<NT, 1>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[0]);
   if (e != hipSuccess) return e;
-  return ang_fold_grid_k(k_ang_fold_h<NT, 2>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[1]);
+  e = ang_fold_grid_k(k_ang_fold_h<NT, 2>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[1]);
+  if (e != hipSuccess) return e;
+  // part 3 up to three tiles (at four it would spill registers)
+  if constexpr (NT <= 3)
+    return ang_fold_grid_k(k_ang_fold_h<NT, 3>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[2]);
+  return hipSuccess;
 }
 
 static hipError_t ang_cor_setup(Ctx* c, int nt) {
@@ -2188,9 +2253,12 @@ static hipError_t ang_cor_setup(Ctx* c, int nt) {
 }
 
 // part 1: psi -> F into the FBR slab (function order); part 2: the
-// coefficient pass's output -> the angular term
+// coefficient pass's output -> the angular term; part 3: the FBR slab ->
+// the angular term with the coefficients and the Omega +- 1 terms formed in
+// the kernel (no coefficient pass)
 template <int NT, int PART>
 static void launch_ang_cor_nt(Ctx* c, const float* psi, hipStream_t st) {
+  static_assert(PART != 3 || NT <= 3, "part 3 up to three tiles");
   const HostTables& t = c->t;
   const int gdiv = c->opt_ang_grid > 0 ? c->opt_ang_grid : 1;
   const int nbc = std::max(1, c->ang_cor_nbc[PART - 1] / gdiv);
@@ -2198,7 +2266,8 @@ static void launch_ang_cor_nt(Ctx* c, const float* psi, hipStream_t st) {
                      st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
                      (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups, c->ang_ngroups,
                      nbc, t.na, t.n1, t.n2, (float*)c->d_fbr, (const float*)c->d_phi,
-                     c->d_fold_jmap, t.nJ);
+                     c->d_fold_jmap, t.nJ, (const float*)c->d_fold_bm, (const float*)c->d_fold_bp,
+                     c->d_nbr);
 }
 
 template <int PART>
@@ -2207,7 +2276,10 @@ static hipError_t launch_ang_cor(Ctx* c, const float* psi, hipStream_t st) {
     case 1: launch_ang_cor_nt<1, PART>(c, psi, st); break;
     case 2: launch_ang_cor_nt<2, PART>(c, psi, st); break;
     case 3: launch_ang_cor_nt<3, PART>(c, psi, st); break;
-    case 4: launch_ang_cor_nt<4, PART>(c, psi, st); break;
+    case 4:
+      if constexpr (PART == 3) return hipErrorInvalidValue;
+      else launch_ang_cor_nt<4, PART>(c, psi, st);
+      break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -2244,6 +2316,7 @@ static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
   std::vector<double> Df((size_t)nc * NR * NC2, 0.0), af((size_t)nc * NC2 * t.n1, 0.0),
       bf((size_t)nc * NC2 * t.n2, 0.0);
   std::vector<int32_t> jmap((size_t)nc * NC2, -1);
+  std::vector<double> bmf((size_t)nc * NC2 * t.n2, 0.0), bpf((size_t)nc * NC2 * t.n2, 0.0);
   for (int ch = 0; ch < nc; ch++) {
     int slot[2] = {0, NR};
     for (int j = 0; j < nJ; j++) {
@@ -2253,9 +2326,12 @@ static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
         Df[((size_t)ch * NR + ia) * NC2 + col] = t.Dmat[((size_t)ch * na + ia) * t.nJp + j];
       for (int r = 0; r < t.n1; r++)
         af[((size_t)ch * NC2 + col) * t.n1 + r] = t.acoef[((size_t)ch * nJ + j) * t.n1 + r];
-      for (int R = 0; R < t.n2; R++)
+      for (int R = 0; R < t.n2; R++) {
         bf[((size_t)ch * NC2 + col) * t.n2 + R] =
             t.bcoef[(((size_t)ch * 3 + 1) * nJ + j) * t.n2 + R];
+        bmf[((size_t)ch * NC2 + col) * t.n2 + R] = t.bcoef[(((size_t)ch * 3 + 0) * nJ + j) * t.n2 + R];
+        bpf[((size_t)ch * NC2 + col) * t.n2 + R] = t.bcoef[(((size_t)ch * 3 + 2) * nJ + j) * t.n2 + R];
+      }
     }
   }
   hipError_t e;
@@ -2285,6 +2361,8 @@ static hipError_t build_ang_fold(Ctx* c, int* nt_out) {
   };
   if ((e = put_table(&c->d_fold_a, af)) != hipSuccess) return e;
   if ((e = put_table(&c->d_fold_b, bf)) != hipSuccess) return e;
+  if ((e = put_table(&c->d_fold_bm, bmf)) != hipSuccess) return e;
+  if ((e = put_table(&c->d_fold_bp, bpf)) != hipSuccess) return e;
   if ((e = hipMalloc(&c->d_fold_jmap, jmap.size() * sizeof(int32_t))) != hipSuccess) return e;
   if ((e = hipMemcpy(c->d_fold_jmap, jmap.data(), jmap.size() * sizeof(int32_t),
                      hipMemcpyHostToDevice)) != hipSuccess)
@@ -4226,6 +4304,9 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // folded parts beside the Coriolis pass (mixed precision, FP16 products)
   const bool use_cor = sizeof(T) == 4 && c->ang_fold_cor && c->opt_angular == 0 && c->opt_fp16 &&
                        !use_fold && !use_one && !c->ang_fused;
+  // its back part forms the coefficients and the Omega +- 1 terms itself (no
+  // coefficient pass), up to three tiles
+  const bool use_cor3 = use_cor && c->ang_fold_cor <= 3;
   // the forward R pass first (measured 1.3% faster per FP64 step than after
   // the angular term)
   timer_begin(c, K_DST_R);
@@ -4273,6 +4354,14 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
       if (e != hipSuccess) return e;
     }
     timer_end(c, K_ANG_FWD, sa);
+  }
+  if (use_cor3) {
+    if constexpr (sizeof(T) == 4) {
+      timer_begin(c, K_ANG_BWD, sa);
+      if ((e = launch_ang_cor<3>(c, (const float*)c->d_phi, sa)) != hipSuccess) return e;
+      timer_end(c, K_ANG_BWD, sa);
+    }
+  } else if (!use_fold && !use_one && !c->ang_fused) {
     timer_begin(c, K_CORIOLIS, sa);
     const unsigned nall = (unsigned)((int64_t)t.nJ * ((t.n1 + FBR_RB - 1) / FBR_RB));
     switch (c->fbr_all ? t.nc : 0) {
@@ -4299,7 +4388,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
     if ((e = hipGetLastError()) != hipSuccess) return e;
     timer_end(c, K_CORIOLIS, sa);
   }
-  if (!use_one && !use_fold) {
+  if (!use_one && !use_fold && !use_cor3) {
     timer_begin(c, K_ANG_BWD, sa);
     if constexpr (sizeof(T) == 4) {
       if (use_cor) {
