@@ -1825,7 +1825,13 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   const float* fbp = fbr + (int64_t)sp * nJ * pl;
   const char* mc = (const char*)(bmf + (int64_t)ch * NC2 * n2);
   const char* qc = (const char*)(bpf + (int64_t)ch * NC2 * n2);
-  if constexpr (PART != 0) {
+  // part 3 keeps the map in LDS (registers) and loads each group's inputs at
+  // its start
+  __shared__ int jms[PART == 3 ? NC2 : 1];
+  if constexpr (PART == 3) {
+    for (int v = threadIdx.x; v < NC2; v += 64 * ANG_WAVES) jms[v] = jmap[ch * NC2 + v];
+  }
+  if constexpr (PART == 1 || PART == 2) {
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++)
 #pragma unroll
@@ -1846,12 +1852,13 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         }
       return;
     }
-    if constexpr (PART == 3) {  // F only: the coefficients load per group (registers)
+    if constexpr (PART == 3) {  // F only: the coefficients load separately
 #pragma unroll
       for (int t = 0; t < 2 * NT; t++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          const int64_t o = (int64_t)(jm[t][r] >= 0 ? jm[t][r] : 0) * pl + pp;
+          const int jj = jms[16 * t + 4 * gg + r];
+          const int64_t o = (int64_t)(jj >= 0 ? jj : 0) * pl + pp;
           gv[t][r] = fbo[o];
           fm[t][r] = fbm[o];
           fp[t][r] = fbp[o];
@@ -1896,7 +1903,9 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   };
   int gi = 2 * (bl + nbc * (wave >> 1)) + (wave & 1);
   const bool busy = gi < ngroups;
-  if (busy) ldall(16 * groups[gi]);  // in flight during the image copy
+  if constexpr (PART != 3) {
+    if (busy) ldall(16 * groups[gi]);  // in flight during the image copy
+  }
   if constexpr (KP != NR) {  // padded depth: zero images first
     if constexpr (!BACK)
       for (int v = threadIdx.x; v < SF / 4; v += 64 * ANG_WAVES)
@@ -1993,6 +2002,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         for (int r = 0; r < 4; r++) f[t][r] = jm[t][r] >= 0 ? gv[t][r] : 0.0f;
       ldall(cnext);
     } else if constexpr (PART == 3) {
+      ldall(col);
       ldcoef3(col);
 #pragma unroll
       for (int t = 0; t < 2 * NT; t++)
@@ -2000,9 +2010,8 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
         for (int r = 0; r < 4; r++) {
           const float gq = (ca[t][r] + cb[t][r]) * gv[t][r] + wm * cm[t][r] * fm[t][r] +
                            wp * cq[t][r] * fp[t][r];
-          f[t][r] = jm[t][r] >= 0 ? gq : 0.0f;
+          f[t][r] = jms[16 * t + 4 * g + r] >= 0 ? gq : 0.0f;
         }
-      ldall(cnext);
     } else {
     // fold, scale per column and class, split
     float mu = 0.0f, mv = 0.0f;
@@ -2236,10 +2245,7 @@ static hipError_t ang_cor_grid(Ctx* c) {
   if (e != hipSuccess) return e;
   e = ang_fold_grid_k(k_ang_fold_h<NT, 2>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[1]);
   if (e != hipSuccess) return e;
-  // part 3 up to three tiles (at four it would spill registers)
-  if constexpr (NT <= 3)
-    return ang_fold_grid_k(k_ang_fold_h<NT, 3>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[2]);
-  return hipSuccess;
+  return ang_fold_grid_k(k_ang_fold_h<NT, 3>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[2]);
 }
 
 static hipError_t ang_cor_setup(Ctx* c, int nt) {
@@ -2258,7 +2264,6 @@ static hipError_t ang_cor_setup(Ctx* c, int nt) {
 // the kernel (no coefficient pass)
 template <int NT, int PART>
 static void launch_ang_cor_nt(Ctx* c, const float* psi, hipStream_t st) {
-  static_assert(PART != 3 || NT <= 3, "part 3 up to three tiles");
   const HostTables& t = c->t;
   const int gdiv = c->opt_ang_grid > 0 ? c->opt_ang_grid : 1;
   const int nbc = std::max(1, c->ang_cor_nbc[PART - 1] / gdiv);
@@ -2276,10 +2281,7 @@ static hipError_t launch_ang_cor(Ctx* c, const float* psi, hipStream_t st) {
     case 1: launch_ang_cor_nt<1, PART>(c, psi, st); break;
     case 2: launch_ang_cor_nt<2, PART>(c, psi, st); break;
     case 3: launch_ang_cor_nt<3, PART>(c, psi, st); break;
-    case 4:
-      if constexpr (PART == 3) return hipErrorInvalidValue;
-      else launch_ang_cor_nt<4, PART>(c, psi, st);
-      break;
+    case 4: launch_ang_cor_nt<4, PART>(c, psi, st); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -4305,8 +4307,8 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   const bool use_cor = sizeof(T) == 4 && c->ang_fold_cor && c->opt_angular == 0 && c->opt_fp16 &&
                        !use_fold && !use_one && !c->ang_fused;
   // its back part forms the coefficients and the Omega +- 1 terms itself (no
-  // coefficient pass), up to three tiles
-  const bool use_cor3 = use_cor && c->ang_fold_cor <= 3;
+  // coefficient pass)
+  const bool use_cor3 = use_cor;
   // the forward R pass first (measured 1.3% faster per FP64 step than after
   // the angular term)
   timer_begin(c, K_DST_R);
