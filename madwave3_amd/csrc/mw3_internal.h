@@ -222,7 +222,7 @@ struct Ctx {
   bool ang_fold_h = false;         // FP32 folded angular kernel with compensated FP16 products
   int ang_fold_h_nbc = 0;          // its blocks per channel
   int ang_fold_cor = 0;            // tiles of the folded forward / back parts with Coriolis (mixed)
-  int ang_cor_nbc[3] = {0, 0, 0};  // their blocks per channel (forward, back, back with the stencil)
+  int ang_cor_nbc[3] = {0, 0, 0};  // their blocks per channel (index part - 1: parts 1 and 3)
   int32_t* d_fold_jmap = nullptr;  // [nc][32 tiles] function j at each class-order position, or -1
   void* d_fold_bm = nullptr;       // [nc][32 tiles][n2] b_j(R) of the Omega - 1 neighbour, class order
   void* d_fold_bp = nullptr;       // the same for Omega + 1

@@ -1770,14 +1770,20 @@ __global__ __launch_bounds__(64 * ANG_WAVES, sizeof(T) == 4 ? 4 : 1) void k_ang_
 //     Di stores each class's columns permuted to that order (16-byte reads).
 // Both images are built from the FP32 table once per block.  Per group the
 // next group's psi values and coefficients are loaded while this one runs.
+// PART 0 is the whole pass.  With Coriolis-coupled channels the two products
+// run as two launches: PART 1 stores F, unscaled, into the FBR slab at the
+// rows of its functions (jmap: function at each class-order position, -1
+// for padding); PART 3 forms G = (a + b) F + b_m F(Omega-1) + b_p F(Omega+1)
+// from that slab (bmf, bpf: the neighbour coefficients in class order; nbr:
+// the neighbour channels) and runs the back product.
 // ---------------------------------------------------------------------------
 template <int NT, int PART>
 __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     const float* __restrict__ psi, const float* __restrict__ Dfold,
     const float* __restrict__ afold, const float* __restrict__ bfold, float* __restrict__ ang,
     const int32_t* __restrict__ groups, int ngroups, int nbc, int na, int n1, int n2,
-    float* __restrict__ fbr, const float* __restrict__ phi, const int32_t* __restrict__ jmap,
-    int nJ, const float* __restrict__ bmf, const float* __restrict__ bpf,
+    float* __restrict__ fbr, const int32_t* __restrict__ jmap, int nJ,
+    const float* __restrict__ bmf, const float* __restrict__ bpf,
     const int32_t* __restrict__ nbr) {
   typedef _Float16 H8 __attribute__((ext_vector_type(8)));
   typedef _Float16 H4 __attribute__((ext_vector_type(4)));
@@ -1787,7 +1793,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   constexpr int KP = 32 * MK, NTP = 2 * MK;     // padded depth and output tiles per class
   constexpr int PF = KP + 8, PB = 2 * KP + 8;  // image pitches (halves)
   constexpr int SF = NC2 * PF, SB = KP * PB;
-  constexpr bool BACK = PART >= 2;  // back product only
+  constexpr bool BACK = PART == 3;  // back product only
   __shared__ __align__(16) _Float16 Dj[BACK ? 8 : 2 * SF];  // hi, lo
   __shared__ __align__(16) _Float16 Di[PART == 1 ? 8 : 2 * SB];
   const int pl = n1 * n2, h = na >> 1;
@@ -1831,27 +1837,16 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
   if constexpr (PART == 3) {
     for (int v = threadIdx.x; v < NC2; v += 64 * ANG_WAVES) jms[v] = jmap[ch * NC2 + v];
   }
-  if constexpr (PART == 1 || PART == 2) {
+  if constexpr (PART == 1) {
 #pragma unroll
     for (int t = 0; t < 2 * NT; t++)
 #pragma unroll
       for (int r = 0; r < 4; r++) jm[t][r] = jmap[ch * NC2 + 16 * t + 4 * g + r];
   }
-  const float* phc = phi + (int64_t)ch * nJ * pl;
   auto ldall = [&](int col0) {
     int pp = col0 + lane_opaque(i);
     pp = pp < pl ? pp : pl - 1;
     const int gg = lane_opaque(g), ll = lane_opaque(lane);
-    if constexpr (PART == 2) {
-#pragma unroll
-      for (int t = 0; t < 2 * NT; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          const int j = jm[t][r] >= 0 ? jm[t][r] : 0;
-          gv[t][r] = phc[(int64_t)j * pl + pp];
-        }
-      return;
-    }
     if constexpr (PART == 3) {  // F only: the coefficients load separately
 #pragma unroll
       for (int t = 0; t < 2 * NT; t++)
@@ -1994,14 +1989,7 @@ __global__ __launch_bounds__(64 * ANG_WAVES, 2) void k_ang_fold_h(
     const int cnext = 16 * groups[gi + stride < ngroups ? gi + stride : gi];
     const int p = col + i;
     V f[2 * NT];
-    if constexpr (PART == 2) {
-      // G from the slab (padding positions zero), then the next group's
-#pragma unroll
-      for (int t = 0; t < 2 * NT; t++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) f[t][r] = jm[t][r] >= 0 ? gv[t][r] : 0.0f;
-      ldall(cnext);
-    } else if constexpr (PART == 3) {
+    if constexpr (PART == 3) {
       ldall(col);
       ldcoef3(col);
 #pragma unroll
@@ -2201,8 +2189,8 @@ static void launch_ang_fold_nt(Ctx* c, const T* psi, hipStream_t st) {
       hipLaunchKernelGGL((k_ang_fold_h<NT, 0>), dim3((unsigned)(nbh * t.nc)), dim3(64 * ANG_WAVES),
                          0, st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
                          (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups,
-                         c->ang_ngroups, nbh, t.na, t.n1, t.n2, nullptr, nullptr, nullptr, 0,
-                         nullptr, nullptr, nullptr);
+                         c->ang_ngroups, nbh, t.na, t.n1, t.n2, nullptr, nullptr, 0, nullptr,
+                         nullptr, nullptr);
       return;
     }
   }
@@ -2243,8 +2231,6 @@ template <int NT>
 static hipError_t ang_cor_grid(Ctx* c) {
   hipError_t e = ang_fold_grid_k(k_ang_fold_h<NT, 1>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[0]);
   if (e != hipSuccess) return e;
-  e = ang_fold_grid_k(k_ang_fold_h<NT, 2>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[1]);
-  if (e != hipSuccess) return e;
   return ang_fold_grid_k(k_ang_fold_h<NT, 3>, c->t.nc, c->ang_ngroups, &c->ang_cor_nbc[2]);
 }
 
@@ -2258,10 +2244,9 @@ static hipError_t ang_cor_setup(Ctx* c, int nt) {
   }
 }
 
-// part 1: psi -> F into the FBR slab (function order); part 2: the
-// coefficient pass's output -> the angular term; part 3: the FBR slab ->
-// the angular term with the coefficients and the Omega +- 1 terms formed in
-// the kernel (no coefficient pass)
+// part 1: psi -> F into the FBR slab (function order); part 3: the FBR slab
+// -> the angular term, with the coefficients and the Omega +- 1 terms formed
+// in the kernel
 template <int NT, int PART>
 static void launch_ang_cor_nt(Ctx* c, const float* psi, hipStream_t st) {
   const HostTables& t = c->t;
@@ -2270,8 +2255,7 @@ static void launch_ang_cor_nt(Ctx* c, const float* psi, hipStream_t st) {
   hipLaunchKernelGGL((k_ang_fold_h<NT, PART>), dim3((unsigned)(nbc * t.nc)), dim3(64 * ANG_WAVES), 0,
                      st, psi, (const float*)c->d_fold_D, (const float*)c->d_fold_a,
                      (const float*)c->d_fold_b, (float*)c->d_ang, c->d_ang_groups, c->ang_ngroups,
-                     nbc, t.na, t.n1, t.n2, (float*)c->d_fbr, (const float*)c->d_phi,
-                     c->d_fold_jmap, t.nJ, (const float*)c->d_fold_bm, (const float*)c->d_fold_bp,
+                     nbc, t.na, t.n1, t.n2, (float*)c->d_fbr, c->d_fold_jmap, t.nJ, (const float*)c->d_fold_bm, (const float*)c->d_fold_bp,
                      c->d_nbr);
 }
 
@@ -4306,8 +4290,8 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   // folded parts beside the Coriolis pass (mixed precision, FP16 products)
   const bool use_cor = sizeof(T) == 4 && c->ang_fold_cor && c->opt_angular == 0 && c->opt_fp16 &&
                        !use_fold && !use_one && !c->ang_fused;
-  // its back part forms the coefficients and the Omega +- 1 terms itself (no
-  // coefficient pass)
+  // (its back part forms the coefficients and the Omega +- 1 terms itself:
+  // no coefficient pass)
   const bool use_cor3 = use_cor;
   // the forward R pass first (measured 1.3% faster per FP64 step than after
   // the angular term)
@@ -4360,6 +4344,7 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   if (use_cor3) {
     if constexpr (sizeof(T) == 4) {
       timer_begin(c, K_ANG_BWD, sa);
+      // (the psi argument is not read by part 3)
       if ((e = launch_ang_cor<3>(c, (const float*)c->d_phi, sa)) != hipSuccess) return e;
       timer_end(c, K_ANG_BWD, sa);
     }
@@ -4392,17 +4377,9 @@ static hipError_t hpsi_impl(Ctx* c, const double* psi64) {
   }
   if (!use_one && !use_fold && !use_cor3) {
     timer_begin(c, K_ANG_BWD, sa);
-    if constexpr (sizeof(T) == 4) {
-      if (use_cor) {
-        // (the psi argument is not read by the back part)
-        if ((e = launch_ang_cor<2>(c, (const float*)c->d_phi, sa)) != hipSuccess) return e;
-      }
-    }
-    if (!use_cor) {
-      e = run_gemm<G_ANG_BWD, T, T, T, T>(sa, (const T*)c->d_Dmat, (const T*)c->d_phi,
-                                          (T*)c->d_ang, c->p_angB);
-      if (e != hipSuccess) return e;
-    }
+    e = run_gemm<G_ANG_BWD, T, T, T, T>(sa, (const T*)c->d_Dmat, (const T*)c->d_phi,
+                                        (T*)c->d_ang, c->p_angB);
+    if (e != hipSuccess) return e;
     timer_end(c, K_ANG_BWD, sa);
   }
   if (sa != s) {
